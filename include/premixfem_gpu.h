@@ -1,0 +1,158 @@
+/*
+ * premixfem_gpu.h -- C ABI of the MI355X implementation of PreMixFEM's
+ * three-level GMsFEM preconditioner for the 3D mixed-FEM (RT0/P0,
+ * velocity-eliminated) Darcy pressure system.
+ *
+ * Every entry point replaces one function of the reference's
+ * src/PreMixFEM_3D.h (or the PETSc call it relies on); the citation is given
+ * above each declaration.  Signatures use plain pointers and sizes only.
+ * All entry points return 0 on success or one of the PMX_ERR_* codes, whose
+ * values equal the PETSc error codes the reference raises in the same
+ * situation (PETSC_ERR_ARG_WRONG for bad options, PETSC_ERR_USER when the
+ * eigensolver cannot deliver enough eigenvectors, ...).
+ *
+ * Host arrays are in the reference's natural DMDA order arr[ez][ey][ex]
+ * (x fastest), length M*N*P.  One context drives one GPU (the current HIP
+ * device when pmx_init is called); calls on one context must not overlap.
+ */
+#ifndef PREMIXFEM_GPU_H
+#define PREMIXFEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: the PETSc values the reference's PetscCheck/PetscCall use. */
+#define PMX_SUCCESS 0
+#define PMX_ERR_MEM 55            /* PETSC_ERR_MEM */
+#define PMX_ERR_SUP 56            /* PETSC_ERR_SUP: option not implemented */
+#define PMX_ERR_ARG_WRONG 62      /* PETSC_ERR_ARG_WRONG (src/PreMixFEM_3D.c:1653-1690) */
+#define PMX_ERR_ARG_OUTOFRANGE 63 /* PETSC_ERR_ARG_OUTOFRANGE */
+#define PMX_ERR_ORDER 73          /* PETSC_ERR_ORDER: call sequence violated */
+#define PMX_ERR_LIB 76            /* PETSC_ERR_LIB: HIP runtime failure */
+#define PMX_ERR_USER 83           /* PETSC_ERR_USER: eigensolver shortfall (:367, :1098) */
+#define PMX_ERR_ARG_NULL 85       /* PETSC_ERR_ARG_NULL */
+#define PMX_ERR_NOT_CONVERGED 91  /* PETSC_ERR_NOT_CONVERGED */
+
+/* Eigen mass operators (src/PreMixFEM_3D.h:28-30). */
+#define PMX_EIG_OP_MOD 0
+#define PMX_EIG_OP_MEAN 1
+#define PMX_EIG_OP_DIAG 2
+
+/* Stage timer ids (src/PreMixFEM_3D.h:16-26). */
+#define PMX_STAGE_SU_LV1 0
+#define PMX_STAGE_SU_LV2_1 1
+#define PMX_STAGE_SU_LV2_2 2
+#define PMX_STAGE_SU_LV3_1 3
+#define PMX_STAGE_SU_LV3_2 4
+#define PMX_STAGE_AV_LV1 5
+#define PMX_STAGE_AV_LV2 6
+#define PMX_STAGE_AV_LV3 7
+#define PMX_STAGE_SU 8
+#define PMX_STAGE_AV 9
+#define PMX_MAX_LOG_STAGES 10
+
+/* PCG convergence reasons (PETSc KSPConvergedReason values). */
+#define PMX_CONVERGED_RTOL 2
+#define PMX_CONVERGED_ATOL 3
+#define PMX_DIVERGED_ITS (-3)
+#define PMX_DIVERGED_DTOL (-4)
+#define PMX_DIVERGED_INDEFINITE_MAT (-8)
+
+typedef struct pmx_ctx pmx_ctx;
+
+/* Statistics PC_print_stat reports (src/PreMixFEM_3D.c:1938-1999), plus the
+ * eigensolver and outer-solver counters of this implementation. */
+typedef struct {
+  int M, N, P;
+  int sd[3];
+  int nsub;
+  int size_coarse;           /* level-2 coarse size (sum of eigen counts) */
+  int size_mat_lv3;          /* level-3 size (eigen_num_lv2) */
+  int eigen_num_lv1_range[2];
+  double eigen_min_lv1_range[2];
+  double eigen_max_lv1_range[2];
+  int eigen_num_lv2;
+  double eigen_min_lv2, eigen_max_lv2;
+  double t_stages[PMX_MAX_LOG_STAGES]; /* seconds, same ids as the reference */
+  int eig_iters_max;         /* LOBPCG iterations, worst subdomain */
+  double eig_iters_mean;
+  int eig_fallbacks;         /* subdomains restarted from a random block */
+  int lv3_iters;
+  int pc_precision;          /* 64 or 32 */
+  int its;                   /* last pmx_solve iteration count */
+  double rnorm, rnorm0;      /* last pmx_solve final / initial residual norms */
+  int reason;                /* last pmx_solve convergence reason */
+} pmx_stats;
+
+/* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),
+ * cells mesh[3] (M, N, P) and an options string with the reference's option
+ * names (-sd -en_lv1 -en_lv2 -si_lv1 -si_lv2 -eg_op -eb_lv1 -eb_lv2
+ * -use_W_cycle -no_shift_A_cc -use_full_Cholesky_lv1 -use_2level), plus
+ * -sdx/-sdy/-sdz (per-direction subdomains), -pin x,y,z, -pc_precision 32|64,
+ * -eig_tol, -eig_maxit, -warm_start none|cnn|basis, -seed.  Allocates the
+ * context and selects the current HIP device. */
+int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
+
+/* PC_print_info (src/PreMixFEM_3D.c:1743-1755). */
+int pmx_print_info(const pmx_ctx *ctx);
+
+/* Caller fills s_ctx.kappa[0..2] (src/Expe_scal.c:11-42): the three
+ * permeability components, host arrays of M*N*P doubles. */
+int pmx_set_kappa(pmx_ctx *ctx, const double *kx, const double *ky, const double *kz);
+
+/* PC_create_A (src/PreMixFEM_3D.c:1757-1824): the 7-point operator as seven
+ * diagonals in natural order: diag[c] and the couplings to the -x,+x,-y,+y,
+ * -z,+z neighbours (off[6*c + d], zero where there is no neighbour).  The
+ * pin term, if requested, is included in diag. */
+int pmx_create_A(pmx_ctx *ctx, double *diag, double *off);
+
+/* PC_setup (src/PreMixFEM_3D.c:1826-1838 -> _PC_setup :1392-1427): builds
+ * level 1 (block ICC(0)), level 2 (per-subdomain eigenproblems by batched
+ * LOBPCG, Galerkin coarse matrix and its FP64 factorisation) and level 3. */
+int pmx_setup(pmx_ctx *ctx);
+
+/* PC_setup with the level-2 eigenvectors supplied by the caller (host,
+ * natural order, en_lv1 columns of M*N*P doubles each, column j of every
+ * subdomain in vector j; eigen_num per subdomain, may be NULL = en_lv1).
+ * Replaces _PC_setup_lv2_eigen (:224-421) by the given basis; everything
+ * else as pmx_setup.  Used for restarts from a previous basis. */
+int pmx_setup_from_basis(pmx_ctx *ctx, const double *phi, const int *eigen_num,
+                         const double *eigvals);
+
+/* PC_apply_vec (src/PreMixFEM_3D.c:1840-1936): y = B x, host arrays. */
+int pmx_apply(pmx_ctx *ctx, const double *x, double *y);
+
+/* PETSc KSPSolve with -ksp_type cg, KSP_NORM_UNPRECONDITIONED and the
+ * PCSHELL above (src/Expe_scal.c:101-119), run entirely on the GPU.
+ * b, x: host arrays.  x0_nonzero selects the spe10-style initial guess
+ * (src/spe10.c:191).  On return *its and *rnorm hold the iteration count and
+ * the final unpreconditioned residual norm. */
+int pmx_solve(pmx_ctx *ctx, const double *b, double *x, int x0_nonzero, double rtol,
+              int max_it, int *its, double *rnorm);
+
+/* PC_print_stat / PC_get_range (src/PreMixFEM_3D.c:1938-1999). */
+int pmx_get_stats(const pmx_ctx *ctx, pmx_stats *st);
+int pmx_print_stat(const pmx_ctx *ctx);
+
+/* The ms_bases_c / ms_bases_cc vectors of PCCtx (src/PreMixFEM_3D.h:34):
+ * level 2 (level=2): en_lv1 vectors of M*N*P doubles (zero outside each
+ * subdomain; vector j holds eigenvector j of every subdomain); level 3
+ * (level=3): eigen_num_lv2 vectors.  eigvals (optional) receives the
+ * eigenvalues ([nsub][en_lv1] for level 2, [eigen_num_lv2] for level 3). */
+int pmx_get_basis(const pmx_ctx *ctx, int level, double *out, double *eigvals);
+
+/* The level-2 Galerkin matrix Phi^T A Phi (dense, size_coarse^2, row-major). */
+int pmx_get_coarse_matrix(const pmx_ctx *ctx, double *out);
+
+/* PC_final / PC_final_default (src/PreMixFEM_3D.c:2001-2046). */
+int pmx_final(pmx_ctx *ctx);
+
+/* Last error message of this thread (empty if none). */
+const char *pmx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PREMIXFEM_GPU_H */

@@ -1,0 +1,715 @@
+// Fine-grid kernels: layout permutation, face coefficients, eigen mass,
+// level-1 block ICC(0) (factor and wavefront solve), residual + restriction,
+// prolongation and the PCG vector kernels.  All are HBM-bandwidth bound; each
+// workgroup owns one subdomain brick so every access is to a contiguous block.
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+// ---------------------------------------------------------------------------
+// Natural <-> brick permutation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int brick_of_natural(const Geo& g, int c) {
+  const int x = c % g.M;
+  const int y = (c / g.M) % g.N;
+  const int z = c / (g.M * g.N);
+  const int s = g.ax_sub[0][x] + g.sdx * (g.ax_sub[1][y] + g.sdy * g.ax_sub[2][z]);
+  const Brick& b = g.bricks[s];
+  return b.off + (g.ax_loc[2][z] * b.ny + g.ax_loc[1][y]) * b.nx + g.ax_loc[0][x];
+}
+
+template <typename S, typename D>
+__global__ void k_nat2brick(Geo g, const S* __restrict__ src, D* __restrict__ dst, int nvec,
+                            int src_stride, int dst_stride) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const int bi = brick_of_natural(g, c);
+  for (int v = 0; v < nvec; ++v) dst[(size_t)v * dst_stride + bi] = (D)src[(size_t)v * src_stride + c];
+}
+
+template <typename S, typename D>
+__global__ void k_brick2nat(Geo g, const S* __restrict__ src, D* __restrict__ dst, int nvec,
+                            int src_stride, int dst_stride) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const int bi = brick_of_natural(g, c);
+  for (int v = 0; v < nvec; ++v) dst[(size_t)v * dst_stride + c] = (D)src[(size_t)v * src_stride + bi];
+}
+
+// Row-major [cell][k] basis <-> natural column vectors.
+__global__ void k_basis_nat2rows(Geo g, const double* __restrict__ src, double* __restrict__ rows,
+                                 int k) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const int bi = brick_of_natural(g, c);
+  for (int j = 0; j < k; ++j) rows[(size_t)bi * k + j] = src[(size_t)j * g.ncell + c];
+}
+
+__global__ void k_basis_rows2nat(Geo g, const double* __restrict__ rows, double* __restrict__ dst,
+                                 int k) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ncell) return;
+  const int bi = brick_of_natural(g, c);
+  for (int j = 0; j < k; ++j) dst[(size_t)j * g.ncell + c] = rows[(size_t)bi * k + j];
+}
+
+// ---------------------------------------------------------------------------
+// Face coefficients T = (|F|^2/|E|) * 2/(1/k_L + 1/k_R)
+// (src/PreMixFEM_3D.c:1785-1786, :1797-1798, :1809-1810).  T[d][c] is the
+// coefficient of the face between c and its lower neighbour in direction d,
+// 0 on the domain boundary (no-flow).
+// ---------------------------------------------------------------------------
+__global__ void k_trans(Geo g, const double* __restrict__ kx, const double* __restrict__ ky,
+                        const double* __restrict__ kz, double cx, double cy, double cz,
+                        double* __restrict__ Tx, double* __restrict__ Ty, double* __restrict__ Tz,
+                        float* __restrict__ Tx32, float* __restrict__ Ty32,
+                        float* __restrict__ Tz32) {
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    double t[3];
+    const double* kk[3] = {kx, ky, kz};
+    const double cc[3] = {cx, cy, cz};
+    for (int d = 0; d < 3; ++d) {
+      const int nb = nbr_index(g, b, c, lx, ly, lz, 2 * d);
+      t[d] = nb < 0 ? 0.0 : cc[d] * (2.0 / (1.0 / kk[d][nb] + 1.0 / kk[d][c]));
+    }
+    Tx[c] = t[0];
+    Ty[c] = t[1];
+    Tz[c] = t[2];
+    if (Tx32) {
+      Tx32[c] = (float)t[0];
+      Ty32[c] = (float)t[1];
+      Tz32[c] = (float)t[2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Eigen mass diagonal (src/PreMixFEM_3D.c:265-343).  MOD: per cell, in the
+// reference's accumulation order: [lower x if first cell of the row], upper x,
+// lower y, upper y, lower z, upper z, [upper x again if not last of the row];
+// each term 2*kbar/H/H, then scaled by |E|.  MEAN: kx+ky+kz.  DIAG: diagonal
+// of the Neumann block.
+// ---------------------------------------------------------------------------
+__global__ void k_mass(Geo g, int op, const double* __restrict__ kx, const double* __restrict__ ky,
+                       const double* __restrict__ kz, const double* __restrict__ Tx,
+                       const double* __restrict__ Ty, const double* __restrict__ Tz, double Hx,
+                       double Hy, double Hz, double meas_elem, double* __restrict__ m) {
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    if (op == PMX_EIG_OP_MEAN) {
+      m[c] = kx[c] + ky[c] + kz[c];
+      continue;
+    }
+    if (op == PMX_EIG_OP_DIAG) {
+      double v = 0.0;
+      if (lx > 0) v += Tx[c];
+      if (lx < b.nx - 1) v += Tx[c + 1];
+      if (ly > 0) v += Ty[c];
+      if (ly < b.ny - 1) v += Ty[c + b.nx];
+      if (lz > 0) v += Tz[c];
+      if (lz < b.nz - 1) v += Tz[c + b.nx * b.ny];
+      m[c] = v;
+      continue;
+    }
+    const double* kk[3] = {kx, ky, kz};
+    const double H[3] = {Hx, Hy, Hz};
+    double lo[3], hi[3];
+    bool has_lo[3], has_hi[3];
+    for (int d = 0; d < 3; ++d) {
+      const int nl = nbr_index(g, b, c, lx, ly, lz, 2 * d);
+      const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
+      has_lo[d] = nl >= 0;
+      has_hi[d] = nh >= 0;
+      lo[d] = has_lo[d] ? 2.0 * (2.0 / (1.0 / kk[d][nl] + 1.0 / kk[d][c])) / H[d] / H[d] : 0.0;
+      hi[d] = has_hi[d] ? 2.0 * (2.0 / (1.0 / kk[d][c] + 1.0 / kk[d][nh])) / H[d] / H[d] : 0.0;
+    }
+    double v = 0.0;
+    if (lx == 0 && has_lo[0]) v = lo[0];
+    if (has_hi[0]) v = v + hi[0];
+    if (has_lo[1]) v = v + lo[1];
+    if (has_hi[1]) v = v + hi[1];
+    if (has_lo[2]) v = v + lo[2];
+    if (has_hi[2]) v = v + hi[2];
+    if (lx < b.nx - 1) v = v + hi[0];
+    m[c] = v * meas_elem;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level-1 block ICC(0) factor by local hyperplanes lx+ly+lz = s.
+// The 7-point graph has no triangles, so ICC(0) in natural order keeps only
+// the pivots: d = a - cz^2/d_{-z} - cy^2/d_{-y} - cx^2/d_{-x} (couplings inside
+// the subdomain only).  mode 0: a = diagonal of A[Omega_i, Omega_i] in the
+// accumulation order of _PC_setup_lv1_J (src/PreMixFEM_3D.c:93-197), + pin;
+// mode 1: a = Neumann diagonal + sigma * mass (the LOBPCG preconditioner).
+// Writes plane-ordered {dinv, cx, cy, cz} for the lower and upper couplings.
+// ---------------------------------------------------------------------------
+template <typename F4T>
+__global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
+                             const double* __restrict__ Ty, const double* __restrict__ Tz,
+                             const double* __restrict__ mass, double sigma,
+                             F4T* __restrict__ fac_f, F4T* __restrict__ fac_b) {
+  extern __shared__ double dsh[];
+  const Brick b = g.bricks[blockIdx.x];
+  const int nxny = b.nx * b.ny;
+  const int* ps = g.plane_start + g.shape_pstart[b.shape];
+  const int* pc = g.plane_cell + g.shape_coff[b.shape];
+  const double* TT[3] = {Tx, Ty, Tz};
+  for (int s = 0; s < b.nplanes; ++s) {
+    for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
+      const int pk = pc[pos];
+      const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+      const int li = (lz * b.ny + ly) * b.nx + lx;
+      const int c = b.off + li;
+      double lo[3], hi[3];
+      bool in_hi[3];
+      const int lc[3] = {lx, ly, lz};
+      const int ln[3] = {b.nx, b.ny, b.nz};
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = TT[d][c];  // 0 at the domain boundary
+        const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
+        hi[d] = nh >= 0 ? TT[d][nh] : 0.0;
+        in_hi[d] = lc[d] < ln[d] - 1;
+      }
+      double a;
+      if (mode == 0) {
+        a = lo[0];
+        if (!in_hi[0]) a = a + hi[0];
+        a = a + lo[1];
+        if (!in_hi[1]) a = a + hi[1];
+        a = a + lo[2];
+        if (!in_hi[2]) a = a + hi[2];
+        if (in_hi[0]) a = a + hi[0];
+        if (in_hi[1]) a = a + hi[1];
+        if (in_hi[2]) a = a + hi[2];
+        if (c == g.pin) a = a + g.tau;
+      } else {
+        a = 0.0;
+        if (lx > 0) a += lo[0];
+        if (in_hi[0]) a += hi[0];
+        if (ly > 0) a += lo[1];
+        if (in_hi[1]) a += hi[1];
+        if (lz > 0) a += lo[2];
+        if (in_hi[2]) a += hi[2];
+        a += sigma * mass[c];
+      }
+      if (lz > 0) a = a - lo[2] * lo[2] / dsh[li - nxny];
+      if (ly > 0) a = a - lo[1] * lo[1] / dsh[li - b.nx];
+      if (lx > 0) a = a - lo[0] * lo[0] / dsh[li - 1];
+      dsh[li] = a;
+    }
+    __syncthreads();
+  }
+  const int n = b.nx * b.ny * b.nz;
+  for (int pos = threadIdx.x; pos < n; pos += blockDim.x) {
+    const int pk = pc[pos];
+    const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+    const int li = (lz * b.ny + ly) * b.nx + lx;
+    const int c = b.off + li;
+    const double dinv = 1.0 / dsh[li];
+    F4T f, u;
+    f.x = dinv;
+    f.y = lx > 0 ? Tx[c] : 0.0;
+    f.z = ly > 0 ? Ty[c] : 0.0;
+    f.w = lz > 0 ? Tz[c] : 0.0;
+    u.x = dinv;
+    u.y = lx < b.nx - 1 ? Tx[c + 1] : 0.0;
+    u.z = ly < b.ny - 1 ? Ty[c + b.nx] : 0.0;
+    u.w = lz < b.nz - 1 ? Tz[c + nxny] : 0.0;
+    fac_f[b.off + pos] = f;
+    fac_b[b.off + pos] = u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront sweeps of the block ICC(0) solve on a brick held in LDS:
+// forward  w = (r + cx w_{-x} + cy w_{-y} + cz w_{-z}) * dinv
+// backward y = w + (cx' y_{+x} + cy' y_{+y} + cz' y_{+z}) * dinv
+// i.e. y = [(D+L) D^{-1} (D+L^T)]^{-1} r.  Factor data is plane-ordered so
+// the loads of one hyperplane are contiguous.
+// ---------------------------------------------------------------------------
+template <typename S, typename F4T>
+__device__ __forceinline__ void dic_sweeps(const Brick& b, const int* ps, const int* pc,
+                                           const F4T* __restrict__ fac_f,
+                                           const F4T* __restrict__ fac_b, S* v) {
+  const int nxny = b.nx * b.ny;
+  for (int s = 0; s < b.nplanes; ++s) {
+    for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
+      const int pk = pc[pos];
+      const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+      const int li = (lz * b.ny + ly) * b.nx + lx;
+      const F4T f = fac_f[b.off + pos];
+      S acc = v[li];
+      if (lx > 0) acc += (S)f.y * v[li - 1];
+      if (ly > 0) acc += (S)f.z * v[li - b.nx];
+      if (lz > 0) acc += (S)f.w * v[li - nxny];
+      v[li] = acc * (S)f.x;
+    }
+    __syncthreads();
+  }
+  for (int s = b.nplanes - 1; s >= 0; --s) {
+    for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
+      const int pk = pc[pos];
+      const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+      const int li = (lz * b.ny + ly) * b.nx + lx;
+      const F4T u = fac_b[b.off + pos];
+      S acc = (S)0;
+      if (lx < b.nx - 1) acc += (S)u.y * v[li + 1];
+      if (ly < b.ny - 1) acc += (S)u.z * v[li + b.nx];
+      if (lz < b.nz - 1) acc += (S)u.w * v[li + nxny];
+      v[li] = v[li] + acc * (S)u.x;
+    }
+    __syncthreads();
+  }
+}
+
+// A x for one cell (brick order) with face coefficients of type TT.
+template <typename TT, typename XT, typename ACC>
+__device__ __forceinline__ ACC stencil_cell(const Geo& g, const Brick& b, int c, int lx, int ly,
+                                            int lz, const TT* __restrict__ Tx,
+                                            const TT* __restrict__ Ty,
+                                            const TT* __restrict__ Tz, const XT* __restrict__ x) {
+  const ACC xc = (ACC)x[c];
+  ACC acc = (ACC)0;
+  ACC diag = (ACC)0;
+  const TT* TT3[3] = {Tx, Ty, Tz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int nl = nbr_index(g, b, c, lx, ly, lz, 2 * d);
+    if (nl >= 0) {
+      const ACC t = (ACC)TT3[d][c];
+      diag += t;
+      acc += t * (ACC)x[nl];
+    }
+    const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
+    if (nh >= 0) {
+      const ACC t = (ACC)TT3[d][nh];
+      diag += t;
+      acc += t * (ACC)x[nh];
+    }
+  }
+  if (c == g.pin) diag += (ACC)g.tau;
+  return diag * xc - acc;
+}
+
+// Stage "lv1" of PC_apply_vec with y = 0 (so r = x exactly): y = B1 x.
+template <typename S, typename F4T>
+__global__ void k_lv1_first(Geo g, const double* __restrict__ x, S* __restrict__ y,
+                            const F4T* __restrict__ fac_f, const F4T* __restrict__ fac_b) {
+  extern __shared__ char shraw[];
+  S* v = reinterpret_cast<S*>(shraw);
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) v[li] = (S)x[b.off + li];
+  __syncthreads();
+  dic_sweeps<S, F4T>(b, g.plane_start + g.shape_pstart[b.shape],
+                     g.plane_cell + g.shape_coff[b.shape], fac_f, fac_b, v);
+  for (int li = threadIdx.x; li < n; li += blockDim.x) y[b.off + li] = v[li];
+}
+
+// Final lv1 stage: r = x - A y, z = y + B1 r, and partial sums of z.x
+// (the PCG's beta = z'r, the PC input x being the PCG residual).
+template <typename S, typename TT, typename F4T>
+__global__ void k_lv1_last(Geo g, const double* __restrict__ x, const S* __restrict__ y,
+                           const TT* __restrict__ Tx, const TT* __restrict__ Ty,
+                           const TT* __restrict__ Tz, const F4T* __restrict__ fac_f,
+                           const F4T* __restrict__ fac_b, double* __restrict__ z,
+                           S* __restrict__ yout, double* __restrict__ partial) {
+  extern __shared__ char shraw[];
+  S* v = reinterpret_cast<S*>(shraw);
+  __shared__ double red[8];
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    const S ay = stencil_cell<TT, S, S>(g, b, c, lx, ly, lz, Tx, Ty, Tz, y);
+    v[li] = (S)x[c] - ay;
+  }
+  __syncthreads();
+  dic_sweeps<S, F4T>(b, g.plane_start + g.shape_pstart[b.shape],
+                     g.plane_cell + g.shape_coff[b.shape], fac_f, fac_b, v);
+  double acc = 0.0;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int c = b.off + li;
+    const S yn = y[c] + v[li];
+    if (z) {
+      const double zz = (double)yn;
+      z[c] = zz;
+      acc += zz * x[c];
+    }
+    if (yout) yout[c] = yn;
+  }
+  if (partial) {
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  }
+}
+
+// r = x - A y and c[dof+j] = sum_cells phi[cell][j] r[cell]  (the VecMDot of
+// _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1564).  y == nullptr means y = 0.
+template <typename S, typename TT>
+__global__ void k_resid_restrict(Geo g, const double* __restrict__ x, const S* __restrict__ y,
+                                 const TT* __restrict__ Tx, const TT* __restrict__ Ty,
+                                 const TT* __restrict__ Tz, const S* __restrict__ phi, int kmax,
+                                 double* __restrict__ cvec) {
+  __shared__ double red[8];
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  double acc[kMaxEig];
+#pragma unroll
+  for (int j = 0; j < kMaxEig; ++j) acc[j] = 0.0;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    double r = x[c];
+    if (y) r -= (double)stencil_cell<TT, S, S>(g, b, c, lx, ly, lz, Tx, Ty, Tz, y);
+    const S* ph = phi + (size_t)c * kmax;
+#pragma unroll
+    for (int j = 0; j < kMaxEig; ++j)
+      if (j < b.neig) acc[j] += (double)ph[j] * r;
+  }
+  for (int j = 0; j < b.neig; ++j) {
+    const double s = block_sum(acc[j], red);
+    if (threadIdx.x == 0) cvec[b.dof + j] = s;
+  }
+}
+
+// y += phi c  (the VecMAXPY of _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1578).
+template <typename S>
+__global__ void k_prolong(Geo g, S* __restrict__ y, const S* __restrict__ phi, int kmax,
+                          const double* __restrict__ cvec) {
+  __shared__ double cs[kMaxEig];
+  const Brick b = g.bricks[blockIdx.x];
+  if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
+  __syncthreads();
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int c = b.off + li;
+    const S* ph = phi + (size_t)c * kmax;
+    S acc = (S)0;
+    for (int j = 0; j < b.neig; ++j) acc += ph[j] * (S)cs[j];
+    y[c] = y[c] + acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCG kernels (FP64, KSPSolve_CG of PETSc; scalars live on the device).
+// scal[]: 0 beta=z'r, 1 betaold, 2 dpi=p'w, 3 dpiold, 4 rnorm^2, 5 flags
+// ---------------------------------------------------------------------------
+// p_new = z + (beta/betaold) p_old (p_new = z on the first iteration),
+// w = A p_new, partial p_new'w.  p_old is read at neighbours, so p_new goes to
+// a second buffer.
+__global__ void k_cg_p_spmv(Geo g, const double* __restrict__ z, const double* __restrict__ pold,
+                            double* __restrict__ pnew, double* __restrict__ w,
+                            const double* __restrict__ Tx, const double* __restrict__ Ty,
+                            const double* __restrict__ Tz, const double* __restrict__ scal,
+                            int first, double* __restrict__ partial) {
+  __shared__ double red[8];
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  const double bc = first ? 0.0 : scal[0] / scal[1];
+  double acc = 0.0;
+  const double* TT3[3] = {Tx, Ty, Tz};
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    const double pc = first ? z[c] : z[c] + bc * pold[c];
+    double diag = 0.0, off = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int nl = nbr_index(g, b, c, lx, ly, lz, 2 * d);
+      if (nl >= 0) {
+        const double t = TT3[d][c];
+        diag += t;
+        off += t * (first ? z[nl] : z[nl] + bc * pold[nl]);
+      }
+      const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
+      if (nh >= 0) {
+        const double t = TT3[d][nh];
+        diag += t;
+        off += t * (first ? z[nh] : z[nh] + bc * pold[nh]);
+      }
+    }
+    if (c == g.pin) diag += g.tau;
+    const double wc = diag * pc - off;
+    pnew[c] = pc;
+    w[c] = wc;
+    acc += pc * wc;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// x += a p, r -= a w, partial r'r with a = beta/dpi.
+__global__ void k_cg_update(int n, double* __restrict__ x, double* __restrict__ r,
+                            const double* __restrict__ p, const double* __restrict__ w,
+                            const double* __restrict__ scal, double* __restrict__ partial) {
+  __shared__ double red[8];
+  const double a = scal[0] / scal[2];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] += a * p[i];
+    const double rr = r[i] - a * w[i];
+    r[i] = rr;
+    acc += rr * rr;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// r = b - A x, partial r'r.
+__global__ void k_residual(Geo g, const double* __restrict__ bvec, const double* __restrict__ x,
+                           double* __restrict__ r, const double* __restrict__ Tx,
+                           const double* __restrict__ Ty, const double* __restrict__ Tz,
+                           double* __restrict__ partial) {
+  __shared__ double red[8];
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  double acc = 0.0;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    double rr = bvec[c];
+    if (x) rr -= stencil_cell<double, double, double>(g, b, c, lx, ly, lz, Tx, Ty, Tz, x);
+    r[c] = rr;
+    acc += rr * rr;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// y = A x (brick order), FP64.
+__global__ void k_spmv(Geo g, const double* __restrict__ x, double* __restrict__ y,
+                       const double* __restrict__ Tx, const double* __restrict__ Ty,
+                       const double* __restrict__ Tz) {
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+    const int c = b.off + li;
+    y[c] = stencil_cell<double, double, double>(g, b, c, lx, ly, lz, Tx, Ty, Tz, x);
+  }
+}
+
+// Deterministic finalisation of per-block partial sums into scal[].
+// op 0: beta (z'r): scal[1] = scal[0]; scal[0] = sum
+// op 1: dpi  (p'w): scal[3] = scal[2]; scal[2] = sum; flag indefinite
+// op 2: rnorm^2   : scal[4] = sum
+__global__ void k_finalize(const double* __restrict__ partial, int nb, double* __restrict__ scal,
+                           int op, int first) {
+  __shared__ double red[8];
+  double acc = 0.0;
+  // Fixed assignment of partials to threads and a fixed tree: bitwise
+  // reproducible from run to run.
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    if (op == 0) {
+      scal[1] = scal[0];
+      scal[0] = acc;
+    } else if (op == 1) {
+      scal[3] = scal[2];
+      scal[2] = acc;
+      const bool flip = !first && ((acc > 0.0) != (scal[3] > 0.0));
+      if (acc == 0.0 || flip) scal[5] = 1.0;
+    } else {
+      scal[4] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+// ---------------------------------------------------------------------------
+size_t brick_lds_bytes(const pmx_ctx* ctx, size_t elem) {
+  int mx = 0;
+  for (const Brick& b : ctx->h_bricks) mx = std::max(mx, b.nx * b.ny * b.nz);
+  return (size_t)mx * elem;
+}
+
+int launch_nat2brick(pmx_ctx* ctx, const double* src, double* dst, int nvec, int sstride,
+                     int dstride) {
+  const int nb = (ctx->ncell + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((k_nat2brick<double, double>), dim3(nb), dim3(kBlock), 0, ctx->stream,
+                     ctx->geo, src, dst, nvec, sstride, dstride);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_brick2nat(pmx_ctx* ctx, const double* src, double* dst, int nvec, int sstride,
+                     int dstride) {
+  const int nb = (ctx->ncell + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((k_brick2nat<double, double>), dim3(nb), dim3(kBlock), 0, ctx->stream,
+                     ctx->geo, src, dst, nvec, sstride, dstride);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_basis_nat2rows(pmx_ctx* ctx, const double* src, double* rows, int k) {
+  const int nb = (ctx->ncell + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_basis_nat2rows, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->geo, src,
+                     rows, k);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_basis_rows2nat(pmx_ctx* ctx, const double* rows, double* dst, int k) {
+  const int nb = (ctx->ncell + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_basis_rows2nat, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->geo, rows,
+                     dst, k);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_trans(pmx_ctx* ctx) {
+  const bool f32 = ctx->pc_precision == 32;
+  hipLaunchKernelGGL(k_trans, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                     ctx->kappa[0].p, ctx->kappa[1].p, ctx->kappa[2].p, ctx->coef[0],
+                     ctx->coef[1], ctx->coef[2], ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                     f32 ? ctx->T32[0].p : nullptr, f32 ? ctx->T32[1].p : nullptr,
+                     f32 ? ctx->T32[2].p : nullptr);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_mass(pmx_ctx* ctx) {
+  hipLaunchKernelGGL(k_mass, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, ctx->eg_op,
+                     ctx->kappa[0].p, ctx->kappa[1].p, ctx->kappa[2].p, ctx->T[0].p, ctx->T[1].p,
+                     ctx->T[2].p, ctx->Hx, ctx->Hy, ctx->Hz, ctx->meas_elem, ctx->mass.p);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* fac_b,
+                      bool f32) {
+  const size_t lds = brick_lds_bytes(ctx, sizeof(double));
+  if (f32)
+    hipLaunchKernelGGL(k_dic_factor<float4>, dim3(ctx->nsub), dim3(kBlock), lds, ctx->stream,
+                       ctx->geo, mode, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->mass.p, sigma,
+                       (float4*)fac_f, (float4*)fac_b);
+  else
+    hipLaunchKernelGGL(k_dic_factor<double4>, dim3(ctx->nsub), dim3(kBlock), lds, ctx->stream,
+                       ctx->geo, mode, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->mass.p, sigma,
+                       (double4*)fac_f, (double4*)fac_b);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_lv1_first(pmx_ctx* ctx, const double* x) {
+  if (ctx->pc_precision == 32) {
+    hipLaunchKernelGGL((k_lv1_first<float, float4>), dim3(ctx->nsub), dim3(kBlock),
+                       brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
+                       ctx->w_y32.p, (const float4*)ctx->fac32_f.p,
+                       (const float4*)ctx->fac32_b.p);
+  } else {
+    hipLaunchKernelGGL((k_lv1_first<double, double4>), dim3(ctx->nsub), dim3(kBlock),
+                       brick_lds_bytes(ctx, sizeof(double)), ctx->stream, ctx->geo, x,
+                       ctx->w_y.p, (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p);
+  }
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, double* partial) {
+  if (ctx->pc_precision == 32) {
+    hipLaunchKernelGGL((k_lv1_last<float, float, float4>), dim3(ctx->nsub), dim3(kBlock),
+                       brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
+                       ctx->w_y32.p, ctx->T32[0].p, ctx->T32[1].p, ctx->T32[2].p,
+                       (const float4*)ctx->fac32_f.p, (const float4*)ctx->fac32_b.p, z,
+                       keep_y ? ctx->w_y32.p : nullptr, partial);
+  } else {
+    hipLaunchKernelGGL((k_lv1_last<double, double, double4>), dim3(ctx->nsub), dim3(kBlock),
+                       brick_lds_bytes(ctx, sizeof(double)), ctx->stream, ctx->geo, x,
+                       ctx->w_y.p, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                       (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p, z,
+                       keep_y ? ctx->w_y.p : nullptr, partial);
+  }
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec) {
+  if (ctx->pc_precision == 32)
+    hipLaunchKernelGGL((k_resid_restrict<float, float>), dim3(ctx->nsub), dim3(kBlock), 0,
+                       ctx->stream, ctx->geo, x, y_zero ? nullptr : ctx->w_y32.p, ctx->T32[0].p,
+                       ctx->T32[1].p, ctx->T32[2].p, ctx->phi32.p, ctx->en_lv1, cvec);
+  else
+    hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub), dim3(kBlock), 0,
+                       ctx->stream, ctx->geo, x, y_zero ? nullptr : ctx->w_y.p, ctx->T[0].p,
+                       ctx->T[1].p, ctx->T[2].p, ctx->phi.p, ctx->en_lv1, cvec);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_prolong(pmx_ctx* ctx, const double* cvec) {
+  if (ctx->pc_precision == 32)
+    hipLaunchKernelGGL(k_prolong<float>, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->w_y32.p, ctx->phi32.p, ctx->en_lv1, cvec);
+  else
+    hipLaunchKernelGGL(k_prolong<double>, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->w_y.p, ctx->phi.p, ctx->en_lv1, cvec);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w,
+                     int first) {
+  hipLaunchKernelGGL(k_cg_p_spmv, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, z,
+                     pold, pnew, w, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->scal.p, first,
+                     ctx->partial.p);
+  PMX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
+                     ctx->nsub, ctx->scal.p, 1, first);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int cg_update_blocks(const pmx_ctx* ctx) {
+  int nb = (ctx->ncell + kBlock * 4 - 1) / (kBlock * 4);
+  return std::max(1, std::min(nb, 2048));
+}
+
+int launch_cg_update(pmx_ctx* ctx, double* x, double* r, const double* p, const double* w) {
+  const int nb = cg_update_blocks(ctx);
+  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->ncell, x, r, p, w,
+                     ctx->scal.p, ctx->partial.p);
+  PMX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p, nb,
+                     ctx->scal.p, 2, 0);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
+  hipLaunchKernelGGL(k_residual, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, b, x, r,
+                     ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->partial.p);
+  PMX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
+                     ctx->nsub, ctx->scal.p, 2, 0);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_finalize_beta(pmx_ctx* ctx) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
+                     ctx->nsub, ctx->scal.p, 0, 0);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_spmv(pmx_ctx* ctx, const double* x, double* y) {
+  hipLaunchKernelGGL(k_spmv, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, x, y,
+                     ctx->T[0].p, ctx->T[1].p, ctx->T[2].p);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+}  // namespace pmx

@@ -1,0 +1,249 @@
+// Internal declarations of the MI355X implementation of PreMixFEM's
+// three-level GMsFEM preconditioner.  The public C ABI is include/premixfem_gpu.h.
+//
+// Device data layout ("brick order"): cells are stored subdomain-major, each
+// subdomain (brick) contiguous in its own local x-fastest order
+// (ez-startz)*ny*nx + (ey-starty)*nx + (ex-startx), i.e. the local vector
+// order of src/PreMixFEM_3D.c:104-107, bricks in x-fastest subdomain order
+// (src/PreMixFEM_3D.c:23-27).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "premixfem_gpu.h"
+
+namespace pmx {
+
+constexpr int kBlock = 256;        // threads per workgroup for cell kernels
+constexpr int kMaxShapes = 8;      // uneven partitions give at most 2 lengths per axis
+constexpr int kMaxEig = 16;        // largest en_lv1 / en_lv2 supported on device
+constexpr double kNil = 1.0e-12;   // NIL of src/PreMixFEM_3D.h:14
+
+void set_error(const char* fmt, ...);
+
+#define PMX_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      ::pmx::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #call,                 \
+                       hipGetErrorString(e_));                                     \
+      return PMX_ERR_LIB;                                                          \
+    }                                                                              \
+  } while (0)
+
+#define PMX_TRY(call)                                                              \
+  do {                                                                             \
+    int rc_ = (call);                                                              \
+    if (rc_ != PMX_SUCCESS) return rc_;                                            \
+  } while (0)
+
+// One subdomain on the device.
+struct Brick {
+  int off;          // first cell (brick order)
+  int nx, ny, nz;   // local extents
+  int gx, gy, gz;   // global start
+  int shape;        // index into the shape tables
+  int nb[6];        // neighbour bricks: -x, +x, -y, +y, -z, +z (-1 = domain boundary)
+  int dof;          // first coarse dof (prefix sum of eigen counts)
+  int neig;         // eigen count of this subdomain
+  int plane_off;    // offset of this shape in the plane-position tables
+  int nplanes;      // nx+ny+nz-2
+};
+
+// Geometry passed by value to kernels.
+struct Geo {
+  int M, N, P;
+  int sdx, sdy, sdz;
+  int nsub;
+  int ncell;
+  const Brick* bricks;
+  const int* ax_sub[3];   // per axis: subdomain index of each global coordinate
+  const int* ax_loc[3];   // per axis: local coordinate
+  const int* plane_start; // per shape: nplanes+1 prefix counts (at shape's plane_off + shape*? see below)
+  const int* plane_cell;  // per shape: packed (lx | ly<<10 | lz<<20) in plane order
+  const int* shape_pstart;// per shape: offset into plane_start
+  const int* shape_coff;  // per shape: offset into plane_cell
+  int pin;                // brick index of the pinned cell, -1 if none
+  double tau;             // pin strength
+};
+
+// Neighbour index in brick order, or -1.
+__device__ __forceinline__ int nbr_index(const Geo& g, const Brick& b, int li, int lx, int ly,
+                                         int lz, int dir) {
+  switch (dir) {
+    case 0:
+      if (lx > 0) return li - 1;
+      if (b.nb[0] >= 0) {
+        const Brick& o = g.bricks[b.nb[0]];
+        return o.off + (lz * o.ny + ly) * o.nx + (o.nx - 1);
+      }
+      return -1;
+    case 1:
+      if (lx < b.nx - 1) return li + 1;
+      if (b.nb[1] >= 0) {
+        const Brick& o = g.bricks[b.nb[1]];
+        return o.off + (lz * o.ny + ly) * o.nx;
+      }
+      return -1;
+    case 2:
+      if (ly > 0) return li - b.nx;
+      if (b.nb[2] >= 0) {
+        const Brick& o = g.bricks[b.nb[2]];
+        return o.off + (lz * o.ny + (o.ny - 1)) * o.nx + lx;
+      }
+      return -1;
+    case 3:
+      if (ly < b.ny - 1) return li + b.nx;
+      if (b.nb[3] >= 0) {
+        const Brick& o = g.bricks[b.nb[3]];
+        return o.off + (lz * o.ny) * o.nx + lx;
+      }
+      return -1;
+    case 4:
+      if (lz > 0) return li - b.nx * b.ny;
+      if (b.nb[4] >= 0) {
+        const Brick& o = g.bricks[b.nb[4]];
+        return o.off + ((o.nz - 1) * o.ny + ly) * o.nx + lx;
+      }
+      return -1;
+    default:
+      if (lz < b.nz - 1) return li + b.nx * b.ny;
+      if (b.nb[5] >= 0) {
+        const Brick& o = g.bricks[b.nb[5]];
+        return o.off + ly * o.nx + lx;
+      }
+      return -1;
+  }
+}
+
+// Block-wide deterministic sum of one double (256 threads).
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    sh[0] = r;
+  }
+  __syncthreads();
+  r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Device buffer helper.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count == n && p) return PMX_SUCCESS;
+    release();
+    if (count == 0) return PMX_SUCCESS;
+    PMX_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+    return PMX_SUCCESS;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace pmx
+
+// Full context behind the opaque pmx_ctx handle.
+struct pmx_ctx {
+  // ---- options (src/PreMixFEM_3D.c:1662-1705) ----
+  double dom[3];
+  int mesh[3];
+  int sd[3];
+  int si_lv1 = 1, si_lv2 = 1;
+  int en_lv1 = 4, en_lv2 = 4;
+  int eg_op = PMX_EIG_OP_MOD;
+  double eb_lv1 = 1.0 / pmx::kNil, eb_lv2 = 1.0 / pmx::kNil;
+  int use_W_cycle = 0, no_shift_A_cc = 0, use_full_Cholesky_lv1 = 0, use_2level = 0;
+  int pc_precision = 64;
+  int pin[3] = {-1, -1, -1};
+  int warm_start = 0;           // 0 random, 1 CNN prediction, 2 caller basis
+  double eig_tol = 1.0e-8;      // LOBPCG relative residual tolerance
+  int eig_maxit = 400;
+  int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1
+  uint64_t seed = 12345;
+
+  // ---- derived geometry ----
+  double Hx, Hy, Hz, meas_elem;
+  double coef[3];               // |F|^2/|E| per direction
+  int ncell = 0, nsub = 0;
+  std::vector<pmx::Brick> h_bricks;
+  std::vector<int> h_ax_sub[3], h_ax_loc[3];
+  std::vector<int> h_starts[3], h_lens[3];
+  std::vector<int> h_plane_start, h_plane_cell, h_shape_pstart, h_shape_coff;
+  int nshape = 0;
+
+  pmx::DBuf<pmx::Brick> d_bricks;
+  pmx::DBuf<int> d_ax[6];
+  pmx::DBuf<int> d_plane_start, d_plane_cell, d_shape_pstart, d_shape_coff;
+  pmx::Geo geo;
+
+  // ---- stream ----
+  hipStream_t stream = nullptr;
+
+  // ---- fields (brick order) ----
+  int have_kappa = 0;
+  pmx::DBuf<double> kappa[3];
+  pmx::DBuf<double> T[3];       // lower-face coefficient of each cell per direction
+  pmx::DBuf<float> T32[3];
+  double tau = 0.0;
+  int pin_brick = -1;
+
+  // ---- level 1 ----
+  pmx::DBuf<double> fac_f, fac_b;   // plane-major {dinv, cx, cy, cz} lower / upper couplings
+  pmx::DBuf<float> fac32_f, fac32_b;
+
+  // ---- level 2 ----
+  pmx::DBuf<double> mass;           // eigen mass diagonal
+  pmx::DBuf<double> phi;            // [cell][en_lv1] basis rows
+  pmx::DBuf<float> phi32;
+  std::vector<int> h_eigen_num;     // per subdomain
+  std::vector<double> h_eig_lv1;    // [nsub][en_lv1] eigenvalues
+  std::vector<double> h_eigen_min_lv1, h_eigen_max_lv1;
+  std::vector<int> h_dof;           // prefix sums
+  int nc = 0;                        // coarse size
+  pmx::DBuf<double> Ac;             // dense nc x nc Galerkin matrix
+  pmx::DBuf<double> Acb;            // block-sparse rows: [nsub][7][kmax][kmax]
+  pmx::DBuf<double> Ainv;           // dense inverse of the level-2 coarse matrix
+  int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
+
+  // ---- level 3 ----
+  int eigen_num_lv2 = 0;
+  double eigen_min_lv2 = 0.0, eigen_max_lv2 = 0.0;
+  std::vector<double> h_eig_lv2;
+  pmx::DBuf<double> V2;             // [nc][en_lv2]
+  pmx::DBuf<double> Acc_lu;         // k2 x k2 LU (row-major) + pivots packed after
+  int lv3_iters = 0;
+
+  // ---- work vectors ----
+  pmx::DBuf<double> w_x, w_y, w_r, w_b, w_p, w_z, w_q, w_tmp;   // brick order FP64
+  pmx::DBuf<float> w_y32;
+  pmx::DBuf<double> c_a, c_b, c_c, c_d, c_e;                       // coarse vectors
+  pmx::DBuf<double> partial;                                       // per-block partial sums
+  pmx::DBuf<double> scal;                                          // device scalars
+  pmx::DBuf<double> nat_buf;                                       // natural-order staging
+
+  // ---- state / stats ----
+  int setup_done = 0;
+  double t_stages[PMX_MAX_LOG_STAGES] = {0};
+  int last_its = 0;
+  double last_rnorm = 0.0, last_rnorm0 = 0.0;
+  int last_reason = 0;
+};

@@ -1,0 +1,46 @@
+// Kernel launch wrappers shared between the translation units.
+#pragma once
+
+#include <algorithm>
+
+#include "pmx_internal.h"
+
+namespace pmx {
+
+size_t brick_lds_bytes(const pmx_ctx* ctx, size_t elem);
+
+// fine grid (pmx_fine.hip)
+int launch_nat2brick(pmx_ctx* ctx, const double* src, double* dst, int nvec, int sstride,
+                     int dstride);
+int launch_brick2nat(pmx_ctx* ctx, const double* src, double* dst, int nvec, int sstride,
+                     int dstride);
+int launch_basis_nat2rows(pmx_ctx* ctx, const double* src, double* rows, int k);
+int launch_basis_rows2nat(pmx_ctx* ctx, const double* rows, double* dst, int k);
+int launch_trans(pmx_ctx* ctx);
+int launch_mass(pmx_ctx* ctx);
+int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* fac_b, bool f32);
+int launch_lv1_first(pmx_ctx* ctx, const double* x);
+int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, double* partial);
+int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec);
+int launch_prolong(pmx_ctx* ctx, const double* cvec);
+int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w,
+                     int first);
+int cg_update_blocks(const pmx_ctx* ctx);
+int launch_cg_update(pmx_ctx* ctx, double* x, double* r, const double* p, const double* w);
+int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r);
+int launch_finalize_beta(pmx_ctx* ctx);
+int launch_spmv(pmx_ctx* ctx, const double* x, double* y);
+
+// coarse levels (pmx_coarse.hip)
+int launch_galerkin(pmx_ctx* ctx);
+int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift);
+int level3_setup(pmx_ctx* ctx);
+int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n);
+int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, int first,
+                         int update_t);
+int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d);
+
+// level-2 eigensolver (pmx_lobpcg.hip)
+int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows);
+
+}  // namespace pmx
