@@ -83,6 +83,7 @@ typedef struct {
   int its;                   /* last pmx_solve iteration count */
   double rnorm, rnorm0;      /* last pmx_solve final / initial residual norms */
   int reason;                /* last pmx_solve convergence reason */
+  double t_solve;            /* last pmx_solve wall seconds (PCG iterations only) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),

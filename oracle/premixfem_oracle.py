@@ -178,7 +178,7 @@ def stencil_apply(lay: Layout, T, x, pin=None):
     """y = A x with the face arrays (matrix-free form of ``assemble_A``)."""
     Tx, Ty, Tz = T
     M, N, P = lay.mesh
-    x = x.reshape(P, N, M)
+    x = np.asarray(x, dtype=np.float64).reshape(P, N, M)
     y = np.zeros_like(x)
     d = Tx * (x[:, :, :-1] - x[:, :, 1:])
     y[:, :, :-1] += d
@@ -672,3 +672,40 @@ def solve_problem(prob, ghep_fn=None):
     orc.setup(ghep_fn)
     x, its, hist, reason = pcg(orc.matvec, prob.rhs.ravel(), orc.apply, rtol=prob.rtol)
     return orc, x, its, hist, reason
+
+
+def create_A_diagonals(lay: Layout, T, pin_cell=None):
+    """PC_create_A's matrix as seven diagonals, natural order: diag[c] and
+    off[c, d] for the -x,+x,-y,+y,-z,+z neighbours.  The diagonal follows the
+    accumulation order of the reference's face loop (:1777-1816): a cell's
+    lower x, y, z faces (its own iteration), then its upper x, y, z faces
+    (the iterations of the upper neighbours); the pin term last."""
+    M, N, P = lay.mesh
+    xl, xh, yl, yh, zl, zh = cell_faces(lay, T)
+    d = xl.copy()
+    d = d + yl
+    d = d + zl
+    d = d + xh
+    d = d + yh
+    d = d + zh
+    d = d.ravel()
+    if pin_cell is not None:
+        x, y, z = pin_cell
+        d[(z * N + y) * M + x] += pin_tau(lay, T, pin_cell)
+    off = -np.stack([a.ravel() for a in (xl, xh, yl, yh, zl, zh)], axis=1)
+    return d, off
+
+
+def basis_columns(orc: "Oracle"):
+    """The level-2 basis as en_lv1 natural-order vectors (vector j holds
+    eigenvector j of every subdomain, zero where a subdomain has fewer), the
+    per-subdomain counts and the eigenvalues [nsub][en_lv1]."""
+    n = orc.A.shape[0]
+    k = orc.en_lv1
+    phi = np.zeros((k, n))
+    ev = np.zeros((orc.lay.nsub, k))
+    for s in range(orc.lay.nsub):
+        V = orc.phi[s]
+        phi[:V.shape[1], orc.sub_cells[s]] = V.T
+        ev[s, :len(orc.eig_lv1[s])] = orc.eig_lv1[s][:k]
+    return phi, orc.eigen_num_lv1.astype(np.int32), ev

@@ -1,0 +1,944 @@
+// Host side of the C ABI (include/premixfem_gpu.h): options, geometry and
+// brick tables, setup orchestration (src/PreMixFEM_3D.c:_PC_setup), the
+// V-cycle (PC_apply_vec) and the GPU-resident PCG (PETSc KSPSolve_CG).
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <tuple>
+
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Per axis: len_i = n/sd + (n%sd > i), start by prefix (src/PreMixFEM_3D.c:62-72).
+static void partition(int n, int sd, std::vector<int>& st, std::vector<int>& ln) {
+  st.assign(sd, 0);
+  ln.assign(sd, 0);
+  for (int i = 0; i < sd; ++i) ln[i] = n / sd + ((n % sd) > i ? 1 : 0);
+  for (int i = 1; i < sd; ++i) st[i] = st[i - 1] + ln[i - 1];
+}
+
+static int parse_options(pmx_ctx* c, const char* options) {
+  if (!options) return PMX_SUCCESS;
+  std::istringstream is(options);
+  std::vector<std::string> tok;
+  std::string t;
+  while (is >> t) tok.push_back(t);
+  auto need = [&](size_t i) -> bool { return i + 1 < tok.size(); };
+  for (size_t i = 0; i < tok.size(); ++i) {
+    const std::string& k = tok[i];
+    auto geti = [&](int& dst) -> int {
+      if (!need(i)) {
+        set_error("option %s needs a value", k.c_str());
+        return PMX_ERR_ARG_WRONG;
+      }
+      char* end = nullptr;
+      long v = strtol(tok[++i].c_str(), &end, 10);
+      if (*end) {
+        set_error("bad integer for %s: %s", k.c_str(), tok[i].c_str());
+        return PMX_ERR_ARG_WRONG;
+      }
+      dst = (int)v;
+      return PMX_SUCCESS;
+    };
+    auto getd = [&](double& dst) -> int {
+      if (!need(i)) {
+        set_error("option %s needs a value", k.c_str());
+        return PMX_ERR_ARG_WRONG;
+      }
+      char* end = nullptr;
+      double v = strtod(tok[++i].c_str(), &end);
+      if (*end) {
+        set_error("bad number for %s: %s", k.c_str(), tok[i].c_str());
+        return PMX_ERR_ARG_WRONG;
+      }
+      dst = v;
+      return PMX_SUCCESS;
+    };
+    if (k == "-sd") {
+      int v;
+      PMX_TRY(geti(v));
+      c->sd[0] = c->sd[1] = c->sd[2] = v;
+    } else if (k == "-sdx") {
+      PMX_TRY(geti(c->sd[0]));
+    } else if (k == "-sdy") {
+      PMX_TRY(geti(c->sd[1]));
+    } else if (k == "-sdz") {
+      PMX_TRY(geti(c->sd[2]));
+    } else if (k == "-si_lv1") {
+      PMX_TRY(geti(c->si_lv1));
+    } else if (k == "-si_lv2") {
+      PMX_TRY(geti(c->si_lv2));
+    } else if (k == "-en_lv1") {
+      PMX_TRY(geti(c->en_lv1));
+    } else if (k == "-en_lv2") {
+      PMX_TRY(geti(c->en_lv2));
+    } else if (k == "-eg_op") {
+      PMX_TRY(geti(c->eg_op));
+    } else if (k == "-eb_lv1") {
+      PMX_TRY(getd(c->eb_lv1));
+    } else if (k == "-eb_lv2") {
+      PMX_TRY(getd(c->eb_lv2));
+    } else if (k == "-use_W_cycle") {
+      c->use_W_cycle = 1;
+    } else if (k == "-no_shift_A_cc") {
+      c->no_shift_A_cc = 1;
+    } else if (k == "-use_full_Cholesky_lv1") {
+      c->use_full_Cholesky_lv1 = 1;
+    } else if (k == "-use_2level") {
+      c->use_2level = 1;
+    } else if (k == "-pc_precision") {
+      PMX_TRY(geti(c->pc_precision));
+    } else if (k == "-eig_tol") {
+      PMX_TRY(getd(c->eig_tol));
+    } else if (k == "-eig_maxit") {
+      PMX_TRY(geti(c->eig_maxit));
+    } else if (k == "-eig_sigma") {
+      PMX_TRY(getd(c->eig_sigma));
+    } else if (k == "-eig_guard") {
+      PMX_TRY(geti(c->eig_guard));
+    } else if (k == "-log_stages") {
+      c->log_stages = 1;
+    } else if (k == "-seed") {
+      int v;
+      PMX_TRY(geti(v));
+      c->seed = (uint64_t)v;
+    } else if (k == "-pin") {
+      if (!need(i)) {
+        set_error("-pin needs x,y,z");
+        return PMX_ERR_ARG_WRONG;
+      }
+      if (sscanf(tok[++i].c_str(), "%d,%d,%d", &c->pin[0], &c->pin[1], &c->pin[2]) != 3) {
+        set_error("-pin needs x,y,z");
+        return PMX_ERR_ARG_WRONG;
+      }
+    } else if (k == "-warm_start") {
+      if (!need(i)) {
+        set_error("-warm_start needs none|cnn|basis");
+        return PMX_ERR_ARG_WRONG;
+      }
+      const std::string v = tok[++i];
+      c->warm_start = v == "cnn" ? 1 : (v == "basis" ? 2 : 0);
+    } else {
+      set_error("unknown option %s", k.c_str());
+      return PMX_ERR_ARG_WRONG;
+    }
+  }
+  return PMX_SUCCESS;
+}
+
+template <typename T>
+static int upload(DBuf<T>& d, const std::vector<T>& h) {
+  PMX_TRY(d.alloc(h.size()));
+  PMX_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PMX_SUCCESS;
+}
+
+static int build_geometry(pmx_ctx* c) {
+  const int M = c->mesh[0], N = c->mesh[1], P = c->mesh[2];
+  c->ncell = M * N * P;
+  for (int a = 0; a < 3; ++a) {
+    partition(c->mesh[a], c->sd[a], c->h_starts[a], c->h_lens[a]);
+    c->h_ax_sub[a].assign(c->mesh[a], 0);
+    c->h_ax_loc[a].assign(c->mesh[a], 0);
+    for (int i = 0; i < c->sd[a]; ++i)
+      for (int l = 0; l < c->h_lens[a][i]; ++l) {
+        c->h_ax_sub[a][c->h_starts[a][i] + l] = i;
+        c->h_ax_loc[a][c->h_starts[a][i] + l] = l;
+      }
+  }
+  const int sdx = c->sd[0], sdy = c->sd[1], sdz = c->sd[2];
+  c->nsub = sdx * sdy * sdz;
+  c->h_bricks.assign(c->nsub, Brick{});
+  std::map<std::tuple<int, int, int>, int> shapes;
+  c->h_plane_start.clear();
+  c->h_plane_cell.clear();
+  c->h_shape_pstart.clear();
+  c->h_shape_coff.clear();
+  int off = 0;
+  for (int sz = 0; sz < sdz; ++sz)
+    for (int sy = 0; sy < sdy; ++sy)
+      for (int sx = 0; sx < sdx; ++sx) {
+        const int s = sx + sdx * (sy + sdy * sz);
+        Brick& b = c->h_bricks[s];
+        b.off = off;
+        b.nx = c->h_lens[0][sx];
+        b.ny = c->h_lens[1][sy];
+        b.nz = c->h_lens[2][sz];
+        b.gx = c->h_starts[0][sx];
+        b.gy = c->h_starts[1][sy];
+        b.gz = c->h_starts[2][sz];
+        b.nb[0] = sx > 0 ? s - 1 : -1;
+        b.nb[1] = sx < sdx - 1 ? s + 1 : -1;
+        b.nb[2] = sy > 0 ? s - sdx : -1;
+        b.nb[3] = sy < sdy - 1 ? s + sdx : -1;
+        b.nb[4] = sz > 0 ? s - sdx * sdy : -1;
+        b.nb[5] = sz < sdz - 1 ? s + sdx * sdy : -1;
+        b.nplanes = b.nx + b.ny + b.nz - 2;
+        b.dof = 0;
+        b.neig = 0;
+        const auto key = std::make_tuple(b.nx, b.ny, b.nz);
+        auto it = shapes.find(key);
+        if (it == shapes.end()) {
+          const int id = (int)shapes.size();
+          shapes[key] = id;
+          c->h_shape_pstart.push_back((int)c->h_plane_start.size());
+          c->h_shape_coff.push_back((int)c->h_plane_cell.size());
+          int cnt = 0;
+          for (int p = 0; p < b.nplanes; ++p) {
+            c->h_plane_start.push_back(cnt);
+            for (int lz = 0; lz < b.nz; ++lz)
+              for (int ly = 0; ly < b.ny; ++ly) {
+                const int lx = p - ly - lz;
+                if (lx < 0 || lx >= b.nx) continue;
+                c->h_plane_cell.push_back(lx | (ly << 10) | (lz << 20));
+                ++cnt;
+              }
+          }
+          c->h_plane_start.push_back(cnt);
+          b.shape = id;
+        } else {
+          b.shape = it->second;
+        }
+        b.plane_off = c->h_shape_coff[b.shape];
+        off += b.nx * b.ny * b.nz;
+      }
+  c->nshape = (int)shapes.size();
+  c->h_nat2brick.assign(c->ncell, 0);
+  for (int z = 0; z < P; ++z)
+    for (int y = 0; y < N; ++y)
+      for (int x = 0; x < M; ++x) {
+        const int s = c->h_ax_sub[0][x] + sdx * (c->h_ax_sub[1][y] + sdy * c->h_ax_sub[2][z]);
+        const Brick& b = c->h_bricks[s];
+        c->h_nat2brick[(size_t)(z * N + y) * M + x] =
+            b.off + (c->h_ax_loc[2][z] * b.ny + c->h_ax_loc[1][y]) * b.nx + c->h_ax_loc[0][x];
+      }
+  PMX_TRY(upload(c->d_bricks, c->h_bricks));
+  for (int a = 0; a < 3; ++a) {
+    PMX_TRY(upload(c->d_ax[a], c->h_ax_sub[a]));
+    PMX_TRY(upload(c->d_ax[3 + a], c->h_ax_loc[a]));
+  }
+  PMX_TRY(upload(c->d_plane_start, c->h_plane_start));
+  PMX_TRY(upload(c->d_plane_cell, c->h_plane_cell));
+  PMX_TRY(upload(c->d_shape_pstart, c->h_shape_pstart));
+  PMX_TRY(upload(c->d_shape_coff, c->h_shape_coff));
+  Geo& g = c->geo;
+  g.M = M;
+  g.N = N;
+  g.P = P;
+  g.sdx = sdx;
+  g.sdy = sdy;
+  g.sdz = sdz;
+  g.nsub = c->nsub;
+  g.ncell = c->ncell;
+  g.bricks = c->d_bricks.p;
+  for (int a = 0; a < 3; ++a) {
+    g.ax_sub[a] = c->d_ax[a].p;
+    g.ax_loc[a] = c->d_ax[3 + a].p;
+  }
+  g.plane_start = c->d_plane_start.p;
+  g.plane_cell = c->d_plane_cell.p;
+  g.shape_pstart = c->d_shape_pstart.p;
+  g.shape_coff = c->d_shape_coff.p;
+  g.pin = -1;
+  g.tau = 0.0;
+  return PMX_SUCCESS;
+}
+
+static int alloc_work(pmx_ctx* c) {
+  const size_t n = c->ncell;
+  for (DBuf<double>* b : {&c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p, &c->w_z, &c->w_q,
+                          &c->w_tmp, &c->nat_buf})
+    PMX_TRY(b->alloc(n));
+  if (c->pc_precision == 32) {
+    PMX_TRY(c->w_y32.alloc(n));
+    PMX_TRY(c->w_y32b.alloc(n));
+  }
+  PMX_TRY(c->partial.alloc(std::max<size_t>(c->nsub, 4096)));
+  PMX_TRY(c->scal.alloc(16));
+  PMX_HIP(hipMemset(c->scal.p, 0, 16 * sizeof(double)));
+  return PMX_SUCCESS;
+}
+
+// Level-2 and level-3 setup from the basis rows in ctx->phi.
+static int setup_coarse(pmx_ctx* c) {
+  double t0 = now();
+  // Eigen counts against eb_lv1 (src/PreMixFEM_3D.c:368-384) and coarse dofs.
+  c->h_eigen_num.assign(c->nsub, c->en_lv1);
+  c->h_eigen_min_lv1.assign(c->nsub, 0.0);
+  c->h_eigen_max_lv1.assign(c->nsub, 0.0);
+  c->h_dof.assign(c->nsub + 1, 0);
+  for (int s = 0; s < c->nsub; ++s) {
+    int num = c->en_lv1;
+    if (!c->h_eigen_num_given.empty()) {
+      num = std::max(1, std::min(c->en_lv1, c->h_eigen_num_given[s]));
+    } else {
+      for (int j = 0; j < c->en_lv1; ++j)
+        if (c->h_eig_lv1[(size_t)s * c->en_lv1 + j] >= c->eb_lv1) {
+          num = j + 1;
+          break;
+        }
+    }
+    c->h_eigen_num[s] = num;
+    c->h_eigen_min_lv1[s] = c->h_eig_lv1[(size_t)s * c->en_lv1];
+    c->h_eigen_max_lv1[s] = c->h_eig_lv1[(size_t)s * c->en_lv1 + num - 1];
+    c->h_dof[s + 1] = c->h_dof[s] + num;
+    c->h_bricks[s].dof = c->h_dof[s];
+    c->h_bricks[s].neig = num;
+  }
+  c->nc = c->h_dof[c->nsub];
+  PMX_HIP(hipMemcpy(c->d_bricks.p, c->h_bricks.data(), c->h_bricks.size() * sizeof(Brick),
+                    hipMemcpyHostToDevice));
+  // Galerkin coarse matrix and its inverse.
+  PMX_TRY(c->Ac.alloc((size_t)c->nc * c->nc));
+  PMX_TRY(c->Acb.alloc((size_t)c->nsub * 7 * c->en_lv1 * c->en_lv1));
+  PMX_TRY(c->Ainv.alloc((size_t)c->nc * c->nc));
+  for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
+  PMX_TRY(launch_galerkin(c));
+  const double shift = (c->use_2level && !c->no_shift_A_cc) ? kNil : 0.0;
+  PMX_TRY(coarse_invert(c, c->Ac.p, c->Ainv.p, shift));
+  if (c->pc_precision == 32) {
+    std::vector<double> h((size_t)c->ncell * c->en_lv1);
+    PMX_HIP(hipMemcpy(h.data(), c->phi.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<float> hf(h.begin(), h.end());
+    PMX_TRY(upload(c->phi32, hf));
+  }
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  double t1 = now();
+  c->t_stages[PMX_STAGE_SU_LV2_2] += t1 - t0;
+  if (!c->use_2level) {
+    PMX_TRY(level3_setup(c));
+    double t2 = now();
+    c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
+  }
+  return PMX_SUCCESS;
+}
+
+// Operator and level 1.
+static int setup_fine(pmx_ctx* c) {
+  if (!c->have_kappa) {
+    set_error("kappa has not been set (pmx_set_kappa)");
+    return PMX_ERR_ORDER;
+  }
+  double t0 = now();
+  PMX_TRY(launch_trans(c));
+  if (c->pin[0] >= 0) {
+    const int x = c->pin[0], y = c->pin[1], z = c->pin[2];
+    const int bi = c->h_nat2brick[(size_t)(z * c->mesh[1] + y) * c->mesh[0] + x];
+    double tau = 0.0;
+    const int nat = (z * c->mesh[1] + y) * c->mesh[0] + x;
+    const int nbh[3] = {x + 1 < c->mesh[0] ? nat + 1 : -1,
+                        y + 1 < c->mesh[1] ? nat + c->mesh[0] : -1,
+                        z + 1 < c->mesh[2] ? nat + c->mesh[0] * c->mesh[1] : -1};
+    for (int d = 0; d < 3; ++d) {
+      double v = 0.0;
+      PMX_HIP(hipMemcpy(&v, c->T[d].p + bi, sizeof(double), hipMemcpyDeviceToHost));
+      tau = std::max(tau, v);
+      if (nbh[d] >= 0) {
+        PMX_HIP(hipMemcpy(&v, c->T[d].p + c->h_nat2brick[nbh[d]], sizeof(double),
+                          hipMemcpyDeviceToHost));
+        tau = std::max(tau, v);
+      }
+    }
+    c->tau = tau;
+    c->pin_brick = bi;
+    c->geo.pin = bi;
+    c->geo.tau = tau;
+  } else {
+    c->geo.pin = -1;
+    c->geo.tau = 0.0;
+  }
+  PMX_TRY(c->fac_f.alloc((size_t)c->ncell * 4));
+  PMX_TRY(c->fac_b.alloc((size_t)c->ncell * 4));
+  PMX_TRY(launch_dic_factor(c, 0, 0.0, c->fac_f.p, c->fac_b.p, false));
+  if (c->pc_precision == 32) {
+    PMX_TRY(c->fac32_f.alloc((size_t)c->ncell * 4));
+    PMX_TRY(c->fac32_b.alloc((size_t)c->ncell * 4));
+    PMX_TRY(launch_dic_factor(c, 0, 0.0, c->fac32_f.p, c->fac32_b.p, true));
+  }
+  PMX_TRY(c->mass.alloc(c->ncell));
+  PMX_TRY(launch_mass(c));
+  PMX_TRY(c->phi.alloc((size_t)c->ncell * c->en_lv1));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  c->t_stages[PMX_STAGE_SU_LV1] += now() - t0;
+  return PMX_SUCCESS;
+}
+
+// V-cycle of PC_apply_vec (src/PreMixFEM_3D.c:1840-1936) on brick-order
+// device vectors.  The fine-grid stages are level 1 (before and after); all
+// level-2/level-3 stages in between act on the coarse residual
+// t = Phi^T (x - A y): a stage with correction d updates t -= A_c d (since
+// A_c = Phi^T A Phi) and accumulates u += d, and y += Phi u is applied once.
+// z receives the output; partial (if not null) the per-brick z.x sums.
+static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial) {
+  const bool f32 = c->pc_precision == 32;
+  auto tick = [&](int stage, double& t) -> int {
+    if (!c->log_stages) return PMX_SUCCESS;
+    PMX_HIP(hipStreamSynchronize(c->stream));
+    const double tn = now();
+    c->t_stages[stage] += tn - t;
+    t = tn;
+    return PMX_SUCCESS;
+  };
+  double t = now();
+  const double t_start = t;
+  // Pre-smoothing: y = B1 x, then si_lv1-1 residual-corrected sweeps.
+  PMX_TRY(launch_lv1_first(c, x));
+  for (int i = 1; i < c->si_lv1; ++i) {
+    PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
+  }
+  PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
+  PMX_TRY(launch_resid_restrict(c, x, false, c->c_a.p));
+  double* tv = c->c_a.p;
+  double* u = c->c_c.p;
+  double* d = c->c_b.p;
+  std::vector<int> prog;  // 2 = lv2 stage, 3 = lv3 stage
+  if (c->use_2level) {
+    prog.push_back(2);
+  } else if (c->use_W_cycle) {
+    for (int r = 0; r < 2; ++r) {
+      for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+      prog.push_back(3);
+    }
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+  } else {
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+    prog.push_back(3);
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+  }
+  for (size_t k = 0; k < prog.size(); ++k) {
+    const bool last = k + 1 == prog.size();
+    if (prog[k] == 2) {
+      PMX_TRY(launch_gemv(c, c->Ainv.p, tv, d, c->nc));
+    } else {
+      PMX_TRY(launch_lv3_apply(c, tv, d));
+    }
+    PMX_TRY(launch_coarse_update(c, d, tv, u, k == 0, last ? 0 : 1));
+  }
+  PMX_TRY(tick(PMX_STAGE_AV_LV2, t));
+  PMX_TRY(launch_prolong(c, u));
+  // Post-smoothing.
+  for (int i = 1; i < c->si_lv1; ++i) PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
+  PMX_TRY(launch_lv1_last(c, x, z, false, partial));
+  PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
+  (void)f32;
+  if (c->log_stages) c->t_stages[PMX_STAGE_AV] += now() - t_start;
+  return PMX_SUCCESS;
+}
+
+}  // namespace pmx
+
+using namespace pmx;
+
+extern "C" {
+
+const char* pmx_last_error(void) { return g_err.c_str(); }
+
+int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* options) {
+  if (!out || !dom || !mesh) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  *out = nullptr;
+  if (!(dom[0] > 0.0 && dom[1] > 0.0 && dom[2] > 0.0)) {
+    set_error("Errors in dom(L, W, H)=[%.5f, %.5f, %.5f].", dom[0], dom[1], dom[2]);
+    return PMX_ERR_ARG_WRONG;
+  }
+  if (!(mesh[0] > 0 && mesh[1] > 0 && mesh[2] > 0)) {
+    set_error("Errors in mesh(M, N, P)=[%d, %d, %d].", mesh[0], mesh[1], mesh[2]);
+    return PMX_ERR_ARG_WRONG;
+  }
+  pmx_ctx* c = new pmx_ctx();
+  for (int a = 0; a < 3; ++a) {
+    c->dom[a] = dom[a];
+    c->mesh[a] = mesh[a];
+    c->sd[a] = 2;
+  }
+  int rc = parse_options(c, options);
+  auto fail = [&](int code) {
+    delete c;
+    return code;
+  };
+  if (rc) return fail(rc);
+  for (int a = 0; a < 3; ++a) {
+    if (c->sd[a] < 1) {
+      set_error("Error in sub_domains=%d.", c->sd[a]);
+      return fail(PMX_ERR_ARG_WRONG);
+    }
+    if (c->sd[a] == 1) c->sd[a] = 2;  // src/PreMixFEM_3D.c:1718-1721
+    if (c->sd[a] > c->mesh[a]) {
+      set_error("sub_domains=%d exceeds the %d cells of axis %d", c->sd[a], c->mesh[a], a);
+      return fail(PMX_ERR_ARG_WRONG);
+    }
+  }
+  if (c->en_lv1 < 1) {
+    set_error("Error in max_eigen_num_lv1=%d for the level-1 problem.", c->en_lv1);
+    return fail(PMX_ERR_ARG_WRONG);
+  }
+  if (c->en_lv2 < 1) {
+    set_error("Error in max_eigen_num_lv2=%d for the level-2 problem.", c->en_lv2);
+    return fail(PMX_ERR_ARG_WRONG);
+  }
+  if (c->en_lv1 > kMaxEig || c->en_lv2 > kMaxEig) {
+    set_error("eigen counts above %d are not supported", kMaxEig);
+    return fail(PMX_ERR_SUP);
+  }
+  if (c->eg_op != PMX_EIG_OP_MOD && c->eg_op != PMX_EIG_OP_MEAN && c->eg_op != PMX_EIG_OP_DIAG) {
+    set_error("Error in lv2_eigen_op=%d.", c->eg_op);
+    return fail(PMX_ERR_ARG_WRONG);
+  }
+  if (c->pc_precision != 32 && c->pc_precision != 64) {
+    set_error("-pc_precision must be 32 or 64");
+    return fail(PMX_ERR_ARG_WRONG);
+  }
+  if (c->use_full_Cholesky_lv1) {
+    set_error("-use_full_Cholesky_lv1 (exact level-1 blocks) is not implemented");
+    return fail(PMX_ERR_SUP);
+  }
+  c->si_lv1 = std::max(1, c->si_lv1);
+  c->si_lv2 = std::max(1, c->si_lv2);
+  if (c->eb_lv1 < 0.0) c->eb_lv1 = 1.0 / kNil;
+  if (c->eb_lv2 < 0.0) c->eb_lv2 = 1.0 / kNil;
+  if (c->pin[0] >= 0 &&
+      (c->pin[0] >= mesh[0] || c->pin[1] < 0 || c->pin[1] >= mesh[1] || c->pin[2] < 0 ||
+       c->pin[2] >= mesh[2])) {
+    set_error("-pin cell outside the mesh");
+    return fail(PMX_ERR_ARG_OUTOFRANGE);
+  }
+  c->Hx = dom[0] / (double)mesh[0];
+  c->Hy = dom[1] / (double)mesh[1];
+  c->Hz = dom[2] / (double)mesh[2];
+  c->meas_elem = c->Hx * c->Hy * c->Hz;
+  const double fyz = c->Hy * c->Hz, fzx = c->Hz * c->Hx, fxy = c->Hx * c->Hy;
+  c->coef[0] = fyz * fyz / c->meas_elem;
+  c->coef[1] = fzx * fzx / c->meas_elem;
+  c->coef[2] = fxy * fxy / c->meas_elem;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("no HIP device available");
+    return fail(PMX_ERR_LIB);
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(PMX_ERR_LIB);
+  }
+  rc = build_geometry(c);
+  if (!rc) rc = alloc_work(c);
+  if (rc) {
+    pmx_final(c);
+    return rc;
+  }
+  *out = c;
+  return PMX_SUCCESS;
+}
+
+int pmx_print_info(const pmx_ctx* c) {
+  if (!c) return PMX_ERR_ARG_NULL;
+  printf("DIM=%d, M=%d, N=%d, P=%d.\n", 3, c->mesh[0], c->mesh[1], c->mesh[2]);
+  printf("GPU subdomain bricks: X=%d, Y=%d, Z=%d.\n", c->sd[0], c->sd[1], c->sd[2]);
+  printf("smoothing_iters_lv1=%d, smoothing_iters_lv2=%d, sub_domains=%d, lv2_eigen_op=%d\n",
+         c->si_lv1, c->si_lv2, c->sd[0], c->eg_op);
+  printf("max_eigen_num_lv1=%d, eigen_bd_lv1=%.5E\n", c->en_lv1, c->eb_lv1);
+  printf("max_eigen_num_lv2=%d, eigen_bd_lv2=%.5E\n", c->en_lv2, c->eb_lv2);
+  if (c->use_2level) printf("Only use the 2-level version!\n");
+  printf("pc_precision=%d\n", c->pc_precision);
+  return PMX_SUCCESS;
+}
+
+int pmx_set_kappa(pmx_ctx* c, const double* kx, const double* ky, const double* kz) {
+  if (!c || !kx || !ky || !kz) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  const double* src[3] = {kx, ky, kz};
+  for (int a = 0; a < 3; ++a) {
+    for (int i = 0; i < c->ncell; ++i)
+      if (!(src[a][i] > 0.0) || !std::isfinite(src[a][i])) {
+        set_error("kappa[%d][%d]=%g is not positive and finite", a, i, src[a][i]);
+        return PMX_ERR_ARG_WRONG;
+      }
+    PMX_TRY(c->kappa[a].alloc(c->ncell));
+    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, src[a], c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->kappa[a].p, 1, c->ncell, c->ncell));
+    PMX_TRY(c->T[a].alloc(c->ncell));
+    if (c->pc_precision == 32) PMX_TRY(c->T32[a].alloc(c->ncell));
+  }
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  c->have_kappa = 1;
+  c->setup_done = 0;
+  return PMX_SUCCESS;
+}
+
+int pmx_create_A(pmx_ctx* c, double* diag, double* off) {
+  if (!c || !diag || !off) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  if (!c->have_kappa) {
+    set_error("kappa has not been set (pmx_set_kappa)");
+    return PMX_ERR_ORDER;
+  }
+  if (!c->setup_done) PMX_TRY(launch_trans(c));
+  const size_t n = c->ncell;
+  std::vector<double> T[3];
+  for (int a = 0; a < 3; ++a) {
+    T[a].resize(n);
+    PMX_HIP(hipMemcpyAsync(T[a].data(), c->T[a].p, n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  }
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  const int M = c->mesh[0], N = c->mesh[1], P = c->mesh[2];
+  const long st[3] = {1, M, (long)M * N};
+  // Accumulation order of PC_create_A's face loop for one cell: its lower
+  // faces (x, y, z) from its own iteration, then its upper faces (x, y, z).
+  for (int z = 0; z < P; ++z)
+    for (int y = 0; y < N; ++y)
+      for (int x = 0; x < M; ++x) {
+        const size_t cn = (size_t)(z * N + y) * M + x;
+        const int co[3] = {x, y, z};
+        const int ext[3] = {M, N, P};
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = co[a] > 0 ? T[a][c->h_nat2brick[cn]] : 0.0;
+          hi[a] = co[a] < ext[a] - 1 ? T[a][c->h_nat2brick[cn + st[a]]] : 0.0;
+          off[6 * cn + 2 * a] = -lo[a];
+          off[6 * cn + 2 * a + 1] = -hi[a];
+        }
+        double dsum = lo[0];
+        dsum += lo[1];
+        dsum += lo[2];
+        dsum += hi[0];
+        dsum += hi[1];
+        dsum += hi[2];
+        if (c->pin[0] == x && c->pin[1] == y && c->pin[2] == z) {
+          double tau = 0.0;
+          for (int a = 0; a < 3; ++a) tau = std::max(tau, std::max(lo[a], hi[a]));
+          dsum += tau;
+        }
+        diag[cn] = dsum;
+      }
+  return PMX_SUCCESS;
+}
+
+static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
+                        const double* eigvals) {
+  std::fill(c->t_stages, c->t_stages + PMX_MAX_LOG_STAGES, 0.0);
+  const double t0 = now();
+  PMX_TRY(setup_fine(c));
+  const double t1 = now();
+  if (phi_nat) {
+    // Caller basis (natural-order columns) -> brick rows.
+    PMX_HIP(hipMemsetAsync(c->phi.p, 0, (size_t)c->ncell * c->en_lv1 * sizeof(double), c->stream));
+    DBuf<double> tmp;
+    PMX_TRY(tmp.alloc((size_t)c->ncell * c->en_lv1));
+    PMX_HIP(hipMemcpyAsync(tmp.p, phi_nat, (size_t)c->ncell * c->en_lv1 * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+    PMX_TRY(launch_basis_nat2rows(c, tmp.p, c->phi.p, c->en_lv1));
+    PMX_HIP(hipStreamSynchronize(c->stream));
+    c->h_eig_lv1.assign((size_t)c->nsub * c->en_lv1, 0.0);
+    if (eigvals) memcpy(c->h_eig_lv1.data(), eigvals, c->h_eig_lv1.size() * sizeof(double));
+    if (eigen_num)
+      c->h_eigen_num_given.assign(eigen_num, eigen_num + c->nsub);
+    else
+      c->h_eigen_num_given.assign(c->nsub, c->en_lv1);
+    c->eig_iters_max = c->eig_iters_sum = 0;
+  } else {
+    c->h_eigen_num_given.clear();
+    PMX_TRY(lobpcg_setup(c, nullptr));
+  }
+  c->t_stages[PMX_STAGE_SU_LV2_1] += now() - t1;
+  PMX_TRY(setup_coarse(c));
+  c->t_stages[PMX_STAGE_SU] += now() - t0;
+  c->setup_done = 1;
+  return PMX_SUCCESS;
+}
+
+int pmx_setup(pmx_ctx* c) {
+  if (!c) return PMX_ERR_ARG_NULL;
+  return setup_common(c, nullptr, nullptr, nullptr);
+}
+
+int pmx_setup_from_basis(pmx_ctx* c, const double* phi, const int* eigen_num,
+                         const double* eigvals) {
+  if (!c || !phi) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  return setup_common(c, phi, eigen_num, eigvals);
+}
+
+int pmx_apply(pmx_ctx* c, const double* x, double* y) {
+  if (!c || !x || !y) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  if (!c->setup_done) {
+    set_error("PC_apply_vec before PC_setup");
+    return PMX_ERR_ORDER;
+  }
+  PMX_HIP(hipMemcpyAsync(c->nat_buf.p, x, c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                         c->stream));
+  PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->w_x.p, 1, c->ncell, c->ncell));
+  PMX_TRY(apply_device(c, c->w_x.p, c->w_z.p, nullptr));
+  PMX_TRY(launch_brick2nat(c, c->w_z.p, c->nat_buf.p, 1, c->ncell, c->ncell));
+  PMX_HIP(hipMemcpyAsync(y, c->nat_buf.p, c->ncell * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  return PMX_SUCCESS;
+}
+
+int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rtol, int max_it,
+              int* its, double* rnorm) {
+  if (!c || !b || !x) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  if (!c->setup_done) {
+    set_error("KSPSolve before PC_setup");
+    return PMX_ERR_ORDER;
+  }
+  const double abstol = 1e-50, dtol = 1e5;
+  const size_t nb = c->ncell * sizeof(double);
+  double* xb = c->w_x.p;
+  double* bb = c->w_b.p;
+  double* r = c->w_r.p;
+  double* z = c->w_z.p;
+  double* p0 = c->w_p.p;
+  double* p1 = c->w_tmp.p;
+  double* w = c->w_q.p;
+  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
+  PMX_HIP(hipMemcpyAsync(c->nat_buf.p, b, nb, hipMemcpyHostToDevice, c->stream));
+  PMX_TRY(launch_nat2brick(c, c->nat_buf.p, bb, 1, c->ncell, c->ncell));
+  if (x0_nonzero) {
+    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, x, nb, hipMemcpyHostToDevice, c->stream));
+    PMX_TRY(launch_nat2brick(c, c->nat_buf.p, xb, 1, c->ncell, c->ncell));
+    PMX_TRY(launch_residual(c, bb, xb, r));
+  } else {
+    PMX_HIP(hipMemsetAsync(xb, 0, nb, c->stream));
+    PMX_TRY(launch_residual(c, bb, nullptr, r));
+  }
+  double hs[16];
+  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  const double t_solve0 = now();
+  double dp = std::sqrt(hs[4]);
+  const double rnorm0 = dp;
+  const double ttol = std::max(rtol * rnorm0, abstol);
+  int reason = 0, k = 0;
+  if (dp <= ttol) {
+    reason = dp > abstol ? PMX_CONVERGED_RTOL : PMX_CONVERGED_ATOL;
+  } else {
+    PMX_TRY(apply_device(c, r, z, c->partial.p));
+    PMX_TRY(launch_finalize_beta(c));
+    int i = 0;
+    while (true) {
+      k = i + 1;
+      PMX_TRY(launch_cg_p_spmv(c, z, p0, p1, w, i == 0));
+      PMX_TRY(launch_cg_update(c, xb, r, p1, w));
+      PMX_HIP(hipMemcpyAsync(hs, c->scal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+      PMX_HIP(hipStreamSynchronize(c->stream));
+      if (hs[5] != 0.0) {
+        reason = PMX_DIVERGED_INDEFINITE_MAT;
+        break;
+      }
+      dp = std::sqrt(hs[4]);
+      if (dp <= ttol) {
+        reason = PMX_CONVERGED_RTOL;
+        break;
+      }
+      if (dp >= dtol * rnorm0) {
+        reason = PMX_DIVERGED_DTOL;
+        break;
+      }
+      if (k >= max_it) {
+        reason = PMX_DIVERGED_ITS;
+        break;
+      }
+      PMX_TRY(apply_device(c, r, z, c->partial.p));
+      PMX_TRY(launch_finalize_beta(c));
+      std::swap(p0, p1);
+      ++i;
+    }
+  }
+  PMX_TRY(launch_brick2nat(c, xb, c->nat_buf.p, 1, c->ncell, c->ncell));
+  PMX_HIP(hipMemcpyAsync(x, c->nat_buf.p, nb, hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  c->last_its = k;
+  c->last_rnorm = dp;
+  c->last_rnorm0 = rnorm0;
+  c->last_reason = reason;
+  c->t_solve = now() - t_solve0;
+  if (its) *its = k;
+  if (rnorm) *rnorm = dp;
+  return reason > 0 ? PMX_SUCCESS : PMX_ERR_NOT_CONVERGED;
+}
+
+int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
+  if (!c || !st) return PMX_ERR_ARG_NULL;
+  memset(st, 0, sizeof(*st));
+  st->M = c->mesh[0];
+  st->N = c->mesh[1];
+  st->P = c->mesh[2];
+  for (int a = 0; a < 3; ++a) st->sd[a] = c->sd[a];
+  st->nsub = c->nsub;
+  st->size_coarse = c->nc;
+  st->size_mat_lv3 = c->eigen_num_lv2;
+  if (!c->h_eigen_num.empty()) {
+    st->eigen_num_lv1_range[0] = st->eigen_num_lv1_range[1] = c->h_eigen_num[0];
+    st->eigen_min_lv1_range[0] = st->eigen_min_lv1_range[1] = c->h_eigen_min_lv1[0];
+    st->eigen_max_lv1_range[0] = st->eigen_max_lv1_range[1] = c->h_eigen_max_lv1[0];
+    for (int s = 1; s < c->nsub; ++s) {
+      st->eigen_num_lv1_range[0] = std::min(st->eigen_num_lv1_range[0], c->h_eigen_num[s]);
+      st->eigen_num_lv1_range[1] = std::max(st->eigen_num_lv1_range[1], c->h_eigen_num[s]);
+      st->eigen_min_lv1_range[0] = std::min(st->eigen_min_lv1_range[0], c->h_eigen_min_lv1[s]);
+      st->eigen_min_lv1_range[1] = std::max(st->eigen_min_lv1_range[1], c->h_eigen_min_lv1[s]);
+      st->eigen_max_lv1_range[0] = std::min(st->eigen_max_lv1_range[0], c->h_eigen_max_lv1[s]);
+      st->eigen_max_lv1_range[1] = std::max(st->eigen_max_lv1_range[1], c->h_eigen_max_lv1[s]);
+    }
+  }
+  st->eigen_num_lv2 = c->eigen_num_lv2;
+  st->eigen_min_lv2 = c->eigen_min_lv2;
+  st->eigen_max_lv2 = c->eigen_max_lv2;
+  for (int i = 0; i < PMX_MAX_LOG_STAGES; ++i) st->t_stages[i] = c->t_stages[i];
+  st->eig_iters_max = c->eig_iters_max;
+  st->eig_iters_mean = c->nsub ? (double)c->eig_iters_sum / c->nsub : 0.0;
+  st->eig_fallbacks = c->eig_fallbacks;
+  st->lv3_iters = c->lv3_iters;
+  st->pc_precision = c->pc_precision;
+  st->its = c->last_its;
+  st->rnorm = c->last_rnorm;
+  st->rnorm0 = c->last_rnorm0;
+  st->reason = c->last_reason;
+  st->t_solve = c->t_solve;
+  return PMX_SUCCESS;
+}
+
+int pmx_print_stat(const pmx_ctx* c) {
+  pmx_stats s;
+  PMX_TRY(pmx_get_stats(c, &s));
+  printf("eigen_max_lv1_range=[%.5E, %.5E], \teigen_min_lv1_range=[%.5E, %.5E], "
+         "\teigen_num_lv1_range=[%d, %d].\n",
+         s.eigen_max_lv1_range[0], s.eigen_max_lv1_range[1], s.eigen_min_lv1_range[0],
+         s.eigen_min_lv1_range[1], s.eigen_num_lv1_range[0], s.eigen_num_lv1_range[1]);
+  printf("eigen_max_lv2_range=[%.5E, %.5E], \teigen_min_lv2_range=[%.5E, %.5E], "
+         "\teigen_num_lv2_range=[%d, %d], \tsize_mat_lv3=%d.\n",
+         s.eigen_max_lv2, s.eigen_max_lv2, s.eigen_min_lv2, s.eigen_min_lv2, s.eigen_num_lv2,
+         s.eigen_num_lv2, s.size_mat_lv3);
+  const double* t = s.t_stages;
+  printf("su_lv1=[%.5f, %.5f], \tsu_lv2_1=[%.5f, %.5f], \tsu_lv2_2=[%.5f, %.5f], "
+         "\tsu_lv3_1=[%.5f, %.5f], \tsu_lv3_2=[%.5f, %.5f], \tsu=[%.5f, %.5f].\n",
+         t[0], t[0], t[1], t[1], t[2], t[2], t[3], t[3], t[4], t[4], t[8], t[8]);
+  printf("av_lv1=[%.5f, %.5f], \tav_lv2=[%.5f, %.5f], \tav_lv3=[%.5f, %.5f], \tav=[%.5f, %.5f].\n",
+         t[5], t[5], t[6], t[6], t[7], t[7], t[9], t[9]);
+  return PMX_SUCCESS;
+}
+
+int pmx_get_basis(const pmx_ctx* cc, int level, double* out, double* eigvals) {
+  pmx_ctx* c = const_cast<pmx_ctx*>(cc);
+  if (!c || !out) return PMX_ERR_ARG_NULL;
+  if (!c->setup_done) {
+    set_error("basis requested before setup");
+    return PMX_ERR_ORDER;
+  }
+  if (level == 2) {
+    DBuf<double> tmp;
+    PMX_TRY(tmp.alloc((size_t)c->ncell * c->en_lv1));
+    PMX_TRY(launch_basis_rows2nat(c, c->phi.p, tmp.p, c->en_lv1));
+    PMX_HIP(hipMemcpyAsync(out, tmp.p, (size_t)c->ncell * c->en_lv1 * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP(hipStreamSynchronize(c->stream));
+    if (eigvals) memcpy(eigvals, c->h_eig_lv1.data(), c->h_eig_lv1.size() * sizeof(double));
+    return PMX_SUCCESS;
+  }
+  if (level == 3) {
+    if (c->use_2level) {
+      set_error("no level-3 basis in the 2-level variant");
+      return PMX_ERR_ORDER;
+    }
+    // Phi_cc = Phi V on the host.
+    const int k2 = c->eigen_num_lv2, kmax = c->en_lv1;
+    std::vector<double> V((size_t)c->nc * k2), rows((size_t)c->ncell * kmax);
+    PMX_HIP(hipMemcpy(V.data(), c->V2.p, V.size() * sizeof(double), hipMemcpyDeviceToHost));
+    PMX_HIP(hipMemcpy(rows.data(), c->phi.p, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < c->ncell; ++n) {
+      const int bi = c->h_nat2brick[n];
+      int s = 0;
+      // find the brick of bi
+      int lo = 0, hi = c->nsub - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (c->h_bricks[mid].off <= bi)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      s = lo;
+      const Brick& b = c->h_bricks[s];
+      for (int j = 0; j < k2; ++j) {
+        double v = 0.0;
+        for (int i = 0; i < b.neig; ++i) v += rows[(size_t)bi * kmax + i] * V[(size_t)j * c->nc + b.dof + i];
+        out[(size_t)j * c->ncell + n] = v;
+      }
+    }
+    if (eigvals) memcpy(eigvals, c->h_eig_lv2.data(), c->h_eig_lv2.size() * sizeof(double));
+    return PMX_SUCCESS;
+  }
+  set_error("level must be 2 or 3");
+  return PMX_ERR_ARG_WRONG;
+}
+
+int pmx_get_coarse_matrix(const pmx_ctx* c, double* out) {
+  if (!c || !out) return PMX_ERR_ARG_NULL;
+  if (!c->setup_done) {
+    set_error("coarse matrix requested before setup");
+    return PMX_ERR_ORDER;
+  }
+  PMX_HIP(hipMemcpy(out, c->Ac.p, (size_t)c->nc * c->nc * sizeof(double), hipMemcpyDeviceToHost));
+  return PMX_SUCCESS;
+}
+
+int pmx_final(pmx_ctx* c) {
+  if (!c) return PMX_SUCCESS;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->d_bricks.release();
+  for (auto& d : c->d_ax) d.release();
+  c->d_plane_start.release();
+  c->d_plane_cell.release();
+  c->d_shape_pstart.release();
+  c->d_shape_coff.release();
+  for (int a = 0; a < 3; ++a) {
+    c->kappa[a].release();
+    c->T[a].release();
+    c->T32[a].release();
+  }
+  for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
+                          &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
+                          &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,
+                          &c->partial, &c->scal, &c->nat_buf})
+    b->release();
+  for (DBuf<float>* b : {&c->fac32_f, &c->fac32_b, &c->phi32, &c->w_y32, &c->w_y32b}) b->release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PMX_SUCCESS;
+}
+
+}  // extern "C"

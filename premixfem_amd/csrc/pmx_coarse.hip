@@ -1,0 +1,669 @@
+// Coarse levels: Galerkin assembly of Phi^T A Phi, its dense FP64 SPD
+// inversion (block Gauss-Jordan sweep), the coarse-space operators of the
+// V-cycle, and the level-3 eigenproblem / coarse-coarse matrix.
+#include <cmath>
+#include <random>
+
+#include "pmx_dense.h"
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+// ---------------------------------------------------------------------------
+// Galerkin coarse matrix A_c = Phi^T A Phi (src/PreMixFEM_3D.c:737-890).
+// Diagonal block of subdomain s: Phi_s^T A[Omega_s,Omega_s] Phi_s (the
+// principal submatrix keeps the faces to other subdomains on its diagonal);
+// off-diagonal block with the upper neighbour s' across a face:
+// -sum_face T phi_s,i phi_s',j.  One workgroup per subdomain, rows in chunks
+// of 4 to bound registers; only the upper triangle of the diagonal block is
+// reduced and mirrored, so A_c is exactly symmetric.
+// Output: dense nc x nc (row-major) and block rows Acb[s][slot][kmax][kmax]
+// with slot 0 = self, 1..6 = -x,+x,-y,+y,-z,+z.
+// ---------------------------------------------------------------------------
+__global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
+                           const double* __restrict__ Tx, const double* __restrict__ Ty,
+                           const double* __restrict__ Tz, int nc, double* __restrict__ Ac,
+                           double* __restrict__ Acb) {
+  __shared__ double red[8];
+  const int s = blockIdx.x;
+  const Brick b = g.bricks[s];
+  const int n = b.nx * b.ny * b.nz;
+  const int ke = b.neig;
+  const double* TT3[3] = {Tx, Ty, Tz};
+  double* blk = Acb + (size_t)s * 7 * kmax * kmax;
+  for (int i0 = 0; i0 < ke; i0 += 4) {
+    double acc[4][kMaxEig];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int j = 0; j < kMaxEig; ++j) acc[a][j] = 0.0;
+    for (int li = threadIdx.x; li < n; li += blockDim.x) {
+      const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
+      const int c = b.off + li;
+      double diag = 0.0;
+      double aphi[kMaxEig];
+      const double* pc = phi + (size_t)c * kmax;
+#pragma unroll
+      for (int j = 0; j < kMaxEig; ++j) aphi[j] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const int nbi = nbr_index(g, b, c, lx, ly, lz, d);
+        if (nbi < 0) continue;
+        const double t = (d & 1) ? TT3[d >> 1][nbi] : TT3[d >> 1][c];
+        diag += t;
+        const int lcoord = (d >> 1) == 0 ? lx : ((d >> 1) == 1 ? ly : lz);
+        const int lext = (d >> 1) == 0 ? b.nx : ((d >> 1) == 1 ? b.ny : b.nz);
+        const bool inside = (d & 1) ? (lcoord < lext - 1) : (lcoord > 0);
+        if (inside) {
+          const double* pn = phi + (size_t)nbi * kmax;
+#pragma unroll
+          for (int j = 0; j < kMaxEig; ++j)
+            if (j < ke) aphi[j] -= t * pn[j];
+        }
+      }
+      if (c == g.pin) diag += g.tau;
+#pragma unroll
+      for (int j = 0; j < kMaxEig; ++j)
+        if (j < ke) aphi[j] += diag * pc[j];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (i0 + a >= ke) break;
+        const double pi = pc[i0 + a];
+#pragma unroll
+        for (int j = 0; j < kMaxEig; ++j) acc[a][j] += pi * aphi[j];
+      }
+    }
+    for (int a = 0; a < 4; ++a) {
+      const int i = i0 + a;
+      if (i >= ke) break;
+      for (int j = i; j < ke; ++j) {
+        const double v = block_sum(acc[a][j], red);
+        if (threadIdx.x == 0) {
+          Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
+          Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
+          blk[i * kmax + j] = v;
+          blk[j * kmax + i] = v;
+        }
+      }
+    }
+  }
+  // Faces to the upper neighbours.
+  for (int dir = 1; dir < 6; dir += 2) {
+    const int sn = b.nb[dir];
+    if (sn < 0) continue;
+    const Brick o = g.bricks[sn];
+    const int axis = dir >> 1;
+    const int nf = axis == 0 ? b.ny * b.nz : (axis == 1 ? b.nx * b.nz : b.nx * b.ny);
+    for (int i0 = 0; i0 < ke; i0 += 4) {
+      double acc[4][kMaxEig];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int j = 0; j < kMaxEig; ++j) acc[a][j] = 0.0;
+      for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+        int lx, ly, lz;
+        if (axis == 0) {
+          lx = b.nx - 1;
+          ly = f % b.ny;
+          lz = f / b.ny;
+        } else if (axis == 1) {
+          ly = b.ny - 1;
+          lx = f % b.nx;
+          lz = f / b.nx;
+        } else {
+          lz = b.nz - 1;
+          lx = f % b.nx;
+          ly = f / b.nx;
+        }
+        const int c = b.off + (lz * b.ny + ly) * b.nx + lx;
+        const int nbi = nbr_index(g, b, c, lx, ly, lz, dir);
+        const double t = TT3[axis][nbi];
+        const double* pc = phi + (size_t)c * kmax;
+        const double* pn = phi + (size_t)nbi * kmax;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (i0 + a >= ke) break;
+          const double pi = t * pc[i0 + a];
+#pragma unroll
+          for (int j = 0; j < kMaxEig; ++j)
+            if (j < o.neig) acc[a][j] -= pi * pn[j];
+        }
+      }
+      for (int a = 0; a < 4; ++a) {
+        const int i = i0 + a;
+        if (i >= ke) break;
+        for (int j = 0; j < o.neig; ++j) {
+          const double v = block_sum(acc[a][j], red);
+          if (threadIdx.x == 0) {
+            Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
+            Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
+            blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
+            Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
+          }
+        }
+      }
+    }
+  }
+}
+
+int launch_galerkin(pmx_ctx* ctx) {
+  PMX_HIP(hipMemsetAsync(ctx->Ac.p, 0, sizeof(double) * (size_t)ctx->nc * ctx->nc, ctx->stream));
+  PMX_HIP(hipMemsetAsync(ctx->Acb.p, 0, sizeof(double) * ctx->Acb.n, ctx->stream));
+  hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                     ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->nc,
+                     ctx->Ac.p, ctx->Acb.p);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Dense SPD inversion by the block sweep operator: for each 64-wide pivot
+// block k with D = A_kk^{-1}:  A_ij -= A_ik D A_kj (i,j != k),
+// A_ik <- A_ik D, A_kj <- D A_kj, A_kk <- -D.  After all blocks A = -A^{-1}.
+// Pivots are Schur complements of an SPD matrix, so no pivoting is needed.
+// ---------------------------------------------------------------------------
+constexpr int kSw = 64;
+
+__global__ void k_sweep_pivot(const double* __restrict__ A, int n, int k0, int bs,
+                              double* __restrict__ D) {
+  __shared__ double M[kSw][kSw + 1];
+  __shared__ double rowt[kSw], fcol[kSw];
+  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+    const int i = e / bs, j = e % bs;
+    M[i][j] = A[(size_t)(k0 + i) * n + k0 + j];
+  }
+  __syncthreads();
+  for (int t = 0; t < bs; ++t) {
+    const double piv = 1.0 / M[t][t];
+    if (threadIdx.x < bs) {
+      const int j = threadIdx.x;
+      rowt[j] = (j == t ? 1.0 : M[t][j]) * piv;
+      fcol[j] = M[j][t];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+      const int i = e / bs, j = e % bs;
+      if (i == t)
+        M[i][j] = rowt[j];
+      else
+        M[i][j] = (j == t ? 0.0 : M[i][j]) - fcol[i] * rowt[j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
+    const int i = e / bs, j = e % bs;
+    // Symmetrise the pivot inverse.
+    D[i * kSw + j] = 0.5 * (M[i][j] + M[j][i]);
+  }
+}
+
+// P[r][t] = A[r][k0+t]; U[r][t] = sum_s A[r][k0+s] D[s][t].  4 rows per block.
+__global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int bs,
+                              const double* __restrict__ D, double* __restrict__ P,
+                              double* __restrict__ U) {
+  __shared__ double Ds[kSw][kSw + 1];
+  __shared__ double Ar[4][kSw];
+  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) Ds[e / bs][e % bs] = D[(e / bs) * kSw + e % bs];
+  const int r0 = blockIdx.x * 4;
+  const int rr = threadIdx.x / kSw, t = threadIdx.x % kSw;
+  const int r = r0 + rr;
+  if (r < n && t < bs) Ar[rr][t] = A[(size_t)r * n + k0 + t];
+  __syncthreads();
+  if (r < n && t < bs) {
+    double s = 0.0;
+    for (int q = 0; q < bs; ++q) s += Ar[rr][q] * Ds[q][t];
+    P[(size_t)r * kSw + t] = Ar[rr][t];
+    U[(size_t)r * kSw + t] = s;
+  }
+}
+
+// Trailing update on 64x64 tiles, 4x4 outputs per thread.
+__global__ void k_sweep_update(double* __restrict__ A, int n, int k0, int bs,
+                               const double* __restrict__ P, const double* __restrict__ U,
+                               const double* __restrict__ D) {
+  __shared__ double Us[kSw][kSw + 1];  // Us[t][i]
+  __shared__ double Ps[kSw][kSw + 1];  // Ps[t][j]
+  const int I0 = blockIdx.y * kSw, J0 = blockIdx.x * kSw;
+  const int kb = k0 / kSw;
+  const int ti = blockIdx.y, tj = blockIdx.x;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  if (ti == kb || tj == kb) {
+    for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
+      const int i = I0 + e / kSw, j = J0 + e % kSw;
+      if (i >= n || j >= n) continue;
+      double v;
+      if (ti == kb && tj == kb)
+        v = -D[(i - k0) * kSw + (j - k0)];
+      else if (ti == kb)
+        v = U[(size_t)j * kSw + (i - k0)];
+      else
+        v = U[(size_t)i * kSw + (j - k0)];
+      A[(size_t)i * n + j] = v;
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
+    const int rI = e / kSw, t = e % kSw;
+    Us[t][rI] = (I0 + rI < n && t < bs) ? U[(size_t)(I0 + rI) * kSw + t] : 0.0;
+    Ps[t][rI] = (J0 + rI < n && t < bs) ? P[(size_t)(J0 + rI) * kSw + t] : 0.0;
+  }
+  __syncthreads();
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int t = 0; t < bs; ++t) {
+    double u[4], p[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) u[a] = Us[t][ty + 16 * a];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) p[c] = Ps[t][tx + 16 * c];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[a][c] += u[a] * p[c];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = I0 + ty + 16 * a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = J0 + tx + 16 * c;
+      if (j < n) A[(size_t)i * n + j] -= acc[a][c];
+    }
+  }
+}
+
+__global__ void k_copy_shift(const double* __restrict__ A, double* __restrict__ B, int n,
+                             double shift, double scale) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double v = A[e] * scale;
+    if (e / n == e % n) v += shift;
+    B[e] = v;
+  }
+}
+
+int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift) {
+  const int n = ctx->nc;
+  DBuf<double> D, P, U;
+  PMX_TRY(D.alloc(kSw * kSw));
+  PMX_TRY(P.alloc((size_t)n * kSw));
+  PMX_TRY(U.alloc((size_t)n * kSw));
+  const int nblk = 1024;
+  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, A, Ainv, n, shift,
+                     1.0);
+  PMX_HIP(hipGetLastError());
+  const int nt = (n + kSw - 1) / kSw;
+  for (int k0 = 0; k0 < n; k0 += kSw) {
+    const int bs = std::min(kSw, n - k0);
+    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kBlock), 0, ctx->stream, Ainv, n, k0, bs,
+                       D.p);
+    hipLaunchKernelGGL(k_sweep_panel, dim3((n + 3) / 4), dim3(kBlock), 0, ctx->stream, Ainv, n,
+                       k0, bs, D.p, P.p, U.p);
+    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(kBlock), 0, ctx->stream, Ainv, n, k0,
+                       bs, P.p, U.p, D.p);
+    PMX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, Ainv, Ainv, n, 0.0,
+                     -1.0);
+  PMX_HIP(hipGetLastError());
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  D.release();
+  P.release();
+  U.release();
+  return PMX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-space operators
+// ---------------------------------------------------------------------------
+// y = A x for a dense row-major n x n matrix; one wavefront per row.
+__global__ void k_gemv(const double* __restrict__ A, const double* __restrict__ x,
+                       double* __restrict__ y, int n) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* a = A + (size_t)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += a[j] * x[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) y[row] = s;
+}
+
+int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n) {
+  const int wpb = kBlock / 64;
+  hipLaunchKernelGGL(k_gemv, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream, A, x, y,
+                     n);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+// Row r of A_c d from the block rows.
+__device__ __forceinline__ double coarse_row(const Geo& g, const double* __restrict__ Acb, int kmax,
+                                             int s, int i, const double* __restrict__ d) {
+  const Brick& b = g.bricks[s];
+  const double* blk = Acb + (size_t)s * 7 * kmax * kmax;
+  double acc = 0.0;
+  for (int j = 0; j < b.neig; ++j) acc += blk[i * kmax + j] * d[b.dof + j];
+  for (int dir = 0; dir < 6; ++dir) {
+    const int sn = b.nb[dir];
+    if (sn < 0) continue;
+    const Brick& o = g.bricks[sn];
+    const double* bb = blk + (size_t)(dir + 1) * kmax * kmax;
+    for (int j = 0; j < o.neig; ++j) acc += bb[i * kmax + j] * d[o.dof + j];
+  }
+  return acc;
+}
+
+// u = d (first) or u += d; if update_t: t -= A_c d.
+__global__ void k_coarse_update(Geo g, const double* __restrict__ Acb, int kmax,
+                                const double* __restrict__ d, double* __restrict__ t,
+                                double* __restrict__ u, int first, int update_t) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid / kmax, i = gid % kmax;
+  if (s >= g.nsub) return;
+  const Brick& b = g.bricks[s];
+  if (i >= b.neig) return;
+  const int r = b.dof + i;
+  u[r] = first ? d[r] : u[r] + d[r];
+  if (update_t) t[r] -= coarse_row(g, Acb, kmax, s, i, d);
+}
+
+int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, int first,
+                         int update_t) {
+  const int n = ctx->nsub * ctx->en_lv1;
+  hipLaunchKernelGGL(k_coarse_update, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     ctx->stream, ctx->geo, ctx->Acb.p, ctx->en_lv1, d, t, u, first, update_t);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+// Y = A_c X for m column vectors (column-major, stride nc).
+__global__ void k_coarse_spmm(Geo g, const double* __restrict__ Acb, int kmax,
+                              const double* __restrict__ X, double* __restrict__ Y, int nc,
+                              int m) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid / kmax, i = gid % kmax;
+  if (s >= g.nsub) return;
+  const Brick& b = g.bricks[s];
+  if (i >= b.neig) return;
+  for (int c = 0; c < m; ++c)
+    Y[(size_t)c * nc + b.dof + i] = coarse_row(g, Acb, kmax, s, i, X + (size_t)c * nc);
+}
+
+// d = V (A_cc^{-1} (V^T t)) with the LU of A_cc (level-3 apply,
+// _PC_apply_vec_lv3, src/PreMixFEM_3D.c:1597-1648, written in the coarse
+// space since Phi_cc = Phi V).  One workgroup.
+__global__ void k_lv3_apply(const double* __restrict__ V, int nc, int k2,
+                            const double* __restrict__ LU, const double* __restrict__ t,
+                            double* __restrict__ d) {
+  __shared__ double red[8];
+  __shared__ double c[kMaxEig];
+  for (int j = 0; j < k2; ++j) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nc; r += blockDim.x) s += V[(size_t)j * nc + r] * t[r];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) c[j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // LU with row pivots stored after the k2 x k2 factors.
+    const double* piv = LU + k2 * k2;
+    double b[kMaxEig];
+    for (int i = 0; i < k2; ++i) b[i] = c[i];
+    for (int i = 0; i < k2; ++i) {
+      const int p = (int)piv[i];
+      const double tmp = b[i];
+      b[i] = b[p];
+      b[p] = tmp;
+    }
+    for (int i = 0; i < k2; ++i)
+      for (int j = 0; j < i; ++j) b[i] -= LU[i * k2 + j] * b[j];
+    for (int i = k2 - 1; i >= 0; --i) {
+      for (int j = i + 1; j < k2; ++j) b[i] -= LU[i * k2 + j] * b[j];
+      b[i] /= LU[i * k2 + i];
+    }
+    for (int i = 0; i < k2; ++i) c[i] = b[i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < nc; r += blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < k2; ++j) s += V[(size_t)j * nc + r] * c[j];
+    d[r] = s;
+  }
+}
+
+int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d) {
+  hipLaunchKernelGGL(k_lv3_apply, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, ctx->nc,
+                     ctx->eigen_num_lv2, ctx->Acc_lu.p, t, d);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Level 3 (src/PreMixFEM_3D.c:918-1390): smallest en_lv2 eigenpairs of the
+// coarse matrix by block inverse (subspace) iteration on the dense inverse
+// with Rayleigh-Ritz, then Phi_cc = Phi V and A_cc = V^T A_c V + NIL I with
+// an LU factorisation.
+// ---------------------------------------------------------------------------
+// Y[:, c] = Ainv X[:, c] for m columns; one wavefront per row.
+__global__ void k_gemm_tall(const double* __restrict__ A, const double* __restrict__ X,
+                            double* __restrict__ Y, int n, int m) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* a = A + (size_t)row * n;
+  double s[2 * kMaxEig];
+  for (int c = 0; c < m; ++c) s[c] = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double av = a[j];
+    for (int c = 0; c < m; ++c) s[c] += av * X[(size_t)c * n + j];
+  }
+  for (int c = 0; c < m; ++c) {
+    double v = s[c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) Y[(size_t)c * n + row] = v;
+  }
+}
+
+// One-workgroup Rayleigh-Ritz step of the subspace iteration on column-major
+// blocks of n rows.  mode 0: G = Y^T Y, Y <- Y L^{-T} (Cholesky QR).
+// mode 1: H = Y^T Z, H = Q diag(w) Q^T, X = Y Q, Z <- Z Q, res[c] =
+// ||Z q_c - w_c X q_c||.
+__global__ void k_lv3_rr(double* __restrict__ Y, double* __restrict__ Z, double* __restrict__ X,
+                         int n, int m, int mode, double* __restrict__ w_out,
+                         double* __restrict__ res, int* __restrict__ fail) {
+  constexpr int L = 2 * kMaxEig;
+  __shared__ double G[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
+  __shared__ double red[8];
+  __shared__ int flag;
+  for (int e = 0; e < m * m; ++e) {
+    const int i = e / m, j = e % m;
+    if (j < i) continue;
+    const double* a = Y + (size_t)i * n;
+    const double* bb = (mode == 0 ? Y : Z) + (size_t)j * n;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) s += a[r] * bb[r];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) {
+      G[i * L + j] = s;
+      G[j * L + i] = s;
+    }
+  }
+  __syncthreads();
+  if (mode == 0) {
+    const bool ok = wg_cholesky(G, m, L, &flag);
+    if (!ok && threadIdx.x == 0) *fail = 1;
+    wg_trinv_lower(G, tmp, m, L);  // G = L^{-1}
+    // Y <- Y L^{-T}: new column c = sum_k Y[:,k] * Linv[c][k]
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      double v[L];
+      for (int k = 0; k < m; ++k) v[k] = Y[(size_t)k * n + r];
+      for (int c = 0; c < m; ++c) {
+        double s = 0.0;
+        for (int k = 0; k <= c; ++k) s += v[k] * G[c * L + k];
+        Y[(size_t)c * n + r] = s;
+      }
+    }
+    return;
+  }
+  wg_syev(G, Q, tmp, w, rot, m, L);
+  if (threadIdx.x < m) w_out[threadIdx.x] = w[threadIdx.x];
+  double rs[L];
+  for (int c = 0; c < m; ++c) rs[c] = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    double yv[L], zv[L];
+    for (int k = 0; k < m; ++k) {
+      yv[k] = Y[(size_t)k * n + r];
+      zv[k] = Z[(size_t)k * n + r];
+    }
+    for (int c = 0; c < m; ++c) {
+      double xs = 0.0, zs = 0.0;
+      for (int k = 0; k < m; ++k) {
+        xs += yv[k] * Q[k * L + c];
+        zs += zv[k] * Q[k * L + c];
+      }
+      X[(size_t)c * n + r] = xs;
+      Z[(size_t)c * n + r] = zs;
+      const double e = zs - w[c] * xs;
+      rs[c] += e * e;
+    }
+  }
+  for (int c = 0; c < m; ++c) {
+    const double s = block_sum(rs[c], red);
+    if (threadIdx.x == 0) res[c] = sqrt(s);
+  }
+}
+
+// Sign normalisation (positive inner product with 1 + frac(0.618..(r+1))),
+// A_cc = V^T A_c V + shift I and its LU with partial pivoting.
+__global__ void k_lv3_finish(double* __restrict__ V, const double* __restrict__ AV, int n, int k2,
+                             double shift, double* __restrict__ LU) {
+  __shared__ double red[8];
+  __shared__ double sg[kMaxEig];
+  __shared__ double A[kMaxEig * kMaxEig];
+  for (int j = 0; j < k2; ++j) {
+    double s = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const double wgt = 1.0 + fmod(0.6180339887498949 * (r + 1.0), 1.0);
+      s += wgt * V[(size_t)j * n + r];
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) sg[j] = s < 0.0 ? -1.0 : 1.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < k2; ++j)
+    for (int r = threadIdx.x; r < n; r += blockDim.x) V[(size_t)j * n + r] *= sg[j];
+  __syncthreads();
+  for (int i = 0; i < k2; ++i)
+    for (int j = 0; j < k2; ++j) {
+      double s = 0.0;
+      for (int r = threadIdx.x; r < n; r += blockDim.x) s += V[(size_t)i * n + r] * AV[(size_t)j * n + r];
+      s = block_sum(s, red);
+      if (threadIdx.x == 0) A[i * kMaxEig + j] = s * sg[j] + (i == j ? shift : 0.0);
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double piv[kMaxEig];
+    for (int k = 0; k < k2; ++k) {
+      int p = k;
+      for (int i = k + 1; i < k2; ++i)
+        if (fabs(A[i * kMaxEig + k]) > fabs(A[p * kMaxEig + k])) p = i;
+      piv[k] = p;
+      if (p != k)
+        for (int j = 0; j < k2; ++j) {
+          const double t = A[k * kMaxEig + j];
+          A[k * kMaxEig + j] = A[p * kMaxEig + j];
+          A[p * kMaxEig + j] = t;
+        }
+      for (int i = k + 1; i < k2; ++i) {
+        const double f = A[i * kMaxEig + k] / A[k * kMaxEig + k];
+        A[i * kMaxEig + k] = f;
+        for (int j = k + 1; j < k2; ++j) A[i * kMaxEig + j] -= f * A[k * kMaxEig + j];
+      }
+    }
+    for (int i = 0; i < k2; ++i)
+      for (int j = 0; j < k2; ++j) LU[i * k2 + j] = A[i * kMaxEig + j];
+    for (int i = 0; i < k2; ++i) LU[k2 * k2 + i] = piv[i];
+  }
+}
+
+int level3_setup(pmx_ctx* ctx) {
+  const int n = ctx->nc;
+  const int k2 = std::min(ctx->en_lv2, n);
+  const int m = std::min(std::max(2 * k2, k2 + 4), std::min(n, 2 * kMaxEig));
+  DBuf<double> X, Y, Z, w, res;
+  DBuf<int> fail;
+  PMX_TRY(X.alloc((size_t)n * m));
+  PMX_TRY(Y.alloc((size_t)n * m));
+  PMX_TRY(Z.alloc((size_t)n * m));
+  PMX_TRY(w.alloc(m));
+  PMX_TRY(res.alloc(m));
+  PMX_TRY(fail.alloc(1));
+  std::vector<double> h((size_t)n * m);
+  std::mt19937_64 rng(ctx->seed + 7);
+  std::normal_distribution<double> nd;
+  for (auto& v : h) v = nd(rng);
+  PMX_HIP(hipMemcpyAsync(X.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
+  const int wpb = kBlock / 64;
+  const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
+  std::vector<double> hw(m), hres(m);
+  int it = 0;
+  const int maxit = 300;
+  for (it = 1; it <= maxit; ++it) {
+    hipLaunchKernelGGL(k_gemm_tall, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream,
+                       ctx->Ainv.p, X.p, Y.p, n, m);
+    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 0,
+                       w.p, res.p, fail.p);
+    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n, m);
+    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 1,
+                       w.p, res.p, fail.p);
+    PMX_HIP(hipGetLastError());
+    PMX_HIP(hipMemcpyAsync(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PMX_HIP(hipMemcpyAsync(hres.data(), res.p, m * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));
+    bool conv = true;
+    const double scale = std::fabs(hw[m - 1]);
+    for (int j = 0; j < k2; ++j)
+      if (!(hres[j] <= 1e-10 * std::max(std::fabs(hw[j]), 1e-14 * scale))) conv = false;
+    if (conv) break;
+  }
+  int hfail = 0;
+  PMX_HIP(hipMemcpy(&hfail, fail.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (it > maxit || hfail) {
+    set_error("level-3 eigensolver did not converge (%d iterations, fail=%d)", it, hfail);
+    return PMX_ERR_USER;
+  }
+  ctx->lv3_iters = it;
+  // Adaptive count against eb_lv2 (src/PreMixFEM_3D.c:1100-1115).
+  int num = k2;
+  for (int j = 0; j < k2; ++j)
+    if (hw[j] >= ctx->eb_lv2) {
+      num = j + 1;
+      break;
+    }
+  ctx->eigen_num_lv2 = num;
+  ctx->eigen_min_lv2 = hw[0];
+  ctx->eigen_max_lv2 = hw[num - 1];
+  ctx->h_eig_lv2.assign(hw.begin(), hw.begin() + k2);
+  PMX_TRY(ctx->V2.alloc((size_t)n * num));
+  PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
+  // AV = A_c V (Z holds A_c X before the sign flip; k_lv3_finish applies it).
+  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, Z.p, n, num,
+                     ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
+  PMX_HIP(hipGetLastError());
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  return PMX_SUCCESS;
+}
+
+}  // namespace pmx

@@ -618,19 +618,23 @@ int launch_lv1_first(pmx_ctx* ctx, const double* x) {
   return PMX_SUCCESS;
 }
 
+// keep_y: y <- y + B1 (x - A y) written to the second buffer (other bricks
+// read y as halo while it is being updated), then the buffers swap.
 int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, double* partial) {
   if (ctx->pc_precision == 32) {
     hipLaunchKernelGGL((k_lv1_last<float, float, float4>), dim3(ctx->nsub), dim3(kBlock),
                        brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
                        ctx->w_y32.p, ctx->T32[0].p, ctx->T32[1].p, ctx->T32[2].p,
                        (const float4*)ctx->fac32_f.p, (const float4*)ctx->fac32_b.p, z,
-                       keep_y ? ctx->w_y32.p : nullptr, partial);
+                       keep_y ? ctx->w_y32b.p : nullptr, partial);
+    if (keep_y) std::swap(ctx->w_y32, ctx->w_y32b);
   } else {
     hipLaunchKernelGGL((k_lv1_last<double, double, double4>), dim3(ctx->nsub), dim3(kBlock),
                        brick_lds_bytes(ctx, sizeof(double)), ctx->stream, ctx->geo, x,
                        ctx->w_y.p, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
                        (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p, z,
-                       keep_y ? ctx->w_y.p : nullptr, partial);
+                       keep_y ? ctx->w_yb.p : nullptr, partial);
+    if (keep_y) std::swap(ctx->w_y, ctx->w_yb);
   }
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;

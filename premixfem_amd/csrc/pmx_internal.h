@@ -178,6 +178,8 @@ struct pmx_ctx {
   double eig_tol = 1.0e-8;      // LOBPCG relative residual tolerance
   int eig_maxit = 400;
   int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1
+  double eig_sigma = 1.0e-6;    // shift of the LOBPCG preconditioner A^N + sigma M
+  int log_stages = 0;           // synchronise and time every apply stage
   uint64_t seed = 12345;
 
   // ---- derived geometry ----
@@ -188,6 +190,7 @@ struct pmx_ctx {
   std::vector<int> h_ax_sub[3], h_ax_loc[3];
   std::vector<int> h_starts[3], h_lens[3];
   std::vector<int> h_plane_start, h_plane_cell, h_shape_pstart, h_shape_coff;
+  std::vector<int> h_nat2brick;     // natural index -> brick index
   int nshape = 0;
 
   pmx::DBuf<pmx::Brick> d_bricks;
@@ -234,7 +237,8 @@ struct pmx_ctx {
 
   // ---- work vectors ----
   pmx::DBuf<double> w_x, w_y, w_r, w_b, w_p, w_z, w_q, w_tmp;   // brick order FP64
-  pmx::DBuf<float> w_y32;
+  pmx::DBuf<float> w_y32, w_y32b;
+  pmx::DBuf<double> w_yb;
   pmx::DBuf<double> c_a, c_b, c_c, c_d, c_e;                       // coarse vectors
   pmx::DBuf<double> partial;                                       // per-block partial sums
   pmx::DBuf<double> scal;                                          // device scalars
@@ -246,4 +250,6 @@ struct pmx_ctx {
   int last_its = 0;
   double last_rnorm = 0.0, last_rnorm0 = 0.0;
   int last_reason = 0;
+  double t_solve = 0.0;
+  std::vector<int> h_eigen_num_given;  // caller-fixed eigen counts (setup from a basis)
 };

@@ -1,0 +1,50 @@
+"""Drive one pressure solve through the reference call sequence
+(PC_init -> kappa -> PC_create_A -> PCSHELL setup -> KSPSolve, as in
+``src/Expe_scal.c:66-160``) on the GPU, with wall-clock stage times."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import pc as P
+from .problems import Problem
+
+
+def options_for(prob: Problem, precision: int = 64, extra: str = "") -> str:
+    sdx, sdy, sdz = prob.sd
+    s = (f"-sdx {sdx} -sdy {sdy} -sdz {sdz} -en_lv1 {prob.en_lv1} -en_lv2 {prob.en_lv2} "
+         f"-eg_op {prob.eg_op} -pc_precision {precision}")
+    if prob.pin_cell is not None:
+        s += " -pin {},{},{}".format(*prob.pin_cell)
+    return (s + " " + extra).strip()
+
+
+def run(prob: Problem, precision: int = 64, extra: str = "", basis=None, rtol=None,
+        keep_ctx=False, sync=None):
+    """Set up and solve ``prob``; returns a dict with the solution (natural
+    order), iteration count, residual, setup / solve seconds and stats."""
+    ctx = P.PCCtx()
+    P.PC_init(ctx, prob.dom, prob.mesh, options_for(prob, precision, extra))
+    ctx.kappa = prob.kappa
+    shell = P.PCShell(ctx)
+    try:
+        t0 = time.perf_counter()
+        if basis is None:
+            P.PC_setup(shell)
+        else:
+            P.PC_setup_from_basis(shell, *basis)
+        t1 = time.perf_counter()
+        x, its, rnorm, reason = P.KSPSolve(shell, prob.rhs.ravel(),
+                                           rtol=prob.rtol if rtol is None else rtol)
+        t2 = time.perf_counter()
+        st = P.PC_get_stats(ctx)
+        out = dict(x=x, its=its, rnorm=rnorm, reason=reason, t_setup=t1 - t0, t_solve=t2 - t1,
+                   stats=st)
+        if keep_ctx:
+            out["ctx"] = ctx
+            out["shell"] = shell
+        return out
+    finally:
+        if not keep_ctx:
+            P.PC_final(ctx)
