@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark: setup + solve of the three-level GMsFEM preconditioner with PCG
+to rtol 1e-8 on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 128^3 cells,
+binary high-contrast medium (background k=1, 3000 seeded axis-aligned boxes
+of 2-6 cells per side with k=1e5), 8x8x8 = 512 subdomains of 16^3 cells,
+6 eigenvectors per subdomain, source/sink in opposite corners, one pinned
+cell, PCG to rtol 1e-8.
+
+A step is one full PC_setup (operator, level-1 ICC(0), 512 subdomain
+eigenproblems by batched LOBPCG, Galerkin coarse matrix, coarse inversion,
+level 3) plus one PCG solve from x0 = 0, with kappa, b and x already in HBM.
+Two modes are timed: the plain FP64 form and the FP32 preconditioner inside
+the FP64 CG; the headline value is the FP32-preconditioned form.
+
+Prints one JSON line (see the keys below); --gpus N > 1 is refused (one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "setup+solve s to rtol 1e-8: learned/FP32-prec vs plain FP64 GPU; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name, ncell, k, s, nc):
+    """Algorithmic HBM bytes of one launch (each array element moved once)."""
+    N = ncell
+    return {
+        "k_lv1_first": N * (8 + 9 * s),           # x; ICC(0) fwd+bwd {dinv,3 couplings}; y
+        "k_lv1_last": N * (16 + 12 * s),          # x, y, 3 faces, fwd+bwd factor; z
+        "k_resid_restrict": N * (8 + (4 + k) * s),  # x, y, 3 faces, k basis rows
+        "k_prolong": N * ((2 + k) * s),           # y in/out, k basis rows
+        "k_cg_p_spmv": N * 56,                    # z, p_old, 3 faces; p_new, w
+        "k_cg_update": N * 48,                    # x, r, p, w; x, r
+        "k_gemv": nc * nc * 8 + 16 * nc,          # dense coarse inverse
+    }.get(name)
+
+
+def run_mode(prob, precision, steps, warmup, extra=""):
+    import torch
+
+    from premixfem_amd import _lib
+    from premixfem_amd import pc as P
+    from premixfem_amd import runner
+
+    L = _lib.lib()
+    ctx = P.PCCtx()
+    P.PC_init(ctx, prob.dom, prob.mesh,
+              runner.options_for(prob, precision, "-kernel_timing " + extra))
+    ctx.kappa = prob.kappa
+    P._push_kappa(ctx)
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(np.ascontiguousarray(prob.rhs.ravel())).to(dev)
+    x = torch.zeros_like(b)
+    its = C.c_int(0)
+    rn = C.c_double(0.0)
+
+    def step():
+        _lib.check(L.pmx_setup(ctx.handle))
+        rc = L.pmx_solve_device(ctx.handle, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                0, prob.rtol, 10000, C.byref(its), C.byref(rn))
+        if rc not in (0,):
+            _lib.check(rc)
+
+    for _ in range(warmup):
+        step()
+    ms = np.zeros(8)
+    cnt = np.zeros(8, dtype=np.int32)
+    L.pmx_get_kernel_times(ctx.handle, ms, cnt, 8, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t_setup = 0.0
+    t_solve = 0.0
+    for _ in range(steps):
+        ts = time.perf_counter()
+        _lib.check(L.pmx_setup(ctx.handle))
+        tm = time.perf_counter()
+        rc = L.pmx_solve_device(ctx.handle, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()),
+                                0, prob.rtol, 10000, C.byref(its), C.byref(rn))
+        if rc != 0:
+            _lib.check(rc)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        t_setup += tm - ts
+        t_solve += te - tm
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.pmx_get_kernel_times(ctx.handle, ms, cnt, 8, 1)
+    st = P.PC_get_stats(ctx)
+    xs = x.cpu().numpy()
+    P.PC_final(ctx)
+    return dict(t_step=(t1 - t0) / steps, t_setup=t_setup / steps, t_solve=t_solve / steps,
+                its=its.value, rnorm=rn.value, stats=st, kernel_ms=ms / steps,
+                kernel_count=cnt / steps, x=xs)
+
+
+def cpu_baseline(prob, gpu_its, nsample):
+    """The FP64 oracle (numpy/SciPy restatement of the reference algorithm) on
+    a bounded sample of the same workload, scaled to one setup+solve."""
+    from oracle import premixfem_oracle as O
+
+    t = {}
+    t0 = time.perf_counter()
+    orc = O.Oracle(prob.dom, prob.mesh, prob.kappa, sd=prob.sd, en_lv1=prob.en_lv1,
+                   en_lv2=prob.en_lv2, pin_cell=prob.pin_cell)
+    lay = orc.lay
+    dic = O.DIC(lay, orc.T, O.lv1_block_diag(lay, orc.T, orc.pin))
+    t["operator+lv1"] = time.perf_counter() - t0
+    # Level-2 eigenproblems: a sample of subdomains, scaled to all of them.
+    mdiag = O.mass_diag(lay, prob.kappa, orc.T)
+    M, N, P = lay.mesh
+    gidx = np.arange(M * N * P).reshape(P, N, M)
+    subs = np.linspace(0, lay.nsub - 1, nsample).astype(int)
+    t0 = time.perf_counter()
+    for s in subs:
+        stx, nx, sty, ny, stz, nz = lay.corners(s)
+        cells = gidx[stz:stz + nz, sty:sty + ny, stx:stx + nx].ravel()
+        O.ghep(O.local_neumann(lay, orc.T, s), mdiag[cells], prob.en_lv1)
+    t["lv2_eigen"] = (time.perf_counter() - t0) * lay.nsub / nsample
+    # Galerkin + coarse Cholesky + level 3 on a basis of the same shape.
+    rng = np.random.default_rng(0)
+    nc = lay.nsub * prob.en_lv1
+    t0 = time.perf_counter()
+    import scipy.sparse as sp
+    import scipy.linalg as sla
+    sub = lay.cell_sub().ravel()
+    cols = sub * prob.en_lv1
+    rows = np.repeat(np.arange(M * N * P), prob.en_lv1)
+    colj = (cols[:, None] + np.arange(prob.en_lv1)[None, :]).ravel()
+    Phi = sp.csr_matrix((rng.standard_normal(rows.size), (rows, colj)), shape=(M * N * P, nc))
+    Ac = (Phi.T @ (orc.A @ Phi)).toarray()
+    Ac = Ac @ Ac.T / nc + np.eye(nc)
+    t["lv2_galerkin"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    chol = sla.cho_factor(Ac, lower=True)
+    t["lv2_factor"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sla.eigh(Ac, subset_by_index=[0, prob.en_lv2 - 1])
+    t["lv3"] = time.perf_counter() - t0
+    # PCG: time 3 iterations (apply + operator + vector ops), scale.
+    orc.dic, orc.Phi, orc.chol_c = dic, Phi, chol
+    orc.Phi_cc = np.asarray(Phi[:, :prob.en_lv2].todense())
+    orc.lu_cc = sla.lu_factor(np.eye(prob.en_lv2))
+    b = prob.rhs.ravel()
+    t0 = time.perf_counter()
+    O.pcg(orc.matvec, b, orc.apply, rtol=1e-300, max_it=3)
+    t["pcg"] = (time.perf_counter() - t0) / 3 * gpu_its
+    total = sum(t.values())
+    return total, t
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-sample", type=int, default=16,
+                    help="subdomain eigenproblems timed for the CPU baseline")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,fp32")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    from premixfem_amd import problems
+
+    prob = problems.config(args.config)
+    print(f"# workload {prob.name}: mesh {prob.mesh} sd {prob.sd} k {prob.en_lv1}", file=sys.stderr,
+          flush=True)
+    modes = {}
+    for m in args.modes.split(","):
+        prec = 32 if m == "fp32" else 64
+        r = run_mode(prob, prec, args.steps, args.warmup)
+        modes[m] = r
+        print(f"# {m}: step {r['t_step']:.4f}s setup {r['t_setup']:.4f}s solve {r['t_solve']:.4f}s "
+              f"its {r['its']} lobpcg max/mean {r['stats']['eig_iters_max']}/"
+              f"{r['stats']['eig_iters_mean']:.1f}", file=sys.stderr, flush=True)
+    head = "fp32" if "fp32" in modes else list(modes)[0]
+    hr = modes[head]
+    N = int(np.prod(prob.mesh))
+    nc = hr["stats"]["size_coarse"]
+    s = 4 if head == "fp32" else 8
+    from premixfem_amd._lib import TIMED_KERNELS
+    per_kernel = {}
+    solve_bytes = 0.0
+    for i, name in enumerate(TIMED_KERNELS):
+        cnt = hr["kernel_count"][i]
+        if cnt <= 0:
+            continue
+        avg_ms = hr["kernel_ms"][i] / cnt
+        by = kernel_bytes(name, N, prob.en_lv1, s, nc)
+        if by is not None:
+            solve_bytes += by * cnt
+        per_kernel[name] = dict(launches_per_step=float(cnt), avg_us=avg_ms * 1e3,
+                                bytes_per_launch=by,
+                                gbs=(by / (avg_ms * 1e-3) / 1e9) if by else None)
+    dom = max((k for k in per_kernel if per_kernel[k]["bytes_per_launch"]),
+              key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
+    dk = per_kernel[dom]
+    roofline = dict(kernel=dom, bound="hbm", achieved=round(dk["gbs"], 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(dk["gbs"] / HBM_PEAK_GBS, 4), traffic=None)
+    solve_gbs = solve_bytes / hr["t_solve"] / 1e9 if hr["t_solve"] > 0 else None
+    cpu = None
+    if not args.no_cpu_baseline:
+        total, parts = cpu_baseline(prob, hr["its"], args.cpu_sample)
+        cpu = dict(value=round(total, 2), unit="s", cores=os.cpu_count(), kind="port",
+                   sample=(f"numpy/SciPy FP64 oracle on the same workload: {args.cpu_sample} of "
+                           f"{int(np.prod(prob.sd))} subdomain eigenproblems timed and scaled, "
+                           "Galerkin/coarse Cholesky/level-3 timed in full, PCG timed for 3 "
+                           "iterations and scaled to the GPU iteration count"),
+                   parts={k: round(v, 3) for k, v in parts.items()})
+    out = {
+        "metric": METRIC,
+        "value": round(hr["t_step"], 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["t_step"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 preconditioner / f64 CG" if head == "fp32" else "f64",
+        "data": "synthetic (seeded binary box medium)",
+        "config": {
+            "workload": "128^3 binary box medium (3000 boxes, k=1e5), 8x8x8 subdomains of 16^3, "
+                        "6 eigenvectors, pinned cell, PCG rtol 1e-8" if prob.name == "c2"
+                        else prob.name,
+            "headline_mode": head,
+            "learned_warm_start": False,
+        },
+        "modes": {m: dict(t_step=round(r["t_step"], 5), t_setup=round(r["t_setup"], 5),
+                          t_solve=round(r["t_solve"], 5), its=r["its"],
+                          lobpcg_iters_max=r["stats"]["eig_iters_max"],
+                          lobpcg_iters_mean=round(r["stats"]["eig_iters_mean"], 2),
+                          setup_stages={k: round(v, 5) for k, v in r["stats"]["t_stages"].items()
+                                        if k.startswith("su")})
+                  for m, r in modes.items()},
+        "speedup_vs_fp64": (round(modes["fp64"]["t_step"] / hr["t_step"], 3)
+                            if "fp64" in modes and head != "fp64" else None),
+        "rel_diff_vs_fp64": (float(np.linalg.norm(hr["x"] - modes["fp64"]["x"]) /
+                                   np.linalg.norm(modes["fp64"]["x"]))
+                             if "fp64" in modes and head != "fp64" else None),
+        "solve_hbm_gbs": round(solve_gbs, 1) if solve_gbs else None,
+        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in per_kernel.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -132,6 +132,22 @@ int pmx_apply(pmx_ctx *ctx, const double *x, double *y);
 int pmx_solve(pmx_ctx *ctx, const double *b, double *x, int x0_nonzero, double rtol,
               int max_it, int *its, double *rnorm);
 
+/* pmx_solve with b and x in device memory (natural order, length M*N*P):
+ * the same PCG without the host<->device copies of the vectors. */
+int pmx_solve_device(pmx_ctx *ctx, const double *d_b, double *d_x, int x0_nonzero, double rtol,
+                     int max_it, int *its, double *rnorm);
+
+/* Per-kernel GPU time of the solve phase, measured with HIP events on the
+ * context's stream when the context was created with -kernel_timing.
+ * Kernels, in order: 0 level-1 first sweep, 1 level-1 last sweep (with the
+ * residual), 2 residual + restriction, 3 prolongation, 4 PCG p-update +
+ * SpMV, 5 PCG x/r update, 6 coarse dense solve (GEMV), 7 coarse small ops.
+ * ms[i] receives the summed milliseconds, count[i] the launch count, for
+ * i < n (n <= PMX_NUM_TIMED_KERNELS).  reset clears the counters after
+ * reading. */
+#define PMX_NUM_TIMED_KERNELS 8
+int pmx_get_kernel_times(pmx_ctx *ctx, double *ms, int *count, int n, int reset);
+
 /* PC_print_stat / PC_get_range (src/PreMixFEM_3D.c:1938-1999). */
 int pmx_get_stats(const pmx_ctx *ctx, pmx_stats *st);
 int pmx_print_stat(const pmx_ctx *ctx);

@@ -23,9 +23,11 @@ REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS",
 
 # Every entry point include/premixfem_gpu.h declares.
 EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_setup",
-           "pmx_setup_from_basis", "pmx_apply", "pmx_solve", "pmx_get_stats",
-           "pmx_print_stat", "pmx_get_basis", "pmx_get_coarse_matrix", "pmx_final",
-           "pmx_last_error")
+           "pmx_setup_from_basis", "pmx_apply", "pmx_solve", "pmx_solve_device",
+           "pmx_get_kernel_times", "pmx_get_stats", "pmx_print_stat", "pmx_get_basis",
+           "pmx_get_coarse_matrix", "pmx_final", "pmx_last_error")
+TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
+                 "k_cg_update", "k_gemv", "coarse_small")
 
 
 class PMXError(RuntimeError):
@@ -85,6 +87,9 @@ def lib():
     L.pmx_get_basis.argtypes = [vp, C.c_int, _dp, C.c_void_p]
     L.pmx_get_coarse_matrix.argtypes = [vp, _dp]
     L.pmx_final.argtypes = [vp]
+    L.pmx_solve_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.pmx_get_kernel_times.argtypes = [vp, _dp, _ip, C.c_int, C.c_int]
     L.pmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "pmx_last_error":

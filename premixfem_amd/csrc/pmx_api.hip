@@ -115,6 +115,10 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(getd(c->eig_sigma));
     } else if (k == "-eig_guard") {
       PMX_TRY(geti(c->eig_guard));
+    } else if (k == "-eig_debug") {
+      PMX_TRY(geti(c->eig_debug));
+    } else if (k == "-kernel_timing") {
+      c->kernel_timing = 1;
     } else if (k == "-log_stages") {
       c->log_stages = 1;
     } else if (k == "-seed") {
@@ -314,7 +318,7 @@ static int setup_coarse(pmx_ctx* c) {
   for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
   PMX_TRY(launch_galerkin(c));
   const double shift = (c->use_2level && !c->no_shift_A_cc) ? kNil : 0.0;
-  PMX_TRY(coarse_invert(c, c->Ac.p, c->Ainv.p, shift));
+  PMX_TRY(coarse_solve_setup(c, shift));
   if (c->pc_precision == 32) {
     std::vector<double> h((size_t)c->ncell * c->en_lv1);
     PMX_HIP(hipMemcpy(h.data(), c->phi.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -382,6 +386,36 @@ static int setup_fine(pmx_ctx* c) {
   return PMX_SUCCESS;
 }
 
+// Per-kernel timing with HIP events on the context stream (-kernel_timing).
+static int kt_begin(pmx_ctx* c, int id) {
+  if (!c->kernel_timing) return -1;
+  const int slot = (int)c->kt_pending.size();
+  while ((int)c->kt_events.size() < 2 * (slot + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    c->kt_events.push_back(e);
+  }
+  (void)hipEventRecord(c->kt_events[2 * slot], c->stream);
+  c->kt_pending.push_back(id);
+  return slot;
+}
+
+static void kt_end(pmx_ctx* c, int slot) {
+  if (slot < 0) return;
+  (void)hipEventRecord(c->kt_events[2 * slot + 1], c->stream);
+}
+
+static int kt_collect(pmx_ctx* c) {
+  for (size_t i = 0; i < c->kt_pending.size(); ++i) {
+    float ms = 0.0f;
+    PMX_HIP(hipEventElapsedTime(&ms, c->kt_events[2 * i], c->kt_events[2 * i + 1]));
+    c->kt_ms[c->kt_pending[i]] += ms;
+    c->kt_count[c->kt_pending[i]] += 1;
+  }
+  c->kt_pending.clear();
+  return PMX_SUCCESS;
+}
+
 // V-cycle of PC_apply_vec (src/PreMixFEM_3D.c:1840-1936) on brick-order
 // device vectors.  The fine-grid stages are level 1 (before and after); all
 // level-2/level-3 stages in between act on the coarse residual
@@ -401,12 +435,16 @@ static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial)
   double t = now();
   const double t_start = t;
   // Pre-smoothing: y = B1 x, then si_lv1-1 residual-corrected sweeps.
+  int ev = kt_begin(c, 0);
   PMX_TRY(launch_lv1_first(c, x));
+  kt_end(c, ev);
   for (int i = 1; i < c->si_lv1; ++i) {
     PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
   }
   PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
+  ev = kt_begin(c, 2);
   PMX_TRY(launch_resid_restrict(c, x, false, c->c_a.p));
+  kt_end(c, ev);
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
@@ -427,17 +465,30 @@ static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial)
   for (size_t k = 0; k < prog.size(); ++k) {
     const bool last = k + 1 == prog.size();
     if (prog[k] == 2) {
-      PMX_TRY(launch_gemv(c, c->Ainv.p, tv, d, c->nc));
+      ev = kt_begin(c, 6);
+      PMX_TRY(launch_gemv(c, c->Ainv.p, tv, c->c_e.p, c->nc));
+      kt_end(c, ev);
+      ev = kt_begin(c, 7);
+      PMX_TRY(launch_coarse_fix(c, c->c_e.p, tv, d));
+      kt_end(c, ev);
     } else {
+      ev = kt_begin(c, 7);
       PMX_TRY(launch_lv3_apply(c, tv, d));
+      kt_end(c, ev);
     }
+    ev = kt_begin(c, 7);
     PMX_TRY(launch_coarse_update(c, d, tv, u, k == 0, last ? 0 : 1));
+    kt_end(c, ev);
   }
   PMX_TRY(tick(PMX_STAGE_AV_LV2, t));
+  ev = kt_begin(c, 3);
   PMX_TRY(launch_prolong(c, u));
+  kt_end(c, ev);
   // Post-smoothing.
   for (int i = 1; i < c->si_lv1; ++i) PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
+  ev = kt_begin(c, 1);
   PMX_TRY(launch_lv1_last(c, x, z, false, partial));
+  kt_end(c, ev);
   PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
   (void)f32;
   if (c->log_stages) c->t_stages[PMX_STAGE_AV] += now() - t_start;
@@ -706,18 +757,12 @@ int pmx_apply(pmx_ctx* c, const double* x, double* y) {
   return PMX_SUCCESS;
 }
 
-int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rtol, int max_it,
-              int* its, double* rnorm) {
-  if (!c || !b || !x) {
-    set_error("null argument");
-    return PMX_ERR_ARG_NULL;
-  }
-  if (!c->setup_done) {
-    set_error("KSPSolve before PC_setup");
-    return PMX_ERR_ORDER;
-  }
+// PETSc KSPSolve_CG (standard variant, KSP_NORM_UNPRECONDITIONED, default
+// convergence test) on the brick-order vectors w_b (right-hand side) and w_x
+// (initial guess if x0_nonzero, solution on return).
+static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* its,
+                      double* rnorm) {
   const double abstol = 1e-50, dtol = 1e5;
-  const size_t nb = c->ncell * sizeof(double);
   double* xb = c->w_x.p;
   double* bb = c->w_b.p;
   double* r = c->w_r.p;
@@ -725,19 +770,14 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
   double* p0 = c->w_p.p;
   double* p1 = c->w_tmp.p;
   double* w = c->w_q.p;
-  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
-  PMX_HIP(hipMemcpyAsync(c->nat_buf.p, b, nb, hipMemcpyHostToDevice, c->stream));
-  PMX_TRY(launch_nat2brick(c, c->nat_buf.p, bb, 1, c->ncell, c->ncell));
   if (x0_nonzero) {
-    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, x, nb, hipMemcpyHostToDevice, c->stream));
-    PMX_TRY(launch_nat2brick(c, c->nat_buf.p, xb, 1, c->ncell, c->ncell));
     PMX_TRY(launch_residual(c, bb, xb, r));
   } else {
-    PMX_HIP(hipMemsetAsync(xb, 0, nb, c->stream));
+    PMX_HIP(hipMemsetAsync(xb, 0, c->ncell * sizeof(double), c->stream));
     PMX_TRY(launch_residual(c, bb, nullptr, r));
   }
-  double hs[16];
-  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  double* hs = c->h_scal;
+  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PMX_HIP(hipStreamSynchronize(c->stream));
   const double t_solve0 = now();
   double dp = std::sqrt(hs[4]);
@@ -752,9 +792,14 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
     int i = 0;
     while (true) {
       k = i + 1;
+      int ev = kt_begin(c, 4);
       PMX_TRY(launch_cg_p_spmv(c, z, p0, p1, w, i == 0));
+      kt_end(c, ev);
+      ev = kt_begin(c, 5);
       PMX_TRY(launch_cg_update(c, xb, r, p1, w));
-      PMX_HIP(hipMemcpyAsync(hs, c->scal.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+      kt_end(c, ev);
+      PMX_HIP(hipMemcpyAsync(hs, c->scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
       PMX_HIP(hipStreamSynchronize(c->stream));
       if (hs[5] != 0.0) {
         reason = PMX_DIVERGED_INDEFINITE_MAT;
@@ -779,9 +824,8 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
       ++i;
     }
   }
-  PMX_TRY(launch_brick2nat(c, xb, c->nat_buf.p, 1, c->ncell, c->ncell));
-  PMX_HIP(hipMemcpyAsync(x, c->nat_buf.p, nb, hipMemcpyDeviceToHost, c->stream));
   PMX_HIP(hipStreamSynchronize(c->stream));
+  PMX_TRY(kt_collect(c));
   c->last_its = k;
   c->last_rnorm = dp;
   c->last_rnorm0 = rnorm0;
@@ -790,6 +834,64 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
   if (its) *its = k;
   if (rnorm) *rnorm = dp;
   return reason > 0 ? PMX_SUCCESS : PMX_ERR_NOT_CONVERGED;
+}
+
+static int solve_check(pmx_ctx* c, const void* b, const void* x) {
+  if (!c || !b || !x) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  if (!c->setup_done) {
+    set_error("KSPSolve before PC_setup");
+    return PMX_ERR_ORDER;
+  }
+  return PMX_SUCCESS;
+}
+
+int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rtol, int max_it,
+              int* its, double* rnorm) {
+  PMX_TRY(solve_check(c, b, x));
+  const size_t nb = c->ncell * sizeof(double);
+  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
+  PMX_HIP(hipMemcpyAsync(c->nat_buf.p, b, nb, hipMemcpyHostToDevice, c->stream));
+  PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->w_b.p, 1, c->ncell, c->ncell));
+  if (x0_nonzero) {
+    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, x, nb, hipMemcpyHostToDevice, c->stream));
+    PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->w_x.p, 1, c->ncell, c->ncell));
+  }
+  const int rc = solve_core(c, x0_nonzero, rtol, max_it, its, rnorm);
+  if (rc != PMX_SUCCESS && rc != PMX_ERR_NOT_CONVERGED) return rc;
+  PMX_TRY(launch_brick2nat(c, c->w_x.p, c->nat_buf.p, 1, c->ncell, c->ncell));
+  PMX_HIP(hipMemcpyAsync(x, c->nat_buf.p, nb, hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int pmx_solve_device(pmx_ctx* c, const double* d_b, double* d_x, int x0_nonzero, double rtol,
+                     int max_it, int* its, double* rnorm) {
+  PMX_TRY(solve_check(c, d_b, d_x));
+  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
+  PMX_TRY(launch_nat2brick(c, d_b, c->w_b.p, 1, c->ncell, c->ncell));
+  if (x0_nonzero) PMX_TRY(launch_nat2brick(c, d_x, c->w_x.p, 1, c->ncell, c->ncell));
+  const int rc = solve_core(c, x0_nonzero, rtol, max_it, its, rnorm);
+  if (rc != PMX_SUCCESS && rc != PMX_ERR_NOT_CONVERGED) return rc;
+  PMX_TRY(launch_brick2nat(c, c->w_x.p, d_x, 1, c->ncell, c->ncell));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int pmx_get_kernel_times(pmx_ctx* c, double* ms, int* count, int n, int reset) {
+  if (!c || !ms || !count) return PMX_ERR_ARG_NULL;
+  for (int i = 0; i < n && i < PMX_NUM_TIMED_KERNELS; ++i) {
+    ms[i] = c->kt_ms[i];
+    count[i] = c->kt_count[i];
+  }
+  if (reset)
+    for (int i = 0; i < PMX_NUM_TIMED_KERNELS; ++i) {
+      c->kt_ms[i] = 0.0;
+      c->kt_count[i] = 0;
+    }
+  return PMX_SUCCESS;
 }
 
 int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
@@ -930,12 +1032,14 @@ int pmx_final(pmx_ctx* c) {
     c->T[a].release();
     c->T32[a].release();
   }
+  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs}) b->release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,
                           &c->partial, &c->scal, &c->nat_buf})
     b->release();
   for (DBuf<float>* b : {&c->fac32_f, &c->fac32_b, &c->phi32, &c->w_y32, &c->w_y32b}) b->release();
+  for (hipEvent_t e : c->kt_events) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PMX_SUCCESS;

@@ -653,6 +653,15 @@ int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cv
   return PMX_SUCCESS;
 }
 
+// c = Phi^T x with the FP64 basis (y = 0), whatever the preconditioner precision.
+int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec) {
+  hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub), dim3(kBlock), 0,
+                     ctx->stream, ctx->geo, x, nullptr, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                     ctx->phi.p, ctx->en_lv1, cvec);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
 int launch_prolong(pmx_ctx* ctx, const double* cvec) {
   if (ctx->pc_precision == 32)
     hipLaunchKernelGGL(k_prolong<float>, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,

@@ -180,6 +180,8 @@ struct pmx_ctx {
   int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1
   double eig_sigma = 1.0e-6;    // shift of the LOBPCG preconditioner A^N + sigma M
   int log_stages = 0;           // synchronise and time every apply stage
+  int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels
+  int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
   uint64_t seed = 12345;
 
   // ---- derived geometry ----
@@ -224,7 +226,9 @@ struct pmx_ctx {
   int nc = 0;                        // coarse size
   pmx::DBuf<double> Ac;             // dense nc x nc Galerkin matrix
   pmx::DBuf<double> Acb;            // block-sparse rows: [nsub][7][kmax][kmax]
-  pmx::DBuf<double> Ainv;           // dense inverse of the level-2 coarse matrix
+  pmx::DBuf<double> Ainv;           // dense inverse of the deflated coarse matrix B
+  pmx::DBuf<double> c_z, c_g, c_h, c_cs;  // near-null vector, pin row, B^{-1} g, Woodbury scalars
+  double coarse_gamma = 0.0;
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
 
   // ---- level 3 ----
@@ -234,6 +238,7 @@ struct pmx_ctx {
   pmx::DBuf<double> V2;             // [nc][en_lv2]
   pmx::DBuf<double> Acc_lu;         // k2 x k2 LU (row-major) + pivots packed after
   int lv3_iters = 0;
+  double lv3_floor = 0.0;           // attained residual when the level-3 solver stagnated
 
   // ---- work vectors ----
   pmx::DBuf<double> w_x, w_y, w_r, w_b, w_p, w_z, w_q, w_tmp;   // brick order FP64
@@ -251,5 +256,10 @@ struct pmx_ctx {
   double last_rnorm = 0.0, last_rnorm0 = 0.0;
   int last_reason = 0;
   double t_solve = 0.0;
+  double h_scal[16];
+  std::vector<hipEvent_t> kt_events;
+  std::vector<int> kt_pending;
+  double kt_ms[PMX_NUM_TIMED_KERNELS] = {0};
+  int kt_count[PMX_NUM_TIMED_KERNELS] = {0};
   std::vector<int> h_eigen_num_given;  // caller-fixed eigen counts (setup from a basis)
 };

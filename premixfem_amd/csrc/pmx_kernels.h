@@ -34,6 +34,10 @@ int launch_spmv(pmx_ctx* ctx, const double* x, double* y);
 // coarse levels (pmx_coarse.hip)
 int launch_galerkin(pmx_ctx* ctx);
 int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift);
+int coarse_solve_setup(pmx_ctx* ctx, double delta);
+int launch_coarse_solve(pmx_ctx* ctx, const double* t, double* d, double* ytmp, int ncol);
+int launch_coarse_fix(pmx_ctx* ctx, const double* y, const double* t, double* d);
+int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec);
 int level3_setup(pmx_ctx* ctx);
 int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n);
 int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, int first,

@@ -45,6 +45,7 @@ struct LobArgs {
   double* eig;        // output [nsub][kmax]
   int* its;           // output [nsub]
   int* status;        // output [nsub] 0 = converged
+  int debug;          // subdomain whose iterations are traced with printf (-1: none)
 };
 
 __device__ __forceinline__ double hash_uniform(uint64_t a) {
@@ -353,11 +354,40 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
           }
           wg_reduce_many(acc, m, red, tmpout);
           if (threadIdx.x < m) rho[threadIdx.x] = sqrt(tmpout[threadIdx.x]);
+          __syncthreads();
+        }
+        // ---- W = T R, deflated against the constant vector ----
+        wg_precondition(b, ps, pc, a.pf, a.pb, W, m, pv);
+        wg_multidot(W, m, mass, 1, nullptr, n, red, tmpout, TM, 1);
+        for (int e = threadIdx.x; e < m * n; e += blockDim.x) W[e] -= TM[e / n] / msum;
+        __syncthreads();
+        // ---- convergence: ||T r||_M <= tol for the wanted pairs.  With T
+        // approximating (A^N + sigma M)^{-1} this is the residual of the
+        // shift-inverted problem relative to its eigenvalue, the quantity
+        // SLEPc's relative test bounds under STSINVERT. ----
+        {
+          double acc[kLobM];
+#pragma unroll
+          for (int j = 0; j < kLobM; ++j) acc[j] = 0.0;
+          for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            const double mr = mass[r];
+#pragma unroll
+            for (int j = 0; j < kLobM; ++j)
+              if (j < m) {
+                const double wv = W[(size_t)j * n + r];
+                acc[j] += wv * wv * mr;
+              }
+          }
+          wg_reduce_many(acc, m, red, tmpout);
           if (threadIdx.x == 0) {
             int conv = 1;
             for (int j = 0; j < kw; ++j)
-              if (!(rho[j] <= a.tol * fabs(lam[j]))) conv = 0;
+              if (!(sqrt(tmpout[j]) <= a.tol)) conv = 0;
             s_conv = conv;
+            if (sidx == a.debug)
+              printf("it %d lam0 %.6e lam%d %.6e rho0 %.3e rho%d %.3e Tr0 %.3e Tr%d %.3e np %d\n",
+                     it, lam[0], kw - 1, lam[kw - 1], rho[0], kw - 1, rho[kw - 1], sqrt(tmpout[0]),
+                     kw - 1, sqrt(tmpout[kw - 1]), np);
           }
           __syncthreads();
         }
@@ -365,12 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
           status = 0;
           break;
         }
-        // ---- W = T R ----
-        wg_precondition(b, ps, pc, a.pf, a.pb, W, m, pv);
-        // ---- W -= phi0 (phi0^T M W), X (X^T M W), P (P^T M W) ----
-        wg_multidot(W, m, mass, 1, nullptr, n, red, tmpout, TM, 1);
-        for (int e = threadIdx.x; e < m * n; e += blockDim.x) W[e] -= TM[e / n] / msum;
-        __syncthreads();
+        // ---- W -= X (X^T M W), P (P^T M W) ----
         wg_multidot(X, m, W, m, mass, n, red, tmpout, GB, LD);
         if (np > 0) wg_multidot(P, np, W, m, mass, n, red, tmpout, GB + m * LD, LD);
         for (int r = threadIdx.x; r < n; r += blockDim.x) {
@@ -503,7 +528,37 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
           }
           __syncthreads();
         }
-        np = m;
+        // ---- keep P M-orthogonal to X and M-orthonormal (robust LOBPCG):
+        // P -= X (X^T M P), AP -= AX (X^T M P), then SVQB(P) ----
+        wg_multidot(X, m, P, m, mass, n, red, tmpout, GB, LD);
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+          double xv[kLobM], axv[kLobM];
+#pragma unroll
+          for (int i = 0; i < kLobM; ++i) {
+            xv[i] = i < m ? X[(size_t)i * n + r] : 0.0;
+            axv[i] = i < m ? AX[(size_t)i * n + r] : 0.0;
+          }
+#pragma unroll
+          for (int j = 0; j < kLobM; ++j) {
+            if (j >= m) break;
+            double sp = P[(size_t)j * n + r], sa = AP[(size_t)j * n + r];
+#pragma unroll
+            for (int i = 0; i < kLobM; ++i)
+              if (i < m) {
+                sp -= xv[i] * GB[i * LD + j];
+                sa -= axv[i] * GB[i * LD + j];
+              }
+            P[(size_t)j * n + r] = sp;
+            AP[(size_t)j * n + r] = sa;
+          }
+        }
+        __syncthreads();
+        wg_multidot(P, m, P, m, mass, n, red, tmpout, GB, LD);
+        np = wg_svqb(GB, Z, TM, Q, w, rot, m, LD, 1e-12);
+        if (np > 0) {
+          wg_transform(P, n, m, Z, LD, np);
+          wg_transform(AP, n, m, Z, LD, np);
+        }
       }
       if (status == 0) break;
       if (attempt == 0 && a.x0 == nullptr) break;  // random start already failed
@@ -588,6 +643,7 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   a.eig = eig.p;
   a.its = its.p;
   a.status = status.p;
+  a.debug = ctx->eig_debug;
   const size_t small = (5 * kLobS * kLobS + kLobS + 4 * kLobS + 4 * kLobM * kLobM + kLobM * kLobM) *
                        sizeof(double);
   const size_t prec = (size_t)maxn * kLobCh * sizeof(float);
