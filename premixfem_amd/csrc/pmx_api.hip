@@ -226,6 +226,12 @@ static int build_geometry(pmx_ctx* c) {
         off += b.nx * b.ny * b.nz;
       }
   c->nshape = (int)shapes.size();
+  for (const Brick& b : c->h_bricks)
+    if (b.nx > kBlock || b.ny > 1023 || b.nz > 1023 || b.nx > 1023) {
+      set_error("subdomain of %dx%dx%d cells: x extent above %d is not supported", b.nx, b.ny, b.nz,
+                kBlock);
+      return PMX_ERR_SUP;
+    }
   c->h_nat2brick.assign(c->ncell, 0);
   for (int z = 0; z < P; ++z)
     for (int y = 0; y < N; ++y)
@@ -319,6 +325,23 @@ static int setup_coarse(pmx_ctx* c) {
   PMX_TRY(launch_galerkin(c));
   const double shift = (c->use_2level && !c->no_shift_A_cc) ? kNil : 0.0;
   PMX_TRY(coarse_solve_setup(c, shift));
+  // The coarse stages of PC_apply_vec (src/PreMixFEM_3D.c:1857-1929):
+  // 2 = level 2, 3 = level 3.
+  std::vector<int>& prog = c->coarse_prog;
+  prog.clear();
+  if (c->use_2level) {
+    prog.push_back(2);
+  } else if (c->use_W_cycle) {
+    for (int r = 0; r < 2; ++r) {
+      for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+      prog.push_back(3);
+    }
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+  } else {
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+    prog.push_back(3);
+    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+  }
   if (c->pc_precision == 32) {
     std::vector<double> h((size_t)c->ncell * c->en_lv1);
     PMX_HIP(hipMemcpy(h.data(), c->phi.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -333,6 +356,9 @@ static int setup_coarse(pmx_ctx* c) {
     double t2 = now();
     c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
   }
+  const double t3 = now();
+  PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  c->t_stages[PMX_STAGE_SU_LV3_2] += now() - t3;
   return PMX_SUCCESS;
 }
 
@@ -448,38 +474,10 @@ static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial)
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
-  std::vector<int> prog;  // 2 = lv2 stage, 3 = lv3 stage
-  if (c->use_2level) {
-    prog.push_back(2);
-  } else if (c->use_W_cycle) {
-    for (int r = 0; r < 2; ++r) {
-      for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
-      prog.push_back(3);
-    }
-    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
-  } else {
-    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
-    prog.push_back(3);
-    for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
-  }
-  for (size_t k = 0; k < prog.size(); ++k) {
-    const bool last = k + 1 == prog.size();
-    if (prog[k] == 2) {
-      ev = kt_begin(c, 6);
-      PMX_TRY(launch_gemv(c, c->Ainv.p, tv, c->c_e.p, c->nc));
-      kt_end(c, ev);
-      ev = kt_begin(c, 7);
-      PMX_TRY(launch_coarse_fix(c, c->c_e.p, tv, d));
-      kt_end(c, ev);
-    } else {
-      ev = kt_begin(c, 7);
-      PMX_TRY(launch_lv3_apply(c, tv, d));
-      kt_end(c, ev);
-    }
-    ev = kt_begin(c, 7);
-    PMX_TRY(launch_coarse_update(c, d, tv, u, k == 0, last ? 0 : 1));
-    kt_end(c, ev);
-  }
+  (void)d;
+  ev = kt_begin(c, 6);
+  PMX_TRY(launch_gemv(c, c->Fop.p, tv, u, c->nc));
+  kt_end(c, ev);
   PMX_TRY(tick(PMX_STAGE_AV_LV2, t));
   ev = kt_begin(c, 3);
   PMX_TRY(launch_prolong(c, u));
@@ -1032,7 +1030,7 @@ int pmx_final(pmx_ctx* c) {
     c->T[a].release();
     c->T32[a].release();
   }
-  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs}) b->release();
+  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs, &c->Fop}) b->release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

@@ -506,6 +506,316 @@ __device__ __forceinline__ double coarse_row(const Geo& g, const double* __restr
   return acc;
 }
 
+// One workgroup runs a segment of the coarse program of PC_apply_vec: the
+// stages between two dense GEMVs.  Each op acts on the coarse residual t and
+// the accumulated correction u:
+//   op.kind 2 (level 2): d = Woodbury fix of y = B^{-1} t (the GEMV before the
+//                        segment), only as the segment's first op;
+//   op.kind 3 (level 3): d = V A_cc^{-1} V^T t;
+// then u = d (first) or u += d, and t -= A_c d unless it is the program's
+// last stage.
+struct CoarseOp {
+  int kind, first, update_t;
+};
+struct CoarseSeg {
+  int nops;
+  CoarseOp op[8];
+};
+
+__global__ void k_coarse_seg(Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax,
+                             const double* __restrict__ y, double* __restrict__ t,
+                             double* __restrict__ u, double* __restrict__ d,
+                             const double* __restrict__ z, const double* __restrict__ h,
+                             const double* __restrict__ cs, const double* __restrict__ V, int k2,
+                             const double* __restrict__ LU, int nc) {
+  __shared__ double red[8];
+  __shared__ double cvec[kMaxEig];
+  for (int q = 0; q < seg.nops; ++q) {
+    const CoarseOp op = seg.op[q];
+    if (op.kind == 2) {
+      double zt = 0.0, ht = 0.0;
+      for (int r = threadIdx.x; r < nc; r += blockDim.x) {
+        zt += z[r] * t[r];
+        ht += h[r] * t[r];
+      }
+      zt = block_sum(zt, red);
+      ht = block_sum(ht, red);
+      const double dg = cs[0];
+      const double u0 = zt / dg;
+      const double a0 = cs[1] * u0 + cs[2] * ht;
+      const double a1 = cs[2] * u0 + cs[3] * ht;
+      for (int r = threadIdx.x; r < nc; r += blockDim.x) d[r] = y[r] - (z[r] / dg) * a0 - h[r] * a1;
+    } else {
+      for (int j = 0; j < k2; ++j) {
+        double sj = 0.0;
+        for (int r = threadIdx.x; r < nc; r += blockDim.x) sj += V[(size_t)j * nc + r] * t[r];
+        sj = block_sum(sj, red);
+        if (threadIdx.x == 0) cvec[j] = sj;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const double* piv = LU + k2 * k2;
+        double bb[kMaxEig];
+        for (int i = 0; i < k2; ++i) bb[i] = cvec[i];
+        for (int i = 0; i < k2; ++i) {
+          const int p = (int)piv[i];
+          const double tmp = bb[i];
+          bb[i] = bb[p];
+          bb[p] = tmp;
+        }
+        for (int i = 0; i < k2; ++i)
+          for (int j = 0; j < i; ++j) bb[i] -= LU[i * k2 + j] * bb[j];
+        for (int i = k2 - 1; i >= 0; --i) {
+          for (int j = i + 1; j < k2; ++j) bb[i] -= LU[i * k2 + j] * bb[j];
+          bb[i] /= LU[i * k2 + i];
+        }
+        for (int i = 0; i < k2; ++i) cvec[i] = bb[i];
+      }
+      __syncthreads();
+      for (int r = threadIdx.x; r < nc; r += blockDim.x) {
+        double sv = 0.0;
+        for (int j = 0; j < k2; ++j) sv += V[(size_t)j * nc + r] * cvec[j];
+        d[r] = sv;
+      }
+    }
+    __syncthreads();
+    // u update and t -= A_c d (rows independent; d complete after the sync)
+    for (int gid = threadIdx.x; gid < g.nsub * kmax; gid += blockDim.x) {
+      const int sb = gid / kmax, i = gid % kmax;
+      const Brick& b = g.bricks[sb];
+      if (i >= b.neig) continue;
+      const int r = b.dof + i;
+      u[r] = op.first ? d[r] : u[r] + d[r];
+      if (op.update_t) t[r] -= coarse_row(g, Acb, kmax, sb, i, d);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Composite coarse operator.  Every stage of the coarse program maps the
+// coarse residual t to a correction d = G t (G = A_c^{-1} for level 2,
+// V A_cc^{-1} V^T for level 3) and updates t -= A_c d, u += d.  The program
+// is therefore one linear map u = F t0, composed here once per setup with
+// dense matrices:  T = I, U = 0;  per stage D = G T, U += D, T -= A_c D.
+// ---------------------------------------------------------------------------
+constexpr int kGt = 64;   // GEMM tile
+constexpr int kGk = 16;   // GEMM k-chunk
+
+// C = A B (row-major n x n), 64x64 tiles, 4x4 outputs per thread.
+__global__ void __launch_bounds__(256) k_dgemm(const double* __restrict__ A,
+                                               const double* __restrict__ B,
+                                               double* __restrict__ C, int n) {
+  __shared__ double As[kGk][kGt + 1];  // As[k][i]
+  __shared__ double Bs[kGk][kGt + 1];  // Bs[k][j]
+  const int I0 = blockIdx.y * kGt, J0 = blockIdx.x * kGt;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+  for (int k0 = 0; k0 < n; k0 += kGk) {
+    for (int e = threadIdx.x; e < kGt * kGk; e += blockDim.x) {
+      const int r = e / kGk, k = e % kGk;
+      As[k][r] = (I0 + r < n && k0 + k < n) ? A[(size_t)(I0 + r) * n + k0 + k] : 0.0;
+      const int kb = e / kGt, cb = e % kGt;
+      Bs[kb][cb] = (k0 + kb < n && J0 + cb < n) ? B[(size_t)(k0 + kb) * n + J0 + cb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kGk; ++k) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[k][ty + 16 * a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = Bs[k][tx + 16 * c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] += av[a] * bv[c];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = I0 + ty + 16 * a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = J0 + tx + 16 * c;
+      if (j < n) C[(size_t)i * n + j] = acc[a][c];
+    }
+  }
+}
+
+// S = B^{-1} - [z/dg, h] Cap^{-1} [z/dg, h]^T  (the Woodbury-corrected A_c^{-1}).
+__global__ void k_form_S(const double* __restrict__ Binv, const double* __restrict__ z,
+                         const double* __restrict__ h, const double* __restrict__ cs,
+                         double* __restrict__ S, int n) {
+  const size_t nn = (size_t)n * n;
+  const double dg = cs[0], a00 = cs[1], a01 = cs[2], a11 = cs[3];
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double zi = z[i] / dg, zj = z[j] / dg;
+    S[e] = Binv[e] - zi * (a00 * zj + a01 * h[j]) - h[i] * (a01 * zj + a11 * h[j]);
+  }
+}
+
+// U = D (first) or U += D; if update: T -= A_c D (block rows of A_c).
+__global__ void k_stage_update(Geo g, const double* __restrict__ Acb, int kmax,
+                               const double* __restrict__ D, double* __restrict__ T,
+                               double* __restrict__ U, int n, int first, int update) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    U[e] = first ? D[e] : U[e] + D[e];
+  }
+  if (!update) return;
+  // rows are coarse dofs; column j is independent
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)g.nsub * kmax * n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e % n);
+    const int gid = (int)(e / n);
+    const int sb = gid / kmax, i = gid % kmax;
+    const Brick& b = g.bricks[sb];
+    if (i >= b.neig) continue;
+    const double* blk = Acb + (size_t)sb * 7 * kmax * kmax;
+    double acc = 0.0;
+    for (int q = 0; q < b.neig; ++q) acc += blk[i * kmax + q] * D[(size_t)(b.dof + q) * n + j];
+    for (int dir = 0; dir < 6; ++dir) {
+      const int sn = b.nb[dir];
+      if (sn < 0) continue;
+      const Brick& o = g.bricks[sn];
+      const double* bb = blk + (size_t)(dir + 1) * kmax * kmax;
+      for (int q = 0; q < o.neig; ++q) acc += bb[i * kmax + q] * D[(size_t)(o.dof + q) * n + j];
+    }
+    T[(size_t)(b.dof + i) * n + j] -= acc;
+  }
+}
+
+// Y = A_cc^{-1} (V^T T), k2 x n (column j per thread).
+__global__ void k_lv3_left(const double* __restrict__ V, int k2, const double* __restrict__ LU,
+                           const double* __restrict__ T, double* __restrict__ Y, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double b[kMaxEig];
+  for (int q = 0; q < k2; ++q) b[q] = 0.0;
+  for (int r = 0; r < n; ++r) {
+    const double tv = T[(size_t)r * n + j];
+    for (int q = 0; q < k2; ++q) b[q] += V[(size_t)q * n + r] * tv;
+  }
+  const double* piv = LU + k2 * k2;
+  for (int i = 0; i < k2; ++i) {
+    const int p = (int)piv[i];
+    const double tmp = b[i];
+    b[i] = b[p];
+    b[p] = tmp;
+  }
+  for (int i = 0; i < k2; ++i)
+    for (int q = 0; q < i; ++q) b[i] -= LU[i * k2 + q] * b[q];
+  for (int i = k2 - 1; i >= 0; --i) {
+    for (int q = i + 1; q < k2; ++q) b[i] -= LU[i * k2 + q] * b[q];
+    b[i] /= LU[i * k2 + i];
+  }
+  for (int q = 0; q < k2; ++q) Y[(size_t)q * n + j] = b[q];
+}
+
+// D = V Y (n x n).
+__global__ void k_lv3_right(const double* __restrict__ V, int k2, const double* __restrict__ Y,
+                            double* __restrict__ D, int n) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / n), j = (int)(e % n);
+    double acc = 0.0;
+    for (int q = 0; q < k2; ++q) acc += V[(size_t)q * n + r] * Y[(size_t)q * n + j];
+    D[e] = acc;
+  }
+}
+
+__global__ void k_set_identity(double* __restrict__ T, int n) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x)
+    T[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+
+int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
+  const int n = ctx->nc;
+  const size_t nn = (size_t)n * n;
+  DBuf<double> S, T, D, Y;
+  PMX_TRY(S.alloc(nn));
+  PMX_TRY(T.alloc(nn));
+  PMX_TRY(D.alloc(nn));
+  PMX_TRY(ctx->Fop.alloc(nn));
+  const int eb = 2048;
+  hipLaunchKernelGGL(k_form_S, dim3(eb), dim3(kBlock), 0, ctx->stream, ctx->Ainv.p, ctx->c_z.p,
+                     ctx->c_h.p, ctx->c_cs.p, S.p, n);
+  hipLaunchKernelGGL(k_set_identity, dim3(eb), dim3(kBlock), 0, ctx->stream, T.p, n);
+  PMX_HIP(hipGetLastError());
+  const int nt = (n + kGt - 1) / kGt;
+  const int k2 = ctx->eigen_num_lv2;
+  if (k2 > 0) PMX_TRY(Y.alloc((size_t)k2 * n));
+  bool T_is_I = true;
+  for (size_t k = 0; k < prog.size(); ++k) {
+    const bool last = k + 1 == prog.size();
+    const double* Dp;
+    if (prog[k] == 2) {
+      if (T_is_I) {
+        Dp = S.p;
+      } else {
+        hipLaunchKernelGGL(k_dgemm, dim3(nt, nt), dim3(256), 0, ctx->stream, S.p, T.p, D.p, n);
+        Dp = D.p;
+      }
+    } else {
+      hipLaunchKernelGGL(k_lv3_left, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                         ctx->V2.p, k2, ctx->Acc_lu.p, T.p, Y.p, n);
+      hipLaunchKernelGGL(k_lv3_right, dim3(eb), dim3(kBlock), 0, ctx->stream, ctx->V2.p, k2, Y.p,
+                         D.p, n);
+      Dp = D.p;
+    }
+    hipLaunchKernelGGL(k_stage_update, dim3(eb), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->Acb.p, ctx->en_lv1, Dp, T.p, ctx->Fop.p, n, k == 0, last ? 0 : 1);
+    PMX_HIP(hipGetLastError());
+    if (!last) T_is_I = false;
+  }
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  return PMX_SUCCESS;
+}
+
+// Runs the coarse program prog (2 = level 2, 3 = level 3) on t, leaving the
+// total correction in u: GEMVs with B^{-1} on the whole device, everything
+// else fused per segment.
+int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
+                          double* y, double* d, int kt_gemv, int kt_small,
+                          int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int)) {
+  size_t k = 0;
+  while (k < prog.size()) {
+    CoarseSeg seg{};
+    if (prog[k] == 2) {
+      const int ev = kt_begin(ctx, kt_gemv);
+      PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, y, ctx->nc));
+      kt_end(ctx, ev);
+    }
+    do {
+      CoarseOp& op = seg.op[seg.nops++];
+      op.kind = prog[k];
+      op.first = k == 0;
+      op.update_t = k + 1 < prog.size();
+      ++k;
+    } while (k < prog.size() && prog[k] == 3 && seg.nops < 8);
+    const int ev = kt_begin(ctx, kt_small);
+    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->geo, seg,
+                       ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p, ctx->c_cs.p,
+                       ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc);
+    PMX_HIP(hipGetLastError());
+    kt_end(ctx, ev);
+  }
+  return PMX_SUCCESS;
+}
+
 // u = d (first) or u += d; if update_t: t -= A_c d.
 __global__ void k_coarse_update(Geo g, const double* __restrict__ Acb, int kmax,
                                 const double* __restrict__ d, double* __restrict__ t,

@@ -233,113 +233,141 @@ __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
 // forward  w = (r + cx w_{-x} + cy w_{-y} + cz w_{-z}) * dinv
 // backward y = w + (cx' y_{+x} + cy' y_{+y} + cz' y_{+z}) * dinv
 // i.e. y = [(D+L) D^{-1} (D+L^T)]^{-1} r.  Factor data is plane-ordered so
-// the loads of one hyperplane are contiguous.
+// the loads of one hyperplane are contiguous; the next plane's factor and
+// coordinates are loaded before the current plane is processed so their
+// latency overlaps the LDS work and the barrier.
 // ---------------------------------------------------------------------------
-template <typename S, typename F4T>
-__device__ __forceinline__ void dic_sweeps(const Brick& b, const int* ps, const int* pc,
-                                           const F4T* __restrict__ fac_f,
-                                           const F4T* __restrict__ fac_b, S* v) {
-  const int nxny = b.nx * b.ny;
-  for (int s = 0; s < b.nplanes; ++s) {
-    for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
-      const int pk = pc[pos];
-      const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
-      const int li = (lz * b.ny + ly) * b.nx + lx;
-      const F4T f = fac_f[b.off + pos];
-      S acc = v[li];
-      if (lx > 0) acc += (S)f.y * v[li - 1];
-      if (ly > 0) acc += (S)f.z * v[li - b.nx];
-      if (lz > 0) acc += (S)f.w * v[li - nxny];
-      v[li] = acc * (S)f.x;
-    }
-    __syncthreads();
+template <typename S, typename F4T, bool FWD>
+__device__ __forceinline__ void dic_cell(const BrickView& v, int pk, const F4T& f, S* lv) {
+  const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+  const int li = (lz * v.ny + ly) * v.nx + lx;
+  if (FWD) {
+    S acc = lv[li];
+    if (lx > 0) acc += (S)f.y * lv[li - 1];
+    if (ly > 0) acc += (S)f.z * lv[li - v.nx];
+    if (lz > 0) acc += (S)f.w * lv[li - v.nxny];
+    lv[li] = acc * (S)f.x;
+  } else {
+    S acc = (S)0;
+    if (lx < v.nx - 1) acc += (S)f.y * lv[li + 1];
+    if (ly < v.ny - 1) acc += (S)f.z * lv[li + v.nx];
+    if (lz < v.nz - 1) acc += (S)f.w * lv[li + v.nxny];
+    lv[li] = lv[li] + acc * (S)f.x;
   }
-  for (int s = b.nplanes - 1; s >= 0; --s) {
-    for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
-      const int pk = pc[pos];
-      const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
-      const int li = (lz * b.ny + ly) * b.nx + lx;
-      const F4T u = fac_b[b.off + pos];
-      S acc = (S)0;
-      if (lx < b.nx - 1) acc += (S)u.y * v[li + 1];
-      if (ly < b.ny - 1) acc += (S)u.z * v[li + b.nx];
-      if (lz < b.nz - 1) acc += (S)u.w * v[li + nxny];
-      v[li] = v[li] + acc * (S)u.x;
+}
+
+template <typename S, typename F4T, bool FWD>
+__device__ __forceinline__ void dic_sweep(const BrickView& v, int nplanes, const int* ps,
+                                          const int* pc, const F4T* __restrict__ fac, S* lv) {
+  const int t = threadIdx.x;
+  const int first = FWD ? 0 : nplanes - 1;
+  const int step = FWD ? 1 : -1;
+  F4T fn{};
+  int pkn = 0;
+  {
+    const int pos = ps[first] + t;
+    if (pos < ps[first + 1]) {
+      fn = fac[v.off + pos];
+      pkn = pc[pos];
     }
+  }
+  for (int k = 0; k < nplanes; ++k) {
+    const int s = first + step * k;
+    const int lo = ps[s], hi = ps[s + 1];
+    const F4T f = fn;
+    const int pk = pkn;
+    if (k + 1 < nplanes) {
+      const int sn = s + step;
+      const int pos = ps[sn] + t;
+      if (pos < ps[sn + 1]) {
+        fn = fac[v.off + pos];
+        pkn = pc[pos];
+      }
+    }
+    if (lo + t < hi) dic_cell<S, F4T, FWD>(v, pk, f, lv);
+    for (int pos = lo + t + (int)blockDim.x; pos < hi; pos += blockDim.x)
+      dic_cell<S, F4T, FWD>(v, pc[pos], fac[v.off + pos], lv);
     __syncthreads();
   }
 }
 
-// A x for one cell (brick order) with face coefficients of type TT.
+template <typename S, typename F4T>
+__device__ __forceinline__ void dic_sweeps(const Geo& g, const Brick& b, const BrickView& v,
+                                           const F4T* __restrict__ fac_f,
+                                           const F4T* __restrict__ fac_b, S* lv) {
+  const int* ps = g.plane_start + g.shape_pstart[b.shape];
+  const int* pc = g.plane_cell + g.shape_coff[b.shape];
+  dic_sweep<S, F4T, true>(v, b.nplanes, ps, pc, fac_f, lv);
+  dic_sweep<S, F4T, false>(v, b.nplanes, ps, pc, fac_b, lv);
+}
+
+// (A x)_c for one cell: diag*x_c - sum_faces T x_nb, diag = sum of the
+// existing faces (+ pin).  T[d][c] is the lower face of c (0 on the domain
+// boundary), the upper face is the lower face of the +d neighbour.
 template <typename TT, typename XT, typename ACC>
-__device__ __forceinline__ ACC stencil_cell(const Geo& g, const Brick& b, int c, int lx, int ly,
+__device__ __forceinline__ ACC stencil_fast(const Geo& g, const BrickView& v, int c, int lx, int ly,
                                             int lz, const TT* __restrict__ Tx,
                                             const TT* __restrict__ Ty,
                                             const TT* __restrict__ Tz, const XT* __restrict__ x) {
-  const ACC xc = (ACC)x[c];
-  ACC acc = (ACC)0;
-  ACC diag = (ACC)0;
+  int nb[6];
+  nbr6(v, c, lx, ly, lz, nb);
   const TT* TT3[3] = {Tx, Ty, Tz};
+  ACC diag = (ACC)0, acc = (ACC)0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int nl = nbr_index(g, b, c, lx, ly, lz, 2 * d);
-    if (nl >= 0) {
-      const ACC t = (ACC)TT3[d][c];
-      diag += t;
-      acc += t * (ACC)x[nl];
-    }
-    const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
-    if (nh >= 0) {
-      const ACC t = (ACC)TT3[d][nh];
-      diag += t;
-      acc += t * (ACC)x[nh];
-    }
+    const int nl = nb[2 * d], nh = nb[2 * d + 1];
+    const ACC tl = (ACC)TT3[d][c];  // 0 when nl < 0
+    const ACC th = nh >= 0 ? (ACC)TT3[d][nh] : (ACC)0;
+    diag += tl + th;
+    acc += tl * (ACC)x[nl >= 0 ? nl : c] + th * (ACC)x[nh >= 0 ? nh : c];
   }
   if (c == g.pin) diag += (ACC)g.tau;
-  return diag * xc - acc;
+  return diag * (ACC)x[c] - acc;
 }
 
 // Stage "lv1" of PC_apply_vec with y = 0 (so r = x exactly): y = B1 x.
 template <typename S, typename F4T>
-__global__ void k_lv1_first(Geo g, const double* __restrict__ x, S* __restrict__ y,
-                            const F4T* __restrict__ fac_f, const F4T* __restrict__ fac_b) {
+__global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, const double* __restrict__ x,
+                                                      S* __restrict__ y,
+                                                      const F4T* __restrict__ fac_f,
+                                                      const F4T* __restrict__ fac_b) {
   extern __shared__ char shraw[];
-  S* v = reinterpret_cast<S*>(shraw);
+  S* lv = reinterpret_cast<S*>(shraw);
   const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) v[li] = (S)x[b.off + li];
+  const BrickView v = load_view(g, blockIdx.x);
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) lv[li] = (S)x[v.off + li];
   __syncthreads();
-  dic_sweeps<S, F4T>(b, g.plane_start + g.shape_pstart[b.shape],
-                     g.plane_cell + g.shape_coff[b.shape], fac_f, fac_b, v);
-  for (int li = threadIdx.x; li < n; li += blockDim.x) y[b.off + li] = v[li];
+  dic_sweeps<S, F4T>(g, b, v, fac_f, fac_b, lv);
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) y[v.off + li] = lv[li];
 }
 
 // Final lv1 stage: r = x - A y, z = y + B1 r, and partial sums of z.x
 // (the PCG's beta = z'r, the PC input x being the PCG residual).
 template <typename S, typename TT, typename F4T>
-__global__ void k_lv1_last(Geo g, const double* __restrict__ x, const S* __restrict__ y,
-                           const TT* __restrict__ Tx, const TT* __restrict__ Ty,
-                           const TT* __restrict__ Tz, const F4T* __restrict__ fac_f,
-                           const F4T* __restrict__ fac_b, double* __restrict__ z,
-                           S* __restrict__ yout, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __restrict__ x,
+                                                     const S* __restrict__ y,
+                                                     const TT* __restrict__ Tx,
+                                                     const TT* __restrict__ Ty,
+                                                     const TT* __restrict__ Tz,
+                                                     const F4T* __restrict__ fac_f,
+                                                     const F4T* __restrict__ fac_b,
+                                                     double* __restrict__ z, S* __restrict__ yout,
+                                                     double* __restrict__ partial) {
   extern __shared__ char shraw[];
-  S* v = reinterpret_cast<S*>(shraw);
+  S* lv = reinterpret_cast<S*>(shraw);
   __shared__ double red[8];
   const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
-    const int c = b.off + li;
-    const S ay = stencil_cell<TT, S, S>(g, b, c, lx, ly, lz, Tx, Ty, Tz, y);
-    v[li] = (S)x[c] - ay;
-  }
+  const BrickView v = load_view(g, blockIdx.x);
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    const S ay = stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
+    lv[c - v.off] = (S)x[c] - ay;
+  });
   __syncthreads();
-  dic_sweeps<S, F4T>(b, g.plane_start + g.shape_pstart[b.shape],
-                     g.plane_cell + g.shape_coff[b.shape], fac_f, fac_b, v);
+  dic_sweeps<S, F4T>(g, b, v, fac_f, fac_b, lv);
   double acc = 0.0;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int c = b.off + li;
-    const S yn = y[c] + v[li];
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
+    const int c = v.off + li;
+    const S yn = y[c] + lv[li];
     if (z) {
       const double zz = (double)yn;
       z[c] = zz;
@@ -356,46 +384,64 @@ __global__ void k_lv1_last(Geo g, const double* __restrict__ x, const S* __restr
 // r = x - A y and c[dof+j] = sum_cells phi[cell][j] r[cell]  (the VecMDot of
 // _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1564).  y == nullptr means y = 0.
 template <typename S, typename TT>
-__global__ void k_resid_restrict(Geo g, const double* __restrict__ x, const S* __restrict__ y,
-                                 const TT* __restrict__ Tx, const TT* __restrict__ Ty,
-                                 const TT* __restrict__ Tz, const S* __restrict__ phi, int kmax,
-                                 double* __restrict__ cvec) {
-  __shared__ double red[8];
+__global__ void __launch_bounds__(kBlock) k_resid_restrict(Geo g, const double* __restrict__ x,
+                                                           const S* __restrict__ y,
+                                                           const TT* __restrict__ Tx,
+                                                           const TT* __restrict__ Ty,
+                                                           const TT* __restrict__ Tz,
+                                                           const S* __restrict__ phi, int kmax,
+                                                           double* __restrict__ cvec) {
+  __shared__ double red[4 * kMaxEig];
   const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
+  const BrickView v = load_view(g, blockIdx.x);
+  const int ke = b.neig;
   double acc[kMaxEig];
 #pragma unroll
   for (int j = 0; j < kMaxEig; ++j) acc[j] = 0.0;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
-    const int c = b.off + li;
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
     double r = x[c];
-    if (y) r -= (double)stencil_cell<TT, S, S>(g, b, c, lx, ly, lz, Tx, Ty, Tz, y);
+    if (y) r -= (double)stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
     const S* ph = phi + (size_t)c * kmax;
 #pragma unroll
     for (int j = 0; j < kMaxEig; ++j)
-      if (j < b.neig) acc[j] += (double)ph[j] * r;
+      if (j < ke) acc[j] += (double)ph[j] * r;
+  });
+  // wave sums, then the 4 waves in a fixed order
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kMaxEig; ++j) {
+    if (j < ke) {
+      double s = acc[j];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[w * kMaxEig + j] = s;
+    }
   }
-  for (int j = 0; j < b.neig; ++j) {
-    const double s = block_sum(acc[j], red);
-    if (threadIdx.x == 0) cvec[b.dof + j] = s;
+  __syncthreads();
+  if (threadIdx.x < ke) {
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q * kMaxEig + threadIdx.x];
+    cvec[b.dof + threadIdx.x] = s;
   }
 }
 
 // y += phi c  (the VecMAXPY of _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1578).
 template <typename S>
-__global__ void k_prolong(Geo g, S* __restrict__ y, const S* __restrict__ phi, int kmax,
-                          const double* __restrict__ cvec) {
+__global__ void __launch_bounds__(kBlock) k_prolong(Geo g, S* __restrict__ y,
+                                                    const S* __restrict__ phi, int kmax,
+                                                    const double* __restrict__ cvec) {
   __shared__ double cs[kMaxEig];
   const Brick b = g.bricks[blockIdx.x];
   if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
   __syncthreads();
   const int n = b.nx * b.ny * b.nz;
+  const int ke = b.neig;
   for (int li = threadIdx.x; li < n; li += blockDim.x) {
     const int c = b.off + li;
     const S* ph = phi + (size_t)c * kmax;
     S acc = (S)0;
-    for (int j = 0; j < b.neig; ++j) acc += ph[j] * (S)cs[j];
+#pragma unroll
+    for (int j = 0; j < kMaxEig; ++j)
+      if (j < ke) acc += ph[j] * (S)cs[j];
     y[c] = y[c] + acc;
   }
 }
@@ -407,51 +453,52 @@ __global__ void k_prolong(Geo g, S* __restrict__ y, const S* __restrict__ phi, i
 // p_new = z + (beta/betaold) p_old (p_new = z on the first iteration),
 // w = A p_new, partial p_new'w.  p_old is read at neighbours, so p_new goes to
 // a second buffer.
-__global__ void k_cg_p_spmv(Geo g, const double* __restrict__ z, const double* __restrict__ pold,
-                            double* __restrict__ pnew, double* __restrict__ w,
-                            const double* __restrict__ Tx, const double* __restrict__ Ty,
-                            const double* __restrict__ Tz, const double* __restrict__ scal,
-                            int first, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_cg_p_spmv(Geo g, const double* __restrict__ z,
+                                                      const double* __restrict__ pold,
+                                                      double* __restrict__ pnew,
+                                                      double* __restrict__ w,
+                                                      const double* __restrict__ Tx,
+                                                      const double* __restrict__ Ty,
+                                                      const double* __restrict__ Tz,
+                                                      const double* __restrict__ scal, int first,
+                                                      double* __restrict__ partial) {
   __shared__ double red[8];
-  const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
+  const BrickView v = load_view(g, blockIdx.x);
   const double bc = first ? 0.0 : scal[0] / scal[1];
   double acc = 0.0;
   const double* TT3[3] = {Tx, Ty, Tz};
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
-    const int c = b.off + li;
-    const double pc = first ? z[c] : z[c] + bc * pold[c];
+  auto pval = [&](int i) { return first ? z[i] : z[i] + bc * pold[i]; };
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    int nb[6];
+    nbr6(v, c, lx, ly, lz, nb);
+    const double pc = pval(c);
     double diag = 0.0, off = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const int nl = nbr_index(g, b, c, lx, ly, lz, 2 * d);
-      if (nl >= 0) {
-        const double t = TT3[d][c];
-        diag += t;
-        off += t * (first ? z[nl] : z[nl] + bc * pold[nl]);
-      }
-      const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * d + 1);
-      if (nh >= 0) {
-        const double t = TT3[d][nh];
-        diag += t;
-        off += t * (first ? z[nh] : z[nh] + bc * pold[nh]);
-      }
+      const int nl = nb[2 * d], nh = nb[2 * d + 1];
+      const double tl = TT3[d][c];
+      const double th = nh >= 0 ? TT3[d][nh] : 0.0;
+      diag += tl + th;
+      if (nl >= 0) off += tl * pval(nl);
+      if (nh >= 0) off += th * pval(nh);
     }
     if (c == g.pin) diag += g.tau;
     const double wc = diag * pc - off;
     pnew[c] = pc;
     w[c] = wc;
     acc += pc * wc;
-  }
+  });
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
 // x += a p, r -= a w, partial r'r with a = beta/dpi.
-__global__ void k_cg_update(int n, double* __restrict__ x, double* __restrict__ r,
-                            const double* __restrict__ p, const double* __restrict__ w,
-                            const double* __restrict__ scal, double* __restrict__ partial) {
+__global__ void __launch_bounds__(kBlock) k_cg_update(int n, double* __restrict__ x,
+                                                      double* __restrict__ r,
+                                                      const double* __restrict__ p,
+                                                      const double* __restrict__ w,
+                                                      const double* __restrict__ scal,
+                                                      double* __restrict__ partial) {
   __shared__ double red[8];
   const double a = scal[0] / scal[2];
   double acc = 0.0;
@@ -471,17 +518,14 @@ __global__ void k_residual(Geo g, const double* __restrict__ bvec, const double*
                            const double* __restrict__ Ty, const double* __restrict__ Tz,
                            double* __restrict__ partial) {
   __shared__ double red[8];
-  const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
+  const BrickView v = load_view(g, blockIdx.x);
   double acc = 0.0;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
-    const int c = b.off + li;
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
     double rr = bvec[c];
-    if (x) rr -= stencil_cell<double, double, double>(g, b, c, lx, ly, lz, Tx, Ty, Tz, x);
+    if (x) rr -= stencil_fast<double, double, double>(g, v, c, lx, ly, lz, Tx, Ty, Tz, x);
     r[c] = rr;
     acc += rr * rr;
-  }
+  });
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
@@ -490,13 +534,10 @@ __global__ void k_residual(Geo g, const double* __restrict__ bvec, const double*
 __global__ void k_spmv(Geo g, const double* __restrict__ x, double* __restrict__ y,
                        const double* __restrict__ Tx, const double* __restrict__ Ty,
                        const double* __restrict__ Tz) {
-  const Brick b = g.bricks[blockIdx.x];
-  const int n = b.nx * b.ny * b.nz;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
-    const int c = b.off + li;
-    y[c] = stencil_cell<double, double, double>(g, b, c, lx, ly, lz, Tx, Ty, Tz, x);
-  }
+  const BrickView v = load_view(g, blockIdx.x);
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    y[c] = stencil_fast<double, double, double>(g, v, c, lx, ly, lz, Tx, Ty, Tz, x);
+  });
 }
 
 // Deterministic finalisation of per-block partial sums into scal[].

@@ -121,6 +121,69 @@ __device__ __forceinline__ int nbr_index(const Geo& g, const Brick& b, int li, i
   }
 }
 
+// One brick and the offsets/extents of its six neighbour bricks, held in
+// registers by every thread of the workgroup that owns the brick.  An x
+// neighbour shares ny and nz, a y neighbour nx and nz, a z neighbour nx and ny.
+struct BrickView {
+  int off, nx, ny, nz, nxny, n;
+  int noff[6];   // -1: domain boundary
+  int next[6];   // the neighbour's extent along the shared axis
+};
+
+__device__ __forceinline__ BrickView load_view(const Geo& g, int s) {
+  const Brick& b = g.bricks[s];
+  BrickView v;
+  v.off = b.off;
+  v.nx = b.nx;
+  v.ny = b.ny;
+  v.nz = b.nz;
+  v.nxny = b.nx * b.ny;
+  v.n = v.nxny * b.nz;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int sn = b.nb[d];
+    v.noff[d] = sn >= 0 ? g.bricks[sn].off : -1;
+    const int ax = d >> 1;
+    v.next[d] = sn >= 0 ? (ax == 0 ? g.bricks[sn].nx : (ax == 1 ? g.bricks[sn].ny : g.bricks[sn].nz))
+                        : 0;
+  }
+  return v;
+}
+
+// Brick-order indices of the -x,+x,-y,+y,-z,+z neighbours of cell c at local
+// (lx,ly,lz); -1 on the domain boundary.
+__device__ __forceinline__ void nbr6(const BrickView& v, int c, int lx, int ly, int lz, int nb[6]) {
+  nb[0] = lx > 0 ? c - 1
+                 : (v.noff[0] >= 0 ? v.noff[0] + (lz * v.ny + ly) * v.next[0] + v.next[0] - 1 : -1);
+  nb[1] = lx < v.nx - 1 ? c + 1 : (v.noff[1] >= 0 ? v.noff[1] + (lz * v.ny + ly) * v.next[1] : -1);
+  nb[2] = ly > 0 ? c - v.nx
+                 : (v.noff[2] >= 0 ? v.noff[2] + (lz * v.next[2] + v.next[2] - 1) * v.nx + lx : -1);
+  nb[3] = ly < v.ny - 1 ? c + v.nx : (v.noff[3] >= 0 ? v.noff[3] + lz * v.next[3] * v.nx + lx : -1);
+  nb[4] = lz > 0 ? c - v.nxny
+                 : (v.noff[4] >= 0 ? v.noff[4] + ((v.next[4] - 1) * v.ny + ly) * v.nx + lx : -1);
+  nb[5] = lz < v.nz - 1 ? c + v.nxny : (v.noff[5] >= 0 ? v.noff[5] + ly * v.nx + lx : -1);
+}
+
+// Line-based walk over a brick without divisions in the loop: thread t owns
+// x-coordinate t % nx of lines t/nx, t/nx + lpi, ... (lpi = blockDim/nx).
+// Requires nx <= blockDim.x.
+template <typename F>
+__device__ __forceinline__ void for_cells(const BrickView& v, F&& f) {
+  const int lpi = (int)blockDim.x / v.nx;
+  const int lx = (int)threadIdx.x % v.nx;
+  const int l0 = (int)threadIdx.x / v.nx;
+  if (l0 >= lpi) return;
+  int ly = l0 % v.ny, lz = l0 / v.ny;
+  while (lz < v.nz) {
+    f(v.off + (lz * v.ny + ly) * v.nx + lx, lx, ly, lz);
+    ly += lpi;
+    while (ly >= v.ny) {
+      ly -= v.ny;
+      ++lz;
+    }
+  }
+}
+
 // Block-wide deterministic sum of one double (256 threads).
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -229,6 +292,8 @@ struct pmx_ctx {
   pmx::DBuf<double> Ainv;           // dense inverse of the deflated coarse matrix B
   pmx::DBuf<double> c_z, c_g, c_h, c_cs;  // near-null vector, pin row, B^{-1} g, Woodbury scalars
   double coarse_gamma = 0.0;
+  pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
+  std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
 
   // ---- level 3 ----

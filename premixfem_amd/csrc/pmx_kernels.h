@@ -37,6 +37,10 @@ int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift);
 int coarse_solve_setup(pmx_ctx* ctx, double delta);
 int launch_coarse_solve(pmx_ctx* ctx, const double* t, double* d, double* ytmp, int ncol);
 int launch_coarse_fix(pmx_ctx* ctx, const double* y, const double* t, double* d);
+int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog);
+int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
+                          double* y, double* d, int kt_gemv, int kt_small,
+                          int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int));
 int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec);
 int level3_setup(pmx_ctx* ctx);
 int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n);
