@@ -117,6 +117,9 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(geti(c->eig_guard));
     } else if (k == "-eig_debug") {
       PMX_TRY(geti(c->eig_debug));
+    } else if (k == "-cg_chunk") {
+      PMX_TRY(geti(c->cg_chunk));
+      c->cg_chunk = std::max(1, c->cg_chunk);
     } else if (k == "-kernel_timing") {
       c->kernel_timing = 1;
     } else if (k == "-log_stages") {
@@ -760,7 +763,10 @@ int pmx_apply(pmx_ctx* c, const double* x, double* y) {
 // (initial guess if x0_nonzero, solution on return).
 static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* its,
                       double* rnorm) {
-  const double abstol = 1e-50, dtol = 1e5;
+  c->cg_rtol = rtol;
+  c->cg_abstol = 1e-50;
+  c->cg_dtol = 1e5;
+  c->cg_max_it = max_it;
   double* xb = c->w_x.p;
   double* bb = c->w_b.p;
   double* r = c->w_r.p;
@@ -775,58 +781,53 @@ static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* 
     PMX_TRY(launch_residual(c, bb, nullptr, r));
   }
   double* hs = c->h_scal;
-  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
   PMX_HIP(hipStreamSynchronize(c->stream));
   const double t_solve0 = now();
-  double dp = std::sqrt(hs[4]);
-  const double rnorm0 = dp;
-  const double ttol = std::max(rtol * rnorm0, abstol);
-  int reason = 0, k = 0;
-  if (dp <= ttol) {
-    reason = dp > abstol ? PMX_CONVERGED_RTOL : PMX_CONVERGED_ATOL;
-  } else {
-    PMX_TRY(apply_device(c, r, z, c->partial.p));
-    PMX_TRY(launch_finalize_beta(c));
-    int i = 0;
-    while (true) {
-      k = i + 1;
-      int ev = kt_begin(c, 4);
-      PMX_TRY(launch_cg_p_spmv(c, z, p0, p1, w, i == 0));
-      kt_end(c, ev);
-      ev = kt_begin(c, 5);
-      PMX_TRY(launch_cg_update(c, xb, r, p1, w));
-      kt_end(c, ev);
-      PMX_HIP(hipMemcpyAsync(hs, c->scal.p, 16 * sizeof(double), hipMemcpyDeviceToHost,
-                             c->stream));
-      PMX_HIP(hipStreamSynchronize(c->stream));
-      if (hs[5] != 0.0) {
-        reason = PMX_DIVERGED_INDEFINITE_MAT;
+  // Iterations are enqueued in chunks; the convergence test runs on the
+  // device, which marks the solve done so the remaining kernels of the chunk
+  // return at once.  The host reads the scalars once per chunk.
+  c->geo.stop = c->scal.p + kSDone;
+  int rc = PMX_SUCCESS;
+  if (hs[kSDone] == 0.0) {
+    rc = apply_device(c, r, z, c->partial.p);
+    if (!rc) rc = launch_finalize_beta(c);
+    while (!rc) {
+      for (int q = 0; q < c->cg_chunk && !rc; ++q) {
+        int ev = kt_begin(c, 4);
+        rc = launch_cg_p_spmv(c, z, p0, p1, w, 0);
+        kt_end(c, ev);
+        if (rc) break;
+        ev = kt_begin(c, 5);
+        rc = launch_cg_update(c, xb, r, p1, w);
+        kt_end(c, ev);
+        if (rc) break;
+        rc = apply_device(c, r, z, c->partial.p);
+        if (!rc) rc = launch_finalize_beta(c);
+        std::swap(p0, p1);
+      }
+      if (rc) break;
+      if (hipMemcpyAsync(hs, c->scal.p, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("PCG: reading device scalars failed");
+        rc = PMX_ERR_LIB;
         break;
       }
-      dp = std::sqrt(hs[4]);
-      if (dp <= ttol) {
-        reason = PMX_CONVERGED_RTOL;
-        break;
-      }
-      if (dp >= dtol * rnorm0) {
-        reason = PMX_DIVERGED_DTOL;
-        break;
-      }
-      if (k >= max_it) {
-        reason = PMX_DIVERGED_ITS;
-        break;
-      }
-      PMX_TRY(apply_device(c, r, z, c->partial.p));
-      PMX_TRY(launch_finalize_beta(c));
-      std::swap(p0, p1);
-      ++i;
+      if (hs[kSDone] != 0.0) break;
     }
   }
+  c->geo.stop = nullptr;
+  PMX_TRY(rc);
   PMX_HIP(hipStreamSynchronize(c->stream));
   PMX_TRY(kt_collect(c));
+  const int k = (int)hs[kSIts];
+  const int reason = (int)hs[kSReason];
+  const double dp = std::sqrt(hs[kSRnorm2]);
   c->last_its = k;
   c->last_rnorm = dp;
-  c->last_rnorm0 = rnorm0;
+  c->last_rnorm0 = std::sqrt(hs[kSRnorm0]);
   c->last_reason = reason;
   c->t_solve = now() - t_solve0;
   if (its) *its = k;
@@ -1030,7 +1031,7 @@ int pmx_final(pmx_ctx* c) {
     c->T[a].release();
     c->T32[a].release();
   }
-  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs, &c->Fop}) b->release();
+  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs, &c->Fop, &c->lob_buf}) b->release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

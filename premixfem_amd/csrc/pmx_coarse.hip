@@ -470,7 +470,8 @@ int launch_coarse_solve(pmx_ctx* ctx, const double* t, double* d, double* ytmp, 
 // ---------------------------------------------------------------------------
 // y = A x for a dense row-major n x n matrix; one wavefront per row.
 __global__ void k_gemv(const double* __restrict__ A, const double* __restrict__ x,
-                       double* __restrict__ y, int n) {
+                       double* __restrict__ y, int n, const double* stop) {
+  PMX_STOP_CHECK(stop);
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (row >= n) return;
@@ -484,7 +485,7 @@ __global__ void k_gemv(const double* __restrict__ A, const double* __restrict__ 
 int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n) {
   const int wpb = kBlock / 64;
   hipLaunchKernelGGL(k_gemv, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream, A, x, y,
-                     n);
+                     n, ctx->geo.stop);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

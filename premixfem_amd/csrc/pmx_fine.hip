@@ -331,6 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, const double* __res
                                                       S* __restrict__ y,
                                                       const F4T* __restrict__ fac_f,
                                                       const F4T* __restrict__ fac_b) {
+  PMX_STOP_CHECK(g.stop);
   extern __shared__ char shraw[];
   S* lv = reinterpret_cast<S*>(shraw);
   const Brick b = g.bricks[blockIdx.x];
@@ -353,6 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __rest
                                                      const F4T* __restrict__ fac_b,
                                                      double* __restrict__ z, S* __restrict__ yout,
                                                      double* __restrict__ partial) {
+  PMX_STOP_CHECK(g.stop);
   extern __shared__ char shraw[];
   S* lv = reinterpret_cast<S*>(shraw);
   __shared__ double red[8];
@@ -391,6 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_resid_restrict(Geo g, const double* 
                                                            const TT* __restrict__ Tz,
                                                            const S* __restrict__ phi, int kmax,
                                                            double* __restrict__ cvec) {
+  PMX_STOP_CHECK(g.stop);
   __shared__ double red[4 * kMaxEig];
   const Brick b = g.bricks[blockIdx.x];
   const BrickView v = load_view(g, blockIdx.x);
@@ -429,6 +432,7 @@ template <typename S>
 __global__ void __launch_bounds__(kBlock) k_prolong(Geo g, S* __restrict__ y,
                                                     const S* __restrict__ phi, int kmax,
                                                     const double* __restrict__ cvec) {
+  PMX_STOP_CHECK(g.stop);
   __shared__ double cs[kMaxEig];
   const Brick b = g.bricks[blockIdx.x];
   if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
@@ -460,10 +464,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_p_spmv(Geo g, const double* __res
                                                       const double* __restrict__ Tx,
                                                       const double* __restrict__ Ty,
                                                       const double* __restrict__ Tz,
-                                                      const double* __restrict__ scal, int first,
+                                                      const double* __restrict__ scal, int first_host,
                                                       double* __restrict__ partial) {
+  PMX_STOP_CHECK(g.stop);
   __shared__ double red[8];
   const BrickView v = load_view(g, blockIdx.x);
+  // first iteration: p = z (the device iteration counter is 0 before the
+  // first norm test; first_host = 1 forces it when no counter is kept)
+  const bool first = first_host || scal[kSIts] == 0.0;
   const double bc = first ? 0.0 : scal[0] / scal[1];
   double acc = 0.0;
   const double* TT3[3] = {Tx, Ty, Tz};
@@ -499,6 +507,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int n, double* __restrict_
                                                       const double* __restrict__ w,
                                                       const double* __restrict__ scal,
                                                       double* __restrict__ partial) {
+  PMX_STOP_CHECK(scal + kSDone);
   __shared__ double red[8];
   const double a = scal[0] / scal[2];
   double acc = 0.0;
@@ -517,6 +526,7 @@ __global__ void k_residual(Geo g, const double* __restrict__ bvec, const double*
                            double* __restrict__ r, const double* __restrict__ Tx,
                            const double* __restrict__ Ty, const double* __restrict__ Tz,
                            double* __restrict__ partial) {
+  PMX_STOP_CHECK(g.stop);
   __shared__ double red[8];
   const BrickView v = load_view(g, blockIdx.x);
   double acc = 0.0;
@@ -540,30 +550,58 @@ __global__ void k_spmv(Geo g, const double* __restrict__ x, double* __restrict__
   });
 }
 
-// Deterministic finalisation of per-block partial sums into scal[].
-// op 0: beta (z'r): scal[1] = scal[0]; scal[0] = sum
-// op 1: dpi  (p'w): scal[3] = scal[2]; scal[2] = sum; flag indefinite
-// op 2: rnorm^2   : scal[4] = sum
+// Deterministic finalisation of per-block partial sums into scal[] and the
+// PETSc KSPSolve_CG / KSPConvergedDefault decisions on the device:
+// op 0: beta = z'r (betaold <- beta)
+// op 1: dpi = p'w; zero or sign change -> DIVERGED_INDEFINITE_MAT
+// op 2: ||r||^2; its += 1; ||r|| <= ttol -> CONVERGED_RTOL,
+//       ||r|| >= dtol ||r0|| -> DIVERGED_DTOL, its >= max_it -> DIVERGED_ITS
+// op 3: initial ||r0||^2: ttol = max(rtol ||r0||, abstol), its = 0
 __global__ void k_finalize(const double* __restrict__ partial, int nb, double* __restrict__ scal,
-                           int op, int first) {
+                           int op, double rtol, double abstol, double dtol, int max_it) {
+  if (op != 3 && scal[kSDone] != 0.0) return;
   __shared__ double red[8];
   double acc = 0.0;
   // Fixed assignment of partials to threads and a fixed tree: bitwise
   // reproducible from run to run.
   for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
   acc = block_sum(acc, red);
-  if (threadIdx.x == 0) {
-    if (op == 0) {
-      scal[1] = scal[0];
-      scal[0] = acc;
-    } else if (op == 1) {
-      scal[3] = scal[2];
-      scal[2] = acc;
-      const bool flip = !first && ((acc > 0.0) != (scal[3] > 0.0));
-      if (acc == 0.0 || flip) scal[5] = 1.0;
-    } else {
-      scal[4] = acc;
+  if (threadIdx.x != 0) return;
+  if (op == 0) {
+    scal[kSBetaOld] = scal[kSBeta];
+    scal[kSBeta] = acc;
+  } else if (op == 1) {
+    const bool first = scal[kSIts] == 0.0;
+    scal[kSDpiOld] = scal[kSDpi];
+    scal[kSDpi] = acc;
+    const bool flip = !first && ((acc > 0.0) != (scal[kSDpiOld] > 0.0));
+    if (acc == 0.0 || flip) {
+      scal[kSDone] = 1.0;
+      scal[kSReason] = PMX_DIVERGED_INDEFINITE_MAT;
     }
+  } else if (op == 2) {
+    scal[kSRnorm2] = acc;
+    scal[kSIts] += 1.0;
+    const double dp = sqrt(acc);
+    if (dp <= scal[kSTtol]) {
+      scal[kSDone] = 1.0;
+      scal[kSReason] = PMX_CONVERGED_RTOL;
+    } else if (dp >= dtol * sqrt(scal[kSRnorm0])) {
+      scal[kSDone] = 1.0;
+      scal[kSReason] = PMX_DIVERGED_DTOL;
+    } else if (scal[kSIts] >= (double)max_it) {
+      scal[kSDone] = 1.0;
+      scal[kSReason] = PMX_DIVERGED_ITS;
+    }
+  } else {
+    scal[kSRnorm2] = acc;
+    scal[kSRnorm0] = acc;
+    const double dp = sqrt(acc);
+    const double ttol = fmax(rtol * dp, abstol);
+    scal[kSTtol] = ttol;
+    scal[kSIts] = 0.0;
+    scal[kSDone] = dp <= ttol ? 1.0 : 0.0;
+    scal[kSReason] = dp <= ttol ? (dp > abstol ? PMX_CONVERGED_RTOL : PMX_CONVERGED_ATOL) : 0.0;
   }
 }
 
@@ -721,7 +759,7 @@ int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* 
                      ctx->partial.p);
   PMX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 1, first);
+                     ctx->nsub, ctx->scal.p, 1, 0.0, 0.0, 0.0, 0);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -737,7 +775,7 @@ int launch_cg_update(pmx_ctx* ctx, double* x, double* r, const double* p, const 
                      ctx->scal.p, ctx->partial.p);
   PMX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p, nb,
-                     ctx->scal.p, 2, 0);
+                     ctx->scal.p, 2, 0.0, 0.0, ctx->cg_dtol, ctx->cg_max_it);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -747,14 +785,14 @@ int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
                      ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->partial.p);
   PMX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 2, 0);
+                     ctx->nsub, ctx->scal.p, 3, ctx->cg_rtol, ctx->cg_abstol, 0.0, 0);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
 
 int launch_finalize_beta(pmx_ctx* ctx) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 0, 0);
+                     ctx->nsub, ctx->scal.p, 0, 0.0, 0.0, 0.0, 0);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

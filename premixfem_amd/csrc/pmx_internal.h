@@ -70,7 +70,17 @@ struct Geo {
   const int* shape_coff;  // per shape: offset into plane_cell
   int pin;                // brick index of the pinned cell, -1 if none
   double tau;             // pin strength
+  const double* stop;     // PCG "done" flag: kernels return at once when set (nullptr: none)
 };
+
+// Device-resident PCG scalars (indices into ctx->scal).
+enum ScalIdx {
+  kSBeta = 0, kSBetaOld = 1, kSDpi = 2, kSDpiOld = 3, kSRnorm2 = 4, kSDone = 6, kSIts = 7,
+  kSTtol = 8, kSRnorm0 = 9, kSReason = 12, kSNumScal = 16
+};
+
+#define PMX_STOP_CHECK(ptr) \
+  if ((ptr) != nullptr && *(ptr) != 0.0) return;
 
 // Neighbour index in brick order, or -1.
 __device__ __forceinline__ int nbr_index(const Geo& g, const Brick& b, int li, int lx, int ly,
@@ -295,6 +305,7 @@ struct pmx_ctx {
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
+  pmx::DBuf<double> lob_buf;        // LOBPCG blocks X, AX, W, AW, P, AP (kept for warm starts)
 
   // ---- level 3 ----
   int eigen_num_lv2 = 0;
@@ -321,6 +332,9 @@ struct pmx_ctx {
   double last_rnorm = 0.0, last_rnorm0 = 0.0;
   int last_reason = 0;
   double t_solve = 0.0;
+  double cg_rtol = 1e-8, cg_abstol = 1e-50, cg_dtol = 1e5;
+  int cg_max_it = 10000;
+  int cg_chunk = 8;                 // PCG iterations enqueued between host checks
   double h_scal[16];
   std::vector<hipEvent_t> kt_events;
   std::vector<int> kt_pending;

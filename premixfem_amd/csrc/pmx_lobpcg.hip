@@ -4,13 +4,17 @@
 //
 // * The constant vector is the exact first eigenpair (A_i^N 1 = 0) and is
 //   deflated analytically; LOBPCG iterates on the next en_lv1-1 pairs with a
-//   block of m = en_lv1-1+guard vectors.
+//   block of MB vectors (MB >= en_lv1-1 + 3 guard vectors).
 // * Preconditioner: the FP32 block ICC(0) of A_i^N + sigma M_i, applied with
 //   the same hyperplane sweeps as level 1.
-// * Rayleigh-Ritz on S = [X W P] with SVQB (scaled eigen-orthonormalisation)
-//   of the M-Gram, so dependent directions are dropped instead of breaking a
-//   Cholesky.  Converged when ||A x - lambda M x||_{M^-1} <= tol |lambda| for
-//   the wanted pairs (SLEPc's relative criterion, default tol 1e-8).
+// * The basis S = [X W P] is kept M-orthonormal block by block (W and P are
+//   projected against X (and P) and SVQB-orthonormalised), so the
+//   Rayleigh-Ritz M-Gram is I plus small cross blocks: one Cholesky-reduced
+//   symmetric eigenproblem per iteration; if that Cholesky breaks down the
+//   iteration restarts without P.
+// * Convergence: ||T r||_M <= tol for every wanted pair, T ~ (A^N+sigma M)^{-1}
+//   -- the residual of the shift-inverted problem relative to its eigenvalue,
+//   the quantity SLEPc's relative test bounds under STSINVERT (default 1e-8).
 // * Output: M-orthonormal eigenvectors with a fixed sign convention, written
 //   into the basis rows, eigenvalues ascending with lambda_0 = 0.
 #include <cmath>
@@ -20,12 +24,10 @@
 
 namespace pmx {
 
-constexpr int kLobM = 10;            // largest LOBPCG block
-constexpr int kLobS = 3 * kLobM;     // largest Rayleigh-Ritz basis
-constexpr int kLobCh = 4;            // preconditioner columns per sweep
+constexpr int kLobCh = 4;  // preconditioner columns per sweep pair
 
 struct LobArgs {
-  int m, kwant, maxit, kmax;
+  int kwant, maxit, kmax;
   double tol;
   uint64_t seed;
   double* X;
@@ -40,7 +42,7 @@ struct LobArgs {
   const double* Ty;
   const double* Tz;
   const double* mass;
-  const double* x0;   // optional warm start rows [cell][m]
+  const double* x0;   // optional warm start rows [cell][MB]
   double* phi;        // output rows [cell][kmax]
   double* eig;        // output [nsub][kmax]
   int* its;           // output [nsub]
@@ -56,167 +58,288 @@ __device__ __forceinline__ double hash_uniform(uint64_t a) {
   return (double)(a >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// Reduce K per-thread values over the workgroup into out[0..K) (LDS).
-__device__ inline void wg_reduce_many(double* acc, int K, double* red, double* out) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int k = 0; k < K; ++k) {
-    double v = acc[k];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[w * K + k] = v;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    double s = 0.0;
-    for (int i = 0; i < nw; ++i) s += red[i * K + k];
-    out[k] = s;
-  }
-  __syncthreads();
-}
-
-// out[i*ldo + j] = sum_r L_i[r] wgt[r] R_j[r]  (columns contiguous, stride n).
-__device__ inline void wg_multidot(const double* __restrict__ L, int nl,
-                                   const double* __restrict__ R, int nr,
-                                   const double* __restrict__ wgt, int n, double* red,
-                                   double* tmpout, double* out, int ldo) {
-  double acc[kLobM * kLobM];
+// out[i*ldo + j] = sum_r L_i[r] wgt[r] R_j[r] for i < NL, j < NR (columns
+// contiguous with stride n).  red: 4*NL*NR doubles of LDS.
+template <int NL, int NR>
+__device__ __forceinline__ void wg_multidot(const double* __restrict__ L,
+                                            const double* __restrict__ R,
+                                            const double* __restrict__ wgt, int n, double* red,
+                                            double* out, int ldo) {
+  double acc[NL * NR];
 #pragma unroll
-  for (int e = 0; e < kLobM * kLobM; ++e) acc[e] = 0.0;
+  for (int e = 0; e < NL * NR; ++e) acc[e] = 0.0;
+#pragma unroll 2
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double lv[kLobM], rv[kLobM];
+    double lv[NL], rv[NR];
     const double wr = wgt ? wgt[r] : 1.0;
 #pragma unroll
-    for (int i = 0; i < kLobM; ++i) lv[i] = i < nl ? L[(size_t)i * n + r] * wr : 0.0;
+    for (int i = 0; i < NL; ++i) lv[i] = L[(size_t)i * n + r] * wr;
 #pragma unroll
-    for (int j = 0; j < kLobM; ++j) rv[j] = j < nr ? R[(size_t)j * n + r] : 0.0;
+    for (int j = 0; j < NR; ++j) rv[j] = R[(size_t)j * n + r];
 #pragma unroll
-    for (int i = 0; i < kLobM; ++i)
+    for (int i = 0; i < NL; ++i)
 #pragma unroll
-      for (int j = 0; j < kLobM; ++j) acc[i * kLobM + j] += lv[i] * rv[j];
+      for (int j = 0; j < NR; ++j) acc[i * NR + j] += lv[i] * rv[j];
   }
-  // Wave reduction of the needed entries (static register indices), then the
-  // sum over the waves.
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-  const int K = nl * nr;
 #pragma unroll
-  for (int i = 0; i < kLobM; ++i)
-#pragma unroll
-    for (int j = 0; j < kLobM; ++j) {
-      if (i < nl && j < nr) {
-        double v = acc[i * kLobM + j];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) red[wv * K + i * nr + j] = v;
-      }
-    }
-  __syncthreads();
-  for (int e = threadIdx.x; e < K; e += blockDim.x) {
-    double s = 0.0;
-    for (int q = 0; q < nwv; ++q) s += red[q * K + e];
-    out[(e / nr) * ldo + e % nr] = s;
+  for (int e = 0; e < NL * NR; ++e) {
+    double s = acc[e];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) red[wv * NL * NR + e] = s;
   }
   __syncthreads();
-  (void)tmpout;
+  for (int e = threadIdx.x; e < NL * NR; e += blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < nwv; ++q) s += red[q * NL * NR + e];
+    out[(e / NR) * ldo + e % NR] = s;
+  }
+  __syncthreads();
 }
 
-// Y = A_i^N V for nc columns (interior faces only).
-__device__ inline void wg_apply_AN(const Brick& b, const double* __restrict__ Tx,
-                                   const double* __restrict__ Ty, const double* __restrict__ Tz,
-                                   const double* __restrict__ V, double* __restrict__ Y, int nc) {
-  const int n = b.nx * b.ny * b.nz, nxny = b.nx * b.ny;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / nxny;
-    const int c = b.off + li;
-    const double tl0 = lx > 0 ? Tx[c] : 0.0, th0 = lx < b.nx - 1 ? Tx[c + 1] : 0.0;
-    const double tl1 = ly > 0 ? Ty[c] : 0.0, th1 = ly < b.ny - 1 ? Ty[c + b.nx] : 0.0;
-    const double tl2 = lz > 0 ? Tz[c] : 0.0, th2 = lz < b.nz - 1 ? Tz[c + nxny] : 0.0;
-    for (int j = 0; j < nc; ++j) {
-      const double* v = V + (size_t)j * n;
-      const double vc = v[li];
-      double a = 0.0;
-      if (lx > 0) a += tl0 * (vc - v[li - 1]);
-      if (lx < b.nx - 1) a += th0 * (vc - v[li + 1]);
-      if (ly > 0) a += tl1 * (vc - v[li - b.nx]);
-      if (ly < b.ny - 1) a += th1 * (vc - v[li + b.nx]);
-      if (lz > 0) a += tl2 * (vc - v[li - nxny]);
-      if (lz < b.nz - 1) a += th2 * (vc - v[li + nxny]);
-      Y[(size_t)j * n + li] = a;
+// Per-column sums sum_r wgt[r] B_j[r]^2 (j < NB) into out[j].
+template <int NB>
+__device__ __forceinline__ void wg_colnorm2(const double* __restrict__ B,
+                                            const double* __restrict__ wgt, int n, double* red,
+                                            double* out) {
+  double acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double wr = wgt[r];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const double x = B[(size_t)j * n + r];
+      acc[j] += wr * x * x;
     }
   }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    double s = acc[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) red[wv * NB + j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NB) {
+    double s = 0.0;
+    for (int q = 0; q < nwv; ++q) s += red[q * NB + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Y = A_i^N V for the first nb of NB columns (interior faces only).
+template <int NB>
+__device__ inline void wg_apply_AN(const BrickView& v, const double* __restrict__ Tx,
+                                   const double* __restrict__ Ty, const double* __restrict__ Tz,
+                                   const double* __restrict__ V, double* __restrict__ Y, int nb) {
+  const int n = v.n;
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    const int li = c - v.off;
+    const double tl0 = lx > 0 ? Tx[c] : 0.0, th0 = lx < v.nx - 1 ? Tx[c + 1] : 0.0;
+    const double tl1 = ly > 0 ? Ty[c] : 0.0, th1 = ly < v.ny - 1 ? Ty[c + v.nx] : 0.0;
+    const double tl2 = lz > 0 ? Tz[c] : 0.0, th2 = lz < v.nz - 1 ? Tz[c + v.nxny] : 0.0;
+    const int o0 = lx > 0 ? li - 1 : li, o1 = lx < v.nx - 1 ? li + 1 : li;
+    const int o2 = ly > 0 ? li - v.nx : li, o3 = ly < v.ny - 1 ? li + v.nx : li;
+    const int o4 = lz > 0 ? li - v.nxny : li, o5 = lz < v.nz - 1 ? li + v.nxny : li;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      if (j >= nb) break;
+      const double* x = V + (size_t)j * n;
+      const double xc = x[li];
+      Y[(size_t)j * n + li] = tl0 * (xc - x[o0]) + th0 * (xc - x[o1]) + tl1 * (xc - x[o2]) +
+                              th1 * (xc - x[o3]) + tl2 * (xc - x[o4]) + th2 * (xc - x[o5]);
+    }
+  });
   __syncthreads();
 }
 
 // W <- T W with the FP32 block ICC(0) sweeps, kLobCh columns at a time.
-__device__ inline void wg_precondition(const Brick& b, const int* ps, const int* pc,
-                                       const float4* __restrict__ pf,
+__device__ inline void wg_precondition(const Brick& b, const BrickView& v, const int* ps,
+                                       const int* pc, const float4* __restrict__ pf,
                                        const float4* __restrict__ pb, double* W, int nc,
-                                       float* v) {
-  const int n = b.nx * b.ny * b.nz, nxny = b.nx * b.ny;
+                                       float* lv) {
+  const int n = v.n;
   for (int j0 = 0; j0 < nc; j0 += kLobCh) {
     const int ch = min(kLobCh, nc - j0);
-    for (int e = threadIdx.x; e < ch * n; e += blockDim.x) v[e] = (float)W[(size_t)j0 * n + e];
+    for (int e = threadIdx.x; e < ch * n; e += blockDim.x) lv[e] = (float)W[(size_t)j0 * n + e];
     __syncthreads();
-    for (int s = 0; s < b.nplanes; ++s) {
-      for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
-        const int pk = pc[pos];
-        const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
-        const int li = (lz * b.ny + ly) * b.nx + lx;
-        const float4 f = pf[b.off + pos];
-        for (int q = 0; q < ch; ++q) {
-          float* vv = v + q * n;
-          float acc = vv[li];
-          if (lx > 0) acc += f.y * vv[li - 1];
-          if (ly > 0) acc += f.z * vv[li - b.nx];
-          if (lz > 0) acc += f.w * vv[li - nxny];
-          vv[li] = acc * f.x;
+    for (int dirn = 0; dirn < 2; ++dirn) {
+      const bool fwd = dirn == 0;
+      const float4* fac = fwd ? pf : pb;
+      for (int k = 0; k < b.nplanes; ++k) {
+        const int s = fwd ? k : b.nplanes - 1 - k;
+        for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
+          const int pk = pc[pos];
+          const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
+          const int li = (lz * v.ny + ly) * v.nx + lx;
+          const float4 f = fac[v.off + pos];
+          for (int q = 0; q < ch; ++q) {
+            float* vv = lv + q * n;
+            if (fwd) {
+              float acc = vv[li];
+              if (lx > 0) acc += f.y * vv[li - 1];
+              if (ly > 0) acc += f.z * vv[li - v.nx];
+              if (lz > 0) acc += f.w * vv[li - v.nxny];
+              vv[li] = acc * f.x;
+            } else {
+              float acc = 0.0f;
+              if (lx < v.nx - 1) acc += f.y * vv[li + 1];
+              if (ly < v.ny - 1) acc += f.z * vv[li + v.nx];
+              if (lz < v.nz - 1) acc += f.w * vv[li + v.nxny];
+              vv[li] = vv[li] + acc * f.x;
+            }
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-    for (int s = b.nplanes - 1; s >= 0; --s) {
-      for (int pos = ps[s] + threadIdx.x; pos < ps[s + 1]; pos += blockDim.x) {
-        const int pk = pc[pos];
-        const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
-        const int li = (lz * b.ny + ly) * b.nx + lx;
-        const float4 u = pb[b.off + pos];
-        for (int q = 0; q < ch; ++q) {
-          float* vv = v + q * n;
-          float acc = 0.0f;
-          if (lx < b.nx - 1) acc += u.y * vv[li + 1];
-          if (ly < b.ny - 1) acc += u.z * vv[li + b.nx];
-          if (lz < b.nz - 1) acc += u.w * vv[li + nxny];
-          vv[li] = vv[li] + acc * u.x;
-        }
-      }
-      __syncthreads();
-    }
-    for (int e = threadIdx.x; e < ch * n; e += blockDim.x) W[(size_t)j0 * n + e] = (double)v[e];
+    for (int e = threadIdx.x; e < ch * n; e += blockDim.x) W[(size_t)j0 * n + e] = (double)lv[e];
     __syncthreads();
   }
 }
 
-// Row-wise in-place transform: columns [0,nout) of B <- B(:, 0:nin) C.
-__device__ inline void wg_transform(double* B, int n, int nin, const double* C, int ldc, int nout) {
+// Row-wise in-place transform of an NB-column block: B(:, 0:nout) <- B(:, 0:nin) C.
+template <int NB>
+__device__ inline void wg_transform(double* B, int n, const double* C, int ldc, int nin,
+                                    int nout) {
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double v[kLobM];
+    double x[NB];
 #pragma unroll
-    for (int k = 0; k < kLobM; ++k) v[k] = k < nin ? B[(size_t)k * n + r] : 0.0;
+    for (int k = 0; k < NB; ++k) x[k] = k < nin ? B[(size_t)k * n + r] : 0.0;
 #pragma unroll
-    for (int c = 0; c < kLobM; ++c) {
+    for (int c = 0; c < NB; ++c) {
       if (c >= nout) break;
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < kLobM; ++k)
-        if (k < nin) s += v[k] * C[k * ldc + c];
+      for (int k = 0; k < NB; ++k)
+        if (k < nin) s += x[k] * C[k * ldc + c];
       B[(size_t)c * n + r] = s;
     }
   }
   __syncthreads();
 }
 
-// SVQB of an nb x nb Gram G (destroyed): Z = D U_kept theta^{-1/2} with
-// D = diag(G)^{-1/2}; returns the number of kept columns.
-__device__ inline int wg_svqb(double* G, double* Z, double* tmp, double* Q, double* w, double* rot,
+// Symmetric eigendecomposition by parallel cyclic Jacobi with two barriers
+// per round (ping-pong buffers).  Input in A0 (destroyed); on return the
+// eigenvectors are in Q0 (columns, ascending eigenvalues w).  A1, Q1 are
+// scratch; rot holds 3*ld doubles.  Rotations are applied only where the
+// coupling is significant relative to the diagonal (threshold Jacobi, which
+// keeps small eigenvalues relatively accurate); a sweep without rotations
+// ends the iteration.
+__device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
+                                double* rot, int m, int ld, int max_sweeps = 30) {
+  __shared__ int s_rot[2];
+  double* cc = rot;
+  double* ss = rot + ld;
+  int* part = reinterpret_cast<int*>(rot + 2 * ld);
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x)
+    Q0[(e / m) * ld + e % m] = (e / m == e % m) ? 1.0 : 0.0;
+  if (threadIdx.x == 0) s_rot[0] = s_rot[1] = 0;
+  __syncthreads();
+  const int mp = (m + 1) & ~1;
+  double *Ac = A0, *An = A1, *Qc = Q0, *Qn = Q1;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    const int fl = sweep & 1;
+    for (int r = 0; r < mp - 1; ++r) {
+      for (int k = threadIdx.x; k < mp / 2; k += blockDim.x) {
+        int p, q;
+        if (k == 0) {
+          p = r;
+          q = mp - 1;
+        } else {
+          p = (r + k) % (mp - 1);
+          q = (r - k + (mp - 1)) % (mp - 1);
+        }
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        if (q < m) {
+          const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
+          double c = 1.0, s = 0.0;
+          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
+            s_rot[fl] = 1;
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
+          cc[p] = c;
+          cc[q] = c;
+          ss[p] = -s;
+          ss[q] = s;
+          part[p] = q;
+          part[q] = p;
+        } else if (p < m) {
+          cc[p] = 1.0;
+          ss[p] = 0.0;
+          part[p] = p;
+        }
+      }
+      __syncthreads();
+      // every reader of the previous sweep's flag is past this point
+      if (r == 0 && threadIdx.x == 0) s_rot[fl ^ 1] = 0;
+      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int i = e / m, j = e % m;
+        const int ip = part[i], jp = part[j];
+        const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
+        double x;
+        if (ip == j && ip != i)
+          x = 0.0;
+        else
+          x = ci * cj * Ac[i * ld + j] + ci * sj * Ac[i * ld + jp] + si * cj * Ac[ip * ld + j] +
+              si * sj * Ac[ip * ld + jp];
+        An[i * ld + j] = x;
+        Qn[i * ld + j] = cj * Qc[i * ld + j] + sj * Qc[i * ld + jp];
+      }
+      __syncthreads();
+      double* t = Ac;
+      Ac = An;
+      An = t;
+      t = Qc;
+      Qc = Qn;
+      Qn = t;
+    }
+    if (s_rot[fl] == 0) break;
+  }
+  int* perm = reinterpret_cast<int*>(rot);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < m; ++i) {
+      w[i] = Ac[i * ld + i];
+      perm[i] = i;
+    }
+    for (int i = 0; i < m; ++i) {
+      int best = i;
+      for (int j = i + 1; j < m; ++j)
+        if (w[j] < w[best]) best = j;
+      const double tw = w[i];
+      w[i] = w[best];
+      w[best] = tw;
+      const int tp = perm[i];
+      perm[i] = perm[best];
+      perm[best] = tp;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int i = e / m, j = e % m;
+    An[i * ld + j] = Qc[i * ld + perm[j]];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x)
+    Q0[(e / m) * ld + e % m] = An[(e / m) * ld + e % m];
+  __syncthreads();
+}
+
+// SVQB of an nb x nb M-Gram G (destroyed): Z = D U_kept theta^{-1/2},
+// D = diag(G)^{-1/2}; returns the number of kept columns.  G1 and Q1 are
+// scratch.
+__device__ inline int wg_svqb(double* G, double* G1, double* Z, double* Q1, double* w, double* rot,
                               int nb, int ld, double drop) {
-  __shared__ double dsc[kLobS];
+  __shared__ double dsc[64];
   __shared__ int s_keep, s_first;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) {
     const double d = G[i * ld + i];
@@ -228,7 +351,7 @@ __device__ inline int wg_svqb(double* G, double* Z, double* tmp, double* Q, doub
     G[i * ld + j] *= dsc[i] * dsc[j];
   }
   __syncthreads();
-  wg_syev(G, Q, tmp, w, rot, nb, ld);
+  wg_syev2(G, G1, Z, Q1, w, rot, nb, ld);
   if (threadIdx.x == 0) {
     const double tmax = w[nb - 1];
     int first = nb;
@@ -244,22 +367,29 @@ __device__ inline int wg_svqb(double* G, double* Z, double* tmp, double* Q, doub
   const int keep = s_keep, first = s_first;
   for (int e = threadIdx.x; e < nb * keep; e += blockDim.x) {
     const int i = e / keep, c = e % keep;
-    Z[i * ld + c] = dsc[i] * Q[i * ld + first + c] / sqrt(w[first + c]);
+    G1[i * ld + c] = dsc[i] * Z[i * ld + first + c] / sqrt(w[first + c]);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nb * keep; e += blockDim.x) {
+    const int i = e / keep, c = e % keep;
+    Z[i * ld + c] = G1[i * ld + c];
   }
   __syncthreads();
   return keep;
 }
 
+template <int MB>
 __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
+  constexpr int LD = 3 * MB;
   extern __shared__ double lds[];
   const int sidx = blockIdx.x;
   const Brick b = g.bricks[sidx];
-  const int n = b.nx * b.ny * b.nz;
-  const int m = a.m;
+  const BrickView v = load_view(g, sidx);
+  const int n = v.n;
   const int* ps = g.plane_start + g.shape_pstart[b.shape];
   const int* pc = g.plane_cell + g.shape_coff[b.shape];
-  const double* mass = a.mass + b.off;
-  const size_t base = (size_t)b.off * m;
+  const double* mass = a.mass + v.off;
+  const size_t base = (size_t)v.off * MB;
   double* X = a.X + base;
   double* AX = a.AX + base;
   double* W = a.W + base;
@@ -267,8 +397,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   double* P = a.P + base;
   double* AP = a.AP + base;
 
-  // LDS carve: small matrices; the preconditioner reuses the same space.
-  constexpr int LD = kLobS;
+  // LDS: small matrices (the preconditioner buffer reuses the same space).
   double* GA = lds;
   double* GB = GA + LD * LD;
   double* Z = GB + LD * LD;
@@ -276,251 +405,222 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   double* TM = Q + LD * LD;
   double* w = TM + LD * LD;
   double* rot = w + LD;
-  double* red = rot + 4 * LD;                 // 4 waves x 100 values
-  double* tmpout = red + 4 * kLobM * kLobM;
-  float* pv = reinterpret_cast<float*>(lds);  // preconditioner buffer
-  __shared__ double lam[kLobM], rho[kLobM], msum_s[1];
-  __shared__ int s_conv;
+  double* red = rot + 3 * LD;  // 4 waves x MB*MB
+  float* pv = reinterpret_cast<float*>(lds);
+  __shared__ double lam[MB], nrm[MB], msum_s;
+  __shared__ int s_conv, s_flag;
 
-  // sum of the mass diagonal (for the constant eigenvector)
   {
     double s = 0.0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) s += mass[r];
     s = block_sum(s, red);
-    if (threadIdx.x == 0) msum_s[0] = s;
+    if (threadIdx.x == 0) msum_s = s;
     __syncthreads();
   }
-  const double msum = msum_s[0];
+  const double msum = msum_s;
   const int kw = a.kwant;
-  int it = 0;
-  int status = 0;
+  int it = 0, status = 0, np = 0;
+  unsigned long long tph[10] = {0}, tlast = 0;
+  const bool trace = sidx == a.debug && threadIdx.x == 0;
+#define PMX_PHASE(i)                                                   \
+  if (trace) {                                                         \
+    const unsigned long long tn = __builtin_amdgcn_s_memrealtime();    \
+    if (tlast) tph[i] += tn - tlast;                                   \
+    tlast = tn;                                                        \
+  }
+  // B -= 1 (sum m B / sum m) for the MB columns of B.
+  auto deflate = [&](double* B) {
+    wg_multidot<MB, 1>(B, mass, nullptr, n, red, TM, 1);
+    for (int e = threadIdx.x; e < MB * n; e += blockDim.x) B[e] -= TM[e / n] / msum;
+    __syncthreads();
+  };
+  // G(r0.., c0..) = (L^T diag(wgt) R)(0:nl, 0:nr) through TM.
+  auto gram = [&](const double* L, int nl, const double* R, int nr, const double* wgt, double* G,
+                  int r0, int c0) {
+    wg_multidot<MB, MB>(L, R, wgt, n, red, TM, LD);
+    for (int e = threadIdx.x; e < nl * nr; e += blockDim.x)
+      G[(r0 + e / nr) * LD + c0 + e % nr] = TM[(e / nr) * LD + e % nr];
+    __syncthreads();
+  };
   if (kw > 0) {
     for (int attempt = 0; attempt < 2; ++attempt) {
-      // ---- initial block ----
-      for (int e = threadIdx.x; e < m * n; e += blockDim.x) {
+      // ---- initial block: warm start rows or seeded random ----
+      for (int e = threadIdx.x; e < MB * n; e += blockDim.x) {
         const int j = e / n, r = e % n;
-        double v;
-        if (a.x0 && attempt == 0)
-          v = a.x0[(size_t)(b.off + r) * m + j];
-        else
-          v = hash_uniform(a.seed ^ ((uint64_t)sidx << 40) ^ ((uint64_t)j << 32) ^ (uint64_t)r ^
-                           ((uint64_t)attempt << 60));
-        X[e] = v;
+        X[e] = (a.x0 && attempt == 0)
+                   ? a.x0[(size_t)(v.off + r) * MB + j]
+                   : hash_uniform(a.seed ^ ((uint64_t)sidx << 40) ^ ((uint64_t)j << 32) ^
+                                  (uint64_t)r ^ ((uint64_t)attempt << 60));
       }
       __syncthreads();
-      // deflate the constant: v -= (sum m v / sum m)
-      wg_multidot(X, m, mass, 1, nullptr, n, red, tmpout, TM, 1);
-      for (int e = threadIdx.x; e < m * n; e += blockDim.x) X[e] -= TM[e / n] / msum;
-      __syncthreads();
-      // M-orthonormalise X
-      wg_multidot(X, m, X, m, mass, n, red, tmpout, GB, LD);
-      int keep = wg_svqb(GB, Z, TM, Q, w, rot, m, LD, 1e-14);
-      if (keep < m) {  // degenerate start: fall back to random
-        if (attempt == 0) continue;
+      deflate(X);
+      wg_multidot<MB, MB>(X, X, mass, n, red, GB, LD);
+      const int keep = wg_svqb(GB, GA, Z, Q, w, rot, MB, LD, 1e-14);
+      if (keep < MB) {
         status = 2;
-        break;
+        continue;
       }
-      wg_transform(X, n, m, Z, LD, m);
-      wg_apply_AN(b, a.Tx, a.Ty, a.Tz, X, AX, m);
-      wg_multidot(X, m, AX, m, nullptr, n, red, tmpout, GA, LD);
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int i = e / m, j = e % m;
+      wg_transform<MB>(X, n, Z, LD, MB, MB);
+      wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, X, AX, MB);
+      wg_multidot<MB, MB>(X, AX, nullptr, n, red, GA, LD);
+      for (int e = threadIdx.x; e < MB * MB; e += blockDim.x) {
+        const int i = e / MB, j = e % MB;
         if (j > i) GA[i * LD + j] = GA[j * LD + i] = 0.5 * (GA[i * LD + j] + GA[j * LD + i]);
       }
       __syncthreads();
-      wg_syev(GA, Q, TM, w, rot, m, LD);
-      if (threadIdx.x < m) lam[threadIdx.x] = w[threadIdx.x];
+      wg_syev2(GA, GB, Q, Z, w, rot, MB, LD);
+      if (threadIdx.x < MB) lam[threadIdx.x] = w[threadIdx.x];
       __syncthreads();
-      wg_transform(X, n, m, Q, LD, m);
-      wg_transform(AX, n, m, Q, LD, m);
-      int np = 0;
+      wg_transform<MB>(X, n, Q, LD, MB, MB);
+      wg_transform<MB>(AX, n, Q, LD, MB, MB);
+      np = 0;
       status = 1;
       for (it = 0; it < a.maxit; ++it) {
-        // ---- residuals R = AX - M X Lambda (into W) and their M^-1 norms ----
-        {
-          double acc[kLobM];
-#pragma unroll
-          for (int j = 0; j < kLobM; ++j) acc[j] = 0.0;
-          for (int r = threadIdx.x; r < n; r += blockDim.x) {
-            const double mr = mass[r];
-#pragma unroll
-            for (int j = 0; j < kLobM; ++j) {
-              if (j < m) {
-                const double rv = AX[(size_t)j * n + r] - lam[j] * mr * X[(size_t)j * n + r];
-                W[(size_t)j * n + r] = rv;
-                acc[j] += rv * rv / mr;
-              }
-            }
-          }
-          wg_reduce_many(acc, m, red, tmpout);
-          if (threadIdx.x < m) rho[threadIdx.x] = sqrt(tmpout[threadIdx.x]);
-          __syncthreads();
+        // ---- W = T (AX - M X Lambda), deflated ----
+        for (int e = threadIdx.x; e < MB * n; e += blockDim.x) {
+          const int j = e / n, r = e % n;
+          W[e] = AX[e] - lam[j] * mass[r] * X[e];
         }
-        // ---- W = T R, deflated against the constant vector ----
-        wg_precondition(b, ps, pc, a.pf, a.pb, W, m, pv);
-        wg_multidot(W, m, mass, 1, nullptr, n, red, tmpout, TM, 1);
-        for (int e = threadIdx.x; e < m * n; e += blockDim.x) W[e] -= TM[e / n] / msum;
         __syncthreads();
-        // ---- convergence: ||T r||_M <= tol for the wanted pairs.  With T
-        // approximating (A^N + sigma M)^{-1} this is the residual of the
-        // shift-inverted problem relative to its eigenvalue, the quantity
-        // SLEPc's relative test bounds under STSINVERT. ----
-        {
-          double acc[kLobM];
-#pragma unroll
-          for (int j = 0; j < kLobM; ++j) acc[j] = 0.0;
-          for (int r = threadIdx.x; r < n; r += blockDim.x) {
-            const double mr = mass[r];
-#pragma unroll
-            for (int j = 0; j < kLobM; ++j)
-              if (j < m) {
-                const double wv = W[(size_t)j * n + r];
-                acc[j] += wv * wv * mr;
-              }
-          }
-          wg_reduce_many(acc, m, red, tmpout);
-          if (threadIdx.x == 0) {
-            int conv = 1;
-            for (int j = 0; j < kw; ++j)
-              if (!(sqrt(tmpout[j]) <= a.tol)) conv = 0;
-            s_conv = conv;
-            if (sidx == a.debug)
-              printf("it %d lam0 %.6e lam%d %.6e rho0 %.3e rho%d %.3e Tr0 %.3e Tr%d %.3e np %d\n",
-                     it, lam[0], kw - 1, lam[kw - 1], rho[0], kw - 1, rho[kw - 1], sqrt(tmpout[0]),
-                     kw - 1, sqrt(tmpout[kw - 1]), np);
-          }
-          __syncthreads();
+        PMX_PHASE(0)
+        wg_precondition(b, v, ps, pc, a.pf, a.pb, W, MB, pv);
+        PMX_PHASE(1)
+        deflate(W);
+        // ---- convergence: ||T r||_M <= tol for the wanted pairs ----
+        wg_colnorm2<MB>(W, mass, n, red, nrm);
+        if (threadIdx.x == 0) {
+          int conv = 1;
+          for (int j = 0; j < kw; ++j)
+            if (!(sqrt(nrm[j]) <= a.tol)) conv = 0;
+          s_conv = conv;
+          if (sidx == a.debug && (it < 3 || it % 10 == 0))
+            printf("it %d lam0 %.6e lam%d %.6e Tr0 %.3e Tr%d %.3e np %d\n", it, lam[0], kw - 1,
+                   lam[kw - 1], sqrt(nrm[0]), kw - 1, sqrt(nrm[kw - 1]), np);
         }
+        __syncthreads();
+        PMX_PHASE(2)
         if (s_conv) {
           status = 0;
           break;
         }
-        // ---- W -= X (X^T M W), P (P^T M W) ----
-        wg_multidot(X, m, W, m, mass, n, red, tmpout, GB, LD);
-        if (np > 0) wg_multidot(P, np, W, m, mass, n, red, tmpout, GB + m * LD, LD);
+        // ---- W -= X (X^T M W) + P (P^T M W) ----
+        wg_multidot<MB, MB>(X, W, mass, n, red, GB, LD);
+        if (np > 0) wg_multidot<MB, MB>(P, W, mass, n, red, GB + MB * LD, LD);
         for (int r = threadIdx.x; r < n; r += blockDim.x) {
-          double xv[kLobM], pv2[kLobM];
+          double xv[MB], pv2[MB];
 #pragma unroll
-          for (int i = 0; i < kLobM; ++i) {
-            xv[i] = i < m ? X[(size_t)i * n + r] : 0.0;
+          for (int i = 0; i < MB; ++i) {
+            xv[i] = X[(size_t)i * n + r];
             pv2[i] = i < np ? P[(size_t)i * n + r] : 0.0;
           }
 #pragma unroll
-          for (int j = 0; j < kLobM; ++j) {
-            if (j >= m) break;
+          for (int j = 0; j < MB; ++j) {
             double s = W[(size_t)j * n + r];
 #pragma unroll
-            for (int i = 0; i < kLobM; ++i) {
-              if (i < m) s -= xv[i] * GB[i * LD + j];
-              if (i < np) s -= pv2[i] * GB[(m + i) * LD + j];
+            for (int i = 0; i < MB; ++i) {
+              s -= xv[i] * GB[i * LD + j];
+              if (i < np) s -= pv2[i] * GB[(MB + i) * LD + j];
             }
             W[(size_t)j * n + r] = s;
           }
         }
         __syncthreads();
+        PMX_PHASE(3)
         // ---- SVQB(W) ----
-        wg_multidot(W, m, W, m, mass, n, red, tmpout, GB, LD);
-        int nw = wg_svqb(GB, Z, TM, Q, w, rot, m, LD, 1e-12);
+        wg_multidot<MB, MB>(W, W, mass, n, red, GB, LD);
+        const int nw = wg_svqb(GB, GA, Z, Q, w, rot, MB, LD, 1e-12);
         if (nw == 0) {
           status = 3;
           break;
         }
-        wg_transform(W, n, m, Z, LD, nw);
-        wg_apply_AN(b, a.Tx, a.Ty, a.Tz, W, AW, nw);
-        // ---- Gram matrices of S = [X W P] ----
-        const int ns = m + nw + np;
-        const double* Sb[3] = {X, W, P};
-        const double* ASb[3] = {AX, AW, AP};
-        const int sz[3] = {m, nw, np};
-        const int so[3] = {0, m, m + nw};
-        for (int bi = 0; bi < 3; ++bi)
-          for (int bj = bi; bj < 3; ++bj) {
-            if (sz[bi] == 0 || sz[bj] == 0) continue;
-            wg_multidot(Sb[bi], sz[bi], ASb[bj], sz[bj], nullptr, n, red, tmpout,
-                        GA + so[bi] * LD + so[bj], LD);
-            wg_multidot(Sb[bi], sz[bi], Sb[bj], sz[bj], mass, n, red, tmpout,
-                        GB + so[bi] * LD + so[bj], LD);
-          }
+        wg_transform<MB>(W, n, Z, LD, MB, nw);
+        PMX_PHASE(4)
+        wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, W, AW, nw);
+        PMX_PHASE(5)
+        // ---- Gram blocks of S = [X W P] (compact, ns = MB + nw + np): the
+        // diagonal blocks of G_B are I and G_A(X,X) = Lambda by construction;
+        // the cross blocks are computed. ----
+        const int ns = MB + nw + np;
         for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
           const int i = e / ns, j = e % ns;
-          // blocks were computed for block-row <= block-col; mirror and
-          // symmetrise the diagonal blocks
-          const int bi = i < m ? 0 : (i < m + nw ? 1 : 2);
-          const int bj = j < m ? 0 : (j < m + nw ? 1 : 2);
-          if (bi > bj) {
-            GA[i * LD + j] = GA[j * LD + i];
-            GB[i * LD + j] = GB[j * LD + i];
-          }
+          GA[i * LD + j] = (i == j && i < MB) ? lam[i] : 0.0;
+          GB[i * LD + j] = i == j ? 1.0 : 0.0;
         }
         __syncthreads();
+        gram(X, MB, AW, nw, nullptr, GA, 0, MB);
+        gram(W, nw, AW, nw, nullptr, GA, MB, MB);
+        gram(X, MB, W, nw, mass, GB, 0, MB);
+        if (np > 0) {
+          gram(X, MB, AP, np, nullptr, GA, 0, MB + nw);
+          gram(W, nw, AP, np, nullptr, GA, MB, MB + nw);
+          gram(P, np, AP, np, nullptr, GA, MB + nw, MB + nw);
+          gram(X, MB, P, np, mass, GB, 0, MB + nw);
+          gram(W, nw, P, np, mass, GB, MB, MB + nw);
+        }
         for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
           const int i = e / ns, j = e % ns;
           if (j > i) {
-            const double ga = 0.5 * (GA[i * LD + j] + GA[j * LD + i]);
-            const double gb = 0.5 * (GB[i * LD + j] + GB[j * LD + i]);
-            GA[i * LD + j] = GA[j * LD + i] = ga;
-            GB[i * LD + j] = GB[j * LD + i] = gb;
+            GA[j * LD + i] = GA[i * LD + j];
+            GB[j * LD + i] = GB[i * LD + j];
           }
         }
         __syncthreads();
-        // ---- Rayleigh-Ritz: SVQB of G_B, then eig of Z^T G_A Z ----
-        const int nk = wg_svqb(GB, Z, TM, Q, w, rot, ns, LD, 1e-13);
-        if (nk < m) {
+        PMX_PHASE(6)
+        // ---- Rayleigh-Ritz: G_B = L L^T, H = L^{-1} G_A L^{-T} = Q w Q^T,
+        // C = L^{-T} Q(:, 0:MB) ----
+        if (!wg_cholesky(GB, ns, LD, &s_flag)) {
+          np = 0;
           status = 4;
-          break;
+          continue;
         }
-        // H = Z^T G_A Z  (TM = G_A Z, then GB = Z^T TM)
-        for (int e = threadIdx.x; e < ns * nk; e += blockDim.x) {
-          const int i = e / nk, c = e % nk;
+        wg_trinv_lower(GB, TM, ns, LD);      // GB = L^{-1}
+        wg_matmul(GB, GA, TM, ns, LD, false);  // TM = L^{-1} G_A
+        for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+          const int i = e / ns, j = e % ns;
           double s = 0.0;
-          for (int k = 0; k < ns; ++k) s += GA[i * LD + k] * Z[k * LD + c];
+          for (int k = 0; k <= j; ++k) s += TM[i * LD + k] * GB[j * LD + k];
+          Z[i * LD + j] = s;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+          const int i = e / ns, j = e % ns;
+          if (j > i) Z[i * LD + j] = Z[j * LD + i] = 0.5 * (Z[i * LD + j] + Z[j * LD + i]);
+        }
+        __syncthreads();
+        wg_syev2(Z, TM, Q, GA, w, rot, ns, LD);  // eigenvectors in Q
+        for (int e = threadIdx.x; e < ns * MB; e += blockDim.x) {
+          const int i = e / MB, c = e % MB;
+          double s = 0.0;
+          for (int k = i; k < ns; ++k) s += GB[k * LD + i] * Q[k * LD + c];
           TM[i * LD + c] = s;
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < nk * nk; e += blockDim.x) {
-          const int i = e / nk, c = e % nk;
-          double s = 0.0;
-          for (int k = 0; k < ns; ++k) s += Z[k * LD + i] * TM[k * LD + c];
-          GB[i * LD + c] = s;
-        }
+        for (int e = threadIdx.x; e < ns * MB; e += blockDim.x)
+          GA[(e / MB) * LD + e % MB] = TM[(e / MB) * LD + e % MB];
+        if (threadIdx.x < MB) lam[threadIdx.x] = w[threadIdx.x];
         __syncthreads();
-        for (int e = threadIdx.x; e < nk * nk; e += blockDim.x) {
-          const int i = e / nk, c = e % nk;
-          if (c > i) GB[i * LD + c] = GB[c * LD + i] = 0.5 * (GB[i * LD + c] + GB[c * LD + i]);
-        }
-        __syncthreads();
-        wg_syev(GB, Q, TM, w, rot, nk, LD);
-        // C = Z Q(:, 0:m)  -> GA (GA no longer needed)
-        for (int e = threadIdx.x; e < ns * m; e += blockDim.x) {
-          const int i = e / m, c = e % m;
-          double s = 0.0;
-          for (int k = 0; k < nk; ++k) s += Z[i * LD + k] * Q[k * LD + c];
-          GA[i * LD + c] = s;
-        }
-        if (threadIdx.x < m) lam[threadIdx.x] = w[threadIdx.x];
-        __syncthreads();
+        PMX_PHASE(7)
         // ---- X = S C, P = [W P] C_WP; same for the A-images ----
         for (int pass = 0; pass < 2; ++pass) {
           double* B0 = pass == 0 ? X : AX;
           const double* B1 = pass == 0 ? W : AW;
           double* B2 = pass == 0 ? P : AP;
           for (int r = threadIdx.x; r < n; r += blockDim.x) {
-            double v0[kLobM], v1[kLobM], v2[kLobM];
+            double v0[MB], v1[MB], v2[MB];
 #pragma unroll
-            for (int k = 0; k < kLobM; ++k) {
-              v0[k] = k < m ? B0[(size_t)k * n + r] : 0.0;
+            for (int k = 0; k < MB; ++k) {
+              v0[k] = B0[(size_t)k * n + r];
               v1[k] = k < nw ? B1[(size_t)k * n + r] : 0.0;
               v2[k] = k < np ? B2[(size_t)k * n + r] : 0.0;
             }
 #pragma unroll
-            for (int c = 0; c < kLobM; ++c) {
-              if (c >= m) break;
+            for (int c = 0; c < MB; ++c) {
               double sx = 0.0, sp = 0.0;
 #pragma unroll
-              for (int k = 0; k < kLobM; ++k) {
-                if (k < m) sx += v0[k] * GA[k * LD + c];
-                if (k < nw) sp += v1[k] * GA[(m + k) * LD + c];
-                if (k < np) sp += v2[k] * GA[(m + nw + k) * LD + c];
+              for (int k = 0; k < MB; ++k) {
+                sx += v0[k] * GA[k * LD + c];
+                if (k < nw) sp += v1[k] * GA[(MB + k) * LD + c];
+                if (k < np) sp += v2[k] * GA[(MB + nw + k) * LD + c];
               }
               B0[(size_t)c * n + r] = sx + sp;
               B2[(size_t)c * n + r] = sp;
@@ -528,61 +628,79 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
           }
           __syncthreads();
         }
-        // ---- keep P M-orthogonal to X and M-orthonormal (robust LOBPCG):
-        // P -= X (X^T M P), AP -= AX (X^T M P), then SVQB(P) ----
-        wg_multidot(X, m, P, m, mass, n, red, tmpout, GB, LD);
+        PMX_PHASE(8)
+        // ---- P -= X (X^T M P), AP -= AX (X^T M P); SVQB(P) ----
+        wg_multidot<MB, MB>(X, P, mass, n, red, GB, LD);
         for (int r = threadIdx.x; r < n; r += blockDim.x) {
-          double xv[kLobM], axv[kLobM];
+          double xv[MB], axv[MB];
 #pragma unroll
-          for (int i = 0; i < kLobM; ++i) {
-            xv[i] = i < m ? X[(size_t)i * n + r] : 0.0;
-            axv[i] = i < m ? AX[(size_t)i * n + r] : 0.0;
+          for (int i = 0; i < MB; ++i) {
+            xv[i] = X[(size_t)i * n + r];
+            axv[i] = AX[(size_t)i * n + r];
           }
 #pragma unroll
-          for (int j = 0; j < kLobM; ++j) {
-            if (j >= m) break;
+          for (int j = 0; j < MB; ++j) {
             double sp = P[(size_t)j * n + r], sa = AP[(size_t)j * n + r];
 #pragma unroll
-            for (int i = 0; i < kLobM; ++i)
-              if (i < m) {
-                sp -= xv[i] * GB[i * LD + j];
-                sa -= axv[i] * GB[i * LD + j];
-              }
+            for (int i = 0; i < MB; ++i) {
+              sp -= xv[i] * GB[i * LD + j];
+              sa -= axv[i] * GB[i * LD + j];
+            }
             P[(size_t)j * n + r] = sp;
             AP[(size_t)j * n + r] = sa;
           }
         }
         __syncthreads();
-        wg_multidot(P, m, P, m, mass, n, red, tmpout, GB, LD);
-        np = wg_svqb(GB, Z, TM, Q, w, rot, m, LD, 1e-12);
+        wg_multidot<MB, MB>(P, P, mass, n, red, GB, LD);
+        np = wg_svqb(GB, GA, Z, Q, w, rot, MB, LD, 1e-12);
         if (np > 0) {
-          wg_transform(P, n, m, Z, LD, np);
-          wg_transform(AP, n, m, Z, LD, np);
+          wg_transform<MB>(P, n, Z, LD, MB, np);
+          wg_transform<MB>(AP, n, Z, LD, MB, np);
         }
+        PMX_PHASE(9)
       }
       if (status == 0) break;
-      if (attempt == 0 && a.x0 == nullptr) break;  // random start already failed
+      if (attempt == 0 && a.x0 == nullptr && status != 2) break;
     }
-  } else {
-    status = 0;
   }
+  if (trace) {
+    printf("lobpcg phases (us) its %d: resid %.0f prec %.0f conv %.0f proj %.0f svqbW %.0f AW %.0f "
+           "gram %.0f rr %.0f update %.0f portho %.0f\n",
+           it, tph[0] * 0.01, tph[1] * 0.01, tph[2] * 0.01, tph[3] * 0.01, tph[4] * 0.01,
+           tph[5] * 0.01, tph[6] * 0.01, tph[7] * 0.01, tph[8] * 0.01, tph[9] * 0.01);
+  }
+#undef PMX_PHASE
   // ---- outputs ----
   const double phi0 = 1.0 / sqrt(msum);
-  double* phi = a.phi + (size_t)b.off * a.kmax;
+  double* phi = a.phi + (size_t)v.off * a.kmax;
   for (int r = threadIdx.x; r < n; r += blockDim.x) phi[(size_t)r * a.kmax] = phi0;
   if (kw > 0 && status == 0) {
     // sign: positive inner product with 1 + frac(0.618..(r+1))
-    double acc[kLobM];
-    for (int j = 0; j < kLobM; ++j) acc[j] = 0.0;
+    double acc[MB];
+#pragma unroll
+    for (int j = 0; j < MB; ++j) acc[j] = 0.0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
       const double wgt = 1.0 + fmod(0.6180339887498949 * (r + 1.0), 1.0);
-      for (int j = 0; j < kw; ++j) acc[j] += wgt * X[(size_t)j * n + r];
+#pragma unroll
+      for (int j = 0; j < MB; ++j) acc[j] += wgt * X[(size_t)j * n + r];
     }
-    wg_reduce_many(acc, kw, red, tmpout);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < MB; ++j) {
+      double s = acc[j];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[wv * MB + j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < MB) {
+      double s = 0.0;
+      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q * MB + threadIdx.x];
+      nrm[threadIdx.x] = s < 0.0 ? -1.0 : 1.0;
+    }
+    __syncthreads();
     for (int e = threadIdx.x; e < kw * n; e += blockDim.x) {
       const int j = e / n, r = e % n;
-      const double sg = tmpout[j] < 0.0 ? -1.0 : 1.0;
-      phi[(size_t)r * a.kmax + 1 + j] = sg * X[(size_t)j * n + r];
+      phi[(size_t)r * a.kmax + 1 + j] = nrm[j] * X[(size_t)j * n + r];
     }
   }
   if (threadIdx.x == 0) {
@@ -593,17 +711,39 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   }
 }
 
+template <int MB>
+static size_t lobpcg_lds(int maxn) {
+  constexpr int LD = 3 * MB;
+  const size_t small = (5 * LD * LD + LD + 3 * LD + 4 * MB * MB + 64) * sizeof(double);
+  const size_t prec = (size_t)maxn * kLobCh * sizeof(float);
+  return std::max(small, prec);
+}
+
+template <int MB>
+static void launch_lobpcg_mb(pmx_ctx* ctx, const LobArgs& a, int maxn) {
+  hipLaunchKernelGGL(k_lobpcg<MB>, dim3(ctx->nsub), dim3(kBlock), lobpcg_lds<MB>(maxn),
+                     ctx->stream, ctx->geo, a);
+}
+
+int lobpcg_block_size(int en_lv1, int guard) {
+  const int mb = en_lv1 - 1 + guard;
+  return mb <= 4 ? 4 : (mb <= 6 ? 6 : (mb <= 8 ? 8 : (mb <= 10 ? 10 : 0)));
+}
+
 int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   const int kw = ctx->en_lv1 - 1;
-  const int m = std::min(kLobM, kw + ctx->eig_guard);
-  if (kw > kLobM - 1) {
-    set_error("en_lv1=%d exceeds the LOBPCG block limit %d", ctx->en_lv1, kLobM);
+  const int MB = lobpcg_block_size(ctx->en_lv1, ctx->eig_guard);
+  if (MB == 0) {
+    set_error("en_lv1=%d exceeds the LOBPCG block limit", ctx->en_lv1);
     return PMX_ERR_SUP;
   }
-  int maxn = 0;
-  for (const Brick& b : ctx->h_bricks) maxn = std::max(maxn, b.nx * b.ny * b.nz);
-  if (m > maxn - 1) {
-    set_error("subdomains of %d cells are too small for %d eigenvectors", maxn, ctx->en_lv1);
+  int maxn = 0, minn = 1 << 30;
+  for (const Brick& b : ctx->h_bricks) {
+    maxn = std::max(maxn, b.nx * b.ny * b.nz);
+    minn = std::min(minn, b.nx * b.ny * b.nz);
+  }
+  if (MB > minn - 1) {
+    set_error("subdomains of %d cells are too small for %d eigenvectors", minn, ctx->en_lv1);
     return PMX_ERR_ARG_WRONG;
   }
   // LOBPCG preconditioner: FP32 block ICC(0) of A^N + sigma M.
@@ -611,27 +751,27 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   PMX_TRY(pf.alloc((size_t)ctx->ncell * 4));
   PMX_TRY(pb.alloc((size_t)ctx->ncell * 4));
   PMX_TRY(launch_dic_factor(ctx, 1, ctx->eig_sigma, pf.p, pb.p, true));
-  DBuf<double> buf;
-  const size_t blk = (size_t)ctx->ncell * std::max(m, 1);
-  PMX_TRY(buf.alloc(6 * blk));
+  const size_t blk = (size_t)ctx->ncell * MB;
+  PMX_TRY(ctx->lob_buf.alloc(6 * blk));
+  PMX_HIP(hipMemsetAsync(ctx->lob_buf.p, 0, 6 * blk * sizeof(double), ctx->stream));
   DBuf<double> eig;
   DBuf<int> its, status;
   PMX_TRY(eig.alloc((size_t)ctx->nsub * ctx->en_lv1));
   PMX_TRY(its.alloc(ctx->nsub));
   PMX_TRY(status.alloc(ctx->nsub));
   LobArgs a;
-  a.m = m;
   a.kwant = kw;
   a.maxit = ctx->eig_maxit;
   a.kmax = ctx->en_lv1;
   a.tol = ctx->eig_tol;
   a.seed = ctx->seed;
-  a.X = buf.p;
-  a.AX = buf.p + blk;
-  a.W = buf.p + 2 * blk;
-  a.AW = buf.p + 3 * blk;
-  a.P = buf.p + 4 * blk;
-  a.AP = buf.p + 5 * blk;
+  double* buf = ctx->lob_buf.p;
+  a.X = buf;
+  a.AX = buf + blk;
+  a.W = buf + 2 * blk;
+  a.AW = buf + 3 * blk;
+  a.P = buf + 4 * blk;
+  a.AP = buf + 5 * blk;
   a.pf = (const float4*)pf.p;
   a.pb = (const float4*)pb.p;
   a.Tx = ctx->T[0].p;
@@ -644,11 +784,20 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   a.its = its.p;
   a.status = status.p;
   a.debug = ctx->eig_debug;
-  const size_t small = (5 * kLobS * kLobS + kLobS + 4 * kLobS + 4 * kLobM * kLobM + kLobM * kLobM) *
-                       sizeof(double);
-  const size_t prec = (size_t)maxn * kLobCh * sizeof(float);
-  const size_t lds = std::max(small, prec);
-  hipLaunchKernelGGL(k_lobpcg, dim3(ctx->nsub), dim3(kBlock), lds, ctx->stream, ctx->geo, a);
+  switch (MB) {
+    case 4:
+      launch_lobpcg_mb<4>(ctx, a, maxn);
+      break;
+    case 6:
+      launch_lobpcg_mb<6>(ctx, a, maxn);
+      break;
+    case 8:
+      launch_lobpcg_mb<8>(ctx, a, maxn);
+      break;
+    default:
+      launch_lobpcg_mb<10>(ctx, a, maxn);
+      break;
+  }
   PMX_HIP(hipGetLastError());
   std::vector<int> hits(ctx->nsub), hst(ctx->nsub);
   ctx->h_eig_lv1.resize((size_t)ctx->nsub * ctx->en_lv1);
