@@ -13,7 +13,7 @@ Typical sequence (the one of ``src/Expe_scal.c:66-160``)::
     diag, off = PC_create_A(ctx)
     pc = PCShell(ctx)
     PC_setup(pc)
-    x, its, rnorm = KSPSolve(pc, b, rtol=1e-8)
+    x, its, rnorm, reason = KSPSolve(pc, b, rtol=1e-8)
     PC_print_stat(ctx)
     PC_final(ctx)
 """
