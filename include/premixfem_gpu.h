@@ -57,7 +57,9 @@ extern "C" {
 #define PMX_CONVERGED_ATOL 3
 #define PMX_DIVERGED_ITS (-3)
 #define PMX_DIVERGED_DTOL (-4)
-#define PMX_DIVERGED_INDEFINITE_MAT (-8)
+#define PMX_DIVERGED_INDEFINITE_PC (-8)
+#define PMX_DIVERGED_NANORINF (-9)
+#define PMX_DIVERGED_INDEFINITE_MAT (-10)
 
 typedef struct pmx_ctx pmx_ctx;
 

@@ -19,7 +19,8 @@ ERRORS = {
     83: "PETSC_ERR_USER", 85: "PETSC_ERR_ARG_NULL", 91: "PETSC_ERR_NOT_CONVERGED",
 }
 REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS",
-           -4: "DIVERGED_DTOL", -8: "DIVERGED_INDEFINITE_MAT", 0: "ITERATING"}
+           -4: "DIVERGED_DTOL", -8: "DIVERGED_INDEFINITE_PC",
+           -9: "DIVERGED_NANORINF", -10: "DIVERGED_INDEFINITE_MAT", 0: "ITERATING"}
 
 # Every entry point include/premixfem_gpu.h declares.
 EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_setup",

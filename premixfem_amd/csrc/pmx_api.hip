@@ -3,6 +3,7 @@
 // V-cycle (PC_apply_vec) and the GPU-resident PCG (PETSc KSPSolve_CG).
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <cstdarg>
 #include <cstring>
 #include <map>
@@ -120,6 +121,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
     } else if (k == "-cg_chunk") {
       PMX_TRY(geti(c->cg_chunk));
       c->cg_chunk = std::max(1, c->cg_chunk);
+    } else if (k == "-cg_graph") {
+      PMX_TRY(geti(c->cg_graph));
     } else if (k == "-kernel_timing") {
       c->kernel_timing = 1;
     } else if (k == "-log_stages") {
@@ -177,6 +180,7 @@ static int build_geometry(pmx_ctx* c) {
   c->h_bricks.assign(c->nsub, Brick{});
   std::map<std::tuple<int, int, int>, int> shapes;
   c->h_plane_start.clear();
+  c->max_plane = 0;
   c->h_plane_cell.clear();
   c->h_shape_pstart.clear();
   c->h_shape_coff.clear();
@@ -212,6 +216,7 @@ static int build_geometry(pmx_ctx* c) {
           int cnt = 0;
           for (int p = 0; p < b.nplanes; ++p) {
             c->h_plane_start.push_back(cnt);
+            const int cnt0 = cnt;
             for (int lz = 0; lz < b.nz; ++lz)
               for (int ly = 0; ly < b.ny; ++ly) {
                 const int lx = p - ly - lz;
@@ -219,6 +224,7 @@ static int build_geometry(pmx_ctx* c) {
                 c->h_plane_cell.push_back(lx | (ly << 10) | (lz << 20));
                 ++cnt;
               }
+            c->max_plane = std::max(c->max_plane, cnt - cnt0);
           }
           c->h_plane_start.push_back(cnt);
           b.shape = id;
@@ -285,7 +291,7 @@ static int alloc_work(pmx_ctx* c) {
     PMX_TRY(c->w_y32.alloc(n));
     PMX_TRY(c->w_y32b.alloc(n));
   }
-  PMX_TRY(c->partial.alloc(std::max<size_t>(c->nsub, 4096)));
+  PMX_TRY(c->partial.alloc(std::max<size_t>(6 * (size_t)c->nsub, 4096)));
   PMX_TRY(c->scal.alloc(16));
   PMX_HIP(hipMemset(c->scal.p, 0, 16 * sizeof(double)));
   return PMX_SUCCESS;
@@ -451,7 +457,13 @@ static int kt_collect(pmx_ctx* c) {
 // t = Phi^T (x - A y): a stage with correction d updates t -= A_c d (since
 // A_c = Phi^T A Phi) and accumulates u += d, and y += Phi u is applied once.
 // z receives the output; partial (if not null) the per-brick z.x sums.
-static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial) {
+// cg_partial: the last kernel writes the z'r partials of the PCG.  cg_upd:
+// the first level-1 kernel also performs the x/r update of the current PCG
+// iteration (x is then the PCG residual, xs/p/w the CG vectors) and the
+// prolongation advances the device iteration counter.
+static int apply_device(pmx_ctx* c, double* x, double* z, bool cg_partial, bool cg_upd = false,
+                        double* xs = nullptr, const double* p = nullptr,
+                        const double* w = nullptr) {
   const bool f32 = c->pc_precision == 32;
   auto tick = [&](int stage, double& t) -> int {
     if (!c->log_stages) return PMX_SUCCESS;
@@ -465,10 +477,10 @@ static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial)
   const double t_start = t;
   // Pre-smoothing: y = B1 x, then si_lv1-1 residual-corrected sweeps.
   int ev = kt_begin(c, 0);
-  PMX_TRY(launch_lv1_first(c, x));
+  PMX_TRY(launch_lv1_first(c, x, cg_upd, xs, p, w));
   kt_end(c, ev);
   for (int i = 1; i < c->si_lv1; ++i) {
-    PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
+    PMX_TRY(launch_lv1_last(c, x, nullptr, true, false));
   }
   PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
   ev = kt_begin(c, 2);
@@ -483,12 +495,12 @@ static int apply_device(pmx_ctx* c, const double* x, double* z, double* partial)
   kt_end(c, ev);
   PMX_TRY(tick(PMX_STAGE_AV_LV2, t));
   ev = kt_begin(c, 3);
-  PMX_TRY(launch_prolong(c, u));
+  PMX_TRY(launch_prolong(c, u, cg_upd));
   kt_end(c, ev);
   // Post-smoothing.
-  for (int i = 1; i < c->si_lv1; ++i) PMX_TRY(launch_lv1_last(c, x, nullptr, true, nullptr));
+  for (int i = 1; i < c->si_lv1; ++i) PMX_TRY(launch_lv1_last(c, x, nullptr, true, false));
   ev = kt_begin(c, 1);
-  PMX_TRY(launch_lv1_last(c, x, z, false, partial));
+  PMX_TRY(launch_lv1_last(c, x, z, false, cg_partial));
   kt_end(c, ev);
   PMX_TRY(tick(PMX_STAGE_AV_LV1, t));
   (void)f32;
@@ -588,11 +600,28 @@ int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* opti
     set_error("no HIP device available");
     return fail(PMX_ERR_LIB);
   }
+  if (hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+      hipSuccess) {
+    set_error("hipDeviceGetAttribute failed");
+    return fail(PMX_ERR_LIB);
+  }
   if (hipStreamCreate(&c->stream) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return fail(PMX_ERR_LIB);
   }
   rc = build_geometry(c);
+  if (!rc) {
+    // The level-1 kernels keep a brick in LDS (the factorisation in FP64):
+    // bound the subdomain size by the device's LDS.
+    int mx = 0;
+    for (const Brick& b : c->h_bricks) mx = std::max(mx, b.nx * b.ny * b.nz);
+    const size_t need = (size_t)mx * sizeof(double);
+    if (need > (size_t)c->lds_max) {
+      set_error("subdomains of %d cells need %zu bytes of LDS per workgroup (device: %d); use "
+                "more subdomains (-sd)", mx, need, c->lds_max);
+      rc = PMX_ERR_SUP;
+    }
+  }
   if (!rc) rc = alloc_work(c);
   if (rc) {
     pmx_final(c);
@@ -694,6 +723,7 @@ int pmx_create_A(pmx_ctx* c, double* diag, double* off) {
 static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
                         const double* eigvals) {
   std::fill(c->t_stages, c->t_stages + PMX_MAX_LOG_STAGES, 0.0);
+  ++c->setup_gen;  // buffers may move: a captured PCG chunk is stale
   const double t0 = now();
   PMX_TRY(setup_fine(c));
   const double t1 = now();
@@ -750,7 +780,7 @@ int pmx_apply(pmx_ctx* c, const double* x, double* y) {
   PMX_HIP(hipMemcpyAsync(c->nat_buf.p, x, c->ncell * sizeof(double), hipMemcpyHostToDevice,
                          c->stream));
   PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->w_x.p, 1, c->ncell, c->ncell));
-  PMX_TRY(apply_device(c, c->w_x.p, c->w_z.p, nullptr));
+  PMX_TRY(apply_device(c, c->w_x.p, c->w_z.p, false));
   PMX_TRY(launch_brick2nat(c, c->w_z.p, c->nat_buf.p, 1, c->ncell, c->ncell));
   PMX_HIP(hipMemcpyAsync(y, c->nat_buf.p, c->ncell * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
@@ -761,6 +791,38 @@ int pmx_apply(pmx_ctx* c, const double* x, double* y) {
 // PETSc KSPSolve_CG (standard variant, KSP_NORM_UNPRECONDITIONED, default
 // convergence test) on the brick-order vectors w_b (right-hand side) and w_x
 // (initial guess if x0_nonzero, solution on return).
+static void cg_graph_release(pmx_ctx* c) {
+  if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+  if (c->cg_graph_h) (void)hipGraphDestroy(c->cg_graph_h);
+  c->cg_exec = nullptr;
+  c->cg_graph_h = nullptr;
+}
+
+// Captures `chunk` PCG iterations on the context stream into a HIP graph
+// (once per setup and PCG parameter set) and instantiates it.
+static int cg_graph_prepare(pmx_ctx* c, int chunk,
+                            const std::function<int(int)>& enqueue_chunk) {
+  const CgGraphKey key{c->setup_gen, chunk, c->cg_max_it, c->cg_abstol, c->cg_dtol};
+  if (c->cg_exec && key == c->cg_key) return PMX_SUCCESS;
+  cg_graph_release(c);
+  PMX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  const int rc = enqueue_chunk(chunk);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error("PCG: stream capture failed (%s)", hipGetErrorString(e));
+    return PMX_ERR_LIB;
+  }
+  c->cg_graph_h = g;
+  PMX_HIP(hipGraphInstantiate(&c->cg_exec, g, nullptr, nullptr, 0));
+  c->cg_key = key;
+  return PMX_SUCCESS;
+}
+
 static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* its,
                       double* rnorm) {
   c->cg_rtol = rtol;
@@ -791,23 +853,34 @@ static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* 
   c->geo.stop = c->scal.p + kSDone;
   int rc = PMX_SUCCESS;
   if (hs[kSDone] == 0.0) {
-    rc = apply_device(c, r, z, c->partial.p);
-    if (!rc) rc = launch_finalize_beta(c);
-    while (!rc) {
-      for (int q = 0; q < c->cg_chunk && !rc; ++q) {
+    // z = B r0; its z'r partials go to the slot of iteration 0.
+    rc = apply_device(c, r, z, true);
+    const bool graph = c->cg_graph && !c->kernel_timing && !c->log_stages;
+    auto enqueue_chunk = [&](int n) -> int {
+      for (int q = 0; q < n; ++q) {
         int ev = kt_begin(c, 4);
-        rc = launch_cg_p_spmv(c, z, p0, p1, w, 0);
+        PMX_TRY(launch_cg_p_spmv(c, z, p0, p1, w));
         kt_end(c, ev);
-        if (rc) break;
-        ev = kt_begin(c, 5);
-        rc = launch_cg_update(c, xb, r, p1, w);
-        kt_end(c, ev);
-        if (rc) break;
-        rc = apply_device(c, r, z, c->partial.p);
-        if (!rc) rc = launch_finalize_beta(c);
+        PMX_TRY(apply_device(c, r, z, true, true, xb, p1, w));
         std::swap(p0, p1);
       }
-      if (rc) break;
+      return PMX_SUCCESS;
+    };
+    // An even chunk leaves p0/p1 (and the level-1 buffers) where they started,
+    // so one captured chunk can be replayed.
+    const int chunk = graph ? (c->cg_chunk + 1) / 2 * 2 : c->cg_chunk;
+    if (!rc && graph) rc = cg_graph_prepare(c, chunk, enqueue_chunk);
+    while (!rc) {
+      if (graph) {
+        if (hipGraphLaunch(c->cg_exec, c->stream) != hipSuccess) {
+          set_error("PCG: hipGraphLaunch failed");
+          rc = PMX_ERR_LIB;
+          break;
+        }
+      } else {
+        rc = enqueue_chunk(chunk);
+        if (rc) break;
+      }
       if (hipMemcpyAsync(hs, c->scal.p, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream) != hipSuccess ||
           hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1020,6 +1093,7 @@ int pmx_get_coarse_matrix(const pmx_ctx* c, double* out) {
 int pmx_final(pmx_ctx* c) {
   if (!c) return PMX_SUCCESS;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  cg_graph_release(c);
   c->d_bricks.release();
   for (auto& d : c->d_ax) d.release();
   c->d_plane_start.release();

@@ -468,24 +468,43 @@ int launch_coarse_solve(pmx_ctx* ctx, const double* t, double* d, double* ytmp, 
 // ---------------------------------------------------------------------------
 // Coarse-space operators
 // ---------------------------------------------------------------------------
-// y = A x for a dense row-major n x n matrix; one wavefront per row.
-__global__ void k_gemv(const double* __restrict__ A, const double* __restrict__ x,
-                       double* __restrict__ y, int n, const double* stop) {
+// y = A x for a dense row-major n x n matrix.  One workgroup per row: every
+// lane streams an independent slice of the row with 16-byte loads (many loads
+// in flight per lane), then a fixed-order block reduction.  x is re-read by
+// every row and stays in L2.
+constexpr int kGemvBlock = 256;
+
+__global__ void __launch_bounds__(kGemvBlock) k_gemv(const double* __restrict__ A,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ y, int n,
+                                                     const double* stop) {
   PMX_STOP_CHECK(stop);
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= n) return;
+  __shared__ double red[16];
+  const int row = blockIdx.x;
   const double* a = A + (size_t)row * n;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += a[j] * x[j];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) y[row] = s;
+  double s0 = 0.0, s1 = 0.0;
+  if ((n & 1) == 0) {
+    // rows start 16-byte aligned when n is even
+    const double2* a2 = reinterpret_cast<const double2*>(a);
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    const int n2 = n >> 1;
+#pragma unroll 4
+    for (int j = threadIdx.x; j < n2; j += kGemvBlock) {
+      const double2 av = a2[j], xv = x2[j];
+      s0 += av.x * xv.x;
+      s1 += av.y * xv.y;
+    }
+  } else {
+#pragma unroll 4
+    for (int j = threadIdx.x; j < n; j += kGemvBlock) s0 += a[j] * x[j];
+  }
+  const double s = block_sum(s0 + s1, red);
+  if (threadIdx.x == 0) y[row] = s;
 }
 
 int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n) {
-  const int wpb = kBlock / 64;
-  hipLaunchKernelGGL(k_gemv, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream, A, x, y,
-                     n, ctx->geo.stop);
+  hipLaunchKernelGGL(k_gemv, dim3(n), dim3(kGemvBlock), 0, ctx->stream, A, x, y, n,
+                     ctx->geo.stop);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

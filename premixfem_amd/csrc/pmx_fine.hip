@@ -233,72 +233,105 @@ __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
 // forward  w = (r + cx w_{-x} + cy w_{-y} + cz w_{-z}) * dinv
 // backward y = w + (cx' y_{+x} + cy' y_{+y} + cz' y_{+z}) * dinv
 // i.e. y = [(D+L) D^{-1} (D+L^T)]^{-1} r.  Factor data is plane-ordered so
-// the loads of one hyperplane are contiguous; the next plane's factor and
-// coordinates are loaded before the current plane is processed so their
-// latency overlaps the LDS work and the barrier.
+// the loads of one hyperplane are contiguous.  Each thread keeps a ring of
+// kSweepDepth planes of factor data in flight: the loads for plane k+depth
+// are issued right after plane k is processed, so HBM latency overlaps the
+// LDS work and barriers of the planes in between.  The plane table sits in
+// LDS.
 // ---------------------------------------------------------------------------
+constexpr int kSweepDepth = 4;
+
+// Branch-free cell update: a coupling is 0 on a brick face, so the
+// out-of-brick neighbour index is replaced by the cell itself and its term
+// adds an exact zero (same result as skipping it).
 template <typename S, typename F4T, bool FWD>
 __device__ __forceinline__ void dic_cell(const BrickView& v, int pk, const F4T& f, S* lv) {
   const int lx = pk & 1023, ly = (pk >> 10) & 1023, lz = pk >> 20;
   const int li = (lz * v.ny + ly) * v.nx + lx;
   if (FWD) {
-    S acc = lv[li];
-    if (lx > 0) acc += (S)f.y * lv[li - 1];
-    if (ly > 0) acc += (S)f.z * lv[li - v.nx];
-    if (lz > 0) acc += (S)f.w * lv[li - v.nxny];
+    const S a0 = lv[li];
+    const S ax = lv[lx > 0 ? li - 1 : li];
+    const S ay = lv[ly > 0 ? li - v.nx : li];
+    const S az = lv[lz > 0 ? li - v.nxny : li];
+    S acc = a0;
+    acc += (S)f.y * ax;
+    acc += (S)f.z * ay;
+    acc += (S)f.w * az;
     lv[li] = acc * (S)f.x;
   } else {
+    const S ax = lv[lx < v.nx - 1 ? li + 1 : li];
+    const S ay = lv[ly < v.ny - 1 ? li + v.nx : li];
+    const S az = lv[lz < v.nz - 1 ? li + v.nxny : li];
+    const S a0 = lv[li];
     S acc = (S)0;
-    if (lx < v.nx - 1) acc += (S)f.y * lv[li + 1];
-    if (ly < v.ny - 1) acc += (S)f.z * lv[li + v.nx];
-    if (lz < v.nz - 1) acc += (S)f.w * lv[li + v.nxny];
-    lv[li] = lv[li] + acc * (S)f.x;
+    acc += (S)f.y * ax;
+    acc += (S)f.z * ay;
+    acc += (S)f.w * az;
+    lv[li] = a0 + acc * (S)f.x;
   }
 }
 
-template <typename S, typename F4T, bool FWD>
-__device__ __forceinline__ void dic_sweep(const BrickView& v, int nplanes, const int* ps,
-                                          const int* pc, const F4T* __restrict__ fac, S* lv) {
+// BIG: some hyperplane has more cells than the block has threads (the extra
+// cells are processed without prefetch).
+template <typename S, typename F4T, bool FWD, bool BIG>
+__device__ __forceinline__ void dic_sweep(const BrickView& v, int nplanes, const int* sps,
+                                          const int* __restrict__ pc,
+                                          const F4T* __restrict__ fac, S* lv) {
+  constexpr int D = kSweepDepth;
   const int t = threadIdx.x;
-  const int first = FWD ? 0 : nplanes - 1;
-  const int step = FWD ? 1 : -1;
-  F4T fn{};
-  int pkn = 0;
-  {
-    const int pos = ps[first] + t;
-    if (pos < ps[first + 1]) {
-      fn = fac[v.off + pos];
-      pkn = pc[pos];
-    }
+  F4T f[D];
+  int pk[D];
+  bool ok[D];
+  // Loads are issued unconditionally (clamped to a valid cell) so the
+  // compiler can count them across the unrolled steps.
+  auto fetch = [&](int k, F4T& ff, int& pp, bool& okk) {
+    const int kk = k < nplanes ? k : nplanes - 1;
+    const int s = FWD ? kk : nplanes - 1 - kk;
+    const int lo = sps[s], hi = sps[s + 1];
+    okk = k < nplanes && lo + t < hi;
+    const int pos = min(lo + t, hi - 1);
+    ff = fac[v.off + pos];
+    pp = pc[pos];
+  };
+  // Issue order f[0], f[1], ... (the scheduler must not reorder them, or the
+  // waits in the loop become full drains).
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    fetch(d, f[d], pk[d], ok[d]);
+    __builtin_amdgcn_sched_barrier(0);
   }
-  for (int k = 0; k < nplanes; ++k) {
-    const int s = first + step * k;
-    const int lo = ps[s], hi = ps[s + 1];
-    const F4T f = fn;
-    const int pk = pkn;
-    if (k + 1 < nplanes) {
-      const int sn = s + step;
-      const int pos = ps[sn] + t;
-      if (pos < ps[sn + 1]) {
-        fn = fac[v.off + pos];
-        pkn = pc[pos];
+  // The step count is padded to a multiple of D (steps past the last plane
+  // only fetch and synchronise) so every path through the unrolled body
+  // issues the same loads and the waits stay partial.
+  for (int k0 = 0; k0 < nplanes; k0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int k = k0 + d;
+      if (ok[d]) dic_cell<S, F4T, FWD>(v, pk[d], f[d], lv);
+      if (BIG && k < nplanes) {
+        const int s = FWD ? k : nplanes - 1 - k;
+        for (int pos = sps[s] + t + (int)blockDim.x; pos < sps[s + 1]; pos += blockDim.x)
+          dic_cell<S, F4T, FWD>(v, pc[pos], fac[v.off + pos], lv);
       }
+      fetch(k + D, f[d], pk[d], ok[d]);
+      __syncthreads();
     }
-    if (lo + t < hi) dic_cell<S, F4T, FWD>(v, pk, f, lv);
-    for (int pos = lo + t + (int)blockDim.x; pos < hi; pos += blockDim.x)
-      dic_cell<S, F4T, FWD>(v, pc[pos], fac[v.off + pos], lv);
-    __syncthreads();
   }
 }
 
-template <typename S, typename F4T>
-__device__ __forceinline__ void dic_sweeps(const Geo& g, const Brick& b, const BrickView& v,
-                                           const F4T* __restrict__ fac_f,
-                                           const F4T* __restrict__ fac_b, S* lv) {
+// Loads the brick shape's plane table into LDS (sps: nplanes+1 ints).
+__device__ __forceinline__ void load_planes(const Geo& g, const Brick& b, int* sps) {
   const int* ps = g.plane_start + g.shape_pstart[b.shape];
+  for (int i = threadIdx.x; i <= b.nplanes; i += blockDim.x) sps[i] = ps[i];
+}
+
+template <typename S, typename F4T, bool BIG>
+__device__ __forceinline__ void dic_sweeps(const Geo& g, const Brick& b, const BrickView& v,
+                                           const int* sps, const F4T* __restrict__ fac_f,
+                                           const F4T* __restrict__ fac_b, S* lv) {
   const int* pc = g.plane_cell + g.shape_coff[b.shape];
-  dic_sweep<S, F4T, true>(v, b.nplanes, ps, pc, fac_f, lv);
-  dic_sweep<S, F4T, false>(v, b.nplanes, ps, pc, fac_b, lv);
+  dic_sweep<S, F4T, true, BIG>(v, b.nplanes, sps, pc, fac_f, lv);
+  dic_sweep<S, F4T, false, BIG>(v, b.nplanes, sps, pc, fac_b, lv);
 }
 
 // (A x)_c for one cell: diag*x_c - sum_faces T x_nb, diag = sum of the
@@ -325,157 +358,111 @@ __device__ __forceinline__ ACC stencil_fast(const Geo& g, const BrickView& v, in
   return diag * (ACC)x[c] - acc;
 }
 
-// Stage "lv1" of PC_apply_vec with y = 0 (so r = x exactly): y = B1 x.
-template <typename S, typename F4T>
-__global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, const double* __restrict__ x,
-                                                      S* __restrict__ y,
-                                                      const F4T* __restrict__ fac_f,
-                                                      const F4T* __restrict__ fac_b) {
-  PMX_STOP_CHECK(g.stop);
-  extern __shared__ char shraw[];
-  S* lv = reinterpret_cast<S*>(shraw);
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
-  for (int li = threadIdx.x; li < v.n; li += blockDim.x) lv[li] = (S)x[v.off + li];
-  __syncthreads();
-  dic_sweeps<S, F4T>(g, b, v, fac_f, fac_b, lv);
-  for (int li = threadIdx.x; li < v.n; li += blockDim.x) y[v.off + li] = lv[li];
+// ---------------------------------------------------------------------------
+// PCG scalars without separate reduction launches.
+//
+// Every kernel that needs a scalar of the CG recurrence (beta = z'r,
+// dpi = p'w, ||r||^2) sums the per-brick partials of the kernel that
+// produced them itself: wave 0 of each block adds them in a fixed order, so
+// every block obtains the bitwise same value and takes the same decision.
+// Partials of z'r and p'w alternate between two slots by iteration parity
+// so the previous value (betaold, dpiold) is still there.  Block 0 records
+// the outcome (iterations, reason, norms) in scal[] for the host.
+// ---------------------------------------------------------------------------
+// The index i of the current iteration lives on the device (scal[kSIter]):
+// the prolongation of iteration i advances it to i+1 (no kernel that reads
+// it runs concurrently with that write), so a captured chunk of iterations
+// replays without host-side arguments.
+struct CgCtl {
+  double* scal;
+  double* zr;  // [2][nparts]
+  double* pw;  // [2][nparts]
+  double* rr;  // [nparts]
+  int nparts;
+  double abstol, dtol;
+  int max_it;
+};
+
+// Fixed-order sum of n doubles by one wave (lanes stride through the array,
+// then a butterfly); the result is valid in every lane of the calling wave.
+__device__ __forceinline__ double wave_sum(const double* __restrict__ p, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) s += p[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
 }
 
-// Final lv1 stage: r = x - A y, z = y + B1 r, and partial sums of z.x
-// (the PCG's beta = z'r, the PC input x being the PCG residual).
-template <typename S, typename TT, typename F4T>
-__global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __restrict__ x,
-                                                     const S* __restrict__ y,
-                                                     const TT* __restrict__ Tx,
-                                                     const TT* __restrict__ Ty,
-                                                     const TT* __restrict__ Tz,
-                                                     const F4T* __restrict__ fac_f,
-                                                     const F4T* __restrict__ fac_b,
-                                                     double* __restrict__ z, S* __restrict__ yout,
-                                                     double* __restrict__ partial) {
-  PMX_STOP_CHECK(g.stop);
-  extern __shared__ char shraw[];
-  S* lv = reinterpret_cast<S*>(shraw);
-  __shared__ double red[8];
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
-  for_cells(v, [&](int c, int lx, int ly, int lz) {
-    const S ay = stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
-    lv[c - v.off] = (S)x[c] - ay;
-  });
-  __syncthreads();
-  dic_sweeps<S, F4T>(g, b, v, fac_f, fac_b, lv);
-  double acc = 0.0;
-  for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
-    const int c = v.off + li;
-    const S yn = y[c] + lv[li];
-    if (z) {
-      const double zz = (double)yn;
-      z[c] = zz;
-      acc += zz * x[c];
-    }
-    if (yout) yout[c] = yn;
-  }
-  if (partial) {
-    acc = block_sum(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+__device__ __forceinline__ void cg_finish(const CgCtl& c, int its, int reason, double rnorm2) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    c.scal[kSIts] = its;
+    c.scal[kSReason] = reason;
+    if (rnorm2 >= 0.0) c.scal[kSRnorm2] = rnorm2;
+    c.scal[kSDone] = 1.0;
   }
 }
 
-// r = x - A y and c[dof+j] = sum_cells phi[cell][j] r[cell]  (the VecMDot of
-// _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1564).  y == nullptr means y = 0.
-template <typename S, typename TT>
-__global__ void __launch_bounds__(kBlock) k_resid_restrict(Geo g, const double* __restrict__ x,
-                                                           const S* __restrict__ y,
-                                                           const TT* __restrict__ Tx,
-                                                           const TT* __restrict__ Ty,
-                                                           const TT* __restrict__ Tz,
-                                                           const S* __restrict__ phi, int kmax,
-                                                           double* __restrict__ cvec) {
+// Iteration i, part 1 (src/Expe_scal.c:114 KSPSolve_CG with
+// KSP_NORM_UNPRECONDITIONED):
+//   i > 0: ||r|| of the previous update against ttol / dtol / max_it;
+//   beta checks (beta = 0 -> CONVERGED_ATOL, sign change -> INDEFINITE_PC);
+//   p = z + (beta/betaold) p (p = z when i = 0), w = A p, partial p'w.
+// p_old is read at neighbours, so p goes to a second buffer.
+__global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
+    Geo g, const double* __restrict__ z, const double* __restrict__ pold,
+    double* __restrict__ pnew, double* __restrict__ w, const double* __restrict__ Tx,
+    const double* __restrict__ Ty, const double* __restrict__ Tz, CgCtl cg) {
   PMX_STOP_CHECK(g.stop);
-  __shared__ double red[4 * kMaxEig];
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
-  const int ke = b.neig;
-  double acc[kMaxEig];
-#pragma unroll
-  for (int j = 0; j < kMaxEig; ++j) acc[j] = 0.0;
-  for_cells(v, [&](int c, int lx, int ly, int lz) {
-    double r = x[c];
-    if (y) r -= (double)stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
-    const S* ph = phi + (size_t)c * kmax;
-#pragma unroll
-    for (int j = 0; j < kMaxEig; ++j)
-      if (j < ke) acc[j] += (double)ph[j] * r;
-  });
-  // wave sums, then the 4 waves in a fixed order
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < kMaxEig; ++j) {
-    if (j < ke) {
-      double s = acc[j];
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if (lane == 0) red[w * kMaxEig + j] = s;
+  __shared__ double sh[4];
+  __shared__ double red[16];
+  const int i = (int)cg.scal[kSIter];
+  if (threadIdx.x < 64) {  // wave 0 (a block may have only one wave)
+    const double s0 = wave_sum(cg.zr + (size_t)(i & 1) * cg.nparts, cg.nparts);
+    const double s1 = i > 0 ? wave_sum(cg.zr + (size_t)((i + 1) & 1) * cg.nparts, cg.nparts) : 0.0;
+    const double s2 = i > 0 ? wave_sum(cg.rr, cg.nparts) : 0.0;
+    if (threadIdx.x == 0) {
+      sh[0] = s0;
+      sh[1] = s1;
+      sh[2] = s2;
     }
   }
   __syncthreads();
-  if (threadIdx.x < ke) {
-    double s = 0.0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q * kMaxEig + threadIdx.x];
-    cvec[b.dof + threadIdx.x] = s;
+  const double beta = sh[0];
+  if (i > 0) {
+    const double rr = sh[2];
+    const double dp = sqrt(rr);
+    int reason = 0;
+    if (dp <= cg.scal[kSTtol])
+      reason = dp < cg.abstol ? PMX_CONVERGED_ATOL : PMX_CONVERGED_RTOL;
+    else if (dp >= cg.dtol * sqrt(cg.scal[kSRnorm0]))
+      reason = PMX_DIVERGED_DTOL;
+    else if (dp != dp)
+      reason = PMX_DIVERGED_NANORINF;
+    else if (i >= cg.max_it)
+      reason = PMX_DIVERGED_ITS;
+    if (reason) {
+      cg_finish(cg, i, reason, rr);
+      return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      cg.scal[kSIts] = i;
+      cg.scal[kSRnorm2] = rr;
+    }
   }
-}
-
-// y += phi c  (the VecMAXPY of _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1578).
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k_prolong(Geo g, S* __restrict__ y,
-                                                    const S* __restrict__ phi, int kmax,
-                                                    const double* __restrict__ cvec) {
-  PMX_STOP_CHECK(g.stop);
-  __shared__ double cs[kMaxEig];
-  const Brick b = g.bricks[blockIdx.x];
-  if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
-  __syncthreads();
-  const int n = b.nx * b.ny * b.nz;
-  const int ke = b.neig;
-  for (int li = threadIdx.x; li < n; li += blockDim.x) {
-    const int c = b.off + li;
-    const S* ph = phi + (size_t)c * kmax;
-    S acc = (S)0;
-#pragma unroll
-    for (int j = 0; j < kMaxEig; ++j)
-      if (j < ke) acc += ph[j] * (S)cs[j];
-    y[c] = y[c] + acc;
+  if (beta == 0.0) {
+    cg_finish(cg, i + 1, PMX_CONVERGED_ATOL, -1.0);
+    return;
   }
-}
-
-// ---------------------------------------------------------------------------
-// PCG kernels (FP64, KSPSolve_CG of PETSc; scalars live on the device).
-// scal[]: 0 beta=z'r, 1 betaold, 2 dpi=p'w, 3 dpiold, 4 rnorm^2, 5 flags
-// ---------------------------------------------------------------------------
-// p_new = z + (beta/betaold) p_old (p_new = z on the first iteration),
-// w = A p_new, partial p_new'w.  p_old is read at neighbours, so p_new goes to
-// a second buffer.
-__global__ void __launch_bounds__(kBlock) k_cg_p_spmv(Geo g, const double* __restrict__ z,
-                                                      const double* __restrict__ pold,
-                                                      double* __restrict__ pnew,
-                                                      double* __restrict__ w,
-                                                      const double* __restrict__ Tx,
-                                                      const double* __restrict__ Ty,
-                                                      const double* __restrict__ Tz,
-                                                      const double* __restrict__ scal, int first_host,
-                                                      double* __restrict__ partial) {
-  PMX_STOP_CHECK(g.stop);
-  __shared__ double red[8];
+  if (i > 0 && beta * sh[1] < 0.0) {
+    cg_finish(cg, i + 1, PMX_DIVERGED_INDEFINITE_PC, -1.0);
+    return;
+  }
+  const bool first = i == 0;
+  const double bc = first ? 0.0 : beta / sh[1];
   const BrickView v = load_view(g, blockIdx.x);
-  // first iteration: p = z (the device iteration counter is 0 before the
-  // first norm test; first_host = 1 forces it when no counter is kept)
-  const bool first = first_host || scal[kSIts] == 0.0;
-  const double bc = first ? 0.0 : scal[0] / scal[1];
   double acc = 0.0;
   const double* TT3[3] = {Tx, Ty, Tz};
-  auto pval = [&](int i) { return first ? z[i] : z[i] + bc * pold[i]; };
+  auto pval = [&](int q) { return first ? z[q] : z[q] + bc * pold[q]; };
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     int nb[6];
     nbr6(v, c, lx, ly, lz, nb);
@@ -497,37 +484,186 @@ __global__ void __launch_bounds__(kBlock) k_cg_p_spmv(Geo g, const double* __res
     acc += pc * wc;
   });
   acc = block_sum(acc, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (threadIdx.x == 0) cg.pw[(size_t)(i & 1) * cg.nparts + blockIdx.x] = acc;
 }
 
-// x += a p, r -= a w, partial r'r with a = beta/dpi.
-__global__ void __launch_bounds__(kBlock) k_cg_update(int n, double* __restrict__ x,
-                                                      double* __restrict__ r,
+// Stage "lv1" of PC_apply_vec with y = 0 (so r = x exactly): y = B1 x.
+// UPD: first fuses part 2 of PCG iteration i: dpi = p'w (sign-change check ->
+// INDEFINITE_MAT), a = beta/dpi, x += a p, r -= a w (r is the PC input x),
+// partial ||r||^2.
+template <typename S, typename F4T, bool UPD, bool BIG>
+__global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, double* __restrict__ x,
+                                                      S* __restrict__ y,
+                                                      const F4T* __restrict__ fac_f,
+                                                      const F4T* __restrict__ fac_b,
+                                                      double* __restrict__ xs,
                                                       const double* __restrict__ p,
-                                                      const double* __restrict__ w,
-                                                      const double* __restrict__ scal,
-                                                      double* __restrict__ partial) {
-  PMX_STOP_CHECK(scal + kSDone);
-  __shared__ double red[8];
-  const double a = scal[0] / scal[2];
-  double acc = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += a * p[i];
-    const double rr = r[i] - a * w[i];
-    r[i] = rr;
-    acc += rr * rr;
+                                                      const double* __restrict__ w, CgCtl cg) {
+  PMX_STOP_CHECK(g.stop);
+  extern __shared__ char shraw[];
+  S* lv = reinterpret_cast<S*>(shraw);
+  __shared__ int sps[kMaxPlanes + 1];
+  __shared__ double sh[4];
+  __shared__ double red[16];
+  const Brick b = g.bricks[blockIdx.x];
+  const BrickView v = load_view(g, blockIdx.x);
+  load_planes(g, b, sps);
+  if (UPD) {
+    const int i = (int)cg.scal[kSIter];
+    if (threadIdx.x < 64) {
+      const double s0 = wave_sum(cg.pw + (size_t)(i & 1) * cg.nparts, cg.nparts);
+      const double s1 = wave_sum(cg.zr + (size_t)(i & 1) * cg.nparts, cg.nparts);
+      const double s2 =
+          i > 0 ? wave_sum(cg.pw + (size_t)((i + 1) & 1) * cg.nparts, cg.nparts) : 0.0;
+      if (threadIdx.x == 0) {
+        sh[0] = s0;
+        sh[1] = s1;
+        sh[2] = s2;
+      }
+    }
+    __syncthreads();
+    const double dpi = sh[0];
+    if (dpi == 0.0 || (i > 0 && (dpi > 0.0) != (sh[2] > 0.0))) {
+      cg_finish(cg, i + 1, PMX_DIVERGED_INDEFINITE_MAT, -1.0);
+      return;
+    }
+    const double a = sh[1] / dpi;
+    double acc = 0.0;
+    for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
+      const int c = v.off + li;
+      xs[c] += a * p[c];
+      const double rr = x[c] - a * w[c];
+      x[c] = rr;
+      acc += rr * rr;
+      lv[li] = (S)rr;
+    }
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) cg.rr[blockIdx.x] = acc;
+  } else {
+    for (int li = threadIdx.x; li < v.n; li += blockDim.x) lv[li] = (S)x[v.off + li];
+    __syncthreads();
   }
-  acc = block_sum(acc, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) y[v.off + li] = lv[li];
+}
+
+// Final lv1 stage: r = x - A y, z = y + B1 r, and partial sums of z.x
+// (the PCG's beta = z'r, the PC input x being the PCG residual).
+template <typename S, typename TT, typename F4T, bool BIG>
+__global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __restrict__ x,
+                                                     const S* __restrict__ y,
+                                                     const TT* __restrict__ Tx,
+                                                     const TT* __restrict__ Ty,
+                                                     const TT* __restrict__ Tz,
+                                                     const F4T* __restrict__ fac_f,
+                                                     const F4T* __restrict__ fac_b,
+                                                     double* __restrict__ z, S* __restrict__ yout,
+                                                     CgCtl cg) {
+  PMX_STOP_CHECK(g.stop);
+  extern __shared__ char shraw[];
+  S* lv = reinterpret_cast<S*>(shraw);
+  __shared__ int sps[kMaxPlanes + 1];
+  __shared__ double red[16];
+  const Brick b = g.bricks[blockIdx.x];
+  const BrickView v = load_view(g, blockIdx.x);
+  load_planes(g, b, sps);
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    const S ay = stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
+    lv[c - v.off] = (S)x[c] - ay;
+  });
+  __syncthreads();
+  dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  double acc = 0.0;
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
+    const int c = v.off + li;
+    const S yn = y[c] + lv[li];
+    if (z) {
+      const double zz = (double)yn;
+      z[c] = zz;
+      acc += zz * x[c];
+    }
+    if (yout) yout[c] = yn;
+  }
+  if (cg.zr) {  // z'r partials into the slot of the iteration the counter names
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0)
+      cg.zr[(size_t)((int)cg.scal[kSIter] & 1) * cg.nparts + blockIdx.x] = acc;
+  }
+}
+
+// r = x - A y and c[dof+j] = sum_cells phi[cell][j] r[cell]  (the VecMDot of
+// _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1564).  y == nullptr means y = 0.
+template <typename S, typename TT>
+__global__ void __launch_bounds__(kStreamBlock) k_resid_restrict(
+    Geo g, const double* __restrict__ x, const S* __restrict__ y, const TT* __restrict__ Tx,
+    const TT* __restrict__ Ty, const TT* __restrict__ Tz, const S* __restrict__ phi, int kmax,
+    double* __restrict__ cvec) {
+  PMX_STOP_CHECK(g.stop);
+  __shared__ double red[16 * kMaxEig];
+  const Brick b = g.bricks[blockIdx.x];
+  const BrickView v = load_view(g, blockIdx.x);
+  const int ke = b.neig;
+  double acc[kMaxEig];
+#pragma unroll
+  for (int j = 0; j < kMaxEig; ++j) acc[j] = 0.0;
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    double r = x[c];
+    if (y) r -= (double)stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
+    const S* ph = phi + (size_t)c * kmax;
+#pragma unroll
+    for (int j = 0; j < kMaxEig; ++j)
+      if (j < ke) acc[j] += (double)ph[j] * r;
+  });
+  // wave sums, then the waves in a fixed order
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kMaxEig; ++j) {
+    if (j < ke) {
+      double s = acc[j];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[wv * kMaxEig + j] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ke) {
+    double s = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q * kMaxEig + threadIdx.x];
+    cvec[b.dof + threadIdx.x] = s;
+  }
+}
+
+// y += phi c  (the VecMAXPY of _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1578).
+// iter (not null): the PCG iteration counter, advanced by one.
+template <typename S>
+__global__ void __launch_bounds__(kStreamBlock) k_prolong(Geo g, S* __restrict__ y,
+                                                          const S* __restrict__ phi, int kmax,
+                                                          const double* __restrict__ cvec,
+                                                          double* __restrict__ iter) {
+  PMX_STOP_CHECK(g.stop);
+  __shared__ double cs[kMaxEig];
+  const Brick b = g.bricks[blockIdx.x];
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1.0;
+  if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
+  __syncthreads();
+  const int n = b.nx * b.ny * b.nz;
+  const int ke = b.neig;
+  for (int li = threadIdx.x; li < n; li += blockDim.x) {
+    const int c = b.off + li;
+    const S* ph = phi + (size_t)c * kmax;
+    S acc = (S)0;
+#pragma unroll
+    for (int j = 0; j < kMaxEig; ++j)
+      if (j < ke) acc += ph[j] * (S)cs[j];
+    y[c] = y[c] + acc;
+  }
 }
 
 // r = b - A x, partial r'r.
-__global__ void k_residual(Geo g, const double* __restrict__ bvec, const double* __restrict__ x,
-                           double* __restrict__ r, const double* __restrict__ Tx,
-                           const double* __restrict__ Ty, const double* __restrict__ Tz,
-                           double* __restrict__ partial) {
-  PMX_STOP_CHECK(g.stop);
-  __shared__ double red[8];
+__global__ void __launch_bounds__(kStreamBlock) k_residual(
+    Geo g, const double* __restrict__ bvec, const double* __restrict__ x, double* __restrict__ r,
+    const double* __restrict__ Tx, const double* __restrict__ Ty, const double* __restrict__ Tz,
+    double* __restrict__ partial) {
+  __shared__ double red[16];
   const BrickView v = load_view(g, blockIdx.x);
   double acc = 0.0;
   for_cells(v, [&](int c, int lx, int ly, int lz) {
@@ -541,68 +677,33 @@ __global__ void k_residual(Geo g, const double* __restrict__ bvec, const double*
 }
 
 // y = A x (brick order), FP64.
-__global__ void k_spmv(Geo g, const double* __restrict__ x, double* __restrict__ y,
-                       const double* __restrict__ Tx, const double* __restrict__ Ty,
-                       const double* __restrict__ Tz) {
+__global__ void __launch_bounds__(kStreamBlock) k_spmv(Geo g, const double* __restrict__ x,
+                                                       double* __restrict__ y,
+                                                       const double* __restrict__ Tx,
+                                                       const double* __restrict__ Ty,
+                                                       const double* __restrict__ Tz) {
   const BrickView v = load_view(g, blockIdx.x);
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     y[c] = stencil_fast<double, double, double>(g, v, c, lx, ly, lz, Tx, Ty, Tz, x);
   });
 }
 
-// Deterministic finalisation of per-block partial sums into scal[] and the
-// PETSc KSPSolve_CG / KSPConvergedDefault decisions on the device:
-// op 0: beta = z'r (betaold <- beta)
-// op 1: dpi = p'w; zero or sign change -> DIVERGED_INDEFINITE_MAT
-// op 2: ||r||^2; its += 1; ||r|| <= ttol -> CONVERGED_RTOL,
-//       ||r|| >= dtol ||r0|| -> DIVERGED_DTOL, its >= max_it -> DIVERGED_ITS
-// op 3: initial ||r0||^2: ttol = max(rtol ||r0||, abstol), its = 0
-__global__ void k_finalize(const double* __restrict__ partial, int nb, double* __restrict__ scal,
-                           int op, double rtol, double abstol, double dtol, int max_it) {
-  if (op != 3 && scal[kSDone] != 0.0) return;
-  __shared__ double red[8];
-  double acc = 0.0;
-  // Fixed assignment of partials to threads and a fixed tree: bitwise
-  // reproducible from run to run.
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
-  acc = block_sum(acc, red);
+// Start of the PCG: ||r0||^2 from the residual partials, ttol =
+// max(rtol ||r0||, abstol), its = 0 (KSPConvergedDefault at iteration 0).
+__global__ void k_cg_start(const double* __restrict__ partial, int nb, double* __restrict__ scal,
+                           double rtol, double abstol) {
+  if (threadIdx.x >= 64) return;
+  const double acc = wave_sum(partial, nb);
   if (threadIdx.x != 0) return;
-  if (op == 0) {
-    scal[kSBetaOld] = scal[kSBeta];
-    scal[kSBeta] = acc;
-  } else if (op == 1) {
-    const bool first = scal[kSIts] == 0.0;
-    scal[kSDpiOld] = scal[kSDpi];
-    scal[kSDpi] = acc;
-    const bool flip = !first && ((acc > 0.0) != (scal[kSDpiOld] > 0.0));
-    if (acc == 0.0 || flip) {
-      scal[kSDone] = 1.0;
-      scal[kSReason] = PMX_DIVERGED_INDEFINITE_MAT;
-    }
-  } else if (op == 2) {
-    scal[kSRnorm2] = acc;
-    scal[kSIts] += 1.0;
-    const double dp = sqrt(acc);
-    if (dp <= scal[kSTtol]) {
-      scal[kSDone] = 1.0;
-      scal[kSReason] = PMX_CONVERGED_RTOL;
-    } else if (dp >= dtol * sqrt(scal[kSRnorm0])) {
-      scal[kSDone] = 1.0;
-      scal[kSReason] = PMX_DIVERGED_DTOL;
-    } else if (scal[kSIts] >= (double)max_it) {
-      scal[kSDone] = 1.0;
-      scal[kSReason] = PMX_DIVERGED_ITS;
-    }
-  } else {
-    scal[kSRnorm2] = acc;
-    scal[kSRnorm0] = acc;
-    const double dp = sqrt(acc);
-    const double ttol = fmax(rtol * dp, abstol);
-    scal[kSTtol] = ttol;
-    scal[kSIts] = 0.0;
-    scal[kSDone] = dp <= ttol ? 1.0 : 0.0;
-    scal[kSReason] = dp <= ttol ? (dp > abstol ? PMX_CONVERGED_RTOL : PMX_CONVERGED_ATOL) : 0.0;
-  }
+  scal[kSRnorm2] = acc;
+  scal[kSRnorm0] = acc;
+  const double dp = sqrt(acc);
+  const double ttol = fmax(rtol * dp, abstol);
+  scal[kSTtol] = ttol;
+  scal[kSIts] = 0.0;
+  scal[kSIter] = 0.0;
+  scal[kSDone] = dp <= ttol ? 1.0 : 0.0;
+  scal[kSReason] = dp <= ttol ? (dp < abstol ? PMX_CONVERGED_ATOL : PMX_CONVERGED_RTOL) : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -612,6 +713,33 @@ size_t brick_lds_bytes(const pmx_ctx* ctx, size_t elem) {
   int mx = 0;
   for (const Brick& b : ctx->h_bricks) mx = std::max(mx, b.nx * b.ny * b.nz);
   return (size_t)mx * elem;
+}
+
+// Threads per block for the streaming per-brick kernels: enough waves to keep
+// HBM busy (up to 1024), no more than the largest brick needs, and at least
+// one x-line.
+int stream_threads(const pmx_ctx* ctx) {
+  int mx = 0, nx = 0;
+  for (const Brick& b : ctx->h_bricks) {
+    mx = std::max(mx, b.nx * b.ny * b.nz);
+    nx = std::max(nx, b.nx);
+  }
+  int t = std::min(kStreamBlock, (mx + 63) / 64 * 64);
+  t = std::max(t, (nx + 63) / 64 * 64);
+  return std::min(t, kStreamBlock);
+}
+
+static CgCtl make_cg(pmx_ctx* ctx) {
+  CgCtl c;
+  c.scal = ctx->scal.p;
+  c.nparts = ctx->nsub;
+  c.zr = ctx->partial.p;
+  c.pw = ctx->partial.p + 2 * (size_t)ctx->nsub;
+  c.rr = ctx->partial.p + 4 * (size_t)ctx->nsub;
+  c.abstol = ctx->cg_abstol;
+  c.dtol = ctx->cg_dtol;
+  c.max_it = ctx->cg_max_it;
+  return c;
 }
 
 int launch_nat2brick(pmx_ctx* ctx, const double* src, double* dst, int nvec, int sstride,
@@ -682,16 +810,29 @@ int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* f
   return PMX_SUCCESS;
 }
 
-int launch_lv1_first(pmx_ctx* ctx, const double* x) {
+// y = B1 x.  With upd the kernel first performs the x/r update of the
+// current PCG iteration (x = the PCG residual r, updated in place; xs = the
+// PCG solution).
+int launch_lv1_first(pmx_ctx* ctx, double* x, bool upd, double* xs, const double* p,
+                     const double* w) {
+  const CgCtl cg = make_cg(ctx);
+  const bool big = ctx->max_plane > kBlock;
   if (ctx->pc_precision == 32) {
-    hipLaunchKernelGGL((k_lv1_first<float, float4>), dim3(ctx->nsub), dim3(kBlock),
-                       brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
-                       ctx->w_y32.p, (const float4*)ctx->fac32_f.p,
-                       (const float4*)ctx->fac32_b.p);
+    auto k = upd ? (big ? k_lv1_first<float, float4, true, true>
+                        : k_lv1_first<float, float4, true, false>)
+                 : (big ? k_lv1_first<float, float4, false, true>
+                        : k_lv1_first<float, float4, false, false>);
+    hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), brick_lds_bytes(ctx, sizeof(float)),
+                       ctx->stream, ctx->geo, x, ctx->w_y32.p, (const float4*)ctx->fac32_f.p,
+                       (const float4*)ctx->fac32_b.p, xs, p, w, cg);
   } else {
-    hipLaunchKernelGGL((k_lv1_first<double, double4>), dim3(ctx->nsub), dim3(kBlock),
-                       brick_lds_bytes(ctx, sizeof(double)), ctx->stream, ctx->geo, x,
-                       ctx->w_y.p, (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p);
+    auto k = upd ? (big ? k_lv1_first<double, double4, true, true>
+                        : k_lv1_first<double, double4, true, false>)
+                 : (big ? k_lv1_first<double, double4, false, true>
+                        : k_lv1_first<double, double4, false, false>);
+    hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), brick_lds_bytes(ctx, sizeof(double)),
+                       ctx->stream, ctx->geo, x, ctx->w_y.p, (const double4*)ctx->fac_f.p,
+                       (const double4*)ctx->fac_b.p, xs, p, w, cg);
   }
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
@@ -699,20 +840,24 @@ int launch_lv1_first(pmx_ctx* ctx, const double* x) {
 
 // keep_y: y <- y + B1 (x - A y) written to the second buffer (other bricks
 // read y as halo while it is being updated), then the buffers swap.
-int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, double* partial) {
+int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool cg_partial) {
+  CgCtl cg = make_cg(ctx);
+  if (!cg_partial) cg.zr = nullptr;
+  const bool big = ctx->max_plane > kBlock;
   if (ctx->pc_precision == 32) {
-    hipLaunchKernelGGL((k_lv1_last<float, float, float4>), dim3(ctx->nsub), dim3(kBlock),
-                       brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
-                       ctx->w_y32.p, ctx->T32[0].p, ctx->T32[1].p, ctx->T32[2].p,
-                       (const float4*)ctx->fac32_f.p, (const float4*)ctx->fac32_b.p, z,
-                       keep_y ? ctx->w_y32b.p : nullptr, partial);
+    auto k = big ? k_lv1_last<float, float, float4, true> : k_lv1_last<float, float, float4, false>;
+    hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), brick_lds_bytes(ctx, sizeof(float)),
+                       ctx->stream, ctx->geo, x, ctx->w_y32.p, ctx->T32[0].p, ctx->T32[1].p,
+                       ctx->T32[2].p, (const float4*)ctx->fac32_f.p,
+                       (const float4*)ctx->fac32_b.p, z, keep_y ? ctx->w_y32b.p : nullptr, cg);
     if (keep_y) std::swap(ctx->w_y32, ctx->w_y32b);
   } else {
-    hipLaunchKernelGGL((k_lv1_last<double, double, double4>), dim3(ctx->nsub), dim3(kBlock),
-                       brick_lds_bytes(ctx, sizeof(double)), ctx->stream, ctx->geo, x,
-                       ctx->w_y.p, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
-                       (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p, z,
-                       keep_y ? ctx->w_yb.p : nullptr, partial);
+    auto k = big ? k_lv1_last<double, double, double4, true>
+                 : k_lv1_last<double, double, double4, false>;
+    hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), brick_lds_bytes(ctx, sizeof(double)),
+                       ctx->stream, ctx->geo, x, ctx->w_y.p, ctx->T[0].p, ctx->T[1].p,
+                       ctx->T[2].p, (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p,
+                       z, keep_y ? ctx->w_yb.p : nullptr, cg);
     if (keep_y) std::swap(ctx->w_y, ctx->w_yb);
   }
   PMX_HIP(hipGetLastError());
@@ -720,12 +865,13 @@ int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, doubl
 }
 
 int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec) {
+  const int nt = stream_threads(ctx);
   if (ctx->pc_precision == 32)
-    hipLaunchKernelGGL((k_resid_restrict<float, float>), dim3(ctx->nsub), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_resid_restrict<float, float>), dim3(ctx->nsub), dim3(nt), 0,
                        ctx->stream, ctx->geo, x, y_zero ? nullptr : ctx->w_y32.p, ctx->T32[0].p,
                        ctx->T32[1].p, ctx->T32[2].p, ctx->phi32.p, ctx->en_lv1, cvec);
   else
-    hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub), dim3(nt), 0,
                        ctx->stream, ctx->geo, x, y_zero ? nullptr : ctx->w_y.p, ctx->T[0].p,
                        ctx->T[1].p, ctx->T[2].p, ctx->phi.p, ctx->en_lv1, cvec);
   PMX_HIP(hipGetLastError());
@@ -734,72 +880,49 @@ int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cv
 
 // c = Phi^T x with the FP64 basis (y = 0), whatever the preconditioner precision.
 int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec) {
-  hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub), dim3(kBlock), 0,
-                     ctx->stream, ctx->geo, x, nullptr, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
-                     ctx->phi.p, ctx->en_lv1, cvec);
+  hipLaunchKernelGGL((k_resid_restrict<double, double>), dim3(ctx->nsub),
+                     dim3(stream_threads(ctx)), 0, ctx->stream, ctx->geo, x, nullptr, ctx->T[0].p,
+                     ctx->T[1].p, ctx->T[2].p, ctx->phi.p, ctx->en_lv1, cvec);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
 
-int launch_prolong(pmx_ctx* ctx, const double* cvec) {
+int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter) {
+  double* iter = count_iter ? ctx->scal.p + kSIter : nullptr;
+  const int nt = stream_threads(ctx);
   if (ctx->pc_precision == 32)
-    hipLaunchKernelGGL(k_prolong<float>, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
-                       ctx->w_y32.p, ctx->phi32.p, ctx->en_lv1, cvec);
+    hipLaunchKernelGGL(k_prolong<float>, dim3(ctx->nsub), dim3(nt), 0, ctx->stream, ctx->geo,
+                       ctx->w_y32.p, ctx->phi32.p, ctx->en_lv1, cvec, iter);
   else
-    hipLaunchKernelGGL(k_prolong<double>, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
-                       ctx->w_y.p, ctx->phi.p, ctx->en_lv1, cvec);
+    hipLaunchKernelGGL(k_prolong<double>, dim3(ctx->nsub), dim3(nt), 0, ctx->stream, ctx->geo,
+                       ctx->w_y.p, ctx->phi.p, ctx->en_lv1, cvec, iter);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
 
-int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w,
-                     int first) {
-  hipLaunchKernelGGL(k_cg_p_spmv, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, z,
-                     pold, pnew, w, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->scal.p, first,
-                     ctx->partial.p);
-  PMX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 1, 0.0, 0.0, 0.0, 0);
+int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w) {
+  hipLaunchKernelGGL(k_cg_p_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
+                     ctx->geo, z, pold, pnew, w, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                     make_cg(ctx));
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
 
-int cg_update_blocks(const pmx_ctx* ctx) {
-  int nb = (ctx->ncell + kBlock * 4 - 1) / (kBlock * 4);
-  return std::max(1, std::min(nb, 2048));
-}
-
-int launch_cg_update(pmx_ctx* ctx, double* x, double* r, const double* p, const double* w) {
-  const int nb = cg_update_blocks(ctx);
-  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kBlock), 0, ctx->stream, ctx->ncell, x, r, p, w,
-                     ctx->scal.p, ctx->partial.p);
-  PMX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p, nb,
-                     ctx->scal.p, 2, 0.0, 0.0, ctx->cg_dtol, ctx->cg_max_it);
-  PMX_HIP(hipGetLastError());
-  return PMX_SUCCESS;
-}
-
+// r = b - A x (x == nullptr: x = 0), then the PCG start test.
 int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
-  hipLaunchKernelGGL(k_residual, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, b, x, r,
-                     ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->partial.p);
+  double* part = ctx->partial.p + 4 * (size_t)ctx->nsub;
+  hipLaunchKernelGGL(k_residual, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
+                     ctx->geo, b, x, r, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, part);
   PMX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 3, ctx->cg_rtol, ctx->cg_abstol, 0.0, 0);
-  PMX_HIP(hipGetLastError());
-  return PMX_SUCCESS;
-}
-
-int launch_finalize_beta(pmx_ctx* ctx) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->partial.p,
-                     ctx->nsub, ctx->scal.p, 0, 0.0, 0.0, 0.0, 0);
+  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, ctx->stream, part, ctx->nsub,
+                     ctx->scal.p, ctx->cg_rtol, ctx->cg_abstol);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
 
 int launch_spmv(pmx_ctx* ctx, const double* x, double* y) {
-  hipLaunchKernelGGL(k_spmv, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, x, y,
-                     ctx->T[0].p, ctx->T[1].p, ctx->T[2].p);
+  hipLaunchKernelGGL(k_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
+                     ctx->geo, x, y, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

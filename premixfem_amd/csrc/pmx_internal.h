@@ -20,6 +20,8 @@
 namespace pmx {
 
 constexpr int kBlock = 256;        // threads per workgroup for cell kernels
+constexpr int kStreamBlock = 1024; // largest workgroup of the streaming per-brick kernels
+constexpr int kMaxPlanes = 768;    // hyperplanes of a brick (3 * 256 - 2 at most)
 constexpr int kMaxShapes = 8;      // uneven partitions give at most 2 lengths per axis
 constexpr int kMaxEig = 16;        // largest en_lv1 / en_lv2 supported on device
 constexpr double kNil = 1.0e-12;   // NIL of src/PreMixFEM_3D.h:14
@@ -73,10 +75,20 @@ struct Geo {
   const double* stop;     // PCG "done" flag: kernels return at once when set (nullptr: none)
 };
 
+// What a captured PCG chunk depends on besides the context's buffers.
+struct CgGraphKey {
+  int setup_gen, chunk, max_it;
+  double abstol, dtol;
+  bool operator==(const CgGraphKey& o) const {
+    return setup_gen == o.setup_gen && chunk == o.chunk && max_it == o.max_it &&
+           abstol == o.abstol && dtol == o.dtol;
+  }
+};
+
 // Device-resident PCG scalars (indices into ctx->scal).
 enum ScalIdx {
   kSBeta = 0, kSBetaOld = 1, kSDpi = 2, kSDpiOld = 3, kSRnorm2 = 4, kSDone = 6, kSIts = 7,
-  kSTtol = 8, kSRnorm0 = 9, kSReason = 12, kSNumScal = 16
+  kSTtol = 8, kSRnorm0 = 9, kSReason = 12, kSIter = 13, kSNumScal = 16
 };
 
 #define PMX_STOP_CHECK(ptr) \
@@ -194,7 +206,7 @@ __device__ __forceinline__ void for_cells(const BrickView& v, F&& f) {
   }
 }
 
-// Block-wide deterministic sum of one double (256 threads).
+// Block-wide deterministic sum of one double (up to 16 waves; sh holds 16).
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -304,6 +316,8 @@ struct pmx_ctx {
   double coarse_gamma = 0.0;
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
+  int max_plane = 0;                // most cells in one hyperplane of any brick shape
+  int lds_max = 65536;              // LDS bytes per workgroup of the device
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
   pmx::DBuf<double> lob_buf;        // LOBPCG blocks X, AX, W, AW, P, AP (kept for warm starts)
 
@@ -335,6 +349,11 @@ struct pmx_ctx {
   double cg_rtol = 1e-8, cg_abstol = 1e-50, cg_dtol = 1e5;
   int cg_max_it = 10000;
   int cg_chunk = 8;                 // PCG iterations enqueued between host checks
+  int cg_graph = 1;                 // replay the chunk as a captured HIP graph
+  int setup_gen = 0;                // bumped by every setup (invalidates the graph)
+  pmx::CgGraphKey cg_key{};
+  hipGraph_t cg_graph_h = nullptr;
+  hipGraphExec_t cg_exec = nullptr;
   double h_scal[16];
   std::vector<hipEvent_t> kt_events;
   std::vector<int> kt_pending;

@@ -19,16 +19,14 @@ int launch_basis_rows2nat(pmx_ctx* ctx, const double* rows, double* dst, int k);
 int launch_trans(pmx_ctx* ctx);
 int launch_mass(pmx_ctx* ctx);
 int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* fac_b, bool f32);
-int launch_lv1_first(pmx_ctx* ctx, const double* x);
-int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, double* partial);
+int launch_lv1_first(pmx_ctx* ctx, double* x, bool upd, double* xs, const double* p,
+                     const double* w);
+int stream_threads(const pmx_ctx* ctx);
+int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool cg_partial);
 int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec);
-int launch_prolong(pmx_ctx* ctx, const double* cvec);
-int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w,
-                     int first);
-int cg_update_blocks(const pmx_ctx* ctx);
-int launch_cg_update(pmx_ctx* ctx, double* x, double* r, const double* p, const double* w);
+int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter);
+int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w);
 int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r);
-int launch_finalize_beta(pmx_ctx* ctx);
 int launch_spmv(pmx_ctx* ctx, const double* x, double* y);
 
 // coarse levels (pmx_coarse.hip)

@@ -21,7 +21,7 @@ def options_for(prob: Problem, precision: int = 64, extra: str = "") -> str:
 
 
 def run(prob: Problem, precision: int = 64, extra: str = "", basis=None, rtol=None,
-        keep_ctx=False, sync=None):
+        keep_ctx=False, sync=None, max_it=10000):
     """Set up and solve ``prob``; returns a dict with the solution (natural
     order), iteration count, residual, setup / solve seconds and stats."""
     ctx = P.PCCtx()
@@ -36,7 +36,8 @@ def run(prob: Problem, precision: int = 64, extra: str = "", basis=None, rtol=No
             P.PC_setup_from_basis(shell, *basis)
         t1 = time.perf_counter()
         x, its, rnorm, reason = P.KSPSolve(shell, prob.rhs.ravel(),
-                                           rtol=prob.rtol if rtol is None else rtol)
+                                           rtol=prob.rtol if rtol is None else rtol,
+                                           max_it=max_it)
         t2 = time.perf_counter()
         st = P.PC_get_stats(ctx)
         out = dict(x=x, its=its, rnorm=rnorm, reason=reason, t_setup=t1 - t0, t_solve=t2 - t1,
