@@ -1,6 +1,7 @@
 // Host side of the C ABI (include/premixfem_gpu.h): options, geometry and
 // brick tables, setup orchestration (src/PreMixFEM_3D.c:_PC_setup), the
 // V-cycle (PC_apply_vec) and the GPU-resident PCG (PETSc KSPSolve_CG).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -251,6 +252,44 @@ static int build_geometry(pmx_ctx* c) {
             b.off + (c->h_ax_loc[2][z] * b.ny + c->h_ax_loc[1][y]) * b.nx + c->h_ax_loc[0][x];
       }
   PMX_TRY(upload(c->d_bricks, c->h_bricks));
+  {
+    std::vector<BrickView> views(c->nsub);
+    for (int s = 0; s < c->nsub; ++s) {
+      const Brick& b = c->h_bricks[s];
+      BrickView& v = views[s];
+      v.off = b.off;
+      v.nx = b.nx;
+      v.ny = b.ny;
+      v.nz = b.nz;
+      v.nxny = b.nx * b.ny;
+      v.n = v.nxny * b.nz;
+      for (int d = 0; d < 6; ++d) {
+        const int sn = b.nb[d];
+        const Brick* o = sn >= 0 ? &c->h_bricks[sn] : nullptr;
+        v.noff[d] = o ? o->off : -1;
+        const int ax = d >> 1;
+        v.next[d] = o ? (ax == 0 ? o->nx : (ax == 1 ? o->ny : o->nz)) : 0;
+      }
+    }
+    PMX_TRY(upload(c->d_views, views));
+    // XCD-aware workgroup -> brick order: Morton order of the brick grid,
+    // cut into 8 contiguous runs, run x served by workgroups b = x (mod 8).
+    std::vector<std::pair<uint64_t, int>> key(c->nsub);
+    for (int s = 0; s < c->nsub; ++s) {
+      const int bx = s % c->sd[0], by = (s / c->sd[0]) % c->sd[1], bz = s / (c->sd[0] * c->sd[1]);
+      uint64_t m = 0;
+      for (int bit = 0; bit < 10; ++bit)
+        m |= (uint64_t)((bx >> bit) & 1) << (3 * bit) | (uint64_t)((by >> bit) & 1) << (3 * bit + 1) |
+             (uint64_t)((bz >> bit) & 1) << (3 * bit + 2);
+      key[s] = {m, s};
+    }
+    std::sort(key.begin(), key.end());
+    constexpr int kXcd = 8;
+    std::vector<int> start(kXcd + 1, 0), order(c->nsub);
+    for (int x = 0; x < kXcd; ++x) start[x + 1] = start[x] + (c->nsub - x + kXcd - 1) / kXcd;
+    for (int b = 0; b < c->nsub; ++b) order[b] = key[start[b % kXcd] + b / kXcd].second;
+    PMX_TRY(upload(c->d_order, order));
+  }
   for (int a = 0; a < 3; ++a) {
     PMX_TRY(upload(c->d_ax[a], c->h_ax_sub[a]));
     PMX_TRY(upload(c->d_ax[3 + a], c->h_ax_loc[a]));
@@ -269,6 +308,8 @@ static int build_geometry(pmx_ctx* c) {
   g.nsub = c->nsub;
   g.ncell = c->ncell;
   g.bricks = c->d_bricks.p;
+  g.views = c->d_views.p;
+  g.order = c->d_order.p;
   for (int a = 0; a < 3; ++a) {
     g.ax_sub[a] = c->d_ax[a].p;
     g.ax_loc[a] = c->d_ax[3 + a].p;
@@ -1095,6 +1136,8 @@ int pmx_final(pmx_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   cg_graph_release(c);
   c->d_bricks.release();
+  c->d_views.release();
+  c->d_order.release();
   for (auto& d : c->d_ax) d.release();
   c->d_plane_start.release();
   c->d_plane_cell.release();

@@ -383,13 +383,41 @@ struct CgCtl {
   int max_it;
 };
 
-// Fixed-order sum of n doubles by one wave (lanes stride through the array,
-// then a butterfly); the result is valid in every lane of the calling wave.
-__device__ __forceinline__ double wave_sum(const double* __restrict__ p, int n) {
+// Fixed-order sums of up to three arrays of n doubles by one wave: lane l
+// adds elements l, l+64, l+128, ... in order, then a butterfly.  Eight loads
+// per array are in flight per lane, so the sums cost about one memory
+// round trip for n <= 512.  Results are valid in every lane of the wave.
+__device__ __forceinline__ void wave_sum3(const double* __restrict__ a, const double* __restrict__ b,
+                                          const double* __restrict__ c, int n, double& sa,
+                                          double& sb, double& sc) {
   const int lane = threadIdx.x & 63;
-  double s = 0.0;
-  for (int i = lane; i < n; i += 64) s += p[i];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  sa = sb = sc = 0.0;
+  for (int i0 = 0; i0 < n; i0 += 64 * 8) {
+    double va[8], vb[8], vc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = i0 + lane + 64 * j;
+      va[j] = i < n ? a[i] : 0.0;
+      vb[j] = (b && i < n) ? b[i] : 0.0;
+      vc[j] = (c && i < n) ? c[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      sa += va[j];
+      sb += vb[j];
+      sc += vc[j];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sa += __shfl_xor(sa, o, 64);
+    sb += __shfl_xor(sb, o, 64);
+    sc += __shfl_xor(sc, o, 64);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(const double* __restrict__ p, int n) {
+  double s, u, v;
+  wave_sum3(p, nullptr, nullptr, n, s, u, v);
   return s;
 }
 
@@ -417,9 +445,10 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
   __shared__ double red[16];
   const int i = (int)cg.scal[kSIter];
   if (threadIdx.x < 64) {  // wave 0 (a block may have only one wave)
-    const double s0 = wave_sum(cg.zr + (size_t)(i & 1) * cg.nparts, cg.nparts);
-    const double s1 = i > 0 ? wave_sum(cg.zr + (size_t)((i + 1) & 1) * cg.nparts, cg.nparts) : 0.0;
-    const double s2 = i > 0 ? wave_sum(cg.rr, cg.nparts) : 0.0;
+    double s0, s1, s2;
+    wave_sum3(cg.zr + (size_t)(i & 1) * cg.nparts,
+              i > 0 ? cg.zr + (size_t)((i + 1) & 1) * cg.nparts : nullptr,
+              i > 0 ? cg.rr : nullptr, cg.nparts, s0, s1, s2);
     if (threadIdx.x == 0) {
       sh[0] = s0;
       sh[1] = s1;
@@ -459,7 +488,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
   }
   const bool first = i == 0;
   const double bc = first ? 0.0 : beta / sh[1];
-  const BrickView v = load_view(g, blockIdx.x);
+  const BrickView v = load_view(g, block_brick(g));
   double acc = 0.0;
   const double* TT3[3] = {Tx, Ty, Tz};
   auto pval = [&](int q) { return first ? z[q] : z[q] + bc * pold[q]; };
@@ -505,16 +534,17 @@ __global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, double* __restrict_
   __shared__ int sps[kMaxPlanes + 1];
   __shared__ double sh[4];
   __shared__ double red[16];
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
+  const int sb = block_brick(g);
+  const Brick b = g.bricks[sb];
+  const BrickView v = load_view(g, sb);
   load_planes(g, b, sps);
   if (UPD) {
     const int i = (int)cg.scal[kSIter];
     if (threadIdx.x < 64) {
-      const double s0 = wave_sum(cg.pw + (size_t)(i & 1) * cg.nparts, cg.nparts);
-      const double s1 = wave_sum(cg.zr + (size_t)(i & 1) * cg.nparts, cg.nparts);
-      const double s2 =
-          i > 0 ? wave_sum(cg.pw + (size_t)((i + 1) & 1) * cg.nparts, cg.nparts) : 0.0;
+      double s0, s1, s2;
+      wave_sum3(cg.pw + (size_t)(i & 1) * cg.nparts, cg.zr + (size_t)(i & 1) * cg.nparts,
+                i > 0 ? cg.pw + (size_t)((i + 1) & 1) * cg.nparts : nullptr, cg.nparts, s0, s1,
+                s2);
       if (threadIdx.x == 0) {
         sh[0] = s0;
         sh[1] = s1;
@@ -564,8 +594,9 @@ __global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __rest
   S* lv = reinterpret_cast<S*>(shraw);
   __shared__ int sps[kMaxPlanes + 1];
   __shared__ double red[16];
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
+  const int sb = block_brick(g);
+  const Brick b = g.bricks[sb];
+  const BrickView v = load_view(g, sb);
   load_planes(g, b, sps);
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     const S ay = stencil_fast<TT, S, S>(g, v, c, lx, ly, lz, Tx, Ty, Tz, y);
@@ -600,8 +631,9 @@ __global__ void __launch_bounds__(kStreamBlock) k_resid_restrict(
     double* __restrict__ cvec) {
   PMX_STOP_CHECK(g.stop);
   __shared__ double red[16 * kMaxEig];
-  const Brick b = g.bricks[blockIdx.x];
-  const BrickView v = load_view(g, blockIdx.x);
+  const int sb = block_brick(g);
+  const Brick b = g.bricks[sb];
+  const BrickView v = load_view(g, sb);
   const int ke = b.neig;
   double acc[kMaxEig];
 #pragma unroll
@@ -641,7 +673,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_prolong(Geo g, S* __restrict__
                                                           double* __restrict__ iter) {
   PMX_STOP_CHECK(g.stop);
   __shared__ double cs[kMaxEig];
-  const Brick b = g.bricks[blockIdx.x];
+  const Brick b = g.bricks[block_brick(g)];
   if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1.0;
   if (threadIdx.x < kMaxEig) cs[threadIdx.x] = threadIdx.x < b.neig ? cvec[b.dof + threadIdx.x] : 0.0;
   __syncthreads();
@@ -664,7 +696,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_residual(
     const double* __restrict__ Tx, const double* __restrict__ Ty, const double* __restrict__ Tz,
     double* __restrict__ partial) {
   __shared__ double red[16];
-  const BrickView v = load_view(g, blockIdx.x);
+  const BrickView v = load_view(g, block_brick(g));
   double acc = 0.0;
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     double rr = bvec[c];
@@ -682,7 +714,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_spmv(Geo g, const double* __re
                                                        const double* __restrict__ Tx,
                                                        const double* __restrict__ Ty,
                                                        const double* __restrict__ Tz) {
-  const BrickView v = load_view(g, blockIdx.x);
+  const BrickView v = load_view(g, block_brick(g));
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     y[c] = stencil_fast<double, double, double>(g, v, c, lx, ly, lz, Tx, Ty, Tz, x);
   });

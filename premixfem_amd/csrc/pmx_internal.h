@@ -57,6 +57,14 @@ struct Brick {
   int nplanes;      // nx+ny+nz-2
 };
 
+// Per-brick extents and neighbour offsets, precomputed on the host so a
+// kernel's prologue is one (scalar) load.
+struct BrickView {
+  int off, nx, ny, nz, nxny, n;
+  int noff[6];   // first cell of the neighbour brick, -1: domain boundary
+  int next[6];   // the neighbour's extent along the shared axis
+};
+
 // Geometry passed by value to kernels.
 struct Geo {
   int M, N, P;
@@ -64,6 +72,8 @@ struct Geo {
   int nsub;
   int ncell;
   const Brick* bricks;
+  const BrickView* views;
+  const int* order;       // brick handled by workgroup b of a per-brick launch
   const int* ax_sub[3];   // per axis: subdomain index of each global coordinate
   const int* ax_loc[3];   // per axis: local coordinate
   const int* plane_start; // per shape: nplanes+1 prefix counts (at shape's plane_off + shape*? see below)
@@ -146,31 +156,13 @@ __device__ __forceinline__ int nbr_index(const Geo& g, const Brick& b, int li, i
 // One brick and the offsets/extents of its six neighbour bricks, held in
 // registers by every thread of the workgroup that owns the brick.  An x
 // neighbour shares ny and nz, a y neighbour nx and nz, a z neighbour nx and ny.
-struct BrickView {
-  int off, nx, ny, nz, nxny, n;
-  int noff[6];   // -1: domain boundary
-  int next[6];   // the neighbour's extent along the shared axis
-};
+__device__ __forceinline__ BrickView load_view(const Geo& g, int s) { return g.views[s]; }
 
-__device__ __forceinline__ BrickView load_view(const Geo& g, int s) {
-  const Brick& b = g.bricks[s];
-  BrickView v;
-  v.off = b.off;
-  v.nx = b.nx;
-  v.ny = b.ny;
-  v.nz = b.nz;
-  v.nxny = b.nx * b.ny;
-  v.n = v.nxny * b.nz;
-#pragma unroll
-  for (int d = 0; d < 6; ++d) {
-    const int sn = b.nb[d];
-    v.noff[d] = sn >= 0 ? g.bricks[sn].off : -1;
-    const int ax = d >> 1;
-    v.next[d] = sn >= 0 ? (ax == 0 ? g.bricks[sn].nx : (ax == 1 ? g.bricks[sn].ny : g.bricks[sn].nz))
-                        : 0;
-  }
-  return v;
-}
+// Workgroups are dealt to the 8 XCDs round-robin (b mod 8), and each XCD has
+// its own L2.  order[] gives XCD x a spatially compact run of bricks (a
+// chunk of the Morton order), so the halo reads of neighbouring bricks hit
+// the same L2.
+__device__ __forceinline__ int block_brick(const Geo& g) { return g.order[blockIdx.x]; }
 
 // Brick-order indices of the -x,+x,-y,+y,-z,+z neighbours of cell c at local
 // (lx,ly,lz); -1 on the domain boundary.
@@ -281,6 +273,8 @@ struct pmx_ctx {
   int nshape = 0;
 
   pmx::DBuf<pmx::Brick> d_bricks;
+  pmx::DBuf<pmx::BrickView> d_views;
+  pmx::DBuf<int> d_order;
   pmx::DBuf<int> d_ax[6];
   pmx::DBuf<int> d_plane_start, d_plane_cell, d_shape_pstart, d_shape_coff;
   pmx::Geo geo;
