@@ -167,6 +167,46 @@ int pmx_get_coarse_matrix(const pmx_ctx *ctx, double *out);
 /* PC_final / PC_final_default (src/PreMixFEM_3D.c:2001-2046). */
 int pmx_final(pmx_ctx *ctx);
 
+/* ---- Learned warm start of the level-2 eigensolver ----------------------
+ * No counterpart in the reference: a 3-layer 3D CNN (3x3x3 convolutions,
+ * 4 -> 16 -> 16 -> cout channels, BF16 weights, FP32 accumulation on the
+ * matrix cores) maps a subdomain's log-permeability patch and cell
+ * coordinates to an initial block for its eigenproblem (the eigensolver of
+ * src/PreMixFEM_3D.c:224-421); LOBPCG refines it to the same tolerance and
+ * restarts from a random block where it fails, so the basis is unchanged.
+ * Parameters: pmx_cnn_num_params() FP32 values, per layer W[16][27][16]
+ * (co, tap = (dz+1)*9+(dy+1)*3+(dx+1), ci) then bias[16].  Activations are
+ * device arrays [cell][16] of BF16 (uint16) in brick order, uniform batches
+ * of nb bricks of nx*ny*nz cells (brick b starts at cell b*nx*ny*nz). */
+typedef struct pmx_cnn pmx_cnn;
+int pmx_cnn_num_params(void);
+/* params (host) may be NULL: seeded He-uniform initialisation. */
+int pmx_cnn_create(pmx_cnn **net, int cout, const float *params, unsigned long long seed);
+int pmx_cnn_get_params(const pmx_cnn *net, float *params, float *grad);
+int pmx_cnn_set_params(pmx_cnn *net, const float *params);
+/* d_out: [cell][16] FP32 device array (channels >= cout are 0). */
+int pmx_cnn_predict(pmx_cnn *net, int nb, int nx, int ny, int nz, const unsigned short *d_in,
+                    float *d_out);
+/* Forward + backward against d_target ([cell][16] FP32); the gradient is
+ * kept in the network (pmx_cnn_get_params), the loss returned in *loss.
+ * loss_kind 0: MSE sum_{c<cout} (out - target)^2 / (ncell*cout);
+ * loss_kind 1: subspace distance per brick, sum_b (cout - tr(C^T (G + dI)^{-1}
+ * C)) / (nb*cout) with G = Y^T Y, C = Y^T U/sqrt(n) (Y = the first cout
+ * output channels, U = targets with orthogonal unit-RMS columns, d = 1e-6 n):
+ * invariant to sign and rotation of the eigenvectors. */
+int pmx_cnn_loss_grad(pmx_cnn *net, int nb, int nx, int ny, int nz, const unsigned short *d_in,
+                      const float *d_target, int loss_kind, double *loss);
+int pmx_cnn_adam_step(pmx_cnn *net, float lr, float beta1, float beta2, float eps);
+int pmx_cnn_destroy(pmx_cnn *net);
+/* Training data from a context: the network input of every brick (after
+ * pmx_set_kappa) and the normalised level-2 eigenvectors 1..cout (after
+ * pmx_setup), [ncell][16] device arrays in brick order. */
+int pmx_cnn_inputs(pmx_ctx *ctx, unsigned short *d_in);
+int pmx_cnn_targets(pmx_ctx *ctx, int cout, float *d_target);
+/* Network used by pmx_setup under -warm_start cnn (not owned; keep it alive
+ * while the context uses it). */
+int pmx_set_warm_start(pmx_ctx *ctx, pmx_cnn *net);
+
 /* Last error message of this thread (empty if none). */
 const char *pmx_last_error(void);
 

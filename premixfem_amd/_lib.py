@@ -26,7 +26,10 @@ REASONS = {2: "CONVERGED_RTOL", 3: "CONVERGED_ATOL", -3: "DIVERGED_ITS",
 EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_setup",
            "pmx_setup_from_basis", "pmx_apply", "pmx_solve", "pmx_solve_device",
            "pmx_get_kernel_times", "pmx_get_stats", "pmx_print_stat", "pmx_get_basis",
-           "pmx_get_coarse_matrix", "pmx_final", "pmx_last_error")
+           "pmx_get_coarse_matrix", "pmx_final", "pmx_last_error", "pmx_cnn_num_params",
+           "pmx_cnn_create", "pmx_cnn_get_params", "pmx_cnn_set_params", "pmx_cnn_predict",
+           "pmx_cnn_loss_grad", "pmx_cnn_adam_step", "pmx_cnn_destroy", "pmx_cnn_inputs",
+           "pmx_cnn_targets", "pmx_set_warm_start")
 TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
                  "k_cg_update", "k_gemv", "coarse_small")
 
@@ -91,6 +94,17 @@ def lib():
     L.pmx_solve_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.pmx_get_kernel_times.argtypes = [vp, _dp, _ip, C.c_int, C.c_int]
+    L.pmx_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_void_p, C.c_ulonglong]
+    L.pmx_cnn_get_params.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.pmx_cnn_set_params.argtypes = [vp, C.c_void_p]
+    L.pmx_cnn_predict.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.pmx_cnn_loss_grad.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int,
+                                    C.POINTER(C.c_double)]
+    L.pmx_cnn_adam_step.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.c_float]
+    L.pmx_cnn_destroy.argtypes = [vp]
+    L.pmx_cnn_inputs.argtypes = [vp, vp]
+    L.pmx_cnn_targets.argtypes = [vp, C.c_int, vp]
+    L.pmx_set_warm_start.argtypes = [vp, vp]
     L.pmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "pmx_last_error":

@@ -786,7 +786,20 @@ static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
     c->eig_iters_max = c->eig_iters_sum = 0;
   } else {
     c->h_eigen_num_given.clear();
-    PMX_TRY(lobpcg_setup(c, nullptr));
+    if (c->warm_start == 1) {
+      // learned warm start: the network's block seeds LOBPCG
+      const int mb = lobpcg_block_size(c->en_lv1, c->eig_guard);
+      if (mb == 0) {
+        set_error("en_lv1=%d exceeds the LOBPCG block limit", c->en_lv1);
+        return PMX_ERR_SUP;
+      }
+      DBuf<double> x0;
+      PMX_TRY(x0.alloc((size_t)c->ncell * mb));
+      PMX_TRY(cnn_warm_start(c, mb, x0.p));
+      PMX_TRY(lobpcg_setup(c, x0.p));
+    } else {
+      PMX_TRY(lobpcg_setup(c, nullptr));
+    }
   }
   c->t_stages[PMX_STAGE_SU_LV2_1] += now() - t1;
   PMX_TRY(setup_coarse(c));

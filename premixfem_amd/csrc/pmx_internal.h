@@ -239,6 +239,8 @@ struct DBuf {
 }  // namespace pmx
 
 // Full context behind the opaque pmx_ctx handle.
+struct pmx_cnn;
+
 struct pmx_ctx {
   // ---- options (src/PreMixFEM_3D.c:1662-1705) ----
   double dom[3];
@@ -252,6 +254,7 @@ struct pmx_ctx {
   int pc_precision = 64;
   int pin[3] = {-1, -1, -1};
   int warm_start = 0;           // 0 random, 1 CNN prediction, 2 caller basis
+  pmx_cnn* cnn = nullptr;       // warm-start network (owned by the caller)
   double eig_tol = 1.0e-8;      // LOBPCG relative residual tolerance
   int eig_maxit = 400;
   int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1

@@ -47,6 +47,7 @@ struct LobArgs {
   double* eig;        // output [nsub][kmax]
   int* its;           // output [nsub]
   int* status;        // output [nsub] 0 = converged
+  int* fallback;      // output [nsub] 1 = the warm start failed, restarted at random
   int debug;          // subdomain whose iterations are traced with printf (-1: none)
 };
 
@@ -408,7 +409,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   double* red = rot + 3 * LD;  // 4 waves x MB*MB
   float* pv = reinterpret_cast<float*>(lds);
   __shared__ double lam[MB], nrm[MB], msum_s;
-  __shared__ int s_conv, s_flag;
+  __shared__ int s_conv, s_flag, s_lock[MB];
 
   {
     double s = 0.0;
@@ -442,8 +443,10 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
       G[(r0 + e / nr) * LD + c0 + e % nr] = TM[(e / nr) * LD + e % nr];
     __syncthreads();
   };
+  int attempts = 0;
   if (kw > 0) {
     for (int attempt = 0; attempt < 2; ++attempt) {
+      attempts = attempt;
       // ---- initial block: warm start rows or seeded random ----
       for (int e = threadIdx.x; e < MB * n; e += blockDim.x) {
         const int j = e / n, r = e % n;
@@ -503,28 +506,39 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
           status = 0;
           break;
         }
-        // ---- W -= X (X^T M W) + P (P^T M W) ----
-        wg_multidot<MB, MB>(X, W, mass, n, red, GB, LD);
-        if (np > 0) wg_multidot<MB, MB>(P, W, mass, n, red, GB + MB * LD, LD);
-        for (int r = threadIdx.x; r < n; r += blockDim.x) {
-          double xv[MB], pv2[MB];
-#pragma unroll
-          for (int i = 0; i < MB; ++i) {
-            xv[i] = X[(size_t)i * n + r];
-            pv2[i] = i < np ? P[(size_t)i * n + r] : 0.0;
-          }
-#pragma unroll
-          for (int j = 0; j < MB; ++j) {
-            double s = W[(size_t)j * n + r];
+        // ---- soft locking: converged wanted pairs add no search direction ----
+        if (threadIdx.x < MB)
+          s_lock[threadIdx.x] = threadIdx.x < kw && sqrt(nrm[threadIdx.x]) <= a.tol;
+        __syncthreads();
+        for (int e = threadIdx.x; e < MB * n; e += blockDim.x)
+          if (s_lock[e / n]) W[e] = 0.0;
+        __syncthreads();
+        // ---- W -= X (X^T M W) + P (P^T M W), two passes (a small residual
+        // is mostly cancelled by the first, the second restores
+        // orthogonality to working precision) ----
+        for (int pass = 0; pass < 2; ++pass) {
+          wg_multidot<MB, MB>(X, W, mass, n, red, GB, LD);
+          if (np > 0) wg_multidot<MB, MB>(P, W, mass, n, red, GB + MB * LD, LD);
+          for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            double xv[MB], pv2[MB];
 #pragma unroll
             for (int i = 0; i < MB; ++i) {
-              s -= xv[i] * GB[i * LD + j];
-              if (i < np) s -= pv2[i] * GB[(MB + i) * LD + j];
+              xv[i] = X[(size_t)i * n + r];
+              pv2[i] = i < np ? P[(size_t)i * n + r] : 0.0;
             }
-            W[(size_t)j * n + r] = s;
+#pragma unroll
+            for (int j = 0; j < MB; ++j) {
+              double s = W[(size_t)j * n + r];
+#pragma unroll
+              for (int i = 0; i < MB; ++i) {
+                s -= xv[i] * GB[i * LD + j];
+                if (i < np) s -= pv2[i] * GB[(MB + i) * LD + j];
+              }
+              W[(size_t)j * n + r] = s;
+            }
           }
+          __syncthreads();
         }
-        __syncthreads();
         PMX_PHASE(3)
         // ---- SVQB(W) ----
         wg_multidot<MB, MB>(W, W, mass, n, red, GB, LD);
@@ -708,6 +722,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
     for (int j = 0; j < kw; ++j) a.eig[(size_t)sidx * a.kmax + 1 + j] = lam[j];
     a.its[sidx] = it;
     a.status[sidx] = status;
+    a.fallback[sidx] = a.x0 != nullptr && attempts > 0;
   }
 }
 
@@ -755,10 +770,11 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   PMX_TRY(ctx->lob_buf.alloc(6 * blk));
   PMX_HIP(hipMemsetAsync(ctx->lob_buf.p, 0, 6 * blk * sizeof(double), ctx->stream));
   DBuf<double> eig;
-  DBuf<int> its, status;
+  DBuf<int> its, status, fallback;
   PMX_TRY(eig.alloc((size_t)ctx->nsub * ctx->en_lv1));
   PMX_TRY(its.alloc(ctx->nsub));
   PMX_TRY(status.alloc(ctx->nsub));
+  PMX_TRY(fallback.alloc(ctx->nsub));
   LobArgs a;
   a.kwant = kw;
   a.maxit = ctx->eig_maxit;
@@ -783,6 +799,7 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   a.eig = eig.p;
   a.its = its.p;
   a.status = status.p;
+  a.fallback = fallback.p;
   a.debug = ctx->eig_debug;
   switch (MB) {
     case 4:
@@ -799,7 +816,9 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
       break;
   }
   PMX_HIP(hipGetLastError());
-  std::vector<int> hits(ctx->nsub), hst(ctx->nsub);
+  std::vector<int> hits(ctx->nsub), hst(ctx->nsub), hfb(ctx->nsub);
+  PMX_HIP(hipMemcpyAsync(hfb.data(), fallback.p, ctx->nsub * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream));
   ctx->h_eig_lv1.resize((size_t)ctx->nsub * ctx->en_lv1);
   PMX_HIP(hipMemcpyAsync(hits.data(), its.p, ctx->nsub * sizeof(int), hipMemcpyDeviceToHost,
                          ctx->stream));
@@ -811,6 +830,8 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   int bad = 0, first_bad = -1;
   ctx->eig_iters_max = 0;
   ctx->eig_iters_sum = 0;
+  ctx->eig_fallbacks = 0;
+  for (int s = 0; s < ctx->nsub; ++s) ctx->eig_fallbacks += hfb[s];
   for (int s = 0; s < ctx->nsub; ++s) {
     ctx->eig_iters_max = std::max(ctx->eig_iters_max, hits[s]);
     ctx->eig_iters_sum += hits[s];

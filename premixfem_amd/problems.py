@@ -221,3 +221,24 @@ def config(name: str) -> Problem:
         return Problem("small", dom, mesh, kap, corner_rhs(mesh, dom), sd=(2, 2, 2),
                        en_lv1=4, en_lv2=4, pin_cell=(mesh[0] - 1, 0, 0))
     raise KeyError(name)
+
+
+def training_problem(kind: str, seed: int, mesh=(64, 64, 64)) -> Problem:
+    """A field of the same family as workload ``kind`` ("c2": binary box
+    medium, "c4": lognormal + fractures) on ``mesh`` with 16^3 subdomains,
+    for generating warm-start training samples (seeds disjoint from the
+    workloads')."""
+    dom = (1.0, 1.0, 1.0)
+    vol = mesh[0] * mesh[1] * mesh[2]
+    sd = tuple(m // 16 for m in mesh)
+    if kind == "c2":
+        kap = box_inclusions_kappa(mesh, max(1, round(3000 * vol / 128 ** 3)), 2, 6, 1.0e5, seed)
+        en = 6
+    elif kind == "c4":
+        nfrac = max(1, round(64 * (vol / 256 ** 3) ** (1.0 / 3.0)))
+        kap = fracture_kappa(mesh, 1.0, 8.0, nfrac, 1.0e4, seed)
+        en = 8
+    else:
+        raise KeyError(kind)
+    return Problem(f"{kind}-train-{seed}", dom, mesh, kap, corner_rhs(mesh, dom), sd=sd,
+                   en_lv1=en, en_lv2=4, pin_cell=(mesh[0] - 1, 0, 0))

@@ -21,12 +21,14 @@ def options_for(prob: Problem, precision: int = 64, extra: str = "") -> str:
 
 
 def run(prob: Problem, precision: int = 64, extra: str = "", basis=None, rtol=None,
-        keep_ctx=False, sync=None, max_it=10000):
+        keep_ctx=False, sync=None, max_it=10000, net=None):
     """Set up and solve ``prob``; returns a dict with the solution (natural
     order), iteration count, residual, setup / solve seconds and stats."""
     ctx = P.PCCtx()
     P.PC_init(ctx, prob.dom, prob.mesh, options_for(prob, precision, extra))
     ctx.kappa = prob.kappa
+    if net is not None:
+        net.attach(ctx)
     shell = P.PCShell(ctx)
     try:
         t0 = time.perf_counter()
