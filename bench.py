@@ -11,8 +11,12 @@ cell, PCG to rtol 1e-8.
 A step is one full PC_setup (operator, level-1 ICC(0), 512 subdomain
 eigenproblems by batched LOBPCG, Galerkin coarse matrix, coarse inversion,
 level 3) plus one PCG solve from x0 = 0, with kappa, b and x already in HBM.
-Two modes are timed: the plain FP64 form and the FP32 preconditioner inside
-the FP64 CG; the headline value is the FP32-preconditioned form.
+Three modes are timed: the plain FP64 form, the FP32 preconditioner inside
+the FP64 CG, and the FP32 preconditioner with the learned warm start of the
+subdomain eigensolver ("learned"; its network is trained before the timed
+region on 2048 subdomain eigenvector sets of four seeded fields of the same
+family, training time reported separately); the headline value is the
+learned FP32 form.
 
 Prints one JSON line (see the keys below); --gpus N > 1 is refused (one GPU).
 """
@@ -49,7 +53,7 @@ def kernel_bytes(name, ncell, k, s, nc):
     }.get(name)
 
 
-def run_mode(prob, precision, steps, warmup, extra=""):
+def run_mode(prob, precision, steps, warmup, extra="", net=None):
     import torch
 
     from premixfem_amd import _lib
@@ -59,6 +63,8 @@ def run_mode(prob, precision, steps, warmup, extra=""):
     L = _lib.lib()
     ctx = P.PCCtx()
     P.PC_init(ctx, prob.dom, prob.mesh, runner.options_for(prob, precision, extra))
+    if net is not None:
+        net.attach(ctx)
     ctx.kappa = prob.kappa
     P._push_kappa(ctx)
     dev = torch.device("cuda:0")
@@ -103,6 +109,8 @@ def run_mode(prob, precision, steps, warmup, extra=""):
     tctx = P.PCCtx()
     P.PC_init(tctx, prob.dom, prob.mesh,
               runner.options_for(prob, precision, "-kernel_timing " + extra))
+    if net is not None:
+        net.attach(tctx)
     tctx.kappa = prob.kappa
     P._push_kappa(tctx)
     _lib.check(L.pmx_setup(tctx.handle))
@@ -120,6 +128,27 @@ def run_mode(prob, precision, steps, warmup, extra=""):
     return dict(t_step=(t1 - t0) / steps, t_setup=t_setup / steps, t_solve=t_solve / steps,
                 its=its.value, rnorm=rn.value, stats=st, kernel_ms=ms, kernel_count=cnt,
                 kernel_its=its_t.value, t_solve_eager=t_solve_eager, x=xs)
+
+
+def train_warm_start(prob, nfields=4, epochs=30, batch=64):
+    """Network for the learned mode: trained on the eigenvectors of
+    ``nfields`` seeded fields of the workload's family (same mesh and
+    subdomains, seeds 10000+)."""
+    from premixfem_amd import cnn, problems
+
+    t0 = time.perf_counter()
+    data = [cnn.sample_problem(problems.training_problem(prob.name, 10000 + s, prob.mesh))
+            for s in range(nfields)]
+    t1 = time.perf_counter()
+    net = cnn.WarmStartNet(cout=prob.en_lv1 - 1, seed=1)
+    hist = net.train(data, epochs=epochs, batch=batch, lr=2e-3)
+    t2 = time.perf_counter()
+    info = dict(fields=nfields, samples=sum(d.nb for d in data), epochs=epochs,
+                data_s=round(t1 - t0, 2), train_s=round(t2 - t1, 2),
+                loss_first=round(hist[0], 4), loss_last=round(hist[-1], 4),
+                loss="subspace distance of the predicted block from the eigenvectors (0..1)")
+    del data
+    return net, info
 
 
 def cpu_baseline(prob, gpu_its, nsample):
@@ -186,7 +215,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16,
                     help="subdomain eigenproblems timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--modes", default="fp64,fp32,learned")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 "
@@ -199,18 +228,27 @@ def main():
     print(f"# workload {prob.name}: mesh {prob.mesh} sd {prob.sd} k {prob.en_lv1}", file=sys.stderr,
           flush=True)
     modes = {}
+    training = None
     for m in args.modes.split(","):
-        prec = 32 if m == "fp32" else 64
-        r = run_mode(prob, prec, args.steps, args.warmup)
+        prec = 64 if m == "fp64" else 32
+        net = None
+        extra = ""
+        if m == "learned":
+            net, training = train_warm_start(prob)
+            print(f"# warm-start network: {training}", file=sys.stderr, flush=True)
+            extra = "-warm_start cnn"
+        r = run_mode(prob, prec, args.steps, args.warmup, extra, net)
+        if net is not None:
+            net.close()
         modes[m] = r
         print(f"# {m}: step {r['t_step']:.4f}s setup {r['t_setup']:.4f}s solve {r['t_solve']:.4f}s "
               f"its {r['its']} lobpcg max/mean {r['stats']['eig_iters_max']}/"
               f"{r['stats']['eig_iters_mean']:.1f}", file=sys.stderr, flush=True)
-    head = "fp32" if "fp32" in modes else list(modes)[0]
+    head = "learned" if "learned" in modes else ("fp32" if "fp32" in modes else list(modes)[0])
     hr = modes[head]
     N = int(np.prod(prob.mesh))
     nc = hr["stats"]["size_coarse"]
-    s = 4 if head == "fp32" else 8
+    s = 8 if head == "fp64" else 4
     from premixfem_amd._lib import TIMED_KERNELS
     per_kernel = {}
     solve_bytes = 0.0
@@ -251,19 +289,21 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 preconditioner / f64 CG" if head == "fp32" else "f64",
+        "dtype": "f64" if head == "fp64" else "f32 preconditioner / f64 CG",
         "data": "synthetic (seeded binary box medium)",
         "config": {
             "workload": "128^3 binary box medium (3000 boxes, k=1e5), 8x8x8 subdomains of 16^3, "
                         "6 eigenvectors, pinned cell, PCG rtol 1e-8" if prob.name == "c2"
                         else prob.name,
             "headline_mode": head,
-            "learned_warm_start": False,
+            "learned_warm_start": head == "learned",
         },
         "modes": {m: dict(t_step=round(r["t_step"], 5), t_setup=round(r["t_setup"], 5),
                           t_solve=round(r["t_solve"], 5), its=r["its"],
                           t_solve_instrumented=round(r["t_solve_eager"], 5),
                           lobpcg_iters_max=r["stats"]["eig_iters_max"],
+                          lobpcg_fallbacks=r["stats"]["eig_fallbacks"],
+                          lobpcg_stagnated=r["stats"]["eig_stagnated"],
                           lobpcg_iters_mean=round(r["stats"]["eig_iters_mean"], 2),
                           setup_stages={k: round(v, 5) for k, v in r["stats"]["t_stages"].items()
                                         if k.startswith("su")})
@@ -273,6 +313,7 @@ def main():
         "rel_diff_vs_fp64": (float(np.linalg.norm(hr["x"] - modes["fp64"]["x"]) /
                                    np.linalg.norm(modes["fp64"]["x"]))
                              if "fp64" in modes and head != "fp64" else None),
+        "warm_start_training": training,
         "solve_hbm_gbs": round(solve_gbs, 1) if solve_gbs else None,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in per_kernel.items()},

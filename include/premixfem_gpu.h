@@ -86,6 +86,8 @@ typedef struct {
   double rnorm, rnorm0;      /* last pmx_solve final / initial residual norms */
   int reason;                /* last pmx_solve convergence reason */
   double t_solve;            /* last pmx_solve wall seconds (PCG iterations only) */
+  int eig_stagnated;         /* subdomains accepted at the eigensolver's stagnation test
+                                (worst residual within -eig_stag x -eig_tol) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),
@@ -93,7 +95,8 @@ typedef struct {
  * names (-sd -en_lv1 -en_lv2 -si_lv1 -si_lv2 -eg_op -eb_lv1 -eb_lv2
  * -use_W_cycle -no_shift_A_cc -use_full_Cholesky_lv1 -use_2level), plus
  * -sdx/-sdy/-sdz (per-direction subdomains), -pin x,y,z, -pc_precision 32|64,
- * -eig_tol, -eig_maxit, -warm_start none|cnn|basis, -seed.  Allocates the
+ * -eig_tol, -eig_maxit, -eig_stag, -warm_start none|cnn|basis, -seed, -cg_graph,
+ * -cg_chunk, -kernel_timing.  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 

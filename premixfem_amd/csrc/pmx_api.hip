@@ -122,6 +122,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
     } else if (k == "-cg_chunk") {
       PMX_TRY(geti(c->cg_chunk));
       c->cg_chunk = std::max(1, c->cg_chunk);
+    } else if (k == "-eig_stag") {
+      PMX_TRY(getd(c->eig_stag));
     } else if (k == "-cg_graph") {
       PMX_TRY(geti(c->cg_graph));
     } else if (k == "-kernel_timing") {
@@ -1050,6 +1052,7 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_iters_max = c->eig_iters_max;
   st->eig_iters_mean = c->nsub ? (double)c->eig_iters_sum / c->nsub : 0.0;
   st->eig_fallbacks = c->eig_fallbacks;
+  st->eig_stagnated = c->eig_stagnated;
   st->lv3_iters = c->lv3_iters;
   st->pc_precision = c->pc_precision;
   st->its = c->last_its;

@@ -315,7 +315,8 @@ struct pmx_ctx {
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
   int max_plane = 0;                // most cells in one hyperplane of any brick shape
   int lds_max = 65536;              // LDS bytes per workgroup of the device
-  int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0;
+  int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0, eig_stagnated = 0;
+  double eig_stag = 100.0;      // stagnation acceptance factor (x eig_tol)
   pmx::DBuf<double> lob_buf;        // LOBPCG blocks X, AX, W, AW, P, AP (kept for warm starts)
 
   // ---- level 3 ----

@@ -24,7 +24,8 @@
 
 namespace pmx {
 
-constexpr int kLobCh = 4;  // preconditioner columns per sweep pair
+constexpr int kLobCh = 4;      // preconditioner columns per sweep pair
+constexpr int kStagIts = 40;   // iterations without progress before the stagnation test
 
 struct LobArgs {
   int kwant, maxit, kmax;
@@ -48,6 +49,8 @@ struct LobArgs {
   int* its;           // output [nsub]
   int* status;        // output [nsub] 0 = converged
   int* fallback;      // output [nsub] 1 = the warm start failed, restarted at random
+  int* stagnated;     // output [nsub] 1 = accepted at the stagnation test
+  double stag_factor; // stagnation acceptance: worst residual <= stag_factor * tol
   int debug;          // subdomain whose iterations are traced with printf (-1: none)
 };
 
@@ -409,7 +412,8 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   double* red = rot + 3 * LD;  // 4 waves x MB*MB
   float* pv = reinterpret_cast<float*>(lds);
   __shared__ double lam[MB], nrm[MB], msum_s;
-  __shared__ int s_conv, s_flag, s_lock[MB];
+  __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag;
+  __shared__ double s_rbest;
 
   {
     double s = 0.0;
@@ -478,6 +482,12 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
       wg_transform<MB>(AX, n, Q, LD, MB, MB);
       np = 0;
       status = 1;
+      if (threadIdx.x == 0) {
+        s_rbest = 1e300;
+        s_itbest = 0;
+        s_stag = 0;
+      }
+      __syncthreads();
       for (it = 0; it < a.maxit; ++it) {
         // ---- W = T (AX - M X Lambda), deflated ----
         for (int e = threadIdx.x; e < MB * n; e += blockDim.x) {
@@ -493,8 +503,24 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
         wg_colnorm2<MB>(W, mass, n, red, nrm);
         if (threadIdx.x == 0) {
           int conv = 1;
-          for (int j = 0; j < kw; ++j)
-            if (!(sqrt(nrm[j]) <= a.tol)) conv = 0;
+          double rmax = 0.0;
+          for (int j = 0; j < kw; ++j) {
+            const double rj = sqrt(nrm[j]);
+            if (!(rj <= a.tol)) conv = 0;
+            rmax = fmax(rmax, rj);
+          }
+          // Stagnation: the residuals of a few hard subdomains (clusters of
+          // tiny eigenvalues of 1e5-contrast media) level off just above tol
+          // at the rounding floor of T r.  Accept once the worst residual is
+          // within stag_factor * tol and has not halved for kStagIts its.
+          if (rmax < 0.5 * s_rbest) {
+            s_rbest = rmax;
+            s_itbest = it;
+          }
+          if (!conv && it - s_itbest >= kStagIts && rmax <= a.stag_factor * a.tol) {
+            conv = 1;
+            s_stag = 1;
+          }
           s_conv = conv;
           if (sidx == a.debug && (it < 3 || it % 10 == 0))
             printf("it %d lam0 %.6e lam%d %.6e Tr0 %.3e Tr%d %.3e np %d\n", it, lam[0], kw - 1,
@@ -723,6 +749,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
     a.its[sidx] = it;
     a.status[sidx] = status;
     a.fallback[sidx] = a.x0 != nullptr && attempts > 0;
+    a.stagnated[sidx] = status == 0 && s_stag;
   }
 }
 
@@ -770,11 +797,12 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   PMX_TRY(ctx->lob_buf.alloc(6 * blk));
   PMX_HIP(hipMemsetAsync(ctx->lob_buf.p, 0, 6 * blk * sizeof(double), ctx->stream));
   DBuf<double> eig;
-  DBuf<int> its, status, fallback;
+  DBuf<int> its, status, fallback, stag;
   PMX_TRY(eig.alloc((size_t)ctx->nsub * ctx->en_lv1));
   PMX_TRY(its.alloc(ctx->nsub));
   PMX_TRY(status.alloc(ctx->nsub));
   PMX_TRY(fallback.alloc(ctx->nsub));
+  PMX_TRY(stag.alloc(ctx->nsub));
   LobArgs a;
   a.kwant = kw;
   a.maxit = ctx->eig_maxit;
@@ -800,6 +828,8 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   a.its = its.p;
   a.status = status.p;
   a.fallback = fallback.p;
+  a.stagnated = stag.p;
+  a.stag_factor = ctx->eig_stag;
   a.debug = ctx->eig_debug;
   switch (MB) {
     case 4:
@@ -816,7 +846,9 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
       break;
   }
   PMX_HIP(hipGetLastError());
-  std::vector<int> hits(ctx->nsub), hst(ctx->nsub), hfb(ctx->nsub);
+  std::vector<int> hits(ctx->nsub), hst(ctx->nsub), hfb(ctx->nsub), hsg(ctx->nsub);
+  PMX_HIP(hipMemcpyAsync(hsg.data(), stag.p, ctx->nsub * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream));
   PMX_HIP(hipMemcpyAsync(hfb.data(), fallback.p, ctx->nsub * sizeof(int), hipMemcpyDeviceToHost,
                          ctx->stream));
   ctx->h_eig_lv1.resize((size_t)ctx->nsub * ctx->en_lv1);
@@ -831,7 +863,11 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   ctx->eig_iters_max = 0;
   ctx->eig_iters_sum = 0;
   ctx->eig_fallbacks = 0;
-  for (int s = 0; s < ctx->nsub; ++s) ctx->eig_fallbacks += hfb[s];
+  ctx->eig_stagnated = 0;
+  for (int s = 0; s < ctx->nsub; ++s) {
+    ctx->eig_fallbacks += hfb[s];
+    ctx->eig_stagnated += hsg[s];
+  }
   for (int s = 0; s < ctx->nsub; ++s) {
     ctx->eig_iters_max = std::max(ctx->eig_iters_max, hits[s]);
     ctx->eig_iters_sum += hits[s];
