@@ -395,10 +395,9 @@ static int setup_coarse(pmx_ctx* c) {
     for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
   }
   if (c->pc_precision == 32) {
-    std::vector<double> h((size_t)c->ncell * c->en_lv1);
-    PMX_HIP(hipMemcpy(h.data(), c->phi.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-    std::vector<float> hf(h.begin(), h.end());
-    PMX_TRY(upload(c->phi32, hf));
+    const size_t nphi = (size_t)c->ncell * c->en_lv1;
+    PMX_TRY(c->phi32.alloc(nphi));
+    PMX_TRY(launch_to_f32(c, c->phi.p, c->phi32.p, nphi));
   }
   PMX_HIP(hipStreamSynchronize(c->stream));
   double t1 = now();

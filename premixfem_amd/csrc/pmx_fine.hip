@@ -952,6 +952,19 @@ int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
   return PMX_SUCCESS;
 }
 
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+int launch_to_f32(pmx_ctx* ctx, const double* src, float* dst, size_t n) {
+  const int nb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 8192);
+  hipLaunchKernelGGL(k_to_f32, dim3(std::max(nb, 1)), dim3(kBlock), 0, ctx->stream, src, dst, n);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
 int launch_spmv(pmx_ctx* ctx, const double* x, double* y) {
   hipLaunchKernelGGL(k_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
                      ctx->geo, x, y, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p);

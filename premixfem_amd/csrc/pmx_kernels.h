@@ -28,6 +28,7 @@ int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter);
 int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w);
 int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r);
 int launch_spmv(pmx_ctx* ctx, const double* x, double* y);
+int launch_to_f32(pmx_ctx* ctx, const double* src, float* dst, size_t n);
 
 // coarse levels (pmx_coarse.hip)
 int launch_galerkin(pmx_ctx* ctx);
