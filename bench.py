@@ -130,6 +130,27 @@ def run_mode(prob, precision, steps, warmup, extra="", net=None):
                 kernel_its=its_t.value, t_solve_eager=t_solve_eager, x=xs)
 
 
+def measured_traffic(workload, mode, kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (FP32 preconditioner),
+    calibrated against a copy kernel; None when no summary covers it."""
+    if workload != "c2" or mode == "fp64":
+        return None
+    path = os.path.join(ROOT, "profiles", "c2_fp32_solve_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        f = line.split()
+        if len(f) >= 6 and f[0].replace("pmx::", "").startswith(kernel):
+            try:
+                return int(f[-3])
+            except ValueError:
+                return None
+    return None
+
+
 def train_warm_start(prob, nfields=4, epochs=30, batch=64):
     """Network for the learned mode: trained on the eigenvectors of
     ``nfields`` seeded fields of the workload's family (same mesh and
@@ -216,6 +237,9 @@ def main():
                     help="subdomain eigenproblems timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,fp32,learned")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="finish with an elementwise kernel that reads and writes 256 MiB "
+                         "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 "
@@ -267,7 +291,8 @@ def main():
               key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
     dk = per_kernel[dom]
     roofline = dict(kernel=dom, bound="hbm", achieved=round(dk["gbs"], 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(dk["gbs"] / HBM_PEAK_GBS, 4), traffic=None)
+                    unit="GB/s", frac=round(dk["gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(prob.name, head, dom))
     solve_gbs = solve_bytes / hr["t_solve"] / 1e9 if hr["t_solve"] > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
@@ -321,6 +346,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if args.copy_probe:
+        import torch
+
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print("# copy probe: 3 x (268435456 B read + 268435456 B written)", file=sys.stderr)
 
 
 if __name__ == "__main__":
