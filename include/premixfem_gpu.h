@@ -96,7 +96,8 @@ typedef struct {
  * -use_W_cycle -no_shift_A_cc -use_full_Cholesky_lv1 -use_2level), plus
  * -sdx/-sdy/-sdz (per-direction subdomains), -pin x,y,z, -pc_precision 32|64,
  * -eig_tol, -eig_maxit, -eig_stag, -warm_start none|cnn|basis, -seed, -cg_graph,
- * -cg_chunk, -kernel_timing.  Allocates the
+ * -cg_chunk, -kernel_timing, and -lv2_icc (the spe10 fork's level-2 factor:
+ * ICC(0) of the coarse matrix, src/spe10.c:509-516).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 
@@ -106,6 +107,33 @@ int pmx_print_info(const pmx_ctx *ctx);
 /* Caller fills s_ctx.kappa[0..2] (src/Expe_scal.c:11-42): the three
  * permeability components, host arrays of M*N*P doubles. */
 int pmx_set_kappa(pmx_ctx *ctx, const double *kx, const double *ky, const double *kz);
+
+/* pmx_set_kappa with device arrays (natural order, M*N*P doubles each), as
+ * the spe10 fork's INTER_upd_kappa (src/spe10.c:620-648) refreshes kappa
+ * before every pressure solve. */
+int pmx_set_kappa_device(pmx_ctx *ctx, const double *d_kx, const double *d_ky,
+                         const double *d_kz);
+
+/* The producer-well term of the spe10 fork's pressure system: a per-cell
+ * diagonal addition (|E| perm_x lambda_t WI at the producer columns,
+ * src/spe10.c:780-786) that the operator, the level-1 blocks (:921-927),
+ * the eigen stiffness and the DIAG eigen mass (:1005-1012) and, through A,
+ * the coarse matrices (:1128-1131) all include.  wd: M*N*P non-negative
+ * doubles, natural order; NULL removes the term.  Takes effect at the next
+ * pmx_setup. */
+int pmx_set_well_term(pmx_ctx *ctx, const double *wd);
+int pmx_set_well_term_device(pmx_ctx *ctx, const double *d_wd);
+
+/* Velocity recovery: the RT0 face fluxes F = T (p_L - p_R) of a pressure
+ * field p (the face coefficient T of PC_create_A, src/PreMixFEM_3D.c:1785-1810,
+ * used by the spe10 transport at src/spe10.c:1977-2034).  fx[c] is the flux
+ * from the -x neighbour into cell c across its lower x face (0 on the domain
+ * boundary), likewise fy, fz.  All arrays natural order, M*N*P doubles.  The
+ * outflow (fx[c+x]-fx[c]) + (fy[c+y]-fy[c]) + (fz[c+z]-fz[c]) of every cell
+ * equals (A p)_c without the pin and well terms. */
+int pmx_velocity(pmx_ctx *ctx, const double *p, double *fx, double *fy, double *fz);
+int pmx_velocity_device(pmx_ctx *ctx, const double *d_p, double *d_fx, double *d_fy,
+                        double *d_fz);
 
 /* PC_create_A (src/PreMixFEM_3D.c:1757-1824): the 7-point operator as seven
  * diagonals in natural order: diag[c] and the couplings to the -x,+x,-y,+y,
@@ -209,6 +237,63 @@ int pmx_cnn_targets(pmx_ctx *ctx, int cout, float *d_target);
 /* Network used by pmx_setup under -warm_start cnn (not owned; keep it alive
  * while the context uses it). */
 int pmx_set_warm_start(pmx_ctx *ctx, pmx_cnn *net);
+
+/* ---- The spe10 fork's reservoir loop (src/spe10.c) ----------------------
+ * Two-phase IMPES on an M x N x P grid of Hx x Hy x Hz cells: producers at
+ * the four corner columns, the injector at the four centre columns, Corey
+ * relative permeabilities (exponent 2), mu_o / mu_w = 10 (SPE10_init,
+ * :345-405).  The preconditioner context used with it must have the same
+ * mesh and dom = (M Hx, N Hy, P Hz); the reference's settings are
+ * "-eg_op 2 -no_shift_A_cc -lv2_icc" (DIAG mass, no NIL shift, ICC(0) level-2
+ * factor, :509-530). */
+typedef struct pmx_spe10 pmx_spe10;
+
+/* One row of the P_solver.csv log (src/spe10.c:206-252) plus timings. */
+typedef struct {
+  int step;            /* simu_time_ind */
+  double simu_time;
+  int stage;           /* stage_ind */
+  double DS_max;
+  int S_w_steps;
+  int its;             /* PCG iterations of the pressure solve */
+  int reason;          /* its convergence reason */
+  double S_w_min, S_w_max;
+  double delta_t;      /* time advanced by the transport substeps */
+  double t_assemble, t_setup, t_solve, t_transport;  /* wall seconds */
+} pmx_spe10_log;
+
+/* SPE10_init + INTER_init + get_modified_poro + get_init_S_w (:345-405,
+ * :569-590, :2065-2101): perm_{x,y,z} (the reference's units after its
+ * 10.7639 scaling) and poro, host arrays in natural order; S_w = S_wi,
+ * P = 0. */
+int pmx_spe10_create(pmx_spe10 **sp, const int *mesh, const double *H, const double *perm_x,
+                     const double *perm_y, const double *perm_z, const double *poro);
+/* INTER_upd_kappa + get_linear_system (:620-648, :723-800): kappa and the
+ * well term go into ctx (device to device), the rhs stays in sp. */
+int pmx_spe10_assemble(pmx_spe10 *sp, pmx_ctx *ctx);
+/* The assembled fields as host copies (any pointer may be NULL). */
+int pmx_spe10_get_system(pmx_spe10 *sp, double *kx, double *ky, double *kz, double *wd,
+                         double *rhs);
+/* KSPSetUp (PC_setup) + KSPSolve with the previous pressure as the initial
+ * guess (KSPSetInitialGuessNonzero, :191), all on the GPU. */
+int pmx_spe10_solve(pmx_spe10 *sp, pmx_ctx *ctx, double rtol, int max_it, int *its,
+                    double *rnorm);
+/* get_next_S_w_multisteps (:1942-2063): `steps` explicit upwind substeps of
+ * dt = DS_max / max(rate); *delta_t receives their sum. */
+int pmx_spe10_transport(pmx_spe10 *sp, int steps, double DS_max, double *delta_t);
+/* VecMin / VecMax of S_w (:248-249). */
+int pmx_spe10_saturation_range(pmx_spe10 *sp, double *smin, double *smax);
+int pmx_spe10_get_state(const pmx_spe10 *sp, double *S_w, double *P);
+int pmx_spe10_set_state(pmx_spe10 *sp, const double *S_w, const double *P);
+/* The IMPES loop of main (:197-274): up to max_steps of assemble, setup,
+ * solve, S_w bounds check (PMX_ERR_USER outside [0.2, 0.8], :250) and
+ * transport, with the stage tables simu_time_stages[7], DS_max_stages[7],
+ * steps_stages[7] (:117-119).  log (may be NULL) receives one row per step;
+ * *steps_done the number of steps run. */
+int pmx_spe10_run(pmx_spe10 *sp, pmx_ctx *ctx, int max_steps, const double *simu_time_stages,
+                  const double *DS_max_stages, const int *steps_stages, double rtol, int max_it,
+                  pmx_spe10_log *log, int *steps_done);
+int pmx_spe10_destroy(pmx_spe10 *sp);
 
 /* Last error message of this thread (empty if none). */
 const char *pmx_last_error(void);

@@ -298,7 +298,7 @@ class DIC:
 # --------------------------------------------------------------------------
 # Level 2: local eigenproblems  (src/PreMixFEM_3D.c:224-421)
 # --------------------------------------------------------------------------
-def mass_diag(lay: Layout, kappa, T, eg_op=EIG_OP_MOD):
+def mass_diag(lay: Layout, kappa, T, eg_op=EIG_OP_MOD, wd=None):
     """Per-cell diagonal of the eigen mass matrix M_i (natural order).
 
     MOD (:265-343): 2*kbar/H^2 face weights scaled by |E|.  In x a cell gets
@@ -306,7 +306,8 @@ def mass_diag(lay: Layout, kappa, T, eg_op=EIG_OP_MOD):
     the same subdomain (the face term is added to cell ex-1 at :266), and its
     lower face only when it is the subdomain's first cell.  In y and z every
     cell gets both of its faces.  Domain-boundary faces never count.
-    MEAN: kx+ky+kz.  DIAG: diag of the Neumann block A_i^N."""
+    MEAN: kx+ky+kz.  DIAG: diag of the Neumann block A_i^N, including the
+    well term ``wd`` when there is one (``src/spe10.c:990-1012``)."""
     M, N, P = lay.mesh
     kx, ky, kz = kappa
     if eg_op == EIG_OP_MEAN:
@@ -322,7 +323,10 @@ def mass_diag(lay: Layout, kappa, T, eg_op=EIG_OP_MOD):
         m = (np.where(LX > 0, xl, 0.0) + np.where(LX < NX - 1, xh, 0.0)
              + np.where(LY > 0, yl, 0.0) + np.where(LY < NY - 1, yh, 0.0)
              + np.where(LZ > 0, zl, 0.0) + np.where(LZ < NZ - 1, zh, 0.0))
-        return m.ravel()
+        m = m.ravel()
+        if wd is not None:
+            m = m + wd
+        return m
     if eg_op != EIG_OP_MOD:
         raise ValueError("eg_op")
     ax = 2.0 * (2.0 / (1.0 / kx[:, :, :-1] + 1.0 / kx[:, :, 1:])) / lay.Hx / lay.Hx
@@ -352,8 +356,9 @@ def mass_diag(lay: Layout, kappa, T, eg_op=EIG_OP_MOD):
     return (m * lay.meas_elem).ravel()
 
 
-def local_neumann(lay: Layout, T, s):
-    """A_i^N of subdomain s (interior faces only, :254-264), local order."""
+def local_neumann(lay: Layout, T, s, wd=None):
+    """A_i^N of subdomain s (interior faces only, :254-264), local order,
+    plus the subdomain's well term ``wd`` if given (``src/spe10.c:1005-1012``)."""
     stx, nx, sty, ny, stz, nz = lay.corners(s)
     Tx, Ty, Tz = T
     tx = Tx[stz:stz + nz, sty:sty + ny, stx:stx + nx - 1]
@@ -372,6 +377,10 @@ def local_neumann(lay: Layout, T, s):
         vals += [-t, -t]
         np.add.at(diag, a, t)
         np.add.at(diag, b, t)
+    if wd is not None:
+        M, N, _ = lay.mesh
+        diag = diag + np.asarray(wd).reshape(-1, N, M)[stz:stz + nz, sty:sty + ny,
+                                                     stx:stx + nx].ravel()
     rows.append(np.arange(n))
     cols.append(np.arange(n))
     vals.append(diag)
@@ -415,6 +424,78 @@ def ghep(A_N, m, k, dense_max=1200):
 
 
 # --------------------------------------------------------------------------
+# Level 2 of the spe10 fork: ICC(0) of the coarse matrix  (src/spe10.c:509-516,
+# :1072-1213)
+# --------------------------------------------------------------------------
+DENSE_COARSE_MAX = 6000
+
+
+class BlockICC:
+    """PETSc ICC(0) (natural ordering, no fill) of the Galerkin matrix A_c.
+
+    A_c's pattern is block 7-point over the subdomains: every subdomain's
+    self block and its blocks with the six face neighbours are stored whole
+    (``MatSetValues`` of every entry, :1083-1205).  Scalar IC(0) on that
+    pattern equals the block incomplete factorisation below: between a
+    subdomain I and a lower neighbour J the only common lower neighbour is J
+    itself, so L_IJ = A_IJ L_JJ^{-T} exactly, while the dropped fill is that
+    between I and the neighbours of J; inside a block nothing is dropped:
+        D_I = A_II - sum_{J in {I-x, I-y, I-z}} A_IJ D_J^{-1} A_JI,
+        M = (D + L) D^{-1} (D + L^T),  L = block strictly lower part of A_c.
+    A non-positive pivot (which PETSc would answer with its positive-definite
+    shift) raises ``np.linalg.LinAlgError``."""
+
+    def __init__(self, lay: Layout, A_c, dof):
+        self.lay = lay
+        self.dof = np.asarray(dof)
+        A = A_c.tocsr()
+        sdx, sdy, sdz = lay.sd
+        ns = lay.nsub
+        self.nbr_lo, self.nbr_hi = [], []
+        for s in range(ns):
+            sx, sy, sz = s % sdx, (s // sdx) % sdy, s // (sdx * sdy)
+            lo = [t for t, ok in ((s - 1, sx > 0), (s - sdx, sy > 0), (s - sdx * sdy, sz > 0)) if ok]
+            hi = [t for t, ok in ((s + 1, sx < sdx - 1), (s + sdx, sy < sdy - 1),
+                                  (s + sdx * sdy, sz < sdz - 1)) if ok]
+            self.nbr_lo.append(lo)
+            self.nbr_hi.append(hi)
+        self.blk = {}
+
+        def block(i, j):
+            return A[self.dof[i]:self.dof[i + 1], self.dof[j]:self.dof[j + 1]].toarray()
+
+        self.Dinv = []
+        for s in range(ns):
+            D = block(s, s)
+            for t in self.nbr_lo[s]:
+                Ast = block(s, t)
+                self.blk[(s, t)] = Ast
+                D = D - Ast @ self.Dinv[t] @ Ast.T
+            np.linalg.cholesky(D)   # raises on a non-positive pivot
+            self.Dinv.append(np.linalg.inv(D))
+        for s in range(ns):
+            for t in self.nbr_hi[s]:
+                self.blk[(s, t)] = block(s, t)
+
+    def solve(self, r):
+        r = np.asarray(r, dtype=np.float64)
+        ns, d = self.lay.nsub, self.dof
+        w = np.zeros_like(r)
+        for s in range(ns):
+            v = r[d[s]:d[s + 1]].copy()
+            for t in self.nbr_lo[s]:
+                v -= self.blk[(s, t)] @ w[d[t]:d[t + 1]]
+            w[d[s]:d[s + 1]] = self.Dinv[s] @ v
+        y = np.zeros_like(r)
+        for s in reversed(range(ns)):
+            v = np.zeros(d[s + 1] - d[s])
+            for t in self.nbr_hi[s]:
+                v += self.blk[(s, t)] @ y[d[t]:d[t + 1]]
+            y[d[s]:d[s + 1]] = w[d[s]:d[s + 1]] - self.Dinv[s] @ v
+        return y
+
+
+# --------------------------------------------------------------------------
 # The preconditioner
 # --------------------------------------------------------------------------
 class Oracle:
@@ -422,7 +503,8 @@ class Oracle:
 
     def __init__(self, dom, mesh, kappa, sd=2, en_lv1=4, en_lv2=4, eg_op=EIG_OP_MOD,
                  pin_cell=None, si_lv1=1, si_lv2=1, eb_lv1=-1.0, eb_lv2=-1.0,
-                 use_W_cycle=False, no_shift_A_cc=False, use_2level=False):
+                 use_W_cycle=False, no_shift_A_cc=False, use_2level=False, wd=None,
+                 lv2_icc=False):
         if min(dom) <= 0.0:
             raise ValueError("dom must be positive (:1653)")
         if min(mesh) <= 0:
@@ -445,9 +527,21 @@ class Oracle:
         if pin_cell is not None:
             self.pin = (tuple(pin_cell), pin_tau(self.lay, self.T, pin_cell))
         self.A = assemble_A(self.lay, self.T, pin_cell)
+        # Well term of the spe10 fork (src/spe10.c:780-786): a per-cell
+        # diagonal addition to the operator, the level-1 blocks (:921-927),
+        # the eigen stiffness and DIAG mass (:1005-1012) and, through A, the
+        # coarse matrices.  lv2_icc: the level-2 factor is PETSc ICC(0) of
+        # A_c instead of an exact factorisation (:516).
+        self.wd = None if wd is None else np.asarray(wd, dtype=np.float64).ravel()
+        if self.wd is not None:
+            self.A = (self.A + sp.diags(self.wd)).tocsr()
+        self.lv2_icc = bool(lv2_icc)
 
     def matvec(self, x):
-        return stencil_apply(self.lay, self.T, x, self.pin)
+        y = stencil_apply(self.lay, self.T, x, self.pin)
+        if self.wd is not None:
+            y = y + self.wd * np.asarray(x, dtype=np.float64).ravel()
+        return y
 
     # ---- setup ----------------------------------------------------------
     def setup(self, ghep_fn=None):
@@ -455,11 +549,14 @@ class Oracle:
         lay = self.lay
         # Level 1.
         ts = time.perf_counter()
-        self.dic = DIC(lay, self.T, lv1_block_diag(lay, self.T, self.pin))
+        a_diag = lv1_block_diag(lay, self.T, self.pin)
+        if self.wd is not None:
+            a_diag = a_diag + self.wd
+        self.dic = DIC(lay, self.T, a_diag)
         self.t["su_lv1"] += time.perf_counter() - ts
         # Level 2 eigen.
         ts = time.perf_counter()
-        self.mdiag = mass_diag(lay, self.kappa, self.T, self.eg_op)
+        self.mdiag = mass_diag(lay, self.kappa, self.T, self.eg_op, self.wd)
         ghep_fn = ghep_fn or ghep
         M, N, P = lay.mesh
         gidx = np.arange(M * N * P).reshape(P, N, M)
@@ -470,7 +567,7 @@ class Oracle:
         for s in range(lay.nsub):
             stx, nx, sty, ny, stz, nz = lay.corners(s)
             cells = gidx[stz:stz + nz, sty:sty + ny, stx:stx + nx].ravel()
-            AN = local_neumann(lay, self.T, s)
+            AN = local_neumann(lay, self.T, s, self.wd)
             lam, V = ghep_fn(AN, self.mdiag[cells], self.en_lv1)
             # Adaptive count against eb_lv1 (:368-384).
             num = self.en_lv1
@@ -498,21 +595,36 @@ class Oracle:
                 vals.append(V[:, j])
         self.Phi = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                                  shape=(M * N * P, nc))
-        self.A_c = (self.Phi.T @ (self.A @ self.Phi)).toarray()
-        self.A_c = 0.5 * (self.A_c + self.A_c.T)
-        if self.use_2level:
+        A_c = (self.Phi.T @ (self.A @ self.Phi)).tocsr()
+        self.A_c_sp = (0.5 * (A_c + A_c.T)).tocsr()
+        self.A_c = self.A_c_sp.toarray() if nc <= DENSE_COARSE_MAX else None
+        if self.lv2_icc:
+            self.bicc = BlockICC(lay, self.A_c_sp, self.dof_idx)
+            self.t["su_lv2_2"] += time.perf_counter() - ts
+            if self.use_2level:
+                self.t["su"] += time.perf_counter() - t0
+                return self
+        elif self.use_2level:
             # _PC_setup_lv2_c (:423-735): same Galerkin matrix, NIL shift, LU.
             Ac = self.A_c + (0.0 if self.no_shift_A_cc else NIL) * np.eye(nc)
             self.lu_c = sla.lu_factor(Ac)
             self.t["su_lv2_2"] += time.perf_counter() - ts
             self.t["su"] += time.perf_counter() - t0
             return self
-        self.chol_c = sla.cho_factor(self.A_c, lower=True)
-        self.t["su_lv2_2"] += time.perf_counter() - ts
+        if not self.lv2_icc:
+            if self.A_c is None:
+                raise ValueError("dense coarse factorisation needs nc <= %d" % DENSE_COARSE_MAX)
+            self.chol_c = sla.cho_factor(self.A_c, lower=True)
+            self.t["su_lv2_2"] += time.perf_counter() - ts
         # Level 3 eigen (:918-1175): smallest en_lv2 of the coarse matrix.
         ts = time.perf_counter()
         k2 = min(self.en_lv2, nc)
-        lam2, V2 = sla.eigh(self.A_c, subset_by_index=[0, k2 - 1])
+        if self.A_c is not None:
+            lam2, V2 = sla.eigh(self.A_c, subset_by_index=[0, k2 - 1])
+        else:
+            lam2, V2 = spla.eigsh(self.A_c_sp.tocsc(), k=k2, sigma=-NIL, which="LM", tol=1e-12)
+            o = np.argsort(lam2)
+            lam2, V2 = lam2[o], V2[:, o]
         V2 = sign_normalize(V2)
         num2 = k2
         for j in range(k2):
@@ -546,7 +658,9 @@ class Oracle:
     def apply_lv2(self, r):
         ts = time.perf_counter()
         c = self.Phi.T @ r
-        if self.use_2level:
+        if self.lv2_icc:
+            cs = self.bicc.solve(c)
+        elif self.use_2level:
             cs = sla.lu_solve(self.lu_c, c)
         else:
             cs = sla.cho_solve(self.chol_c, c)
@@ -616,6 +730,11 @@ def pcg(matvec, b, apply_pc, rtol=1e-8, abstol=1e-50, dtol=1e5, max_it=10000, x0
     dp = np.linalg.norm(r)
     hist = [dp]
     rnorm0 = dp
+    if x0 is not None:
+        # KSPConvergedDefault with a nonzero initial guess measures against
+        # ||b|| (||r0|| when b = 0).
+        bn = np.linalg.norm(b)
+        rnorm0 = bn if bn > 0.0 else dp
     ttol = max(rtol * rnorm0, abstol)
     if dp <= ttol:
         return x, 0, hist, "CONVERGED_RTOL" if dp > abstol else "CONVERGED_ATOL"

@@ -29,7 +29,12 @@ EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_s
            "pmx_get_coarse_matrix", "pmx_final", "pmx_last_error", "pmx_cnn_num_params",
            "pmx_cnn_create", "pmx_cnn_get_params", "pmx_cnn_set_params", "pmx_cnn_predict",
            "pmx_cnn_loss_grad", "pmx_cnn_adam_step", "pmx_cnn_destroy", "pmx_cnn_inputs",
-           "pmx_cnn_targets", "pmx_set_warm_start")
+           "pmx_cnn_targets", "pmx_set_warm_start", "pmx_set_kappa_device",
+           "pmx_set_well_term", "pmx_set_well_term_device", "pmx_velocity",
+           "pmx_velocity_device", "pmx_spe10_create", "pmx_spe10_assemble",
+           "pmx_spe10_get_system", "pmx_spe10_solve", "pmx_spe10_transport",
+           "pmx_spe10_saturation_range", "pmx_spe10_get_state", "pmx_spe10_set_state",
+           "pmx_spe10_run", "pmx_spe10_destroy")
 TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
                  "k_cg_update", "k_gemv", "coarse_small")
 
@@ -62,6 +67,20 @@ class Stats(C.Structure):
             v = getattr(self, name)
             out[name] = list(v) if hasattr(v, "__len__") else v
         return out
+
+
+class Spe10Log(C.Structure):
+    """One row of the IMPES log (pmx_spe10_log)."""
+    _fields_ = [
+        ("step", C.c_int), ("simu_time", C.c_double), ("stage", C.c_int),
+        ("DS_max", C.c_double), ("S_w_steps", C.c_int), ("its", C.c_int), ("reason", C.c_int),
+        ("S_w_min", C.c_double), ("S_w_max", C.c_double), ("delta_t", C.c_double),
+        ("t_assemble", C.c_double), ("t_setup", C.c_double), ("t_solve", C.c_double),
+        ("t_transport", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -106,6 +125,24 @@ def lib():
     L.pmx_cnn_inputs.argtypes = [vp, vp]
     L.pmx_cnn_targets.argtypes = [vp, C.c_int, vp]
     L.pmx_set_warm_start.argtypes = [vp, vp]
+    L.pmx_set_kappa_device.argtypes = [vp, vp, vp, vp]
+    L.pmx_set_well_term.argtypes = [vp, C.c_void_p]
+    L.pmx_set_well_term_device.argtypes = [vp, vp]
+    L.pmx_velocity.argtypes = [vp, _dp, _dp, _dp, _dp]
+    L.pmx_velocity_device.argtypes = [vp, vp, vp, vp, vp]
+    L.pmx_spe10_create.argtypes = [C.POINTER(vp), _ip, _dp, _dp, _dp, _dp, _dp]
+    L.pmx_spe10_assemble.argtypes = [vp, vp]
+    L.pmx_spe10_get_system.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]
+    L.pmx_spe10_solve.argtypes = [vp, vp, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double)]
+    L.pmx_spe10_transport.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_double)]
+    L.pmx_spe10_saturation_range.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pmx_spe10_get_state.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.pmx_spe10_set_state.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.pmx_spe10_run.argtypes = [vp, vp, C.c_int, _dp, _dp, _ip, C.c_double, C.c_int,
+                                C.c_void_p, C.POINTER(C.c_int)]
+    L.pmx_spe10_destroy.argtypes = [vp]
     L.pmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "pmx_last_error":

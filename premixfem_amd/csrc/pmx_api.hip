@@ -107,6 +107,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_full_Cholesky_lv1 = 1;
     } else if (k == "-use_2level") {
       c->use_2level = 1;
+    } else if (k == "-lv2_icc") {
+      c->lv2_icc = 1;
     } else if (k == "-pc_precision") {
       PMX_TRY(geti(c->pc_precision));
     } else if (k == "-eig_tol") {
@@ -369,14 +371,21 @@ static int setup_coarse(pmx_ctx* c) {
   c->nc = c->h_dof[c->nsub];
   PMX_HIP(hipMemcpy(c->d_bricks.p, c->h_bricks.data(), c->h_bricks.size() * sizeof(Brick),
                     hipMemcpyHostToDevice));
-  // Galerkin coarse matrix and its inverse.
-  PMX_TRY(c->Ac.alloc((size_t)c->nc * c->nc));
+  // Galerkin coarse matrix (block rows; dense too unless -lv2_icc) and its
+  // level-2 factor: the dense inverse, or the block ICC(0) of the spe10 fork.
   PMX_TRY(c->Acb.alloc((size_t)c->nsub * 7 * c->en_lv1 * c->en_lv1));
-  PMX_TRY(c->Ainv.alloc((size_t)c->nc * c->nc));
   for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
-  PMX_TRY(launch_galerkin(c));
-  const double shift = (c->use_2level && !c->no_shift_A_cc) ? kNil : 0.0;
-  PMX_TRY(coarse_solve_setup(c, shift));
+  if (c->lv2_icc) {
+    for (DBuf<double>* b : {&c->Ac, &c->Ainv, &c->Fop}) b->release();
+    PMX_TRY(launch_galerkin(c));
+    PMX_TRY(bicc_setup(c));
+  } else {
+    PMX_TRY(c->Ac.alloc((size_t)c->nc * c->nc));
+    PMX_TRY(c->Ainv.alloc((size_t)c->nc * c->nc));
+    PMX_TRY(launch_galerkin(c));
+    const double shift = (c->use_2level && !c->no_shift_A_cc) ? kNil : 0.0;
+    PMX_TRY(coarse_solve_setup(c, shift));
+  }
   // The coarse stages of PC_apply_vec (src/PreMixFEM_3D.c:1857-1929):
   // 2 = level 2, 3 = level 3.
   std::vector<int>& prog = c->coarse_prog;
@@ -403,12 +412,14 @@ static int setup_coarse(pmx_ctx* c) {
   double t1 = now();
   c->t_stages[PMX_STAGE_SU_LV2_2] += t1 - t0;
   if (!c->use_2level) {
-    PMX_TRY(level3_setup(c));
+    PMX_TRY(c->lv2_icc ? level3_setup_icc(c) : level3_setup(c));
     double t2 = now();
     c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
   }
   const double t3 = now();
-  PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  // The dense path composes the coarse stages into one nc x nc operator; the
+  // ICC(0) path runs them per apply in one workgroup (no nc^2 storage).
+  if (!c->lv2_icc) PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
   c->t_stages[PMX_STAGE_SU_LV3_2] += now() - t3;
   return PMX_SUCCESS;
 }
@@ -421,6 +432,16 @@ static int setup_fine(pmx_ctx* c) {
   }
   double t0 = now();
   PMX_TRY(launch_trans(c));
+  c->geo.wd = nullptr;
+  c->geo.wd32 = nullptr;
+  if (c->have_wd) {
+    c->geo.wd = c->wd.p;
+    if (c->pc_precision == 32) {
+      PMX_TRY(c->wd32.alloc(c->ncell));
+      PMX_TRY(launch_to_f32(c, c->wd.p, c->wd32.p, c->ncell));
+      c->geo.wd32 = c->wd32.p;
+    }
+  }
   if (c->pin[0] >= 0) {
     const int x = c->pin[0], y = c->pin[1], z = c->pin[2];
     const int bi = c->h_nat2brick[(size_t)(z * c->mesh[1] + y) * c->mesh[0] + x];
@@ -531,10 +552,13 @@ static int apply_device(pmx_ctx* c, double* x, double* z, bool cg_partial, bool 
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
-  (void)d;
-  ev = kt_begin(c, 6);
-  PMX_TRY(launch_gemv(c, c->Fop.p, tv, u, c->nc));
-  kt_end(c, ev);
+  if (c->lv2_icc) {
+    PMX_TRY(launch_coarse_program(c, c->coarse_prog, tv, u, c->c_e.p, d, 6, 7, kt_begin, kt_end));
+  } else {
+    ev = kt_begin(c, 6);
+    PMX_TRY(launch_gemv(c, c->Fop.p, tv, u, c->nc));
+    kt_end(c, ev);
+  }
   PMX_TRY(tick(PMX_STAGE_AV_LV2, t));
   ev = kt_begin(c, 3);
   PMX_TRY(launch_prolong(c, u, cg_upd));
@@ -686,22 +710,31 @@ int pmx_print_info(const pmx_ctx* c) {
   return PMX_SUCCESS;
 }
 
-int pmx_set_kappa(pmx_ctx* c, const double* kx, const double* ky, const double* kz) {
-  if (!c || !kx || !ky || !kz) {
-    set_error("null argument");
-    return PMX_ERR_ARG_NULL;
-  }
-  const double* src[3] = {kx, ky, kz};
+// Pushes natural-order permeability (host or device arrays) into brick order.
+static int set_kappa_impl(pmx_ctx* c, const double* const src[3], bool device) {
   for (int a = 0; a < 3; ++a) {
-    for (int i = 0; i < c->ncell; ++i)
-      if (!(src[a][i] > 0.0) || !std::isfinite(src[a][i])) {
-        set_error("kappa[%d][%d]=%g is not positive and finite", a, i, src[a][i]);
+    if (!device) {
+      for (int i = 0; i < c->ncell; ++i)
+        if (!(src[a][i] > 0.0) || !std::isfinite(src[a][i])) {
+          set_error("kappa[%d][%d]=%g is not positive and finite", a, i, src[a][i]);
+          return PMX_ERR_ARG_WRONG;
+        }
+    } else {
+      int bad = -1;
+      PMX_TRY(launch_check_field(c, src[a], c->ncell, true, &bad));
+      if (bad >= 0) {
+        set_error("kappa[%d][%d] is not positive and finite", a, bad);
         return PMX_ERR_ARG_WRONG;
       }
+    }
     PMX_TRY(c->kappa[a].alloc(c->ncell));
-    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, src[a], c->ncell * sizeof(double), hipMemcpyHostToDevice,
-                           c->stream));
-    PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->kappa[a].p, 1, c->ncell, c->ncell));
+    const double* nat = src[a];
+    if (!device) {
+      PMX_HIP(hipMemcpyAsync(c->nat_buf.p, src[a], c->ncell * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+      nat = c->nat_buf.p;
+    }
+    PMX_TRY(launch_nat2brick(c, nat, c->kappa[a].p, 1, c->ncell, c->ncell));
     PMX_TRY(c->T[a].alloc(c->ncell));
     if (c->pc_precision == 32) PMX_TRY(c->T32[a].alloc(c->ncell));
   }
@@ -709,6 +742,119 @@ int pmx_set_kappa(pmx_ctx* c, const double* kx, const double* ky, const double* 
   c->have_kappa = 1;
   c->setup_done = 0;
   return PMX_SUCCESS;
+}
+
+int pmx_set_kappa(pmx_ctx* c, const double* kx, const double* ky, const double* kz) {
+  if (!c || !kx || !ky || !kz) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  const double* src[3] = {kx, ky, kz};
+  return set_kappa_impl(c, src, false);
+}
+
+int pmx_set_kappa_device(pmx_ctx* c, const double* d_kx, const double* d_ky,
+                         const double* d_kz) {
+  if (!c || !d_kx || !d_ky || !d_kz) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  const double* src[3] = {d_kx, d_ky, d_kz};
+  return set_kappa_impl(c, src, true);
+}
+
+// The per-cell diagonal (well) term; null clears it.
+static int set_well_impl(pmx_ctx* c, const double* w, bool device) {
+  c->setup_done = 0;
+  if (!w) {
+    c->have_wd = 0;
+    c->wd.release();
+    c->wd32.release();
+    c->geo.wd = nullptr;
+    c->geo.wd32 = nullptr;
+    return PMX_SUCCESS;
+  }
+  if (!device) {
+    for (int i = 0; i < c->ncell; ++i)
+      if (!(w[i] >= 0.0) || !std::isfinite(w[i])) {
+        set_error("well term [%d]=%g is not non-negative and finite", i, w[i]);
+        return PMX_ERR_ARG_WRONG;
+      }
+  } else {
+    int bad = -1;
+    PMX_TRY(launch_check_field(c, w, c->ncell, false, &bad));
+    if (bad >= 0) {
+      set_error("well term [%d] is not non-negative and finite", bad);
+      return PMX_ERR_ARG_WRONG;
+    }
+  }
+  PMX_TRY(c->wd.alloc(c->ncell));
+  const double* nat = w;
+  if (!device) {
+    PMX_HIP(hipMemcpyAsync(c->nat_buf.p, w, c->ncell * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+    nat = c->nat_buf.p;
+  }
+  PMX_TRY(launch_nat2brick(c, nat, c->wd.p, 1, c->ncell, c->ncell));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  c->have_wd = 1;
+  c->geo.wd = c->wd.p;
+  c->geo.wd32 = nullptr;
+  return PMX_SUCCESS;
+}
+
+int pmx_set_well_term(pmx_ctx* c, const double* wd) {
+  if (!c) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  return set_well_impl(c, wd, false);
+}
+
+int pmx_set_well_term_device(pmx_ctx* c, const double* d_wd) {
+  if (!c) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  return set_well_impl(c, d_wd, true);
+}
+
+// Face fluxes from a pressure field (natural order, device arrays).
+static int velocity_impl(pmx_ctx* c, const double* d_p, double* d_fx, double* d_fy,
+                         double* d_fz) {
+  if (!c->have_kappa) {
+    set_error("kappa has not been set (pmx_set_kappa)");
+    return PMX_ERR_ORDER;
+  }
+  if (!c->setup_done) PMX_TRY(launch_trans(c));
+  PMX_TRY(launch_nat2brick(c, d_p, c->w_tmp.p, 1, c->ncell, c->ncell));
+  PMX_TRY(launch_velocity(c, c->w_tmp.p, d_fx, d_fy, d_fz));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  return PMX_SUCCESS;
+}
+
+int pmx_velocity(pmx_ctx* c, const double* p, double* fx, double* fy, double* fz) {
+  if (!c || !p || !fx || !fy || !fz) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  const size_t nb = c->ncell * sizeof(double);
+  DBuf<double> f;
+  PMX_TRY(f.alloc(3 * (size_t)c->ncell));
+  PMX_HIP(hipMemcpyAsync(c->nat_buf.p, p, nb, hipMemcpyHostToDevice, c->stream));
+  PMX_TRY(velocity_impl(c, c->nat_buf.p, f.p, f.p + c->ncell, f.p + 2 * (size_t)c->ncell));
+  double* out[3] = {fx, fy, fz};
+  for (int a = 0; a < 3; ++a)
+    PMX_HIP(hipMemcpy(out[a], f.p + (size_t)a * c->ncell, nb, hipMemcpyDeviceToHost));
+  return PMX_SUCCESS;
+}
+
+int pmx_velocity_device(pmx_ctx* c, const double* d_p, double* d_fx, double* d_fy, double* d_fz) {
+  if (!c || !d_p || !d_fx || !d_fy || !d_fz) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  return velocity_impl(c, d_p, d_fx, d_fy, d_fz);
 }
 
 int pmx_create_A(pmx_ctx* c, double* diag, double* off) {
@@ -726,6 +872,12 @@ int pmx_create_A(pmx_ctx* c, double* diag, double* off) {
   for (int a = 0; a < 3; ++a) {
     T[a].resize(n);
     PMX_HIP(hipMemcpyAsync(T[a].data(), c->T[a].p, n * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  }
+  std::vector<double> wdb;
+  if (c->have_wd) {
+    wdb.resize(n);
+    PMX_HIP(hipMemcpyAsync(wdb.data(), c->wd.p, n * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   }
   PMX_HIP(hipStreamSynchronize(c->stream));
@@ -757,6 +909,7 @@ int pmx_create_A(pmx_ctx* c, double* diag, double* off) {
           for (int a = 0; a < 3; ++a) tau = std::max(tau, std::max(lo[a], hi[a]));
           dsum += tau;
         }
+        if (c->have_wd) dsum += wdb[c->h_nat2brick[cn]];
         diag[cn] = dsum;
       }
   return PMX_SUCCESS;
@@ -1142,7 +1295,27 @@ int pmx_get_coarse_matrix(const pmx_ctx* c, double* out) {
     set_error("coarse matrix requested before setup");
     return PMX_ERR_ORDER;
   }
-  PMX_HIP(hipMemcpy(out, c->Ac.p, (size_t)c->nc * c->nc * sizeof(double), hipMemcpyDeviceToHost));
+  if (c->Ac.p) {
+    PMX_HIP(hipMemcpy(out, c->Ac.p, (size_t)c->nc * c->nc * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    return PMX_SUCCESS;
+  }
+  // -lv2_icc keeps block rows only: expand them.
+  const int k = c->en_lv1;
+  std::vector<double> blk(c->Acb.n);
+  PMX_HIP(hipMemcpy(blk.data(), c->Acb.p, blk.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::fill(out, out + (size_t)c->nc * c->nc, 0.0);
+  for (int s = 0; s < c->nsub; ++s) {
+    const Brick& b = c->h_bricks[s];
+    for (int slot = 0; slot < 7; ++slot) {
+      const int o = slot == 0 ? s : b.nb[slot - 1];
+      if (o < 0) continue;
+      const Brick& ob = c->h_bricks[o];
+      const double* B = blk.data() + ((size_t)s * 7 + slot) * k * k;
+      for (int i = 0; i < b.neig; ++i)
+        for (int j = 0; j < ob.neig; ++j) out[(size_t)(b.dof + i) * c->nc + ob.dof + j] = B[i * k + j];
+    }
+  }
   return PMX_SUCCESS;
 }
 
@@ -1163,7 +1336,12 @@ int pmx_final(pmx_ctx* c) {
     c->T[a].release();
     c->T32[a].release();
   }
-  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs, &c->Fop, &c->lob_buf}) b->release();
+  for (DBuf<double>* b : {&c->c_z, &c->c_g, &c->c_h, &c->c_cs, &c->Fop, &c->lob_buf, &c->bicc_dinv,
+                          &c->wd})
+    b->release();
+  c->wd32.release();
+  c->bicc_plane.release();
+  c->bicc_elem.release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

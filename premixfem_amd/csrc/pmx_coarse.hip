@@ -62,6 +62,7 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         }
       }
       if (c == g.pin) diag += g.tau;
+      if (g.wd) diag += g.wd[c];
 #pragma unroll
       for (int j = 0; j < kMaxEig; ++j)
         if (j < ke) aphi[j] += diag * pc[j];
@@ -79,8 +80,10 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
       for (int j = i; j < ke; ++j) {
         const double v = block_sum(acc[a][j], red);
         if (threadIdx.x == 0) {
-          Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
-          Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
+          if (Ac) {
+            Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
+            Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
+          }
           blk[i * kmax + j] = v;
           blk[j * kmax + i] = v;
         }
@@ -135,8 +138,10 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         for (int j = 0; j < o.neig; ++j) {
           const double v = block_sum(acc[a][j], red);
           if (threadIdx.x == 0) {
-            Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
-            Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
+            if (Ac) {
+              Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
+              Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
+            }
             blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
             Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
           }
@@ -147,7 +152,8 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
 }
 
 int launch_galerkin(pmx_ctx* ctx) {
-  PMX_HIP(hipMemsetAsync(ctx->Ac.p, 0, sizeof(double) * (size_t)ctx->nc * ctx->nc, ctx->stream));
+  if (ctx->Ac.p)
+    PMX_HIP(hipMemsetAsync(ctx->Ac.p, 0, sizeof(double) * (size_t)ctx->nc * ctx->nc, ctx->stream));
   PMX_HIP(hipMemsetAsync(ctx->Acb.p, 0, sizeof(double) * ctx->Acb.n, ctx->stream));
   hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
                      ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->nc,
@@ -526,12 +532,238 @@ __device__ __forceinline__ double coarse_row(const Geo& g, const double* __restr
   return acc;
 }
 
+// ---------------------------------------------------------------------------
+// Level 2 of the spe10 fork (src/spe10.c:509-516, :1072-1213): PETSc ICC(0)
+// of A_c in natural order.  A_c's pattern is block 7-point over the
+// subdomains with every block stored whole, so scalar IC(0) on it is the
+// block incomplete Cholesky
+//   D_s = A_ss - sum_{t in {s-x, s-y, s-z}} A_st D_t^{-1} A_ts,
+//   M = (D + L) D^{-1} (D + L^T),  L = block strictly lower part of A_c:
+// between s and a lower neighbour t the only common lower neighbour is t, so
+// L_st = A_st L_tt^{-T} is exact, and the dropped fill is the one between s
+// and the other neighbours of t.  Subdomains depend on their -x, -y, -z
+// neighbours only, so the factorisation and both triangular sweeps run by
+// hyperplanes sx+sy+sz = p inside one workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kBiccWaves = 8;
+
+// Factorisation: one wavefront per subdomain of a hyperplane; the waves of
+// the workgroup step through the plane in uniform rounds.  dinv[s] holds
+// D_s^{-1} (k x k, row stride kmax).  A non-positive pivot sets *fail to the
+// subdomain index + 1.
+__global__ void __launch_bounds__(64 * kBiccWaves) k_bicc_factor(
+    Geo g, const double* __restrict__ Acb, int kmax, const int* __restrict__ plane,
+    const int* __restrict__ elem, int nplanes, double* __restrict__ dinv,
+    int* __restrict__ fail) {
+  __shared__ double Dsh[kBiccWaves][kMaxEig * kMaxEig];
+  __shared__ double Csh[kBiccWaves][kMaxEig * kMaxEig];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t kk = (size_t)kmax * kmax;
+  double* D = Dsh[wv];
+  double* C = Csh[wv];
+  for (int p = 0; p < nplanes; ++p) {
+    const int cnt = plane[p + 1] - plane[p];
+    for (int r0 = 0; r0 < cnt; r0 += kBiccWaves) {
+      const bool act = r0 + wv < cnt;
+      const int s = act ? elem[plane[p] + r0 + wv] : 0;
+      const Brick& b = g.bricks[s];
+      const int k = b.neig;
+      const double* blk = Acb + (size_t)s * 7 * kk;
+      if (act)
+        for (int e = lane; e < k * k; e += 64) D[e] = blk[(e / k) * kmax + e % k];
+      __syncthreads();
+      for (int dir = 0; dir < 6; dir += 2) {
+        const int t = act ? b.nb[dir] : -1;
+        const int kt = t >= 0 ? g.bricks[t].neig : 0;
+        const double* B = blk + (size_t)(dir + 1) * kk;  // A_st, columns over t's dofs
+        const double* Dt = dinv + (size_t)(t >= 0 ? t : 0) * kk;
+        for (int e = lane; e < k * kt; e += 64) {  // C = A_st D_t^{-1}
+          const int i = e / kt, j = e % kt;
+          double acc = 0.0;
+          for (int l = 0; l < kt; ++l) acc += B[i * kmax + l] * Dt[l * kmax + j];
+          C[e] = acc;
+        }
+        __syncthreads();
+        for (int e = lane; e < k * k && kt > 0; e += 64) {  // D -= C A_st^T
+          const int i = e / k, j = e % k;
+          double acc = 0.0;
+          for (int l = 0; l < kt; ++l) acc += C[i * kt + l] * B[j * kmax + l];
+          D[e] -= acc;
+        }
+        __syncthreads();
+      }
+      if (act && lane == 0) {
+        // Cholesky D = L L^T (lower triangle of C), then D^{-1} = L^{-T} L^{-1}.
+        bool ok = true;
+        for (int j = 0; j < k && ok; ++j) {
+          double dj = 0.5 * (D[j * k + j] + D[j * k + j]);
+          for (int l = 0; l < j; ++l) dj -= C[j * k + l] * C[j * k + l];
+          if (!(dj > 0.0)) {
+            ok = false;
+            break;
+          }
+          const double ljj = sqrt(dj);
+          C[j * k + j] = ljj;
+          for (int i = j + 1; i < k; ++i) {
+            double v = 0.5 * (D[i * k + j] + D[j * k + i]);
+            for (int l = 0; l < j; ++l) v -= C[i * k + l] * C[j * k + l];
+            C[i * k + j] = v / ljj;
+          }
+        }
+        if (!ok) {
+          atomicMax(fail, s + 1);
+        } else {
+          // D <- L^{-1} (lower), column by column.
+          for (int j = 0; j < k; ++j) {
+            for (int i = 0; i < k; ++i) D[i * k + j] = 0.0;
+            D[j * k + j] = 1.0 / C[j * k + j];
+            for (int i = j + 1; i < k; ++i) {
+              double v = 0.0;
+              for (int l = j; l < i; ++l) v -= C[i * k + l] * D[l * k + j];
+              D[i * k + j] = v / C[i * k + i];
+            }
+          }
+          double* out = dinv + (size_t)s * kk;
+          for (int i = 0; i < k; ++i)
+            for (int j = 0; j <= i; ++j) {
+              double v = 0.0;
+              for (int l = i; l < k; ++l) v += D[l * k + i] * D[l * k + j];
+              out[i * kmax + j] = v;
+              out[j * kmax + i] = v;
+            }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// d = M^{-1} t with the block ICC(0) factor: forward w_s = D_s^{-1} (t_s -
+// sum_{lower t} A_st w_t), backward y_s = w_s - D_s^{-1} sum_{upper t} A_st
+// y_t (w, then y, live in d).  Lane group of kp >= neig lanes per subdomain,
+// one row each; vsh: blockDim doubles of LDS.  Must be called by the whole
+// workgroup.
+__device__ void bicc_sweeps(const Geo& g, const double* __restrict__ Acb, int kmax,
+                            const double* __restrict__ dinv, const int* __restrict__ plane,
+                            const int* __restrict__ elem, int nplanes, int kp,
+                            const double* __restrict__ t, double* __restrict__ d, double* vsh) {
+  const size_t kk = (size_t)kmax * kmax;
+  const int grp = threadIdx.x / kp, i = threadIdx.x % kp;
+  const int epr = blockDim.x / kp;
+  for (int dirn = 0; dirn < 2; ++dirn) {
+    const bool fwd = dirn == 0;
+    for (int pp = 0; pp < nplanes; ++pp) {
+      const int p = fwd ? pp : nplanes - 1 - pp;
+      const int cnt = plane[p + 1] - plane[p];
+      for (int r0 = 0; r0 < cnt; r0 += epr) {
+        const bool grp_act = r0 + grp < cnt && grp < epr;
+        const int s = grp_act ? elem[plane[p] + r0 + grp] : 0;
+        const Brick& b = g.bricks[s];
+        const bool act = grp_act && i < b.neig;
+        const double* blk = Acb + (size_t)s * 7 * kk;
+        double v = 0.0;
+        if (act) {
+          v = fwd ? t[b.dof + i] : 0.0;
+          for (int dir = fwd ? 0 : 1; dir < 6; dir += 2) {
+            const int o = b.nb[dir];
+            if (o < 0) continue;
+            const Brick& ob = g.bricks[o];
+            const double* B = blk + (size_t)(dir + 1) * kk + (size_t)i * kmax;
+            double acc = 0.0;
+            for (int j = 0; j < ob.neig; ++j) acc += B[j] * d[ob.dof + j];
+            v = fwd ? v - acc : v + acc;
+          }
+        }
+        vsh[threadIdx.x] = v;
+        __syncthreads();
+        if (act) {
+          const double* Di = dinv + (size_t)s * kk + (size_t)i * kmax;
+          const double* vg = vsh + grp * kp;
+          double acc = 0.0;
+          for (int j = 0; j < b.neig; ++j) acc += Di[j] * vg[j];
+          d[b.dof + i] = fwd ? acc : d[b.dof + i] - acc;
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// d[:, c] = M^{-1} t[:, c] for column c = blockIdx.x (stride nc).
+__global__ void __launch_bounds__(1024) k_bicc_solve(Geo g, const double* __restrict__ Acb,
+                                                     int kmax, const double* __restrict__ dinv,
+                                                     const int* __restrict__ plane,
+                                                     const int* __restrict__ elem, int nplanes,
+                                                     int kp, const double* __restrict__ t,
+                                                     double* __restrict__ d, int nc) {
+  __shared__ double vsh[1024];
+  bicc_sweeps(g, Acb, kmax, dinv, plane, elem, nplanes, kp, t + (size_t)blockIdx.x * nc,
+              d + (size_t)blockIdx.x * nc, vsh);
+}
+
+static int bicc_lanes(int kmax) {
+  int kp = 1;
+  while (kp < kmax) kp *= 2;
+  return kp;
+}
+
+int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
+  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), 0, ctx->stream, ctx->geo, ctx->Acb.p,
+                     ctx->en_lv1, ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p,
+                     ctx->bicc_nplanes, bicc_lanes(ctx->en_lv1), t, d, ctx->nc);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int bicc_setup(pmx_ctx* ctx) {
+  const int sdx = ctx->sd[0], sdy = ctx->sd[1], sdz = ctx->sd[2];
+  const int np = sdx + sdy + sdz - 2;
+  std::vector<int> start(np + 1, 0), elems;
+  elems.reserve(ctx->nsub);
+  for (int p = 0; p < np; ++p) {
+    start[p] = (int)elems.size();
+    for (int s = 0; s < ctx->nsub; ++s) {
+      const int sx = s % sdx, sy = (s / sdx) % sdy, sz = s / (sdx * sdy);
+      if (sx + sy + sz == p) elems.push_back(s);
+    }
+  }
+  start[np] = (int)elems.size();
+  PMX_TRY(ctx->bicc_plane.alloc(np + 1));
+  PMX_TRY(ctx->bicc_elem.alloc(ctx->nsub));
+  PMX_HIP(hipMemcpy(ctx->bicc_plane.p, start.data(), (np + 1) * sizeof(int),
+                    hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(ctx->bicc_elem.p, elems.data(), ctx->nsub * sizeof(int),
+                    hipMemcpyHostToDevice));
+  ctx->bicc_nplanes = np;
+  const size_t kk = (size_t)ctx->en_lv1 * ctx->en_lv1;
+  PMX_TRY(ctx->bicc_dinv.alloc((size_t)ctx->nsub * kk));
+  PMX_HIP(hipMemsetAsync(ctx->bicc_dinv.p, 0, ctx->bicc_dinv.n * sizeof(double), ctx->stream));
+  DBuf<int> fail;
+  PMX_TRY(fail.alloc(1));
+  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_bicc_factor, dim3(1), dim3(64 * kBiccWaves), 0, ctx->stream, ctx->geo,
+                     ctx->Acb.p, ctx->en_lv1, ctx->bicc_plane.p, ctx->bicc_elem.p, np,
+                     ctx->bicc_dinv.p, fail.p);
+  PMX_HIP(hipGetLastError());
+  int hf = 0;
+  PMX_HIP(hipMemcpyAsync(&hf, fail.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  if (hf) {
+    set_error("level-2 ICC(0) of the coarse matrix: non-positive pivot in subdomain %d "
+              "(PETSc would apply its positive-definite shift; not implemented)", hf - 1);
+    return PMX_ERR_NOT_CONVERGED;
+  }
+  return PMX_SUCCESS;
+}
+
 // One workgroup runs a segment of the coarse program of PC_apply_vec: the
 // stages between two dense GEMVs.  Each op acts on the coarse residual t and
 // the accumulated correction u:
 //   op.kind 2 (level 2): d = Woodbury fix of y = B^{-1} t (the GEMV before the
 //                        segment), only as the segment's first op;
 //   op.kind 3 (level 3): d = V A_cc^{-1} V^T t;
+//   op.kind 4 (level 2 with the block ICC(0) factor of the spe10 fork):
+//                        d = M^{-1} t by the hyperplane sweeps;
 // then u = d (first) or u += d, and t -= A_c d unless it is the program's
 // last stage.
 struct CoarseOp {
@@ -542,17 +774,26 @@ struct CoarseSeg {
   CoarseOp op[8];
 };
 
-__global__ void k_coarse_seg(Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax,
-                             const double* __restrict__ y, double* __restrict__ t,
-                             double* __restrict__ u, double* __restrict__ d,
-                             const double* __restrict__ z, const double* __restrict__ h,
-                             const double* __restrict__ cs, const double* __restrict__ V, int k2,
-                             const double* __restrict__ LU, int nc) {
-  __shared__ double red[8];
+struct BiccArgs {
+  const double* dinv;
+  const int* plane;
+  const int* elem;
+  int nplanes, kp;
+};
+
+__global__ void __launch_bounds__(1024) k_coarse_seg(
+    Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax, const double* __restrict__ y,
+    double* __restrict__ t, double* __restrict__ u, double* __restrict__ d,
+    const double* __restrict__ z, const double* __restrict__ h, const double* __restrict__ cs,
+    const double* __restrict__ V, int k2, const double* __restrict__ LU, int nc, BiccArgs bi) {
+  __shared__ double red[16];
   __shared__ double cvec[kMaxEig];
+  __shared__ double vsh[1024];
   for (int q = 0; q < seg.nops; ++q) {
     const CoarseOp op = seg.op[q];
-    if (op.kind == 2) {
+    if (op.kind == 4) {
+      bicc_sweeps(g, Acb, kmax, bi.dinv, bi.plane, bi.elem, bi.nplanes, bi.kp, t, d, vsh);
+    } else if (op.kind == 2) {
       double zt = 0.0, ht = 0.0;
       for (int r = threadIdx.x; r < nc; r += blockDim.x) {
         zt += z[r] * t[r];
@@ -811,25 +1052,30 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
 int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
                           double* y, double* d, int kt_gemv, int kt_small,
                           int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int)) {
+  const bool icc = ctx->lv2_icc != 0;
+  const BiccArgs bi{ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p, ctx->bicc_nplanes,
+                    bicc_lanes(ctx->en_lv1)};
   size_t k = 0;
   while (k < prog.size()) {
     CoarseSeg seg{};
-    if (prog[k] == 2) {
+    if (prog[k] == 2 && !icc) {
       const int ev = kt_begin(ctx, kt_gemv);
       PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, y, ctx->nc));
       kt_end(ctx, ev);
     }
+    // With the dense inverse a segment ends before the next level-2 stage
+    // (its GEMV runs on the whole device); the ICC(0) sweeps run inside.
     do {
       CoarseOp& op = seg.op[seg.nops++];
-      op.kind = prog[k];
+      op.kind = (prog[k] == 2 && icc) ? 4 : prog[k];
       op.first = k == 0;
       op.update_t = k + 1 < prog.size();
       ++k;
-    } while (k < prog.size() && prog[k] == 3 && seg.nops < 8);
+    } while (k < prog.size() && (icc || prog[k] == 3) && seg.nops < 8);
     const int ev = kt_begin(ctx, kt_small);
-    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->geo, seg,
-                       ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p, ctx->c_cs.p,
-                       ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc);
+    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(icc ? 1024 : kBlock), 0, ctx->stream, ctx->geo,
+                       seg, ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p,
+                       ctx->c_cs.p, ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc, bi);
     PMX_HIP(hipGetLastError());
     kt_end(ctx, ev);
   }
@@ -1221,6 +1467,103 @@ int level3_setup(pmx_ctx* ctx) {
                          ctx->stream));
   PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
   // AV = A_c V (Z holds A_c X before the sign flip; k_lv3_finish applies it).
+  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, Z.p, n, num,
+                     ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
+  PMX_HIP(hipGetLastError());
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  return PMX_SUCCESS;
+}
+
+// Level 3 of the spe10 fork (src/spe10.c:1255-1417): the smallest en_lv2
+// eigenpairs of A_c without a dense factorisation, by block preconditioned
+// inverse iteration with the level-2 ICC(0) factor M and Rayleigh-Ritz over
+// m >= 4 en_lv2 vectors:  Y = X - M^{-1} (A_c X - X diag(w)).  Converged
+// when ||A_c x - w x|| <= eig_tol |w| for every wanted pair (x has unit
+// norm); a residual that stagnates (no 2x gain over 20 iterations) below
+// 100 eig_tol is accepted and reported in lv3_floor.
+int level3_setup_icc(pmx_ctx* ctx) {
+  const int n = ctx->nc;
+  const int k2 = std::min(ctx->en_lv2, n);
+  const int m = std::min(std::max(4 * k2, k2 + 8), std::min(n, 2 * kMaxEig));
+  DBuf<double> X, Y, Z, R, w, res;
+  DBuf<int> fail;
+  for (DBuf<double>* b : {&X, &Y, &Z, &R}) PMX_TRY(b->alloc((size_t)n * m));
+  PMX_TRY(w.alloc(m));
+  PMX_TRY(res.alloc(m));
+  PMX_TRY(fail.alloc(1));
+  std::vector<double> h((size_t)n * m);
+  std::mt19937_64 rng(ctx->seed + 7);
+  std::normal_distribution<double> nd;
+  for (auto& v : h) v = nd(rng);
+  PMX_HIP(hipMemcpyAsync(R.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
+  PMX_TRY(launch_bicc_solve(ctx, R.p, Y.p, m));
+  const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
+  std::vector<double> hw(m), hres(m), hist;
+  const int maxit = std::max(ctx->eig_maxit, 2000);
+  int it = 0;
+  bool conv = false;
+  ctx->lv3_floor = 0.0;
+  for (it = 1; it <= maxit; ++it) {
+    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 0,
+                       w.p, res.p, fail.p);
+    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n, m);
+    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 1,
+                       w.p, res.p, fail.p);
+    PMX_HIP(hipGetLastError());
+    PMX_HIP(hipMemcpyAsync(hres.data(), res.p, m * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PMX_HIP(hipMemcpyAsync(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));
+    double mres = 0.0;
+    for (int j = 0; j < k2; ++j) mres = std::max(mres, hres[j] / std::max(std::fabs(hw[j]), 1e-300));
+    hist.push_back(mres);
+    if (ctx->eig_debug >= 0 && (it < 20 || it % 50 == 0))
+      printf("lv3 icc it %d res %.3e w0 %.6e\n", it, mres, hw[0]);
+    if (mres <= ctx->eig_tol) {
+      conv = true;
+      break;
+    }
+    if (it >= 40 && mres <= 100.0 * ctx->eig_tol) {
+      double best_old = 1e300;
+      for (size_t q = 0; q + 20 < hist.size(); ++q) best_old = std::min(best_old, hist[q]);
+      if (mres > 0.5 * best_old) {
+        conv = true;
+        ctx->lv3_floor = mres;
+        break;
+      }
+    }
+    hipLaunchKernelGGL(k_lv3_refine, dim3(std::min((n * m + kBlock - 1) / kBlock, 4096)),
+                       dim3(kBlock), 0, ctx->stream, Z.p, X.p, w.p, R.p, n, m);
+    PMX_TRY(launch_bicc_solve(ctx, R.p, Y.p, m));
+    hipLaunchKernelGGL(k_lv3_update, dim3(m), dim3(kBlock), 0, ctx->stream, X.p, Y.p, n, res.p);
+    PMX_HIP(hipGetLastError());
+  }
+  int hfail = 0;
+  PMX_HIP(hipMemcpy(&hfail, fail.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (!conv || hfail) {
+    set_error("level-3 eigensolver (ICC(0)-preconditioned) did not converge (%d iterations, "
+              "fail=%d)", it, hfail);
+    return PMX_ERR_USER;
+  }
+  ctx->lv3_iters = it;
+  int num = k2;
+  for (int j = 0; j < k2; ++j)
+    if (hw[j] >= ctx->eb_lv2) {
+      num = j + 1;
+      break;
+    }
+  ctx->eigen_num_lv2 = num;
+  ctx->eigen_min_lv2 = hw[0];
+  ctx->eigen_max_lv2 = hw[num - 1];
+  ctx->h_eig_lv2.assign(hw.begin(), hw.begin() + k2);
+  PMX_TRY(ctx->V2.alloc((size_t)n * num));
+  PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
   hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, Z.p, n, num,
                      ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
   PMX_HIP(hipGetLastError());

@@ -92,7 +92,7 @@ __global__ void k_trans(Geo g, const double* __restrict__ kx, const double* __re
 // reference's accumulation order: [lower x if first cell of the row], upper x,
 // lower y, upper y, lower z, upper z, [upper x again if not last of the row];
 // each term 2*kbar/H/H, then scaled by |E|.  MEAN: kx+ky+kz.  DIAG: diagonal
-// of the Neumann block.
+// of the Neumann block, with the well term (src/spe10.c:990-1012).
 // ---------------------------------------------------------------------------
 __global__ void k_mass(Geo g, int op, const double* __restrict__ kx, const double* __restrict__ ky,
                        const double* __restrict__ kz, const double* __restrict__ Tx,
@@ -115,6 +115,7 @@ __global__ void k_mass(Geo g, int op, const double* __restrict__ kx, const doubl
       if (ly < b.ny - 1) v += Ty[c + b.nx];
       if (lz > 0) v += Tz[c];
       if (lz < b.nz - 1) v += Tz[c + b.nx * b.ny];
+      if (g.wd) v += g.wd[c];
       m[c] = v;
       continue;
     }
@@ -147,8 +148,9 @@ __global__ void k_mass(Geo g, int op, const double* __restrict__ kx, const doubl
 // The 7-point graph has no triangles, so ICC(0) in natural order keeps only
 // the pivots: d = a - cz^2/d_{-z} - cy^2/d_{-y} - cx^2/d_{-x} (couplings inside
 // the subdomain only).  mode 0: a = diagonal of A[Omega_i, Omega_i] in the
-// accumulation order of _PC_setup_lv1_J (src/PreMixFEM_3D.c:93-197), + pin;
-// mode 1: a = Neumann diagonal + sigma * mass (the LOBPCG preconditioner).
+// accumulation order of _PC_setup_lv1_J (src/PreMixFEM_3D.c:93-197), + pin,
+// + well term (src/spe10.c:921-927); mode 1: a = Neumann diagonal (+ well
+// term) + sigma * mass (the LOBPCG preconditioner).
 // Writes plane-ordered {dinv, cx, cy, cz} for the lower and upper couplings.
 // ---------------------------------------------------------------------------
 template <typename F4T>
@@ -190,6 +192,7 @@ __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
         if (in_hi[1]) a = a + hi[1];
         if (in_hi[2]) a = a + hi[2];
         if (c == g.pin) a = a + g.tau;
+        if (g.wd) a = a + g.wd[c];
       } else {
         a = 0.0;
         if (lx > 0) a += lo[0];
@@ -198,6 +201,7 @@ __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
         if (in_hi[1]) a += hi[1];
         if (lz > 0) a += lo[2];
         if (in_hi[2]) a += hi[2];
+        if (g.wd) a += g.wd[c];
         a += sigma * mass[c];
       }
       if (lz > 0) a = a - lo[2] * lo[2] / dsh[li - nxny];
@@ -355,6 +359,8 @@ __device__ __forceinline__ ACC stencil_fast(const Geo& g, const BrickView& v, in
     acc += tl * (ACC)x[nl >= 0 ? nl : c] + th * (ACC)x[nh >= 0 ? nh : c];
   }
   if (c == g.pin) diag += (ACC)g.tau;
+  const TT* W = well_term<TT>(g);
+  if (W) diag += (ACC)W[c];
   return diag * (ACC)x[c] - acc;
 }
 
@@ -507,6 +513,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
       if (nh >= 0) off += th * pval(nh);
     }
     if (c == g.pin) diag += g.tau;
+    if (g.wd) diag += g.wd[c];
     const double wc = diag * pc - off;
     pnew[c] = pc;
     w[c] = wc;
@@ -690,22 +697,29 @@ __global__ void __launch_bounds__(kStreamBlock) k_prolong(Geo g, S* __restrict__
   }
 }
 
-// r = b - A x, partial r'r.
+// r = b - A x, partial r'r; with an initial guess (x != null) also the
+// partial b'b, written nb entries further on.
 __global__ void __launch_bounds__(kStreamBlock) k_residual(
     Geo g, const double* __restrict__ bvec, const double* __restrict__ x, double* __restrict__ r,
     const double* __restrict__ Tx, const double* __restrict__ Ty, const double* __restrict__ Tz,
     double* __restrict__ partial) {
   __shared__ double red[16];
   const BrickView v = load_view(g, block_brick(g));
-  double acc = 0.0;
+  double acc = 0.0, bacc = 0.0;
   for_cells(v, [&](int c, int lx, int ly, int lz) {
-    double rr = bvec[c];
+    const double bc = bvec[c];
+    double rr = bc;
     if (x) rr -= stencil_fast<double, double, double>(g, v, c, lx, ly, lz, Tx, Ty, Tz, x);
     r[c] = rr;
     acc += rr * rr;
+    bacc += bc * bc;
   });
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+  if (x) {
+    bacc = block_sum(bacc, red);
+    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = bacc;
+  }
 }
 
 // y = A x (brick order), FP64.
@@ -720,22 +734,60 @@ __global__ void __launch_bounds__(kStreamBlock) k_spmv(Geo g, const double* __re
   });
 }
 
-// Start of the PCG: ||r0||^2 from the residual partials, ttol =
-// max(rtol ||r0||, abstol), its = 0 (KSPConvergedDefault at iteration 0).
-__global__ void k_cg_start(const double* __restrict__ partial, int nb, double* __restrict__ scal,
-                           double rtol, double abstol) {
+// Start of the PCG: ||r0||^2 from the residual partials, its = 0 and the
+// reference norm of KSPConvergedDefault at iteration 0: ||r0|| for a zero
+// initial guess, ||b|| for a nonzero one (||r0|| if b = 0); ttol =
+// max(rtol * rnorm0, abstol).  bpart: the b'b partials, null for a zero guess.
+__global__ void k_cg_start(const double* __restrict__ partial, const double* __restrict__ bpart,
+                           int nb, double* __restrict__ scal, double rtol, double abstol) {
   if (threadIdx.x >= 64) return;
   const double acc = wave_sum(partial, nb);
+  const double bacc = bpart ? wave_sum(bpart, nb) : 0.0;
   if (threadIdx.x != 0) return;
+  const double ref = bacc > 0.0 ? bacc : acc;
   scal[kSRnorm2] = acc;
-  scal[kSRnorm0] = acc;
+  scal[kSRnorm0] = ref;
   const double dp = sqrt(acc);
-  const double ttol = fmax(rtol * dp, abstol);
+  const double ttol = fmax(rtol * sqrt(ref), abstol);
   scal[kSTtol] = ttol;
   scal[kSIts] = 0.0;
   scal[kSIter] = 0.0;
   scal[kSDone] = dp <= ttol ? 1.0 : 0.0;
   scal[kSReason] = dp <= ttol ? (dp < abstol ? PMX_CONVERGED_ATOL : PMX_CONVERGED_RTOL) : 0.0;
+}
+
+// RT0 face fluxes (velocity recovery): F = T (p_L - p_R) across the lower
+// x/y/z face of every cell (the face coefficient of PC_create_A,
+// src/PreMixFEM_3D.c:1785-1810), 0 on the domain boundary.  p is in brick
+// order, the three flux arrays in natural order.
+__global__ void __launch_bounds__(kStreamBlock) k_velocity(
+    Geo g, const double* __restrict__ p, const double* __restrict__ Tx,
+    const double* __restrict__ Ty, const double* __restrict__ Tz, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz) {
+  const int s = block_brick(g);
+  const BrickView v = load_view(g, s);
+  const Brick b = g.bricks[s];
+  for_cells(v, [&](int c, int lx, int ly, int lz) {
+    int nb[6];
+    nbr6(v, c, lx, ly, lz, nb);
+    const size_t nat = ((size_t)(b.gz + lz) * g.N + (b.gy + ly)) * g.M + (b.gx + lx);
+    const double pc = p[c];
+    fx[nat] = nb[0] >= 0 ? Tx[c] * (p[nb[0]] - pc) : 0.0;
+    fy[nat] = nb[2] >= 0 ? Ty[c] * (p[nb[2]] - pc) : 0.0;
+    fz[nat] = nb[4] >= 0 ? Tz[c] * (p[nb[4]] - pc) : 0.0;
+  });
+}
+
+// First index whose value is not finite or not > 0 (strict) / < 0 (!strict),
+// or stays INT_MAX.
+__global__ void k_check_field(const double* __restrict__ a, int n, int strict,
+                              int* __restrict__ first) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double v = a[i];
+    const bool ok = strict ? (v > 0.0 && v <= 1.7976931348623157e308)
+                           : (v >= 0.0 && v <= 1.7976931348623157e308);
+    if (!ok) atomicMin(first, i);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -946,8 +998,9 @@ int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
   hipLaunchKernelGGL(k_residual, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
                      ctx->geo, b, x, r, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, part);
   PMX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, ctx->stream, part, ctx->nsub,
-                     ctx->scal.p, ctx->cg_rtol, ctx->cg_abstol);
+  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, ctx->stream, part,
+                     x ? part + ctx->nsub : nullptr, ctx->nsub, ctx->scal.p, ctx->cg_rtol,
+                     ctx->cg_abstol);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -969,6 +1022,29 @@ int launch_spmv(pmx_ctx* ctx, const double* x, double* y) {
   hipLaunchKernelGGL(k_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
                      ctx->geo, x, y, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p);
   PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_velocity(pmx_ctx* ctx, const double* p, double* fx, double* fy, double* fz) {
+  hipLaunchKernelGGL(k_velocity, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
+                     ctx->geo, p, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, fx, fy, fz);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+int launch_check_field(pmx_ctx* ctx, const double* a, int n, bool strict, int* bad) {
+  DBuf<int> flag;
+  PMX_TRY(flag.alloc(1));
+  const int init = 0x7fffffff;
+  PMX_HIP(hipMemcpyAsync(flag.p, &init, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  const int nb = std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(k_check_field, dim3(nb), dim3(kBlock), 0, ctx->stream, a, n, strict ? 1 : 0,
+                     flag.p);
+  PMX_HIP(hipGetLastError());
+  int h = init;
+  PMX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  *bad = h == init ? -1 : h;
   return PMX_SUCCESS;
 }
 

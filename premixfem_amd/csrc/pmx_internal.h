@@ -83,7 +83,18 @@ struct Geo {
   int pin;                // brick index of the pinned cell, -1 if none
   double tau;             // pin strength
   const double* stop;     // PCG "done" flag: kernels return at once when set (nullptr: none)
+  const double* wd;       // per-cell diagonal (well) term, brick order; nullptr: none
+  const float* wd32;      // its FP32 copy (FP32 preconditioner)
 };
+
+// The well term of the operator diagonal in the precision of the face
+// coefficients TT (nullptr when there is none).
+template <typename TT>
+__device__ __forceinline__ const TT* well_term(const Geo& g);
+template <>
+__device__ __forceinline__ const double* well_term<double>(const Geo& g) { return g.wd; }
+template <>
+__device__ __forceinline__ const float* well_term<float>(const Geo& g) { return g.wd32; }
 
 // What a captured PCG chunk depends on besides the context's buffers.
 struct CgGraphKey {
@@ -292,6 +303,11 @@ struct pmx_ctx {
   pmx::DBuf<float> T32[3];
   double tau = 0.0;
   int pin_brick = -1;
+  // Per-cell diagonal term of the operator (the producer wells of
+  // src/spe10.c:780-786), brick order; empty = none.
+  int have_wd = 0;
+  pmx::DBuf<double> wd;
+  pmx::DBuf<float> wd32;
 
   // ---- level 1 ----
   pmx::DBuf<double> fac_f, fac_b;   // plane-major {dinv, cx, cy, cz} lower / upper couplings
@@ -313,6 +329,13 @@ struct pmx_ctx {
   double coarse_gamma = 0.0;
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
+  // Level-2 factor of the spe10 fork (src/spe10.c:509-516): ICC(0) of A_c,
+  // i.e. a block incomplete Cholesky over the subdomains (-lv2_icc).
+  int lv2_icc = 0;
+  pmx::DBuf<double> bicc_dinv;      // [nsub][kmax][kmax] inverse pivot blocks
+  pmx::DBuf<int> bicc_plane;        // coarse hyperplane prefix counts (nplanes+1)
+  pmx::DBuf<int> bicc_elem;         // subdomains in hyperplane order
+  int bicc_nplanes = 0;
   int max_plane = 0;                // most cells in one hyperplane of any brick shape
   int lds_max = 65536;              // LDS bytes per workgroup of the device
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0, eig_stagnated = 0;

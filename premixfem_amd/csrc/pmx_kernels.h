@@ -29,6 +29,8 @@ int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* 
 int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r);
 int launch_spmv(pmx_ctx* ctx, const double* x, double* y);
 int launch_to_f32(pmx_ctx* ctx, const double* src, float* dst, size_t n);
+int launch_velocity(pmx_ctx* ctx, const double* p, double* fx, double* fy, double* fz);
+int launch_check_field(pmx_ctx* ctx, const double* a, int n, bool strict, int* bad);
 
 // coarse levels (pmx_coarse.hip)
 int launch_galerkin(pmx_ctx* ctx);
@@ -46,6 +48,10 @@ int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n
 int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, int first,
                          int update_t);
 int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d);
+// level 2 of the spe10 fork: block ICC(0) of A_c
+int bicc_setup(pmx_ctx* ctx);
+int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol);
+int level3_setup_icc(pmx_ctx* ctx);
 
 // level-2 eigensolver (pmx_lobpcg.hip)
 int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows);

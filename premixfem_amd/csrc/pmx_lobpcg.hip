@@ -4,7 +4,9 @@
 //
 // * The constant vector is the exact first eigenpair (A_i^N 1 = 0) and is
 //   deflated analytically; LOBPCG iterates on the next en_lv1-1 pairs with a
-//   block of MB vectors (MB >= en_lv1-1 + 3 guard vectors).
+//   block of MB vectors (MB >= en_lv1-1 + 3 guard vectors).  A subdomain with
+//   a well term (the spe10 producers, src/spe10.c:1005-1012) has no null
+//   vector: there LOBPCG iterates on all en_lv1 pairs without deflation.
 // * Preconditioner: the FP32 block ICC(0) of A_i^N + sigma M_i, applied with
 //   the same hyperplane sweeps as level 1.
 // * The basis S = [X W P] is kept M-orthonormal block by block (W and P are
@@ -137,7 +139,8 @@ __device__ __forceinline__ void wg_colnorm2(const double* __restrict__ B,
 template <int NB>
 __device__ inline void wg_apply_AN(const BrickView& v, const double* __restrict__ Tx,
                                    const double* __restrict__ Ty, const double* __restrict__ Tz,
-                                   const double* __restrict__ V, double* __restrict__ Y, int nb) {
+                                   const double* __restrict__ wd, const double* __restrict__ V,
+                                   double* __restrict__ Y, int nb) {
   const int n = v.n;
   for_cells(v, [&](int c, int lx, int ly, int lz) {
     const int li = c - v.off;
@@ -147,13 +150,15 @@ __device__ inline void wg_apply_AN(const BrickView& v, const double* __restrict_
     const int o0 = lx > 0 ? li - 1 : li, o1 = lx < v.nx - 1 ? li + 1 : li;
     const int o2 = ly > 0 ? li - v.nx : li, o3 = ly < v.ny - 1 ? li + v.nx : li;
     const int o4 = lz > 0 ? li - v.nxny : li, o5 = lz < v.nz - 1 ? li + v.nxny : li;
+    const double wc = wd ? wd[c] : 0.0;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j >= nb) break;
       const double* x = V + (size_t)j * n;
       const double xc = x[li];
       Y[(size_t)j * n + li] = tl0 * (xc - x[o0]) + th0 * (xc - x[o1]) + tl1 * (xc - x[o2]) +
-                              th1 * (xc - x[o3]) + tl2 * (xc - x[o4]) + th2 * (xc - x[o5]);
+                              th1 * (xc - x[o3]) + tl2 * (xc - x[o4]) + th2 * (xc - x[o5]) +
+                              wc * xc;
     }
   });
   __syncthreads();
@@ -415,15 +420,26 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag;
   __shared__ double s_rbest;
 
+  __shared__ int s_well;
   {
-    double s = 0.0;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) s += mass[r];
+    double s = 0.0, wsum = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      s += mass[r];
+      if (g.wd) wsum += g.wd[v.off + r];
+    }
     s = block_sum(s, red);
-    if (threadIdx.x == 0) msum_s = s;
+    wsum = block_sum(wsum, red);
+    if (threadIdx.x == 0) {
+      msum_s = s;
+      s_well = wsum > 0.0;
+    }
     __syncthreads();
   }
   const double msum = msum_s;
-  const int kw = a.kwant;
+  // A subdomain with a well term has no null vector: all en_lv1 pairs are
+  // iterated and the constant is not deflated.
+  const bool well = s_well != 0;
+  const int kw = well ? a.kwant + 1 : a.kwant;
   int it = 0, status = 0, np = 0;
   unsigned long long tph[10] = {0}, tlast = 0;
   const bool trace = sidx == a.debug && threadIdx.x == 0;
@@ -435,6 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   }
   // B -= 1 (sum m B / sum m) for the MB columns of B.
   auto deflate = [&](double* B) {
+    if (well) return;
     wg_multidot<MB, 1>(B, mass, nullptr, n, red, TM, 1);
     for (int e = threadIdx.x; e < MB * n; e += blockDim.x) B[e] -= TM[e / n] / msum;
     __syncthreads();
@@ -468,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
         continue;
       }
       wg_transform<MB>(X, n, Z, LD, MB, MB);
-      wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, X, AX, MB);
+      wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, X, AX, MB);
       wg_multidot<MB, MB>(X, AX, nullptr, n, red, GA, LD);
       for (int e = threadIdx.x; e < MB * MB; e += blockDim.x) {
         const int i = e / MB, j = e % MB;
@@ -575,7 +592,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
         }
         wg_transform<MB>(W, n, Z, LD, MB, nw);
         PMX_PHASE(4)
-        wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, W, AW, nw);
+        wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, W, AW, nw);
         PMX_PHASE(5)
         // ---- Gram blocks of S = [X W P] (compact, ns = MB + nw + np): the
         // diagonal blocks of G_B are I and G_A(X,X) = Lambda by construction;
@@ -713,7 +730,9 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   // ---- outputs ----
   const double phi0 = 1.0 / sqrt(msum);
   double* phi = a.phi + (size_t)v.off * a.kmax;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) phi[(size_t)r * a.kmax] = phi0;
+  const int c0 = well ? 0 : 1;  // first basis column LOBPCG fills
+  if (!well)
+    for (int r = threadIdx.x; r < n; r += blockDim.x) phi[(size_t)r * a.kmax] = phi0;
   if (kw > 0 && status == 0) {
     // sign: positive inner product with 1 + frac(0.618..(r+1))
     double acc[MB];
@@ -740,12 +759,12 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
     __syncthreads();
     for (int e = threadIdx.x; e < kw * n; e += blockDim.x) {
       const int j = e / n, r = e % n;
-      phi[(size_t)r * a.kmax + 1 + j] = nrm[j] * X[(size_t)j * n + r];
+      phi[(size_t)r * a.kmax + c0 + j] = nrm[j] * X[(size_t)j * n + r];
     }
   }
   if (threadIdx.x == 0) {
-    a.eig[(size_t)sidx * a.kmax] = 0.0;
-    for (int j = 0; j < kw; ++j) a.eig[(size_t)sidx * a.kmax + 1 + j] = lam[j];
+    if (!well) a.eig[(size_t)sidx * a.kmax] = 0.0;
+    for (int j = 0; j < kw; ++j) a.eig[(size_t)sidx * a.kmax + c0 + j] = lam[j];
     a.its[sidx] = it;
     a.status[sidx] = status;
     a.fallback[sidx] = a.x0 != nullptr && attempts > 0;
@@ -787,6 +806,11 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   if (MB > minn - 1) {
     set_error("subdomains of %d cells are too small for %d eigenvectors", minn, ctx->en_lv1);
     return PMX_ERR_ARG_WRONG;
+  }
+  if (ctx->have_wd && MB < ctx->en_lv1 + 1) {
+    // well subdomains iterate on all en_lv1 pairs and need one guard vector
+    set_error("en_lv1=%d with a well term needs -eig_guard >= 2", ctx->en_lv1);
+    return PMX_ERR_SUP;
   }
   // LOBPCG preconditioner: FP32 block ICC(0) of A^N + sigma M.
   DBuf<float> pf, pb;

@@ -148,6 +148,24 @@ def layered_kappa(mesh, sigma, corr_cells, kmin, kmax, seed, kz_ratio=0.1):
     return np.stack([kx, kx.copy(), kz_ratio * kx])
 
 
+MD_TO_REF = 9.869233e-16 * 10.7639   # 1 mD in m^2, times the 10.7639 scaling of src/spe10.c:379
+
+
+def reservoir_fields(mesh, seed, sigma=2.5, corr_cells=(20, 20, 2), kmin=1e-3, kmax=2e4,
+                     kz_ratio=0.1):
+    """Synthetic rock for the IMPES workload (the reference reads SPE10
+    model 2 from ``data/spe10/SPE10_rock.h5``, which is not available):
+    layered lognormal kx = ky in mD clipped to [kmin, kmax], kz = kz_ratio kx,
+    converted to the reference's units (m^2 times 10.7639); porosity from the
+    same field, 0.05 + 0.25 * (rank of log k), so high-permeability layers are
+    also porous.  Returns (perm (3, P, N, M), poro (P, N, M))."""
+    k = layered_kappa(mesh, sigma, corr_cells, kmin, kmax, seed, kz_ratio)
+    lk = np.log(k[0])
+    rank = np.argsort(np.argsort(lk.ravel())).reshape(lk.shape) / max(lk.size - 1, 1)
+    poro = 0.05 + 0.25 * rank
+    return k * MD_TO_REF, poro
+
+
 def fracture_kappa(mesh, sigma, corr_cells, nfrac, k_frac, seed):
     """Lognormal background plus ``nfrac`` one-cell-thick axis-aligned planes
     spanning 30-100 % of the domain in each in-plane direction."""
