@@ -97,7 +97,9 @@ typedef struct {
  * -sdx/-sdy/-sdz (per-direction subdomains), -pin x,y,z, -pc_precision 32|64,
  * -eig_tol, -eig_maxit, -eig_stag, -warm_start none|cnn|basis, -seed, -cg_graph,
  * -cg_chunk, -kernel_timing, and -lv2_icc (the spe10 fork's level-2 factor:
- * ICC(0) of the coarse matrix, src/spe10.c:509-516).  Allocates the
+ * ICC(0) of the coarse matrix, src/spe10.c:509-516; its level-3 eigenvectors
+ * come from ICC(0)-preconditioned block LOBPCG, started from the previous
+ * setup's unless -lv3_warm 0).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 

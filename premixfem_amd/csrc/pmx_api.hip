@@ -109,6 +109,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-lv3_warm") {
+      PMX_TRY(geti(c->lv3_warm));
     } else if (k == "-pc_precision") {
       PMX_TRY(geti(c->pc_precision));
     } else if (k == "-eig_tol") {
@@ -1342,6 +1344,7 @@ int pmx_final(pmx_ctx* c) {
   c->wd32.release();
   c->bicc_plane.release();
   c->bicc_elem.release();
+  c->bicc_pos.release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

@@ -639,66 +639,77 @@ __global__ void __launch_bounds__(64 * kBiccWaves) k_bicc_factor(
 }
 
 // d = M^{-1} t with the block ICC(0) factor: forward w_s = D_s^{-1} (t_s -
-// sum_{lower t} A_st w_t), backward y_s = w_s - D_s^{-1} sum_{upper t} A_st
-// y_t (w, then y, live in d).  Lane group of kp >= neig lanes per subdomain,
-// one row each; vsh: blockDim doubles of LDS.  Must be called by the whole
-// workgroup.
-__device__ void bicc_sweeps(const Geo& g, const double* __restrict__ Acb, int kmax,
-                            const double* __restrict__ dinv, const int* __restrict__ plane,
-                            const int* __restrict__ elem, int nplanes, int kp,
-                            const double* __restrict__ t, double* __restrict__ d, double* vsh) {
+// sum_{lower o} A_so w_o), backward y_s = w_s - D_s^{-1} sum_{upper o} A_so
+// y_o.  A subdomain's lower (upper) neighbours all lie on the previous
+// (next) hyperplane, so the values of the last plane live in an LDS ring of
+// two plane slots (slot entry pos[s]*kp + row): the dependency chain of the
+// sweep never waits on global memory.  w and y are also written to d.  Lane
+// group of kp >= neig lanes per subdomain, one row each.
+struct BiccArgs {
+  const double* dinv;
+  const int* plane;
+  const int* elem;
+  const int* pos;     // position of each subdomain inside its hyperplane
+  int nplanes, kp, maxcnt;
+};
+
+__global__ void __launch_bounds__(1024) k_bicc_solve(Geo g, const double* __restrict__ Acb,
+                                                     int kmax, BiccArgs bi,
+                                                     const double* __restrict__ tcol,
+                                                     double* __restrict__ dcol, int nc) {
+  extern __shared__ double ring[];   // [2][maxcnt * kp]
+  __shared__ double vsh[1024];
+  const double* t = tcol + (size_t)blockIdx.x * nc;
+  double* d = dcol + (size_t)blockIdx.x * nc;
   const size_t kk = (size_t)kmax * kmax;
+  const int kp = bi.kp;
   const int grp = threadIdx.x / kp, i = threadIdx.x % kp;
   const int epr = blockDim.x / kp;
+  const int slot = bi.maxcnt * kp;
   for (int dirn = 0; dirn < 2; ++dirn) {
     const bool fwd = dirn == 0;
-    for (int pp = 0; pp < nplanes; ++pp) {
-      const int p = fwd ? pp : nplanes - 1 - pp;
-      const int cnt = plane[p + 1] - plane[p];
+    for (int pp = 0; pp < bi.nplanes; ++pp) {
+      const int p = fwd ? pp : bi.nplanes - 1 - pp;
+      double* cur = ring + (p & 1) * slot;
+      const double* oth = ring + ((p + 1) & 1) * slot;  // plane p-1 (fwd) or p+1 (bwd)
+      const int cnt = bi.plane[p + 1] - bi.plane[p];
       for (int r0 = 0; r0 < cnt; r0 += epr) {
-        const bool grp_act = r0 + grp < cnt && grp < epr;
-        const int s = grp_act ? elem[plane[p] + r0 + grp] : 0;
+        const bool grp_act = r0 + grp < cnt;
+        const int q = r0 + grp;
+        const int s = grp_act ? bi.elem[bi.plane[p] + q] : 0;
         const Brick& b = g.bricks[s];
         const bool act = grp_act && i < b.neig;
         const double* blk = Acb + (size_t)s * 7 * kk;
-        double v = 0.0;
+        double v = 0.0, own = 0.0;
         if (act) {
           v = fwd ? t[b.dof + i] : 0.0;
+          if (!fwd) own = d[b.dof + i];
           for (int dir = fwd ? 0 : 1; dir < 6; dir += 2) {
             const int o = b.nb[dir];
             if (o < 0) continue;
-            const Brick& ob = g.bricks[o];
+            const int ko = g.bricks[o].neig;
             const double* B = blk + (size_t)(dir + 1) * kk + (size_t)i * kmax;
+            const double* xo = oth + bi.pos[o] * kp;
             double acc = 0.0;
-            for (int j = 0; j < ob.neig; ++j) acc += B[j] * d[ob.dof + j];
+            for (int j = 0; j < ko; ++j) acc += B[j] * xo[j];
             v = fwd ? v - acc : v + acc;
           }
         }
         vsh[threadIdx.x] = v;
         __syncthreads();
         if (act) {
-          const double* Di = dinv + (size_t)s * kk + (size_t)i * kmax;
+          const double* Di = bi.dinv + (size_t)s * kk + (size_t)i * kmax;
           const double* vg = vsh + grp * kp;
           double acc = 0.0;
           for (int j = 0; j < b.neig; ++j) acc += Di[j] * vg[j];
-          d[b.dof + i] = fwd ? acc : d[b.dof + i] - acc;
+          const double r = fwd ? acc : own - acc;
+          cur[q * kp + i] = r;
+          d[b.dof + i] = r;
         }
         __syncthreads();
       }
     }
   }
-}
-
-// d[:, c] = M^{-1} t[:, c] for column c = blockIdx.x (stride nc).
-__global__ void __launch_bounds__(1024) k_bicc_solve(Geo g, const double* __restrict__ Acb,
-                                                     int kmax, const double* __restrict__ dinv,
-                                                     const int* __restrict__ plane,
-                                                     const int* __restrict__ elem, int nplanes,
-                                                     int kp, const double* __restrict__ t,
-                                                     double* __restrict__ d, int nc) {
-  __shared__ double vsh[1024];
-  bicc_sweeps(g, Acb, kmax, dinv, plane, elem, nplanes, kp, t + (size_t)blockIdx.x * nc,
-              d + (size_t)blockIdx.x * nc, vsh);
 }
 
 static int bicc_lanes(int kmax) {
@@ -707,10 +718,16 @@ static int bicc_lanes(int kmax) {
   return kp;
 }
 
+static BiccArgs bicc_args(const pmx_ctx* ctx) {
+  return BiccArgs{ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p, ctx->bicc_pos.p,
+                  ctx->bicc_nplanes, bicc_lanes(ctx->en_lv1), ctx->bicc_maxcnt};
+}
+
 int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
-  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), 0, ctx->stream, ctx->geo, ctx->Acb.p,
-                     ctx->en_lv1, ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p,
-                     ctx->bicc_nplanes, bicc_lanes(ctx->en_lv1), t, d, ctx->nc);
+  const BiccArgs bi = bicc_args(ctx);
+  const size_t lds = 2 * (size_t)bi.maxcnt * bi.kp * sizeof(double);
+  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->geo, ctx->Acb.p,
+                     ctx->en_lv1, bi, t, d, ctx->nc);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -728,6 +745,20 @@ int bicc_setup(pmx_ctx* ctx) {
     }
   }
   start[np] = (int)elems.size();
+  std::vector<int> pos(ctx->nsub);
+  int maxcnt = 0;
+  for (int p = 0; p < np; ++p) {
+    maxcnt = std::max(maxcnt, start[p + 1] - start[p]);
+    for (int q = start[p]; q < start[p + 1]; ++q) pos[elems[q]] = q - start[p];
+  }
+  const size_t lds = 2 * (size_t)maxcnt * bicc_lanes(ctx->en_lv1) * sizeof(double);
+  if (lds + 1024 * sizeof(double) > 65536) {
+    set_error("-lv2_icc: a coarse hyperplane of %d subdomains exceeds the LDS plane ring", maxcnt);
+    return PMX_ERR_SUP;
+  }
+  PMX_TRY(ctx->bicc_pos.alloc(ctx->nsub));
+  PMX_HIP(hipMemcpy(ctx->bicc_pos.p, pos.data(), ctx->nsub * sizeof(int), hipMemcpyHostToDevice));
+  ctx->bicc_maxcnt = maxcnt;
   PMX_TRY(ctx->bicc_plane.alloc(np + 1));
   PMX_TRY(ctx->bicc_elem.alloc(ctx->nsub));
   PMX_HIP(hipMemcpy(ctx->bicc_plane.p, start.data(), (np + 1) * sizeof(int),
@@ -762,8 +793,6 @@ int bicc_setup(pmx_ctx* ctx) {
 //   op.kind 2 (level 2): d = Woodbury fix of y = B^{-1} t (the GEMV before the
 //                        segment), only as the segment's first op;
 //   op.kind 3 (level 3): d = V A_cc^{-1} V^T t;
-//   op.kind 4 (level 2 with the block ICC(0) factor of the spe10 fork):
-//                        d = M^{-1} t by the hyperplane sweeps;
 // then u = d (first) or u += d, and t -= A_c d unless it is the program's
 // last stage.
 struct CoarseOp {
@@ -774,26 +803,17 @@ struct CoarseSeg {
   CoarseOp op[8];
 };
 
-struct BiccArgs {
-  const double* dinv;
-  const int* plane;
-  const int* elem;
-  int nplanes, kp;
-};
-
-__global__ void __launch_bounds__(1024) k_coarse_seg(
-    Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax, const double* __restrict__ y,
-    double* __restrict__ t, double* __restrict__ u, double* __restrict__ d,
-    const double* __restrict__ z, const double* __restrict__ h, const double* __restrict__ cs,
-    const double* __restrict__ V, int k2, const double* __restrict__ LU, int nc, BiccArgs bi) {
-  __shared__ double red[16];
+__global__ void k_coarse_seg(Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax,
+                             const double* __restrict__ y, double* __restrict__ t,
+                             double* __restrict__ u, double* __restrict__ d,
+                             const double* __restrict__ z, const double* __restrict__ h,
+                             const double* __restrict__ cs, const double* __restrict__ V, int k2,
+                             const double* __restrict__ LU, int nc) {
+  __shared__ double red[8];
   __shared__ double cvec[kMaxEig];
-  __shared__ double vsh[1024];
   for (int q = 0; q < seg.nops; ++q) {
     const CoarseOp op = seg.op[q];
-    if (op.kind == 4) {
-      bicc_sweeps(g, Acb, kmax, bi.dinv, bi.plane, bi.elem, bi.nplanes, bi.kp, t, d, vsh);
-    } else if (op.kind == 2) {
+    if (op.kind == 2) {
       double zt = 0.0, ht = 0.0;
       for (int r = threadIdx.x; r < nc; r += blockDim.x) {
         zt += z[r] * t[r];
@@ -1052,30 +1072,40 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
 int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
                           double* y, double* d, int kt_gemv, int kt_small,
                           int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int)) {
-  const bool icc = ctx->lv2_icc != 0;
-  const BiccArgs bi{ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p, ctx->bicc_nplanes,
-                    bicc_lanes(ctx->en_lv1)};
+  if (ctx->lv2_icc) {
+    // ICC(0) level 2 and level 3 run in one workgroup each (their work is a
+    // dependency chain / a few dot products); the coarse residual update
+    // t -= A_c d and u += d run on the whole device.
+    for (size_t k = 0; k < prog.size(); ++k) {
+      const int ev = kt_begin(ctx, prog[k] == 2 ? kt_gemv : kt_small);
+      if (prog[k] == 2)
+        PMX_TRY(launch_bicc_solve(ctx, t, d, 1));
+      else
+        PMX_TRY(launch_lv3_apply(ctx, t, d));
+      kt_end(ctx, ev);
+      PMX_TRY(launch_coarse_update(ctx, d, t, u, k == 0, k + 1 < prog.size()));
+    }
+    return PMX_SUCCESS;
+  }
   size_t k = 0;
   while (k < prog.size()) {
     CoarseSeg seg{};
-    if (prog[k] == 2 && !icc) {
+    if (prog[k] == 2) {
       const int ev = kt_begin(ctx, kt_gemv);
       PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, y, ctx->nc));
       kt_end(ctx, ev);
     }
-    // With the dense inverse a segment ends before the next level-2 stage
-    // (its GEMV runs on the whole device); the ICC(0) sweeps run inside.
     do {
       CoarseOp& op = seg.op[seg.nops++];
-      op.kind = (prog[k] == 2 && icc) ? 4 : prog[k];
+      op.kind = prog[k];
       op.first = k == 0;
       op.update_t = k + 1 < prog.size();
       ++k;
-    } while (k < prog.size() && (icc || prog[k] == 3) && seg.nops < 8);
+    } while (k < prog.size() && prog[k] == 3 && seg.nops < 8);
     const int ev = kt_begin(ctx, kt_small);
-    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(icc ? 1024 : kBlock), 0, ctx->stream, ctx->geo,
-                       seg, ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p,
-                       ctx->c_cs.p, ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc, bi);
+    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->geo, seg,
+                       ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p, ctx->c_cs.p,
+                       ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc);
     PMX_HIP(hipGetLastError());
     kt_end(ctx, ev);
   }
@@ -1474,55 +1504,268 @@ int level3_setup(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
-// Level 3 of the spe10 fork (src/spe10.c:1255-1417): the smallest en_lv2
-// eigenpairs of A_c without a dense factorisation, by block preconditioned
-// inverse iteration with the level-2 ICC(0) factor M and Rayleigh-Ritz over
-// m >= 4 en_lv2 vectors:  Y = X - M^{-1} (A_c X - X diag(w)).  Converged
-// when ||A_c x - w x|| <= eig_tol |w| for every wanted pair (x has unit
+// ---------------------------------------------------------------------------
+// Level 3 of the spe10 fork (src/spe10.c:1255-1417) without a dense
+// factorisation: the smallest en_lv2 eigenpairs of A_c by block LOBPCG with
+// the level-2 ICC(0) factor M as preconditioner.  Blocks are column-major
+// n x m; the Gram matrices are summed by many workgroups (partials in fixed
+// order), the ns x ns Cholesky / eigenproblems run in one workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kGramRows = 2048;   // rows per workgroup of a partial Gram
+constexpr int kLv3Max = 32;       // largest LOBPCG basis [X W P] (3 m)
+
+// part[(g*ma + i)*mb + j] = sum over rows of chunk g of A[:, i] . B[:, j].
+__global__ void __launch_bounds__(256) k_gram_part(const double* __restrict__ A, int ma,
+                                                   const double* __restrict__ B, int mb, int n,
+                                                   double* __restrict__ part) {
+  __shared__ double red[4][kLv3Max];
+  const int g = blockIdx.x, i = blockIdx.y;
+  const int r0 = g * kGramRows, r1 = min(n, r0 + kGramRows);
+  double acc[kLv3Max];
+#pragma unroll
+  for (int j = 0; j < kLv3Max; ++j) acc[j] = 0.0;
+  const double* a = A + (size_t)i * n;
+  for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const double av = a[r];
+#pragma unroll
+    for (int j = 0; j < kLv3Max; ++j)
+      if (j < mb) acc[j] += av * B[(size_t)j * n + r];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kLv3Max; ++j) {
+    if (j >= mb) break;
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wv][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < mb) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
+    part[((size_t)g * ma + i) * mb + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_gram_sum(const double* __restrict__ part, int G, int ma, int mb,
+                           double* __restrict__ out) {
+  for (int e = threadIdx.x; e < ma * mb; e += blockDim.x) {
+    double v = 0.0;
+    for (int g = 0; g < G; ++g) v += part[(size_t)g * ma * mb + e];
+    out[e] = v;
+  }
+}
+
+// mode 0: T = L^{-T} for the Cholesky factor L of G (ns x ns, row-major);
+// *flag = 1 if G is not numerically positive definite.  mode 1: G
+// (symmetrised) = Q diag(w) Q^T, T = Q, ascending w.
+__global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
+                            double* __restrict__ T, double* __restrict__ wout,
+                            int* __restrict__ flag) {
+  constexpr int L = kLv3Max;
+  __shared__ double A[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
+  __shared__ int fl;
+  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+    const int i = e / ns, j = e % ns;
+    A[i * L + j] = mode == 0 ? Gm[e] : 0.5 * (Gm[e] + Gm[j * ns + i]);
+  }
+  __syncthreads();
+  if (mode == 0) {
+    const bool ok = wg_cholesky(A, ns, L, &fl);
+    if (threadIdx.x == 0) *flag = ok ? 0 : 1;
+    wg_trinv_lower(A, tmp, ns, L);  // A = L^{-1}
+    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+      const int k = e / ns, c = e % ns;
+      T[e] = A[c * L + k];
+    }
+    return;
+  }
+  wg_syev(A, Q, tmp, w, rot, ns, L);
+  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) T[e] = Q[(e / ns) * L + e % ns];
+  if (threadIdx.x < ns) wout[threadIdx.x] = w[threadIdx.x];
+}
+
+// out[:, c] = sum_{k < kin} S[:, k] T[(r0 + k) * tld + c0 + c] for c < cout.
+__global__ void __launch_bounds__(256) k_xform(const double* __restrict__ S, int n, int kin,
+                                               const double* __restrict__ T, int tld, int r0,
+                                               int c0, int cout, double* __restrict__ out) {
+  __shared__ double Ts[kLv3Max * kLv3Max];
+  for (int e = threadIdx.x; e < kin * cout; e += blockDim.x)
+    Ts[e] = T[(size_t)(r0 + e / cout) * tld + c0 + e % cout];
+  __syncthreads();
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    double sv[kLv3Max];
+#pragma unroll
+    for (int k = 0; k < kLv3Max; ++k) sv[k] = k < kin ? S[(size_t)k * n + r] : 0.0;
+    for (int c = 0; c < cout; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < kLv3Max; ++k)
+        if (k < kin) v += sv[k] * Ts[k * cout + c];
+      out[(size_t)c * n + r] = v;
+    }
+  }
+}
+
+// R = AX - X diag(w), res[j] = ||R_j|| (one workgroup per column).
+__global__ void k_col_resid(const double* __restrict__ AX, const double* __restrict__ X,
+                            const double* __restrict__ w, double* __restrict__ R, int n,
+                            double* __restrict__ res) {
+  __shared__ double red[8];
+  const int j = blockIdx.x;
+  const double wj = w[j];
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double e = AX[(size_t)j * n + r] - wj * X[(size_t)j * n + r];
+    R[(size_t)j * n + r] = e;
+    s += e * e;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) res[j] = sqrt(s);
+}
+
+namespace {
+
+struct Lv3Work {
+  pmx_ctx* ctx;
+  int n, ngram;
+  DBuf<double> part, G, T, wall;
+  DBuf<int> flag;
+
+  int gram(const double* A, int ma, const double* B, int mb) {
+    hipLaunchKernelGGL(k_gram_part, dim3(ngram, ma), dim3(256), 0, ctx->stream, A, ma, B, mb, n,
+                       part.p);
+    hipLaunchKernelGGL(k_gram_sum, dim3(1), dim3(256), 0, ctx->stream, part.p, ngram, ma, mb,
+                       G.p);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
+  int xform(const double* S, int kin, int tld, int r0, int c0, int cout, double* out) {
+    const int nb = std::min((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_xform, dim3(nb), dim3(256), 0, ctx->stream, S, n, kin, T.p, tld, r0, c0,
+                       cout, out);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
+  // S (ns columns) <- S L^{-T}: orthonormal; returns 1 in *bad if the Gram
+  // matrix is not positive definite.
+  int orthonormalize(double* S, int ns, double* tmp, int* bad) {
+    PMX_TRY(gram(S, ns, S, ns));
+    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, G.p, ns, 0, T.p, wall.p,
+                       flag.p);
+    PMX_HIP(hipGetLastError());
+    int h = 0;
+    PMX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));
+    *bad = h;
+    if (h) return PMX_SUCCESS;
+    PMX_TRY(xform(S, ns, ns, 0, 0, ns, tmp));
+    PMX_HIP(hipMemcpyAsync(S, tmp, (size_t)ns * n * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    return PMX_SUCCESS;
+  }
+};
+
+}  // namespace
+
+// Block LOBPCG (Knyazev 2001) on A_c with the ICC(0) preconditioner:
+// S = [X, M^{-1}(A_c X - X diag(w)), P], S orthonormalised by Cholesky QR
+// (P dropped when that fails), Rayleigh-Ritz on S^T A_c S, X = S Q_{:,1:m},
+// P = S_{:,m+1:} Q_{m+1:,1:m}.  m >= en_lv2 + 4 vectors; converged when
+// ||A_c x - w x|| <= eig_tol |w| for the en_lv2 wanted pairs (x of unit
 // norm); a residual that stagnates (no 2x gain over 20 iterations) below
-// 100 eig_tol is accepted and reported in lv3_floor.
+// 100 eig_tol is accepted and reported in lv3_floor.  Warm start: the
+// previous setup's eigenvectors when the coarse space has the same size.
 int level3_setup_icc(pmx_ctx* ctx) {
   const int n = ctx->nc;
   const int k2 = std::min(ctx->en_lv2, n);
-  const int m = std::min(std::max(4 * k2, k2 + 8), std::min(n, 2 * kMaxEig));
-  DBuf<double> X, Y, Z, R, w, res;
-  DBuf<int> fail;
-  for (DBuf<double>* b : {&X, &Y, &Z, &R}) PMX_TRY(b->alloc((size_t)n * m));
-  PMX_TRY(w.alloc(m));
+  const int m = std::min(std::min(k2 + 6, kLv3Max / 3), n);
+  if (m < k2) {
+    set_error("-lv2_icc: en_lv2=%d exceeds the level-3 block limit", k2);
+    return PMX_ERR_SUP;
+  }
+  Lv3Work wk;
+  wk.ctx = ctx;
+  wk.n = n;
+  wk.ngram = (n + kGramRows - 1) / kGramRows;
+  PMX_TRY(wk.part.alloc((size_t)wk.ngram * kLv3Max * kLv3Max));
+  PMX_TRY(wk.G.alloc(kLv3Max * kLv3Max));
+  PMX_TRY(wk.T.alloc(kLv3Max * kLv3Max));
+  PMX_TRY(wk.wall.alloc(kLv3Max));
+  PMX_TRY(wk.flag.alloc(1));
+  DBuf<double> S, Z, tmp, X, AX, P, R, res;
+  for (DBuf<double>* b : {&S, &Z, &tmp}) PMX_TRY(b->alloc((size_t)n * 3 * m));
+  for (DBuf<double>* b : {&X, &AX, &P, &R}) PMX_TRY(b->alloc((size_t)n * m));
   PMX_TRY(res.alloc(m));
-  PMX_TRY(fail.alloc(1));
+  const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
+  const size_t colb = (size_t)n * sizeof(double);
+
+  // Initial block: previous eigenvectors (if any) and M^{-1} of random columns.
   std::vector<double> h((size_t)n * m);
   std::mt19937_64 rng(ctx->seed + 7);
   std::normal_distribution<double> nd;
   for (auto& v : h) v = nd(rng);
   PMX_HIP(hipMemcpyAsync(R.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
-  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
-  PMX_TRY(launch_bicc_solve(ctx, R.p, Y.p, m));
-  const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
+  PMX_TRY(launch_bicc_solve(ctx, R.p, S.p, m));
+  const bool warm = ctx->lv3_warm && ctx->V2.p && ctx->lv3_prev_n == n && ctx->eigen_num_lv2 > 0;
+  if (warm)
+    PMX_HIP(hipMemcpyAsync(S.p, ctx->V2.p, (size_t)ctx->eigen_num_lv2 * colb,
+                           hipMemcpyDeviceToDevice, ctx->stream));
   std::vector<double> hw(m), hres(m), hist;
-  const int maxit = std::max(ctx->eig_maxit, 2000);
-  int it = 0;
+  int it = 0, np = 0, ns = m;
   bool conv = false;
   ctx->lv3_floor = 0.0;
-  for (it = 1; it <= maxit; ++it) {
-    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 0,
-                       w.p, res.p, fail.p);
+  const int maxit = std::max(ctx->eig_maxit, 2000);
+  for (it = 0; it <= maxit; ++it) {
+    if (it > 0) {
+      // W = M^{-1} R into S[:, m:2m], X into S[:, 0:m], P into S[:, 2m:3m].
+      PMX_TRY(launch_bicc_solve(ctx, R.p, S.p + (size_t)m * n, m));
+      PMX_HIP(hipMemcpyAsync(S.p, X.p, m * colb, hipMemcpyDeviceToDevice, ctx->stream));
+      if (np) PMX_HIP(hipMemcpyAsync(S.p + (size_t)2 * m * n, P.p, m * colb,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+      ns = np ? 3 * m : 2 * m;
+    }
+    int bad = 0;
+    PMX_TRY(wk.orthonormalize(S.p, ns, tmp.p, &bad));
+    if (bad && ns == 3 * m) {
+      // restart without P (S is unchanged when the Cholesky fails)
+      ns = 2 * m;
+      np = 0;
+      PMX_TRY(wk.orthonormalize(S.p, ns, tmp.p, &bad));
+    }
+    if (bad) {
+      set_error("level-3 eigensolver (ICC(0)-preconditioned): basis lost rank at iteration %d",
+                it);
+      return PMX_ERR_USER;
+    }
+    // Rayleigh-Ritz on S^T A_c S.
     hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
-                       ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n, m);
-    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 1,
-                       w.p, res.p, fail.p);
+                       ctx->Acb.p, ctx->en_lv1, S.p, Z.p, n, ns);
+    PMX_TRY(wk.gram(S.p, ns, Z.p, ns));
+    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, wk.G.p, ns, 1, wk.T.p,
+                       wk.wall.p, wk.flag.p);
+    PMX_HIP(hipGetLastError());
+    PMX_TRY(wk.xform(S.p, ns, ns, 0, 0, m, X.p));
+    PMX_TRY(wk.xform(Z.p, ns, ns, 0, 0, m, AX.p));
+    if (ns > m) {
+      PMX_TRY(wk.xform(S.p + (size_t)m * n, ns - m, ns, m, 0, m, P.p));
+      np = 1;
+    }
+    hipLaunchKernelGGL(k_col_resid, dim3(m), dim3(kBlock), 0, ctx->stream, AX.p, X.p, wk.wall.p,
+                       R.p, n, res.p);
     PMX_HIP(hipGetLastError());
     PMX_HIP(hipMemcpyAsync(hres.data(), res.p, m * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
-    PMX_HIP(hipMemcpyAsync(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost,
+    PMX_HIP(hipMemcpyAsync(hw.data(), wk.wall.p, m * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
     PMX_HIP(hipStreamSynchronize(ctx->stream));
     double mres = 0.0;
-    for (int j = 0; j < k2; ++j) mres = std::max(mres, hres[j] / std::max(std::fabs(hw[j]), 1e-300));
+    for (int j = 0; j < k2; ++j)
+      mres = std::max(mres, hres[j] / std::max(std::fabs(hw[j]), 1e-300));
     hist.push_back(mres);
     if (ctx->eig_debug >= 0 && (it < 20 || it % 50 == 0))
-      printf("lv3 icc it %d res %.3e w0 %.6e\n", it, mres, hw[0]);
+      printf("lv3 icc it %d ns %d res %.3e w0 %.6e\n", it, ns, mres, hw[0]);
     if (mres <= ctx->eig_tol) {
       conv = true;
       break;
@@ -1536,17 +1779,10 @@ int level3_setup_icc(pmx_ctx* ctx) {
         break;
       }
     }
-    hipLaunchKernelGGL(k_lv3_refine, dim3(std::min((n * m + kBlock - 1) / kBlock, 4096)),
-                       dim3(kBlock), 0, ctx->stream, Z.p, X.p, w.p, R.p, n, m);
-    PMX_TRY(launch_bicc_solve(ctx, R.p, Y.p, m));
-    hipLaunchKernelGGL(k_lv3_update, dim3(m), dim3(kBlock), 0, ctx->stream, X.p, Y.p, n, res.p);
-    PMX_HIP(hipGetLastError());
   }
-  int hfail = 0;
-  PMX_HIP(hipMemcpy(&hfail, fail.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (!conv || hfail) {
-    set_error("level-3 eigensolver (ICC(0)-preconditioned) did not converge (%d iterations, "
-              "fail=%d)", it, hfail);
+  if (!conv) {
+    set_error("level-3 eigensolver (ICC(0)-preconditioned) did not converge (%d iterations)",
+              it);
     return PMX_ERR_USER;
   }
   ctx->lv3_iters = it;
@@ -1564,10 +1800,11 @@ int level3_setup_icc(pmx_ctx* ctx) {
   PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
                          ctx->stream));
   PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
-  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, Z.p, n, num,
+  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, AX.p, n, num,
                      ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->lv3_prev_n = n;
   return PMX_SUCCESS;
 }
 

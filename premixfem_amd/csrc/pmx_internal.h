@@ -335,7 +335,8 @@ struct pmx_ctx {
   pmx::DBuf<double> bicc_dinv;      // [nsub][kmax][kmax] inverse pivot blocks
   pmx::DBuf<int> bicc_plane;        // coarse hyperplane prefix counts (nplanes+1)
   pmx::DBuf<int> bicc_elem;         // subdomains in hyperplane order
-  int bicc_nplanes = 0;
+  pmx::DBuf<int> bicc_pos;          // position of each subdomain inside its hyperplane
+  int bicc_nplanes = 0, bicc_maxcnt = 0;
   int max_plane = 0;                // most cells in one hyperplane of any brick shape
   int lds_max = 65536;              // LDS bytes per workgroup of the device
   int eig_iters_max = 0, eig_iters_sum = 0, eig_fallbacks = 0, eig_stagnated = 0;
@@ -349,6 +350,8 @@ struct pmx_ctx {
   pmx::DBuf<double> V2;             // [nc][en_lv2]
   pmx::DBuf<double> Acc_lu;         // k2 x k2 LU (row-major) + pivots packed after
   int lv3_iters = 0;
+  int lv3_warm = 1;                 // -lv2_icc: start level 3 from the previous eigenvectors
+  int lv3_prev_n = 0;               // coarse size of the previous level-3 solve
   double lv3_floor = 0.0;           // attained residual when the level-3 solver stagnated
 
   // ---- work vectors ----
