@@ -227,6 +227,102 @@ def cpu_baseline(prob, gpu_its, nsample):
     return total, t
 
 
+IMPES_MESH = (60, 220, 85)
+IMPES_SD = (10, 20, 17)
+
+
+def impes_cpu_baseline(perm, poro, nsample):
+    """The spe10-fork oracle on a bounded sample of the reservoir step:
+    nsample of the 3400 subdomain GHEPs (scaled), the ICC(0) coarse factor
+    and a 3-iteration PCG sample are skipped in favour of the transport
+    (50 substeps, timed in full) and the level-1 factor; returns (seconds per
+    IMPES step estimate, parts)."""
+    from oracle import premixfem_oracle as O
+    from oracle import spe10_oracle as S
+
+    prm = S.Spe10Params()
+    lay = S.layout(IMPES_MESH, IMPES_SD, prm.H)
+    M, N, Pz = IMPES_MESH
+    Sw = np.full((Pz, N, M), prm.S_wi)
+    t = {}
+    t0 = time.perf_counter()
+    kappa, T, wd, rhs = S.linear_system(prm, lay, list(perm), Sw)
+    t["assemble"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.DIC(lay, T, O.lv1_block_diag(lay, T) + wd)
+    t["su_lv1"] = time.perf_counter() - t0
+    mdiag = O.mass_diag(lay, kappa, T, O.EIG_OP_DIAG, wd)
+    gidx = np.arange(M * N * Pz).reshape(Pz, N, M)
+    t0 = time.perf_counter()
+    for s in range(nsample):
+        stx, nx, sty, ny, stz, nz = lay.corners(s * (lay.nsub // nsample))
+        cells = gidx[stz:stz + nz, sty:sty + ny, stx:stx + nx].ravel()
+        O.ghep(O.local_neumann(lay, T, s * (lay.nsub // nsample), wd), mdiag[cells], 8)
+    t["su_lv2_1"] = (time.perf_counter() - t0) / nsample * lay.nsub
+    t0 = time.perf_counter()
+    S.transport(prm, lay, list(perm), kappa, S.modified_poro(poro), np.full(M * N * Pz, 4.0e3),
+                Sw, 50, 0.001)
+    t["transport"] = time.perf_counter() - t0
+    return sum(t.values()), t
+
+
+def impes_main(args):
+    """The IMPES loop of the spe10 fork (src/spe10.c:197-274) on the 60x220x85
+    reservoir grid: per step kappa/wells/rhs assembly, PC_setup, PCG from the
+    previous pressure, 50 saturation substeps; all on the GPU."""
+    from premixfem_amd import problems, spe10
+
+    perm, poro = problems.reservoir_fields(IMPES_MESH, 3001)
+    prec = 64 if args.precision == 64 else 32
+    if args.warmup > 0:
+        res = spe10.Reservoir(IMPES_MESH, perm, poro, IMPES_SD, precision=prec)
+        res.run(max_steps=args.warmup)
+        res.close()
+    res = spe10.Reservoir(IMPES_MESH, perm, poro, IMPES_SD, precision=prec)
+    t0 = time.perf_counter()
+    rows = res.run(max_steps=args.steps)
+    wall = time.perf_counter() - t0
+    res.close()
+    its = [r["its"] for r in rows]
+    for r in rows:
+        print(f"# step {r['step']}: its {r['its']} setup {r['t_setup']:.3f}s solve "
+              f"{r['t_solve']:.3f}s transport {r['t_transport']:.4f}s dt {r['delta_t']:.4e} "
+              f"S_w [{r['S_w_min']:.4f}, {r['S_w_max']:.4f}]", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        total, parts = impes_cpu_baseline(perm, poro, args.cpu_sample)
+        cpu = dict(value=round(total, 2), unit="s per IMPES step (lower bound)", cores=os.cpu_count(),
+                   kind="port",
+                   sample=(f"numpy/SciPy oracle of the spe10 fork on the same grid, step 0: "
+                           f"assembly and level-1 factor in full, {args.cpu_sample} of 3400 "
+                           "subdomain GHEPs timed and scaled, 50 transport substeps in full; "
+                           "coarse levels and PCG not included"),
+                   parts={k: round(v, 3) for k, v in parts.items()})
+    out = {
+        "metric": "IMPES wall s: spe10-fork loop, per step setup + PCG (rtol 1e-8) + 50 saturation substeps",
+        "value": round(wall, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / max(1, len(rows)) * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 64 else "f32 preconditioner / f64 CG",
+        "data": "synthetic (seeded layered lognormal rock, sigma 2.5, corr 20x20x2, kz = 0.1 kx)",
+        "config": {"workload": "60x220x85 reservoir grid of 20x10x2 ft cells, 10x20x17 subdomains "
+                               "of 6x11x5, 8 eigenvectors, 4 producers + injector columns, Corey 2, "
+                               "mu_o/mu_w 10, DIAG mass, ICC(0) level 2"},
+        "its_per_step": its,
+        "t_setup": [round(r["t_setup"], 4) for r in rows],
+        "t_solve": [round(r["t_solve"], 4) for r in rows],
+        "t_transport": [round(r["t_transport"], 5) for r in rows],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,11 +336,19 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="finish with an elementwise kernel that reads and writes 256 MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--workload", default="setup_solve", choices=("setup_solve", "impes"),
+                    help="impes: the spe10-fork IMPES loop on the 60x220x85 reservoir grid "
+                         "(a separate JSON line; --steps IMPES steps)")
+    ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "impes":
+        impes_main(args)
+        return
 
     from premixfem_amd import problems
 

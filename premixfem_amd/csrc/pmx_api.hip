@@ -1344,7 +1344,7 @@ int pmx_final(pmx_ctx* c) {
   c->wd32.release();
   c->bicc_plane.release();
   c->bicc_elem.release();
-  c->bicc_pos.release();
+  c->bicc_slot.release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

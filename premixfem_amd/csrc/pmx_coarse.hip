@@ -642,72 +642,92 @@ __global__ void __launch_bounds__(64 * kBiccWaves) k_bicc_factor(
 // sum_{lower o} A_so w_o), backward y_s = w_s - D_s^{-1} sum_{upper o} A_so
 // y_o.  A subdomain's lower (upper) neighbours all lie on the previous
 // (next) hyperplane, so the values of the last plane live in an LDS ring of
-// two plane slots (slot entry pos[s]*kp + row): the dependency chain of the
-// sweep never waits on global memory.  w and y are also written to d.  Lane
-// group of kp >= neig lanes per subdomain, one row each.
+// two plane slots (slot entry pos*kp + row): the dependency chain of the
+// sweep never waits on global memory.  Each hyperplane position has a
+// precomputed descriptor (subdomain, its dofs, and the ring positions and
+// eigen counts of its neighbours), so a round needs one dependent load
+// before its matrix rows.  w and y are also written to d.  Lane group of
+// kp >= neig lanes per subdomain, one row each.
+struct BiccSlot {
+  int s, dof, neig, pad;
+  int lpos[3], lk[3];   // -x, -y, -z neighbours: ring position (-1: none), eigen count
+  int hpos[3], hk[3];   // +x, +y, +z neighbours
+};
+
 struct BiccArgs {
   const double* dinv;
   const int* plane;
-  const int* elem;
-  const int* pos;     // position of each subdomain inside its hyperplane
+  const BiccSlot* slot;   // descriptors in hyperplane order
   int nplanes, kp, maxcnt;
 };
 
-__global__ void __launch_bounds__(1024) k_bicc_solve(Geo g, const double* __restrict__ Acb,
-                                                     int kmax, BiccArgs bi,
-                                                     const double* __restrict__ tcol,
+constexpr int kBiccMaxPlanes = 512;
+
+__global__ void __launch_bounds__(1024) k_bicc_solve(const double* __restrict__ Acb, int kmax,
+                                                     BiccArgs bi, const double* __restrict__ tcol,
                                                      double* __restrict__ dcol, int nc) {
   extern __shared__ double ring[];   // [2][maxcnt * kp]
-  __shared__ double vsh[1024];
+  __shared__ double vsh[2][1024];
+  __shared__ int pl[kBiccMaxPlanes + 1];
   const double* t = tcol + (size_t)blockIdx.x * nc;
   double* d = dcol + (size_t)blockIdx.x * nc;
+  for (int e = threadIdx.x; e <= bi.nplanes; e += blockDim.x) pl[e] = bi.plane[e];
+  __syncthreads();
   const size_t kk = (size_t)kmax * kmax;
   const int kp = bi.kp;
   const int grp = threadIdx.x / kp, i = threadIdx.x % kp;
   const int epr = blockDim.x / kp;
-  const int slot = bi.maxcnt * kp;
+  const int rs = bi.maxcnt * kp;
+  int round = 0;
   for (int dirn = 0; dirn < 2; ++dirn) {
     const bool fwd = dirn == 0;
     for (int pp = 0; pp < bi.nplanes; ++pp) {
       const int p = fwd ? pp : bi.nplanes - 1 - pp;
-      double* cur = ring + (p & 1) * slot;
-      const double* oth = ring + ((p + 1) & 1) * slot;  // plane p-1 (fwd) or p+1 (bwd)
-      const int cnt = bi.plane[p + 1] - bi.plane[p];
-      for (int r0 = 0; r0 < cnt; r0 += epr) {
-        const bool grp_act = r0 + grp < cnt;
+      double* cur = ring + (p & 1) * rs;
+      const double* oth = ring + ((p + 1) & 1) * rs;  // plane p-1 (fwd) or p+1 (bwd)
+      const int cnt = pl[p + 1] - pl[p];
+      for (int r0 = 0; r0 < cnt; r0 += epr, ++round) {
+        double* vb = vsh[round & 1];
         const int q = r0 + grp;
-        const int s = grp_act ? bi.elem[bi.plane[p] + q] : 0;
-        const Brick& b = g.bricks[s];
-        const bool act = grp_act && i < b.neig;
-        const double* blk = Acb + (size_t)s * 7 * kk;
+        const bool grp_act = q < cnt;
+        BiccSlot sl{};
+        if (grp_act) sl = bi.slot[pl[p] + q];
+        const bool act = grp_act && i < sl.neig;
         double v = 0.0, own = 0.0;
+        double di[16];
         if (act) {
-          v = fwd ? t[b.dof + i] : 0.0;
-          if (!fwd) own = d[b.dof + i];
-          for (int dir = fwd ? 0 : 1; dir < 6; dir += 2) {
-            const int o = b.nb[dir];
-            if (o < 0) continue;
-            const int ko = g.bricks[o].neig;
-            const double* B = blk + (size_t)(dir + 1) * kk + (size_t)i * kmax;
-            const double* xo = oth + bi.pos[o] * kp;
+          const double* Di = bi.dinv + (size_t)sl.s * kk + (size_t)i * kmax;
+#pragma unroll
+          for (int j = 0; j < 16; ++j) di[j] = j < sl.neig ? Di[j] : 0.0;
+          v = fwd ? t[sl.dof + i] : 0.0;
+          if (!fwd) own = d[sl.dof + i];
+          const double* blk = Acb + (size_t)sl.s * 7 * kk + (size_t)i * kmax;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const int op = fwd ? sl.lpos[a] : sl.hpos[a];
+            if (op < 0) continue;
+            const int ko = fwd ? sl.lk[a] : sl.hk[a];
+            const double* B = blk + (size_t)(2 * a + (fwd ? 1 : 2)) * kk;
+            const double* xo = oth + op * kp;
             double acc = 0.0;
             for (int j = 0; j < ko; ++j) acc += B[j] * xo[j];
             v = fwd ? v - acc : v + acc;
           }
         }
-        vsh[threadIdx.x] = v;
+        vb[threadIdx.x] = v;
         __syncthreads();
         if (act) {
-          const double* Di = bi.dinv + (size_t)s * kk + (size_t)i * kmax;
-          const double* vg = vsh + grp * kp;
+          const double* vg = vb + grp * kp;
           double acc = 0.0;
-          for (int j = 0; j < b.neig; ++j) acc += Di[j] * vg[j];
+#pragma unroll
+          for (int j = 0; j < 16; ++j)
+            if (j < sl.neig) acc += di[j] * vg[j];
           const double r = fwd ? acc : own - acc;
           cur[q * kp + i] = r;
-          d[b.dof + i] = r;
+          d[sl.dof + i] = r;
         }
-        __syncthreads();
       }
+      __syncthreads();   // this plane's ring slot is complete before the next plane reads it
     }
   }
 }
@@ -719,14 +739,15 @@ static int bicc_lanes(int kmax) {
 }
 
 static BiccArgs bicc_args(const pmx_ctx* ctx) {
-  return BiccArgs{ctx->bicc_dinv.p, ctx->bicc_plane.p, ctx->bicc_elem.p, ctx->bicc_pos.p,
-                  ctx->bicc_nplanes, bicc_lanes(ctx->en_lv1), ctx->bicc_maxcnt};
+  return BiccArgs{ctx->bicc_dinv.p, ctx->bicc_plane.p,
+                  reinterpret_cast<const BiccSlot*>(ctx->bicc_slot.p), ctx->bicc_nplanes,
+                  bicc_lanes(ctx->en_lv1), ctx->bicc_maxcnt};
 }
 
 int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
   const BiccArgs bi = bicc_args(ctx);
   const size_t lds = 2 * (size_t)bi.maxcnt * bi.kp * sizeof(double);
-  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->geo, ctx->Acb.p,
+  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->Acb.p,
                      ctx->en_lv1, bi, t, d, ctx->nc);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
@@ -752,12 +773,32 @@ int bicc_setup(pmx_ctx* ctx) {
     for (int q = start[p]; q < start[p + 1]; ++q) pos[elems[q]] = q - start[p];
   }
   const size_t lds = 2 * (size_t)maxcnt * bicc_lanes(ctx->en_lv1) * sizeof(double);
-  if (lds + 1024 * sizeof(double) > 65536) {
-    set_error("-lv2_icc: a coarse hyperplane of %d subdomains exceeds the LDS plane ring", maxcnt);
+  if (lds + 2 * 1024 * sizeof(double) + (kBiccMaxPlanes + 1) * sizeof(int) > 65536 ||
+      np > kBiccMaxPlanes || ctx->en_lv1 > 16) {
+    set_error("-lv2_icc: %d coarse hyperplanes of up to %d subdomains exceed the LDS plane ring",
+              np, maxcnt);
     return PMX_ERR_SUP;
   }
-  PMX_TRY(ctx->bicc_pos.alloc(ctx->nsub));
-  PMX_HIP(hipMemcpy(ctx->bicc_pos.p, pos.data(), ctx->nsub * sizeof(int), hipMemcpyHostToDevice));
+  std::vector<BiccSlot> slots(ctx->nsub);
+  for (int q = 0; q < ctx->nsub; ++q) {
+    const Brick& b = ctx->h_bricks[elems[q]];
+    BiccSlot& sl = slots[q];
+    sl.s = elems[q];
+    sl.dof = b.dof;
+    sl.neig = b.neig;
+    sl.pad = 0;
+    for (int a = 0; a < 3; ++a) {
+      const int lo = b.nb[2 * a], hi = b.nb[2 * a + 1];
+      sl.lpos[a] = lo >= 0 ? pos[lo] : -1;
+      sl.lk[a] = lo >= 0 ? ctx->h_bricks[lo].neig : 0;
+      sl.hpos[a] = hi >= 0 ? pos[hi] : -1;
+      sl.hk[a] = hi >= 0 ? ctx->h_bricks[hi].neig : 0;
+    }
+  }
+  static_assert(sizeof(BiccSlot) % sizeof(int) == 0, "descriptor layout");
+  PMX_TRY(ctx->bicc_slot.alloc(ctx->nsub * sizeof(BiccSlot) / sizeof(int)));
+  PMX_HIP(hipMemcpy(ctx->bicc_slot.p, slots.data(), ctx->nsub * sizeof(BiccSlot),
+                    hipMemcpyHostToDevice));
   ctx->bicc_maxcnt = maxcnt;
   PMX_TRY(ctx->bicc_plane.alloc(np + 1));
   PMX_TRY(ctx->bicc_elem.alloc(ctx->nsub));

@@ -335,7 +335,7 @@ struct pmx_ctx {
   pmx::DBuf<double> bicc_dinv;      // [nsub][kmax][kmax] inverse pivot blocks
   pmx::DBuf<int> bicc_plane;        // coarse hyperplane prefix counts (nplanes+1)
   pmx::DBuf<int> bicc_elem;         // subdomains in hyperplane order
-  pmx::DBuf<int> bicc_pos;          // position of each subdomain inside its hyperplane
+  pmx::DBuf<int> bicc_slot;         // per-position sweep descriptors (hyperplane order)
   int bicc_nplanes = 0, bicc_maxcnt = 0;
   int max_plane = 0;                // most cells in one hyperplane of any brick shape
   int lds_max = 65536;              // LDS bytes per workgroup of the device
