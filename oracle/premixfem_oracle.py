@@ -537,6 +537,43 @@ class Oracle:
             self.A = (self.A + sp.diags(self.wd)).tocsr()
         self.lv2_icc = bool(lv2_icc)
 
+    def energy_gram(self, Phi):
+        """Phi^T A Phi in the reference's energy form: every face contributes
+        T (phi_L - phi_R)(phi_L - phi_R)^T, computed from the differences of
+        the basis values (:737-890, :1177-1360), never from A Phi.  For a face
+        between two subdomains the difference row holds phi_L and -phi_R in
+        different columns, which gives the reference's T phi phi^T diagonal
+        and -T phi phi'^T neighbour blocks.  The pin and well terms add
+        d phi phi^T.  Avoids the cancellation of A Phi near the null space."""
+        M, N, P = self.lay.mesh
+        n = M * N * P
+        idx = np.arange(n).reshape(P, N, M)
+        rows, cols, vals, tf = [], [], [], []
+        nf = 0
+        for Tf, a, b in ((self.T[0], idx[:, :, :-1], idx[:, :, 1:]),
+                         (self.T[1], idx[:, :-1, :], idx[:, 1:, :]),
+                         (self.T[2], idx[:-1, :, :], idx[1:, :, :])):
+            a, b = a.ravel(), b.ravel()
+            f = nf + np.arange(a.size)
+            rows += [f, f]
+            cols += [a, b]
+            vals += [np.ones(a.size), -np.ones(a.size)]
+            tf.append(Tf.ravel())
+            nf += a.size
+        inc = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                            shape=(nf, n))
+        G = (inc @ Phi).tocsr()
+        out = G.T @ sp.diags(np.concatenate(tf)) @ G
+        d = np.zeros(n)
+        if self.pin is not None:
+            (px, py, pz), tau = self.pin
+            d[(pz * N + py) * M + px] += tau
+        if self.wd is not None:
+            d = d + self.wd
+        if d.any():
+            out = out + Phi.T @ sp.diags(d) @ Phi
+        return sp.csr_matrix(out)
+
     def matvec(self, x):
         y = stencil_apply(self.lay, self.T, x, self.pin)
         if self.wd is not None:
@@ -595,7 +632,7 @@ class Oracle:
                 vals.append(V[:, j])
         self.Phi = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                                  shape=(M * N * P, nc))
-        A_c = (self.Phi.T @ (self.A @ self.Phi)).tocsr()
+        A_c = self.energy_gram(self.Phi).tocsr()
         self.A_c_sp = (0.5 * (A_c + A_c.T)).tocsr()
         self.A_c = self.A_c_sp.toarray() if nc <= DENSE_COARSE_MAX else None
         if self.lv2_icc:
@@ -639,8 +676,7 @@ class Oracle:
         self.t["su_lv3_1"] += time.perf_counter() - ts
         # Level 3 coarse-coarse (:1177-1390).
         ts = time.perf_counter()
-        APcc = np.column_stack([self.matvec(self.Phi_cc[:, j]) for j in range(num2)])
-        self.A_cc = self.Phi_cc.T @ APcc
+        self.A_cc = self.energy_gram(sp.csr_matrix(self.Phi_cc)).toarray()
         if not self.no_shift_A_cc:
             self.A_cc = self.A_cc + NIL * np.eye(num2)
         self.lu_cc = sla.lu_factor(self.A_cc)

@@ -109,6 +109,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-coarse_deflate") {
+      PMX_TRY(geti(c->coarse_deflate));
     } else if (k == "-lv3_warm") {
       PMX_TRY(geti(c->lv3_warm));
     } else if (k == "-pc_precision") {

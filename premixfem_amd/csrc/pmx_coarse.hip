@@ -10,11 +10,11 @@
 namespace pmx {
 
 // ---------------------------------------------------------------------------
-// Galerkin coarse matrix A_c = Phi^T A Phi (src/PreMixFEM_3D.c:737-890).
-// Diagonal block of subdomain s: Phi_s^T A[Omega_s,Omega_s] Phi_s (the
-// principal submatrix keeps the faces to other subdomains on its diagonal);
-// off-diagonal block with the upper neighbour s' across a face:
-// -sum_face T phi_s,i phi_s',j.  One workgroup per subdomain, rows in chunks
+// Galerkin coarse matrix A_c = Phi^T A Phi (src/PreMixFEM_3D.c:737-890) in
+// the reference's energy form.  Diagonal block of subdomain s: every face
+// inside s adds T (phi_L - phi_R)(phi_L - phi_R)^T, every face to another
+// subdomain (and the pin / well term) adds T phi phi^T; off-diagonal block
+// with the upper neighbour s' across a face: -sum_face T phi_s,i phi_s',j.  One workgroup per subdomain, rows in chunks
 // of 4 to bound registers; only the upper triangle of the diagonal block is
 // reduced and mirrored, so A_c is exactly symmetric.
 // Output: dense nc x nc (row-major) and block rows Acb[s][slot][kmax][kmax]
@@ -40,38 +40,48 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
     for (int li = threadIdx.x; li < n; li += blockDim.x) {
       const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
       const int c = b.off + li;
-      double diag = 0.0;
-      double aphi[kMaxEig];
       const double* pc = phi + (size_t)c * kmax;
+      double pcv[kMaxEig];
 #pragma unroll
-      for (int j = 0; j < kMaxEig; ++j) aphi[j] = 0.0;
+      for (int j = 0; j < kMaxEig; ++j) pcv[j] = j < ke ? pc[j] : 0.0;
+      // Faces to other subdomains, the pin and the well term act on the
+      // diagonal block as d phi phi^T.
+      double sdiag = 0.0;
 #pragma unroll
       for (int d = 0; d < 6; ++d) {
         const int nbi = nbr_index(g, b, c, lx, ly, lz, d);
         if (nbi < 0) continue;
         const double t = (d & 1) ? TT3[d >> 1][nbi] : TT3[d >> 1][c];
-        diag += t;
         const int lcoord = (d >> 1) == 0 ? lx : ((d >> 1) == 1 ? ly : lz);
         const int lext = (d >> 1) == 0 ? b.nx : ((d >> 1) == 1 ? b.ny : b.nz);
         const bool inside = (d & 1) ? (lcoord < lext - 1) : (lcoord > 0);
-        if (inside) {
+        if (!inside) {
+          sdiag += t;
+        } else if ((d & 1) == 0) {
+          // Interior face with the lower neighbour: T (phi_c - phi_nb)(...)^T
+          // from the differences of the basis values, as the reference
+          // assembles it -- no cancellation near the null space.
           const double* pn = phi + (size_t)nbi * kmax;
+          double del[kMaxEig];
 #pragma unroll
-          for (int j = 0; j < kMaxEig; ++j)
-            if (j < ke) aphi[j] -= t * pn[j];
+          for (int j = 0; j < kMaxEig; ++j) del[j] = j < ke ? pcv[j] - pn[j] : 0.0;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            if (i0 + a >= ke) break;
+            const double di = t * del[i0 + a];
+#pragma unroll
+            for (int j = 0; j < kMaxEig; ++j) acc[a][j] += di * del[j];
+          }
         }
       }
-      if (c == g.pin) diag += g.tau;
-      if (g.wd) diag += g.wd[c];
-#pragma unroll
-      for (int j = 0; j < kMaxEig; ++j)
-        if (j < ke) aphi[j] += diag * pc[j];
+      if (c == g.pin) sdiag += g.tau;
+      if (g.wd) sdiag += g.wd[c];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         if (i0 + a >= ke) break;
-        const double pi = pc[i0 + a];
+        const double pi = sdiag * pcv[i0 + a];
 #pragma unroll
-        for (int j = 0; j < kMaxEig; ++j) acc[a][j] += pi * aphi[j];
+        for (int j = 0; j < kMaxEig; ++j) acc[a][j] += pi * pcv[j];
       }
     }
     for (int a = 0; a < 4; ++a) {
@@ -391,6 +401,18 @@ int coarse_solve_setup(pmx_ctx* ctx, double delta) {
   PMX_TRY(ctx->c_g.alloc(n));
   PMX_TRY(ctx->c_h.alloc(n));
   PMX_TRY(ctx->c_cs.alloc(4));
+  if (!ctx->coarse_deflate) {
+    // Plain inverse of A_c (+ delta I), no deflation: the Woodbury fix is the
+    // identity (z = h = 0).
+    PMX_HIP(hipMemsetAsync(ctx->c_z.p, 0, n * sizeof(double), ctx->stream));
+    PMX_HIP(hipMemsetAsync(ctx->c_g.p, 0, n * sizeof(double), ctx->stream));
+    PMX_HIP(hipMemsetAsync(ctx->c_h.p, 0, n * sizeof(double), ctx->stream));
+    const double cs[4] = {1.0, 0.0, 0.0, 0.0};
+    PMX_HIP(hipMemcpy(ctx->c_cs.p, cs, sizeof(cs), hipMemcpyHostToDevice));
+    PMX_TRY(coarse_invert(ctx, ctx->Ac.p, ctx->Ainv.p, delta));
+    ctx->coarse_gamma = 0.0;
+    return PMX_SUCCESS;
+  }
   // z = Phi^T M 1, normalised.
   PMX_TRY(launch_resid_restrict_f64(ctx, ctx->mass.p, ctx->c_z.p));
   std::vector<double> hz(n), hg(n, 0.0);
@@ -1333,13 +1355,16 @@ __global__ void k_lv3_rr(double* __restrict__ Y, double* __restrict__ Z, double*
   }
 }
 
-// Sign normalisation (positive inner product with 1 + frac(0.618..(r+1))),
-// A_cc = V^T A_c V + shift I and its LU with partial pivoting.
-__global__ void k_lv3_finish(double* __restrict__ V, const double* __restrict__ AV, int n, int k2,
-                             double shift, double* __restrict__ LU) {
+// ---------------------------------------------------------------------------
+// Level-3 coarse-coarse matrix in the reference's energy form
+// (_PC_setup_lv3_cc, src/PreMixFEM_3D.c:1177-1360): with Phi_cc = Phi V on
+// the fine grid, A_cc[a][b] = sum over all faces T (dphi_a)(dphi_b) + the pin
+// / well terms d phi_a phi_b, + NIL I unless -no_shift_A_cc; then its LU.
+// ---------------------------------------------------------------------------
+// Sign of every column of V: positive inner product with 1 + frac(0.618..(r+1)).
+__global__ void k_lv3_sign(double* __restrict__ V, int n, int k2) {
   __shared__ double red[8];
   __shared__ double sg[kMaxEig];
-  __shared__ double A[kMaxEig * kMaxEig];
   for (int j = 0; j < k2; ++j) {
     double s = 0.0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
@@ -1352,38 +1377,139 @@ __global__ void k_lv3_finish(double* __restrict__ V, const double* __restrict__ 
   __syncthreads();
   for (int j = 0; j < k2; ++j)
     for (int r = threadIdx.x; r < n; r += blockDim.x) V[(size_t)j * n + r] *= sg[j];
-  __syncthreads();
-  for (int i = 0; i < k2; ++i)
-    for (int j = 0; j < k2; ++j) {
-      double s = 0.0;
-      for (int r = threadIdx.x; r < n; r += blockDim.x) s += V[(size_t)i * n + r] * AV[(size_t)j * n + r];
-      s = block_sum(s, red);
-      if (threadIdx.x == 0) A[i * kMaxEig + j] = s * sg[j] + (i == j ? shift : 0.0);
-    }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double piv[kMaxEig];
-    for (int k = 0; k < k2; ++k) {
-      int p = k;
-      for (int i = k + 1; i < k2; ++i)
-        if (fabs(A[i * kMaxEig + k]) > fabs(A[p * kMaxEig + k])) p = i;
-      piv[k] = p;
-      if (p != k)
-        for (int j = 0; j < k2; ++j) {
-          const double t = A[k * kMaxEig + j];
-          A[k * kMaxEig + j] = A[p * kMaxEig + j];
-          A[p * kMaxEig + j] = t;
-        }
-      for (int i = k + 1; i < k2; ++i) {
-        const double f = A[i * kMaxEig + k] / A[k * kMaxEig + k];
-        A[i * kMaxEig + k] = f;
-        for (int j = k + 1; j < k2; ++j) A[i * kMaxEig + j] -= f * A[k * kMaxEig + j];
-      }
-    }
-    for (int i = 0; i < k2; ++i)
-      for (int j = 0; j < k2; ++j) LU[i * k2 + j] = A[i * kMaxEig + j];
-    for (int i = 0; i < k2; ++i) LU[k2 * k2 + i] = piv[i];
+}
+
+// phicc[c * k2 + a] = sum_j phi[c][j] V[a][dof_s + j] for the cells of brick s.
+__global__ void k_phicc(Geo g, const double* __restrict__ phi, int kmax,
+                        const double* __restrict__ V, int nc, int k2,
+                        double* __restrict__ phicc) {
+  const Brick b = g.bricks[blockIdx.x];
+  const int n = b.nx * b.ny * b.nz;
+  for (int e = threadIdx.x; e < n * k2; e += blockDim.x) {
+    const int li = e / k2, a = e % k2;
+    const size_t c = (size_t)b.off + li;
+    double v = 0.0;
+    for (int j = 0; j < b.neig; ++j) v += phi[c * kmax + j] * V[(size_t)a * nc + b.dof + j];
+    phicc[c * k2 + a] = v;
   }
+}
+
+// Row a = blockIdx.y of brick s = blockIdx.x's share of A_cc: the lower
+// faces of its cells (to any brick) and the diagonal terms.
+__global__ void __launch_bounds__(256) k_acc_part(Geo g, const double* __restrict__ phicc, int k2,
+                                                  const double* __restrict__ Tx,
+                                                  const double* __restrict__ Ty,
+                                                  const double* __restrict__ Tz,
+                                                  double* __restrict__ part) {
+  __shared__ double red[4][kMaxEig];
+  const int s = blockIdx.x, a = blockIdx.y;
+  const BrickView v = load_view(g, s);
+  const double* TT3[3] = {Tx, Ty, Tz};
+  double acc[kMaxEig];
+#pragma unroll
+  for (int q = 0; q < kMaxEig; ++q) acc[q] = 0.0;
+  for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
+    const int lx = li % v.nx, ly = (li / v.nx) % v.ny, lz = li / v.nxny;
+    const int c = v.off + li;
+    int nb[6];
+    nbr6(v, c, lx, ly, lz, nb);
+    const double* pc = phicc + (size_t)c * k2;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int o = nb[2 * d];
+      if (o < 0) continue;
+      const double t = TT3[d][c];
+      const double* po = phicc + (size_t)o * k2;
+      const double da = t * (pc[a] - po[a]);
+#pragma unroll
+      for (int q = 0; q < kMaxEig; ++q)
+        if (q < k2) acc[q] += da * (pc[q] - po[q]);
+    }
+    double dg = 0.0;
+    if (c == g.pin) dg += g.tau;
+    if (g.wd) dg += g.wd[c];
+    if (dg != 0.0) {
+      const double pa = dg * pc[a];
+#pragma unroll
+      for (int q = 0; q < kMaxEig; ++q)
+        if (q < k2) acc[q] += pa * pc[q];
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMaxEig; ++q) {
+    if (q >= k2) break;
+    double x = acc[q];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wv][q] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < k2) {
+    double x = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) x += red[w][threadIdx.x];
+    part[((size_t)s * k2 + a) * k2 + threadIdx.x] = x;
+  }
+}
+
+// A_cc = sum of the brick partials (fixed order), symmetrised, + shift I, and
+// its LU with partial pivoting (row-major factors, pivots after them).
+__global__ void k_acc_lu(const double* __restrict__ part, int nsub, int k2, double shift,
+                         double* __restrict__ LU) {
+  __shared__ double A[kMaxEig * kMaxEig];
+  for (int e = threadIdx.x; e < k2 * k2; e += blockDim.x) {
+    double x = 0.0;
+    for (int s = 0; s < nsub; ++s) x += part[(size_t)s * k2 * k2 + e];
+    A[(e / k2) * kMaxEig + e % k2] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < k2; ++i)
+    for (int j = i + 1; j < k2; ++j) {
+      const double m = 0.5 * (A[i * kMaxEig + j] + A[j * kMaxEig + i]);
+      A[i * kMaxEig + j] = m;
+      A[j * kMaxEig + i] = m;
+    }
+  for (int i = 0; i < k2; ++i) A[i * kMaxEig + i] += shift;
+  double piv[kMaxEig];
+  for (int k = 0; k < k2; ++k) {
+    int p = k;
+    for (int i = k + 1; i < k2; ++i)
+      if (fabs(A[i * kMaxEig + k]) > fabs(A[p * kMaxEig + k])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < k2; ++j) {
+        const double t = A[k * kMaxEig + j];
+        A[k * kMaxEig + j] = A[p * kMaxEig + j];
+        A[p * kMaxEig + j] = t;
+      }
+    for (int i = k + 1; i < k2; ++i) {
+      const double f = A[i * kMaxEig + k] / A[k * kMaxEig + k];
+      A[i * kMaxEig + k] = f;
+      for (int j = k + 1; j < k2; ++j) A[i * kMaxEig + j] -= f * A[k * kMaxEig + j];
+    }
+  }
+  for (int i = 0; i < k2; ++i)
+    for (int j = 0; j < k2; ++j) LU[i * k2 + j] = A[i * kMaxEig + j];
+  for (int i = 0; i < k2; ++i) LU[k2 * k2 + i] = piv[i];
+}
+
+// Sign-normalises ctx->V2 (num columns) and builds the LU of A_cc from it.
+static int lv3_finish_energy(pmx_ctx* ctx, int num) {
+  const int n = ctx->nc;
+  hipLaunchKernelGGL(k_lv3_sign, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, n, num);
+  DBuf<double> phicc, part;
+  PMX_TRY(phicc.alloc((size_t)ctx->ncell * num));
+  PMX_TRY(part.alloc((size_t)ctx->nsub * num * num));
+  hipLaunchKernelGGL(k_phicc, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, ctx->phi.p,
+                     ctx->en_lv1, ctx->V2.p, n, num, phicc.p);
+  hipLaunchKernelGGL(k_acc_part, dim3(ctx->nsub, num), dim3(256), 0, ctx->stream, ctx->geo,
+                     phicc.p, num, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, part.p);
+  PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
+  hipLaunchKernelGGL(k_acc_lu, dim3(1), dim3(kBlock), 0, ctx->stream, part.p, ctx->nsub, num,
+                     ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
+  PMX_HIP(hipGetLastError());
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  return PMX_SUCCESS;
 }
 
 // Refined shift-invert step, part 1: R = A_c X - X diag(w) (Z = A_c X).
@@ -1536,12 +1662,7 @@ int level3_setup(pmx_ctx* ctx) {
   PMX_TRY(ctx->V2.alloc((size_t)n * num));
   PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
                          ctx->stream));
-  PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
-  // AV = A_c V (Z holds A_c X before the sign flip; k_lv3_finish applies it).
-  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, Z.p, n, num,
-                     ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
-  PMX_HIP(hipGetLastError());
-  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  PMX_TRY(lv3_finish_energy(ctx, num));
   return PMX_SUCCESS;
 }
 
@@ -1840,11 +1961,7 @@ int level3_setup_icc(pmx_ctx* ctx) {
   PMX_TRY(ctx->V2.alloc((size_t)n * num));
   PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
                          ctx->stream));
-  PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
-  hipLaunchKernelGGL(k_lv3_finish, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, AX.p, n, num,
-                     ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
-  PMX_HIP(hipGetLastError());
-  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  PMX_TRY(lv3_finish_energy(ctx, num));
   ctx->lv3_prev_n = n;
   return PMX_SUCCESS;
 }
