@@ -109,6 +109,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-coarse_band") {
+      PMX_TRY(geti(c->coarse_band));
     } else if (k == "-coarse_deflate") {
       PMX_TRY(geti(c->coarse_deflate));
     } else if (k == "-lv3_warm") {
@@ -379,10 +381,12 @@ static int setup_coarse(pmx_ctx* c) {
   // level-2 factor: the dense inverse, or the block ICC(0) of the spe10 fork.
   PMX_TRY(c->Acb.alloc((size_t)c->nsub * 7 * c->en_lv1 * c->en_lv1));
   for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
-  if (c->lv2_icc) {
+  c->use_band = !c->lv2_icc && !c->use_2level &&
+                 (c->coarse_band == 1 || (c->coarse_band < 0 && c->nc > kBandAutoNc));
+  if (c->lv2_icc || c->use_band) {
     for (DBuf<double>* b : {&c->Ac, &c->Ainv, &c->Fop}) b->release();
     PMX_TRY(launch_galerkin(c));
-    PMX_TRY(bicc_setup(c));
+    PMX_TRY(c->lv2_icc ? bicc_setup(c) : band_setup(c));
   } else {
     PMX_TRY(c->Ac.alloc((size_t)c->nc * c->nc));
     PMX_TRY(c->Ainv.alloc((size_t)c->nc * c->nc));
@@ -416,14 +420,14 @@ static int setup_coarse(pmx_ctx* c) {
   double t1 = now();
   c->t_stages[PMX_STAGE_SU_LV2_2] += t1 - t0;
   if (!c->use_2level) {
-    PMX_TRY(c->lv2_icc ? level3_setup_icc(c) : level3_setup(c));
+    PMX_TRY((c->lv2_icc || c->use_band) ? level3_setup_icc(c) : level3_setup(c));
     double t2 = now();
     c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
   }
   const double t3 = now();
   // The dense path composes the coarse stages into one nc x nc operator; the
   // ICC(0) path runs them per apply in one workgroup (no nc^2 storage).
-  if (!c->lv2_icc) PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  if (!c->lv2_icc && !c->use_band) PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
   c->t_stages[PMX_STAGE_SU_LV3_2] += now() - t3;
   return PMX_SUCCESS;
 }
@@ -556,7 +560,7 @@ static int apply_device(pmx_ctx* c, double* x, double* z, bool cg_partial, bool 
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
-  if (c->lv2_icc) {
+  if (c->lv2_icc || c->use_band) {
     PMX_TRY(launch_coarse_program(c, c->coarse_prog, tv, u, c->c_e.p, d, 6, 7, kt_begin, kt_end));
   } else {
     ev = kt_begin(c, 6);
@@ -1347,6 +1351,9 @@ int pmx_final(pmx_ctx* c) {
   c->bicc_plane.release();
   c->bicc_elem.release();
   c->bicc_slot.release();
+  c->band.release();
+  c->band_dinv.release();
+  c->band_tmp.release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

@@ -1132,17 +1132,21 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
 // Runs the coarse program prog (2 = level 2, 3 = level 3) on t, leaving the
 // total correction in u: GEMVs with B^{-1} on the whole device, everything
 // else fused per segment.
+int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol) {
+  return ctx->use_band ? launch_band_solve(ctx, t, d, ncol) : launch_bicc_solve(ctx, t, d, ncol);
+}
+
 int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
                           double* y, double* d, int kt_gemv, int kt_small,
                           int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int)) {
-  if (ctx->lv2_icc) {
+  if (ctx->lv2_icc || ctx->use_band) {
     // ICC(0) level 2 and level 3 run in one workgroup each (their work is a
     // dependency chain / a few dot products); the coarse residual update
     // t -= A_c d and u += d run on the whole device.
     for (size_t k = 0; k < prog.size(); ++k) {
       const int ev = kt_begin(ctx, prog[k] == 2 ? kt_gemv : kt_small);
       if (prog[k] == 2)
-        PMX_TRY(launch_bicc_solve(ctx, t, d, 1));
+        PMX_TRY(launch_coarse_prec(ctx, t, d, 1));
       else
         PMX_TRY(launch_lv3_apply(ctx, t, d));
       kt_end(ctx, ev);
@@ -1727,9 +1731,16 @@ __global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
   constexpr int L = kLv3Max;
   __shared__ double A[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
   __shared__ int fl;
+  // mode 0 works on the column-scaled Gram D G D (D = diag(G)^{-1/2}), so
+  // columns of very different norms (small residual directions) stay
+  // well-conditioned; T = D L^{-T}.
+  if (threadIdx.x < ns) w[threadIdx.x] = Gm[threadIdx.x * ns + threadIdx.x] > 0.0
+                                             ? 1.0 / sqrt(Gm[threadIdx.x * ns + threadIdx.x])
+                                             : 1.0;
+  __syncthreads();
   for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
     const int i = e / ns, j = e % ns;
-    A[i * L + j] = mode == 0 ? Gm[e] : 0.5 * (Gm[e] + Gm[j * ns + i]);
+    A[i * L + j] = mode == 0 ? Gm[e] * w[i] * w[j] : 0.5 * (Gm[e] + Gm[j * ns + i]);
   }
   __syncthreads();
   if (mode == 0) {
@@ -1738,7 +1749,7 @@ __global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
     wg_trinv_lower(A, tmp, ns, L);  // A = L^{-1}
     for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
       const int k = e / ns, c = e % ns;
-      T[e] = A[c * L + k];
+      T[e] = w[k] * A[c * L + k];
     }
     return;
   }
@@ -1747,10 +1758,12 @@ __global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
   if (threadIdx.x < ns) wout[threadIdx.x] = w[threadIdx.x];
 }
 
-// out[:, c] = sum_{k < kin} S[:, k] T[(r0 + k) * tld + c0 + c] for c < cout.
+// out[:, c] = beta out[:, c] + alpha sum_{k < kin} S[:, k] T[(r0 + k) * tld + c0 + c]
+// for c < cout.
 __global__ void __launch_bounds__(256) k_xform(const double* __restrict__ S, int n, int kin,
                                                const double* __restrict__ T, int tld, int r0,
-                                               int c0, int cout, double* __restrict__ out) {
+                                               int c0, int cout, double* __restrict__ out,
+                                               double alpha, double beta) {
   __shared__ double Ts[kLv3Max * kLv3Max];
   for (int e = threadIdx.x; e < kin * cout; e += blockDim.x)
     Ts[e] = T[(size_t)(r0 + e / cout) * tld + c0 + e % cout];
@@ -1764,9 +1777,30 @@ __global__ void __launch_bounds__(256) k_xform(const double* __restrict__ S, int
 #pragma unroll
       for (int k = 0; k < kLv3Max; ++k)
         if (k < kin) v += sv[k] * Ts[k * cout + c];
-      out[(size_t)c * n + r] = v;
+      out[(size_t)c * n + r] = beta == 0.0 ? alpha * v : beta * out[(size_t)c * n + r] + alpha * v;
     }
   }
+}
+
+// y += alpha x.
+__global__ void k_axpy(const double* __restrict__ x, double alpha, double* __restrict__ y,
+                       size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
+// res[j] = ||X_j|| (one workgroup per column).
+__global__ void k_col_norm(const double* __restrict__ X, int n, double* __restrict__ res) {
+  __shared__ double red[8];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double e = X[(size_t)j * n + r];
+    s += e * e;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) res[j] = sqrt(s);
 }
 
 // R = AX - X diag(w), res[j] = ||R_j|| (one workgroup per column).
@@ -1802,10 +1836,11 @@ struct Lv3Work {
     PMX_HIP(hipGetLastError());
     return PMX_SUCCESS;
   }
-  int xform(const double* S, int kin, int tld, int r0, int c0, int cout, double* out) {
+  int xform(const double* S, int kin, int tld, int r0, int c0, int cout, double* out,
+            const double* Tm = nullptr, double alpha = 1.0, double beta = 0.0) {
     const int nb = std::min((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_xform, dim3(nb), dim3(256), 0, ctx->stream, S, n, kin, T.p, tld, r0, c0,
-                       cout, out);
+    hipLaunchKernelGGL(k_xform, dim3(nb), dim3(256), 0, ctx->stream, S, n, kin, Tm ? Tm : T.p, tld,
+                       r0, c0, cout, out, alpha, beta);
     PMX_HIP(hipGetLastError());
     return PMX_SUCCESS;
   }
@@ -1838,12 +1873,17 @@ struct Lv3Work {
 // norm); a residual that stagnates (no 2x gain over 20 iterations) below
 // 100 eig_tol is accepted and reported in lv3_floor.  Warm start: the
 // previous setup's eigenvectors when the coarse space has the same size.
+// With the exact band factor instead of ICC(0) the same loop runs the
+// residual-corrected shift-invert subspace iteration of the dense path
+// (basis X - A_c^{-1}(A_c X - X diag(w)) only, m = max(4 en_lv2, en_lv2 + 8))
+// and tests the shift-inverted residual ||x - w A_c^{-1} x||.
 int level3_setup_icc(pmx_ctx* ctx) {
   const int n = ctx->nc;
   const int k2 = std::min(ctx->en_lv2, n);
-  const int m = std::min(std::min(k2 + 6, kLv3Max / 3), n);
+  const int m = ctx->use_band ? std::min(std::min(std::max(4 * k2, k2 + 8), kLv3Max), n)
+                             : std::min(std::min(k2 + 6, kLv3Max / 3), n);
   if (m < k2) {
-    set_error("-lv2_icc: en_lv2=%d exceeds the level-3 block limit", k2);
+    set_error("sparse coarse path: en_lv2=%d exceeds the level-3 block limit", k2);
     return PMX_ERR_SUP;
   }
   Lv3Work wk;
@@ -1869,7 +1909,7 @@ int level3_setup_icc(pmx_ctx* ctx) {
   for (auto& v : h) v = nd(rng);
   PMX_HIP(hipMemcpyAsync(R.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
-  PMX_TRY(launch_bicc_solve(ctx, R.p, S.p, m));
+  PMX_TRY(launch_coarse_prec(ctx, R.p, S.p, m));
   const bool warm = ctx->lv3_warm && ctx->V2.p && ctx->lv3_prev_n == n && ctx->eigen_num_lv2 > 0;
   if (warm)
     PMX_HIP(hipMemcpyAsync(S.p, ctx->V2.p, (size_t)ctx->eigen_num_lv2 * colb,
@@ -1880,9 +1920,16 @@ int level3_setup_icc(pmx_ctx* ctx) {
   ctx->lv3_floor = 0.0;
   const int maxit = std::max(ctx->eig_maxit, 2000);
   for (it = 0; it <= maxit; ++it) {
-    if (it > 0) {
-      // W = M^{-1} R into S[:, m:2m], X into S[:, 0:m], P into S[:, 2m:3m].
-      PMX_TRY(launch_bicc_solve(ctx, R.p, S.p + (size_t)m * n, m));
+    if (it > 0 && ctx->use_band) {
+      // Exact factor: residual-corrected shift-invert subspace iteration,
+      // S = X - A_c^{-1}(A_c X - X diag(w)) = diag(w) A_c^{-1} X.
+      PMX_HIP(hipMemcpyAsync(S.p, X.p, m * colb, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_axpy, dim3(std::min((n * m + kBlock - 1) / kBlock, 4096)),
+                         dim3(kBlock), 0, ctx->stream, S.p + (size_t)m * n, -1.0, S.p,
+                         (size_t)n * m);
+      ns = m;
+    } else if (it > 0) {
+      // X into S[:, 0:m], P into S[:, 2m:3m] (W = M^{-1} R is in S[:, m:2m]).
       PMX_HIP(hipMemcpyAsync(S.p, X.p, m * colb, hipMemcpyDeviceToDevice, ctx->stream));
       if (np) PMX_HIP(hipMemcpyAsync(S.p + (size_t)2 * m * n, P.p, m * colb,
                                      hipMemcpyDeviceToDevice, ctx->stream));
@@ -1896,8 +1943,14 @@ int level3_setup_icc(pmx_ctx* ctx) {
       np = 0;
       PMX_TRY(wk.orthonormalize(S.p, ns, tmp.p, &bad));
     }
+    if (bad && !hist.empty() && hist.back() <= 100.0 * ctx->eig_tol) {
+      // The new directions are rounding noise of converged pairs: accept.
+      conv = true;
+      ctx->lv3_floor = hist.back();
+      break;
+    }
     if (bad) {
-      set_error("level-3 eigensolver (ICC(0)-preconditioned): basis lost rank at iteration %d",
+      set_error("level-3 eigensolver (sparse coarse path): basis lost rank at iteration %d",
                 it);
       return PMX_ERR_USER;
     }
@@ -1916,6 +1969,18 @@ int level3_setup_icc(pmx_ctx* ctx) {
     }
     hipLaunchKernelGGL(k_col_resid, dim3(m), dim3(kBlock), 0, ctx->stream, AX.p, X.p, wk.wall.p,
                        R.p, n, res.p);
+    // Next search directions W = M^{-1} R.  With the exact band factor
+    // ||W_j|| = ||x_j - w_j A_c^{-1} x_j|| is the shift-inverted residual
+    // (scale-free, also for the pinned near-null pair): the convergence test.
+    double* W = S.p + (size_t)m * n;
+    PMX_TRY(launch_coarse_prec(ctx, R.p, W, m));
+    if (ctx->use_band)
+      hipLaunchKernelGGL(k_col_norm, dim3(m), dim3(kBlock), 0, ctx->stream, W, n, res.p);
+    if (!ctx->use_band) {
+      // W -= X (X^T W): the new directions orthogonal to the Ritz vectors.
+      PMX_TRY(wk.gram(X.p, m, W, m));
+      PMX_TRY(wk.xform(X.p, m, m, 0, 0, m, W, wk.G.p, -1.0, 1.0));
+    }
     PMX_HIP(hipGetLastError());
     PMX_HIP(hipMemcpyAsync(hres.data(), res.p, m * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -1924,7 +1989,8 @@ int level3_setup_icc(pmx_ctx* ctx) {
     PMX_HIP(hipStreamSynchronize(ctx->stream));
     double mres = 0.0;
     for (int j = 0; j < k2; ++j)
-      mres = std::max(mres, hres[j] / std::max(std::fabs(hw[j]), 1e-300));
+      mres = std::max(mres, ctx->use_band ? hres[j]
+                                          : hres[j] / std::max(std::fabs(hw[j]), 1e-300));
     hist.push_back(mres);
     if (ctx->eig_debug >= 0 && (it < 20 || it % 50 == 0))
       printf("lv3 icc it %d ns %d res %.3e w0 %.6e\n", it, ns, mres, hw[0]);
@@ -1943,7 +2009,7 @@ int level3_setup_icc(pmx_ctx* ctx) {
     }
   }
   if (!conv) {
-    set_error("level-3 eigensolver (ICC(0)-preconditioned) did not converge (%d iterations)",
+    set_error("level-3 eigensolver (sparse coarse path) did not converge (%d iterations)",
               it);
     return PMX_ERR_USER;
   }

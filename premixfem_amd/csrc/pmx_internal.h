@@ -25,6 +25,7 @@ constexpr int kMaxPlanes = 768;    // hyperplanes of a brick (3 * 256 - 2 at mos
 constexpr int kMaxShapes = 8;      // uneven partitions give at most 2 lengths per axis
 constexpr int kMaxEig = 16;        // largest en_lv1 / en_lv2 supported on device
 constexpr double kNil = 1.0e-12;   // NIL of src/PreMixFEM_3D.h:14
+constexpr int kBandAutoNc = 12288; // coarse size above which level 2 uses the band factor
 
 void set_error(const char* fmt, ...);
 
@@ -333,6 +334,13 @@ struct pmx_ctx {
   // Level-2 factor of the spe10 fork (src/spe10.c:509-516): ICC(0) of A_c,
   // i.e. a block incomplete Cholesky over the subdomains (-lv2_icc).
   int lv2_icc = 0;
+  // Exact level-2 factor in band storage for large coarse spaces (-coarse_band
+  // 1, or automatically above kBandAutoNc unknowns): tiles of the Cholesky
+  // factor of A_c and the inverses of its diagonal tiles.
+  int coarse_band = -1;             // -1 auto, 0 dense, 1 band
+  int use_band = 0;                 // chosen at setup
+  pmx::DBuf<double> band, band_dinv, band_tmp;
+  int band_nt = 0, band_w = 0;
   pmx::DBuf<double> bicc_dinv;      // [nsub][kmax][kmax] inverse pivot blocks
   pmx::DBuf<int> bicc_plane;        // coarse hyperplane prefix counts (nplanes+1)
   pmx::DBuf<int> bicc_elem;         // subdomains in hyperplane order

@@ -52,6 +52,11 @@ int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d);
 int bicc_setup(pmx_ctx* ctx);
 int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol);
 int level3_setup_icc(pmx_ctx* ctx);
+// exact level-2 factor in band storage (pmx_band.hip)
+int band_setup(pmx_ctx* ctx);
+int launch_band_solve(pmx_ctx* ctx, const double* t, double* d, int ncol);
+// level-2 solve of the sparse coarse paths: ICC(0) sweeps or the band factor
+int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol);
 
 // level-2 eigensolver (pmx_lobpcg.hip)
 int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows);

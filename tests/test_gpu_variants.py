@@ -21,6 +21,9 @@ VARIANTS = [
     ("si_lv1=2", dict(si_lv1=2), "-si_lv1 2"),
     ("si_lv2=2", dict(si_lv2=2), "-si_lv2 2"),
     ("W-si2", dict(use_W_cycle=True, si_lv2=2), "-use_W_cycle -si_lv2 2"),
+    # exact level-2 factor in band storage (the large-coarse-space path)
+    ("band", dict(), "-coarse_band 1"),
+    ("band-W", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle"),
 ]
 
 
