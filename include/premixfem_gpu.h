@@ -99,7 +99,12 @@ typedef struct {
  * -cg_chunk, -kernel_timing, and -lv2_icc (the spe10 fork's level-2 factor:
  * ICC(0) of the coarse matrix, src/spe10.c:509-516; its level-3 eigenvectors
  * come from ICC(0)-preconditioned block LOBPCG, started from the previous
- * setup's unless -lv3_warm 0).  Allocates the
+ * setup's unless -lv3_warm 0), -coarse_band 0|1 (exact level-2 Cholesky
+ * factor in band storage; automatic above 12288 coarse unknowns),
+ * -coarse_composite 0|1 (dense path: compose the coarse stages into one
+ * operator, default, or apply them one by one), -coarse_deflate 0|1 (dense
+ * path: invert A_c deflated with a Woodbury correction, default, or
+ * directly).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 

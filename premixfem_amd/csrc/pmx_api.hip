@@ -109,6 +109,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-coarse_composite") {
+      PMX_TRY(geti(c->coarse_composite));
     } else if (k == "-coarse_band") {
       PMX_TRY(geti(c->coarse_band));
     } else if (k == "-coarse_deflate") {
@@ -427,7 +429,8 @@ static int setup_coarse(pmx_ctx* c) {
   const double t3 = now();
   // The dense path composes the coarse stages into one nc x nc operator; the
   // ICC(0) path runs them per apply in one workgroup (no nc^2 storage).
-  if (!c->lv2_icc && !c->use_band) PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  if (!c->lv2_icc && !c->use_band && c->coarse_composite)
+    PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
   c->t_stages[PMX_STAGE_SU_LV3_2] += now() - t3;
   return PMX_SUCCESS;
 }
@@ -560,7 +563,7 @@ static int apply_device(pmx_ctx* c, double* x, double* z, bool cg_partial, bool 
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
-  if (c->lv2_icc || c->use_band) {
+  if (c->lv2_icc || c->use_band || !c->coarse_composite) {
     PMX_TRY(launch_coarse_program(c, c->coarse_prog, tv, u, c->c_e.p, d, 6, 7, kt_begin, kt_end));
   } else {
     ev = kt_begin(c, 6);

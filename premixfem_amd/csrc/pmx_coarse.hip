@@ -850,92 +850,6 @@ int bicc_setup(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
-// One workgroup runs a segment of the coarse program of PC_apply_vec: the
-// stages between two dense GEMVs.  Each op acts on the coarse residual t and
-// the accumulated correction u:
-//   op.kind 2 (level 2): d = Woodbury fix of y = B^{-1} t (the GEMV before the
-//                        segment), only as the segment's first op;
-//   op.kind 3 (level 3): d = V A_cc^{-1} V^T t;
-// then u = d (first) or u += d, and t -= A_c d unless it is the program's
-// last stage.
-struct CoarseOp {
-  int kind, first, update_t;
-};
-struct CoarseSeg {
-  int nops;
-  CoarseOp op[8];
-};
-
-__global__ void k_coarse_seg(Geo g, CoarseSeg seg, const double* __restrict__ Acb, int kmax,
-                             const double* __restrict__ y, double* __restrict__ t,
-                             double* __restrict__ u, double* __restrict__ d,
-                             const double* __restrict__ z, const double* __restrict__ h,
-                             const double* __restrict__ cs, const double* __restrict__ V, int k2,
-                             const double* __restrict__ LU, int nc) {
-  __shared__ double red[8];
-  __shared__ double cvec[kMaxEig];
-  for (int q = 0; q < seg.nops; ++q) {
-    const CoarseOp op = seg.op[q];
-    if (op.kind == 2) {
-      double zt = 0.0, ht = 0.0;
-      for (int r = threadIdx.x; r < nc; r += blockDim.x) {
-        zt += z[r] * t[r];
-        ht += h[r] * t[r];
-      }
-      zt = block_sum(zt, red);
-      ht = block_sum(ht, red);
-      const double dg = cs[0];
-      const double u0 = zt / dg;
-      const double a0 = cs[1] * u0 + cs[2] * ht;
-      const double a1 = cs[2] * u0 + cs[3] * ht;
-      for (int r = threadIdx.x; r < nc; r += blockDim.x) d[r] = y[r] - (z[r] / dg) * a0 - h[r] * a1;
-    } else {
-      for (int j = 0; j < k2; ++j) {
-        double sj = 0.0;
-        for (int r = threadIdx.x; r < nc; r += blockDim.x) sj += V[(size_t)j * nc + r] * t[r];
-        sj = block_sum(sj, red);
-        if (threadIdx.x == 0) cvec[j] = sj;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const double* piv = LU + k2 * k2;
-        double bb[kMaxEig];
-        for (int i = 0; i < k2; ++i) bb[i] = cvec[i];
-        for (int i = 0; i < k2; ++i) {
-          const int p = (int)piv[i];
-          const double tmp = bb[i];
-          bb[i] = bb[p];
-          bb[p] = tmp;
-        }
-        for (int i = 0; i < k2; ++i)
-          for (int j = 0; j < i; ++j) bb[i] -= LU[i * k2 + j] * bb[j];
-        for (int i = k2 - 1; i >= 0; --i) {
-          for (int j = i + 1; j < k2; ++j) bb[i] -= LU[i * k2 + j] * bb[j];
-          bb[i] /= LU[i * k2 + i];
-        }
-        for (int i = 0; i < k2; ++i) cvec[i] = bb[i];
-      }
-      __syncthreads();
-      for (int r = threadIdx.x; r < nc; r += blockDim.x) {
-        double sv = 0.0;
-        for (int j = 0; j < k2; ++j) sv += V[(size_t)j * nc + r] * cvec[j];
-        d[r] = sv;
-      }
-    }
-    __syncthreads();
-    // u update and t -= A_c d (rows independent; d complete after the sync)
-    for (int gid = threadIdx.x; gid < g.nsub * kmax; gid += blockDim.x) {
-      const int sb = gid / kmax, i = gid % kmax;
-      const Brick& b = g.bricks[sb];
-      if (i >= b.neig) continue;
-      const int r = b.dof + i;
-      u[r] = op.first ? d[r] : u[r] + d[r];
-      if (op.update_t) t[r] -= coarse_row(g, Acb, kmax, sb, i, d);
-    }
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Composite coarse operator.  Every stage of the coarse program maps the
 // coarse residual t to a correction d = G t (G = A_c^{-1} for level 2,
@@ -1139,42 +1053,26 @@ int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol) {
 int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,
                           double* y, double* d, int kt_gemv, int kt_small,
                           int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int)) {
-  if (ctx->lv2_icc || ctx->use_band) {
-    // ICC(0) level 2 and level 3 run in one workgroup each (their work is a
-    // dependency chain / a few dot products); the coarse residual update
-    // t -= A_c d and u += d run on the whole device.
-    for (size_t k = 0; k < prog.size(); ++k) {
-      const int ev = kt_begin(ctx, prog[k] == 2 ? kt_gemv : kt_small);
-      if (prog[k] == 2)
-        PMX_TRY(launch_coarse_prec(ctx, t, d, 1));
-      else
-        PMX_TRY(launch_lv3_apply(ctx, t, d));
-      kt_end(ctx, ev);
-      PMX_TRY(launch_coarse_update(ctx, d, t, u, k == 0, k + 1 < prog.size()));
+  // One stage at a time: the level-2 solve (dense inverse GEMV + Woodbury
+  // fix, ICC(0) sweeps or band solves) or the level-3 correction (one
+  // workgroup), then u += d and t -= A_c d on the whole device.
+  const bool sparse = ctx->lv2_icc || ctx->use_band;
+  for (size_t k = 0; k < prog.size(); ++k) {
+    const int ev = kt_begin(ctx, prog[k] == 2 ? kt_gemv : kt_small);
+    if (prog[k] == 2 && sparse) {
+      PMX_TRY(launch_coarse_prec(ctx, t, d, 1));
+    } else if (prog[k] == 2) {
+      if (ctx->coarse_deflate) {
+        PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, y, ctx->nc));
+        PMX_TRY(launch_coarse_fix(ctx, y, t, d));
+      } else {
+        PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, d, ctx->nc));
+      }
+    } else {
+      PMX_TRY(launch_lv3_apply(ctx, t, d));
     }
-    return PMX_SUCCESS;
-  }
-  size_t k = 0;
-  while (k < prog.size()) {
-    CoarseSeg seg{};
-    if (prog[k] == 2) {
-      const int ev = kt_begin(ctx, kt_gemv);
-      PMX_TRY(launch_gemv(ctx, ctx->Ainv.p, t, y, ctx->nc));
-      kt_end(ctx, ev);
-    }
-    do {
-      CoarseOp& op = seg.op[seg.nops++];
-      op.kind = prog[k];
-      op.first = k == 0;
-      op.update_t = k + 1 < prog.size();
-      ++k;
-    } while (k < prog.size() && prog[k] == 3 && seg.nops < 8);
-    const int ev = kt_begin(ctx, kt_small);
-    hipLaunchKernelGGL(k_coarse_seg, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->geo, seg,
-                       ctx->Acb.p, ctx->en_lv1, y, t, u, d, ctx->c_z.p, ctx->c_h.p, ctx->c_cs.p,
-                       ctx->V2.p, ctx->eigen_num_lv2, ctx->Acc_lu.p, ctx->nc);
-    PMX_HIP(hipGetLastError());
     kt_end(ctx, ev);
+    PMX_TRY(launch_coarse_update(ctx, d, t, u, k == 0, k + 1 < prog.size()));
   }
   return PMX_SUCCESS;
 }

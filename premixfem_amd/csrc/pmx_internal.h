@@ -329,6 +329,8 @@ struct pmx_ctx {
   pmx::DBuf<double> c_z, c_g, c_h, c_cs;  // near-null vector, pin row, B^{-1} g, Woodbury scalars
   double coarse_gamma = 0.0;
   int coarse_deflate = 1;           // invert A_c deflated + Woodbury (1) or directly (0)
+  int coarse_composite = 1;         // dense path: one composed coarse operator (1) or the
+                                    // coarse stages one by one with their residual updates (0)
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
   // Level-2 factor of the spe10 fork (src/spe10.c:509-516): ICC(0) of A_c,
