@@ -360,11 +360,11 @@ def main():
     for m in args.modes.split(","):
         prec = 64 if m == "fp64" else 32
         net = None
-        extra = ""
+        extra = "-eig_autotune 1"
         if m == "learned":
             net, training = train_warm_start(prob)
             print(f"# warm-start network: {training}", file=sys.stderr, flush=True)
-            extra = "-warm_start cnn"
+            extra += " -warm_start cnn"
         r = run_mode(prob, prec, args.steps, args.warmup, extra, net)
         if net is not None:
             net.close()

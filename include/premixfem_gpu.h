@@ -88,6 +88,7 @@ typedef struct {
   double t_solve;            /* last pmx_solve wall seconds (PCG iterations only) */
   int eig_stagnated;         /* subdomains accepted at the eigensolver's stagnation test
                                 (worst residual within -eig_stag x -eig_tol) */
+  int eig_block;             /* LOBPCG block size used (chosen by -eig_autotune 1) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),
@@ -101,6 +102,9 @@ typedef struct {
  * come from ICC(0)-preconditioned block LOBPCG, started from the previous
  * setup's unless -lv3_warm 0), -coarse_band 0|1 (exact level-2 Cholesky
  * factor in band storage; automatic above 12288 coarse unknowns),
+ * -eig_guard g (LOBPCG guard vectors), -eig_autotune 0|1 (time the LOBPCG
+ * block sizes on a sample of subdomains and keep the fastest, cached per
+ * subdomain size and eigen count for the process),
  * -coarse_composite 0|1 (dense path: compose the coarse stages into one
  * operator, default, or apply them one by one), -coarse_deflate 0|1 (dense
  * path: invert A_c deflated with a Woodbury correction, default, or

@@ -109,6 +109,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-eig_autotune") {
+      PMX_TRY(geti(c->eig_autotune));
     } else if (k == "-coarse_composite") {
       PMX_TRY(geti(c->coarse_composite));
     } else if (k == "-coarse_band") {
@@ -951,6 +953,7 @@ static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
     c->eig_iters_max = c->eig_iters_sum = 0;
   } else {
     c->h_eigen_num_given.clear();
+    if (c->eig_autotune) PMX_TRY(lobpcg_autotune(c));
     if (c->warm_start == 1) {
       // learned warm start: the network's block seeds LOBPCG
       const int mb = lobpcg_block_size(c->en_lv1, c->eig_guard);
@@ -1216,6 +1219,7 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_iters_mean = c->nsub ? (double)c->eig_iters_sum / c->nsub : 0.0;
   st->eig_fallbacks = c->eig_fallbacks;
   st->eig_stagnated = c->eig_stagnated;
+  st->eig_block = c->eig_block;
   st->lv3_iters = c->lv3_iters;
   st->pc_precision = c->pc_precision;
   st->its = c->last_its;

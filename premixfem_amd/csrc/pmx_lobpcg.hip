@@ -20,6 +20,7 @@
 // * Output: M-orthonormal eigenvectors with a fixed sign convention, written
 //   into the basis rows, eigenvalues ascending with lambda_0 = 0.
 #include <cmath>
+#include <map>
 
 #include "pmx_dense.h"
 #include "pmx_kernels.h"
@@ -31,6 +32,7 @@ constexpr int kStagIts = 40;   // iterations without progress before the stagnat
 
 struct LobArgs {
   int kwant, maxit, kmax;
+  int stride;   // workgroup b solves subdomain b * stride (> 1: autotuner sample)
   double tol;
   uint64_t seed;
   double* X;
@@ -391,7 +393,7 @@ template <int MB>
 __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   constexpr int LD = 3 * MB;
   extern __shared__ double lds[];
-  const int sidx = blockIdx.x;
+  const int sidx = blockIdx.x * a.stride;
   const Brick b = g.bricks[sidx];
   const BrickView v = load_view(g, sidx);
   const int n = v.n;
@@ -782,8 +784,8 @@ static size_t lobpcg_lds(int maxn) {
 
 template <int MB>
 static void launch_lobpcg_mb(pmx_ctx* ctx, const LobArgs& a, int maxn) {
-  hipLaunchKernelGGL(k_lobpcg<MB>, dim3(ctx->nsub), dim3(kBlock), lobpcg_lds<MB>(maxn),
-                     ctx->stream, ctx->geo, a);
+  hipLaunchKernelGGL(k_lobpcg<MB>, dim3((ctx->nsub + a.stride - 1) / a.stride), dim3(kBlock),
+                     lobpcg_lds<MB>(maxn), ctx->stream, ctx->geo, a);
 }
 
 int lobpcg_block_size(int en_lv1, int guard) {
@@ -791,9 +793,11 @@ int lobpcg_block_size(int en_lv1, int guard) {
   return mb <= 4 ? 4 : (mb <= 6 ? 6 : (mb <= 8 ? 8 : (mb <= 10 ? 10 : 0)));
 }
 
-int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
+// One batched LOBPCG launch with block size MB over every stride-th
+// subdomain.  ms == nullptr: the real solve (results and statistics kept);
+// otherwise an autotuner trial, only its duration (*ms) is returned.
+static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, float* ms) {
   const int kw = ctx->en_lv1 - 1;
-  const int MB = lobpcg_block_size(ctx->en_lv1, ctx->eig_guard);
   if (MB == 0) {
     set_error("en_lv1=%d exceeds the LOBPCG block limit", ctx->en_lv1);
     return PMX_ERR_SUP;
@@ -828,6 +832,7 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   PMX_TRY(fallback.alloc(ctx->nsub));
   PMX_TRY(stag.alloc(ctx->nsub));
   LobArgs a;
+  a.stride = stride;
   a.kwant = kw;
   a.maxit = ctx->eig_maxit;
   a.kmax = ctx->en_lv1;
@@ -855,6 +860,12 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   a.stagnated = stag.p;
   a.stag_factor = ctx->eig_stag;
   a.debug = ctx->eig_debug;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    PMX_HIP(hipEventCreate(&e0));
+    PMX_HIP(hipEventCreate(&e1));
+    PMX_HIP(hipEventRecord(e0, ctx->stream));
+  }
   switch (MB) {
     case 4:
       launch_lobpcg_mb<4>(ctx, a, maxn);
@@ -870,6 +881,15 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
       break;
   }
   PMX_HIP(hipGetLastError());
+  if (ms) {
+    PMX_HIP(hipEventRecord(e1, ctx->stream));
+    PMX_HIP(hipEventSynchronize(e1));
+    PMX_HIP(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (ms) return PMX_SUCCESS;
+  ctx->eig_block = MB;
   std::vector<int> hits(ctx->nsub), hst(ctx->nsub), hfb(ctx->nsub), hsg(ctx->nsub);
   PMX_HIP(hipMemcpyAsync(hsg.data(), stag.p, ctx->nsub * sizeof(int), hipMemcpyDeviceToHost,
                          ctx->stream));
@@ -906,6 +926,62 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
               bad, ctx->nsub, first_bad, hst[first_bad]);
     return PMX_ERR_USER;
   }
+  return PMX_SUCCESS;
+}
+
+int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
+  return lobpcg_run(ctx, x0_rows, lobpcg_block_size(ctx->en_lv1, ctx->eig_guard), 1, nullptr);
+}
+
+// Block-size autotuner of the batched LOBPCG (-eig_autotune 1).  The block
+// size MB (wanted pairs + guard vectors) trades work per iteration against
+// iterations; the best value depends on the subdomain size, the eigen count
+// and the medium.  Every admissible MB in {4, 6, 8, 10} is timed from a
+// random start on the whole batch (a strided sample of 1024 subdomains on
+// larger problems: the launch is timed at the occupancy it runs at); the
+// fastest sets -eig_guard, cached per (largest subdomain, en_lv1, well
+// term) for the process.  Tuning only changes how fast LOBPCG reaches the
+// tolerance.
+namespace {
+struct TuneKey {
+  int maxn, k, well;
+  bool operator<(const TuneKey& o) const {
+    return maxn != o.maxn ? maxn < o.maxn : (k != o.k ? k < o.k : well < o.well);
+  }
+};
+std::map<TuneKey, int>& tune_cache() {
+  static std::map<TuneKey, int> m;
+  return m;
+}
+}  // namespace
+
+int lobpcg_autotune(pmx_ctx* ctx) {
+  int maxn = 0, minn = 1 << 30;
+  for (const Brick& b : ctx->h_bricks) {
+    maxn = std::max(maxn, b.nx * b.ny * b.nz);
+    minn = std::min(minn, b.nx * b.ny * b.nz);
+  }
+  const int k = ctx->en_lv1;
+  const TuneKey key{maxn, k, ctx->have_wd};
+  auto it = tune_cache().find(key);
+  if (it == tune_cache().end()) {
+    const int need = ctx->have_wd ? k + 1 : k;   // at least one guard vector
+    const int stride = std::max(1, (ctx->nsub + 1023) / 1024);
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int mb : {4, 6, 8, 10}) {
+      if (mb < need || mb > minn - 1) continue;
+      float ms = 0.0f;
+      PMX_TRY(lobpcg_run(ctx, nullptr, mb, stride, &ms));
+      if (!best || ms < best_ms) {
+        best = mb;
+        best_ms = ms;
+      }
+    }
+    if (!best) return PMX_SUCCESS;   // nothing admissible: keep -eig_guard
+    it = tune_cache().emplace(key, best).first;
+  }
+  ctx->eig_guard = it->second - (k - 1);
   return PMX_SUCCESS;
 }
 

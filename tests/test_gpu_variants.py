@@ -21,6 +21,10 @@ VARIANTS = [
     ("si_lv1=2", dict(si_lv1=2), "-si_lv1 2"),
     ("si_lv2=2", dict(si_lv2=2), "-si_lv2 2"),
     ("W-si2", dict(use_W_cycle=True, si_lv2=2), "-use_W_cycle -si_lv2 2"),
+    # coarse stages one by one (no composed operator), with and without the
+    # deflated inverse
+    ("stagewise", dict(), "-coarse_composite 0"),
+    ("stagewise-plain", dict(), "-coarse_composite 0 -coarse_deflate 0"),
     # exact level-2 factor in band storage (the large-coarse-space path)
     ("band", dict(), "-coarse_band 1"),
     ("band-W", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle"),
@@ -99,3 +103,19 @@ def test_eigen_mass_operators_full_setup(eg_op):
     err = np.linalg.norm(res["x"] - x_ref) / np.linalg.norm(x_ref)
     assert abs(res["its"] - its_ref) <= 1
     assert err <= 1e-7
+
+
+def test_lobpcg_autotuner_keeps_the_basis():
+    """-eig_autotune 1 picks a LOBPCG block size by timing a sample; the
+    eigenvalues and the solve are those of the untuned run (to tolerance)."""
+    prob = _prob()
+    base = runner.run(prob)
+    tuned = runner.run(prob, extra="-eig_autotune 1")
+    st = tuned["stats"]
+    print("autotuned block", st["eig_block"], "default", base["stats"]["eig_block"], flush=True)
+    assert st["eig_block"] in (4, 6, 8, 10)
+    assert abs(tuned["its"] - base["its"]) <= 1
+    assert st["eigen_max_lv1_range"][1] == pytest.approx(base["stats"]["eigen_max_lv1_range"][1],
+                                                         rel=1e-6)
+    err = np.linalg.norm(tuned["x"] - base["x"]) / np.linalg.norm(base["x"])
+    assert err <= 1e-8
