@@ -151,14 +151,15 @@ def measured_traffic(workload, mode, kernel):
     return None
 
 
-def train_warm_start(prob, nfields=4, epochs=30, batch=64):
+def train_warm_start(prob, nfields=4, epochs=30, batch=64, kind=None):
     """Network for the learned mode: trained on the eigenvectors of
-    ``nfields`` seeded fields of the workload's family (same mesh and
-    subdomains, seeds 10000+)."""
+    ``nfields`` seeded fields of the workload's family (``kind``, by default
+    the workload's name; same mesh and subdomains, seeds 10000+)."""
     from premixfem_amd import cnn, problems
 
     t0 = time.perf_counter()
-    data = [cnn.sample_problem(problems.training_problem(prob.name, 10000 + s, prob.mesh))
+    kind = kind or prob.name
+    data = [cnn.sample_problem(problems.training_problem(kind, 10000 + s, prob.mesh))
             for s in range(nfields)]
     t1 = time.perf_counter()
     net = cnn.WarmStartNet(cout=prob.en_lv1 - 1, seed=1)
@@ -323,6 +324,195 @@ def impes_main(args):
     print(json.dumps(out))
 
 
+def batch_cpu_baseline(probs):
+    """The FP64 oracle on a bounded sample of the batch: the smallest member
+    set up and solved in full, scaled by cell count to the whole batch (a
+    lower bound: the coarse factorisations grow faster than the cell
+    count)."""
+    from oracle import premixfem_oracle as O
+
+    small = min(probs, key=lambda p: p.ncell)
+    t0 = time.perf_counter()
+    orc = O.Oracle(small.dom, small.mesh, small.kappa, sd=small.sd, en_lv1=small.en_lv1,
+                   en_lv2=small.en_lv2, pin_cell=small.pin_cell).setup()
+    O.pcg(orc.matvec, small.rhs.ravel(), orc.apply, rtol=small.rtol)
+    t_small = time.perf_counter() - t0
+    total_cells = sum(p.ncell for p in probs)
+    return t_small * total_cells / small.ncell, dict(member=small.name, mesh=small.mesh[0],
+                                                     member_s=round(t_small, 3))
+
+
+def batch_main(args):
+    """Batched uneven solves: 64 independent lognormal Darcy problems of
+    {32,48,64,80,96}^3 cells with 8^3-cell subdomains and 6 eigenvectors;
+    one step = pmx_setup_batch over all of them (every subdomain eigenproblem
+    of every problem in ONE LOBPCG launch, autotuned block size, learned warm
+    start) plus the 64 PCG solves (FP32 preconditioner, FP64 CG, rtol 1e-8,
+    graph-replayed), with kappa, b and x resident in HBM."""
+    import torch
+
+    from premixfem_amd import _lib, problems, runner
+    from premixfem_amd import pc as P
+
+    t0 = time.perf_counter()
+    probs = problems.batch_config(args.batch_count)
+    t_gen = time.perf_counter() - t0
+    sizes = [p.mesh[0] for p in probs]
+    ncell = sum(p.ncell for p in probs)
+    nsub = sum(int(np.prod(p.sd)) for p in probs)
+    hist = {n: sizes.count(n) for n in sorted(set(sizes))}
+    print(f"# batch: {len(probs)} problems, {ncell} cells, {nsub} subdomains, sizes {hist} "
+          f"(generated in {t_gen:.1f}s)", file=sys.stderr, flush=True)
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    modes = {}
+    training = None
+    for m in args.modes.split(","):
+        prec = 64 if m == "fp64" else 32
+        net = None
+        extra = "-eig_autotune 1"
+        if m == "learned":
+            net, training = train_warm_start(problems.training_problem("batch", 0),
+                                             kind="batch")
+            print(f"# warm-start network: {training}", file=sys.stderr, flush=True)
+            extra += " -warm_start cnn"
+        ctxs, bs, xs = [], [], []
+        for prob in probs:
+            ctx = P.PCCtx()
+            P.PC_init(ctx, prob.dom, prob.mesh, runner.options_for(prob, prec, extra))
+            if net is not None:
+                net.attach(ctx)
+            ctx.kappa = prob.kappa
+            P._push_kappa(ctx)
+            ctxs.append(ctx)
+            bs.append(torch.from_numpy(np.ascontiguousarray(prob.rhs.ravel())).to(dev))
+            xs.append(torch.zeros_like(bs[-1]))
+        handles = (C.c_void_p * len(ctxs))(*[c.handle for c in ctxs])
+        its = C.c_int(0)
+        rn = C.c_double(0.0)
+        all_its = []
+
+        def step():
+            ts = time.perf_counter()
+            _lib.check(L.pmx_setup_batch(handles, len(ctxs)))
+            tm = time.perf_counter()
+            all_its.clear()
+            for prob, ctx, b, x in zip(probs, ctxs, bs, xs):
+                rc = L.pmx_solve_device(ctx.handle, C.c_void_p(b.data_ptr()),
+                                        C.c_void_p(x.data_ptr()), 0, prob.rtol, 10000,
+                                        C.byref(its), C.byref(rn))
+                if rc != 0:
+                    _lib.check(rc)
+                all_its.append(its.value)
+            torch.cuda.synchronize()
+            return tm - ts, time.perf_counter() - tm
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_setup = t_solve = 0.0
+        for _ in range(args.steps):
+            a, b_ = step()
+            t_setup += a
+            t_solve += b_
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st = [P.PC_get_stats(c) for c in ctxs]
+        for c in ctxs:
+            P.PC_final(c)
+        r = dict(t_step=(t1 - t0) / args.steps, t_setup=t_setup / args.steps,
+                 t_solve=t_solve / args.steps, its=list(all_its),
+                 lobpcg_iters_max=max(x["eig_iters_max"] for x in st),
+                 lobpcg_block=st[0]["eig_block"],
+                 lobpcg_fallbacks=sum(x["eig_fallbacks"] for x in st),
+                 lobpcg_launch_s=round(st[0]["t_stages"]["su_lv2_1"], 5),
+                 coarse_setup_s=round(sum(x["t_stages"]["su"] - x["t_stages"]["su_lv2_1"]
+                                          for x in st), 5),
+                 stage_sums={k: round(sum(x["t_stages"][k] for x in st), 5)
+                             for k in ("su_lv1", "su_lv2_2", "su_lv3_1", "su_lv3_2")})
+        modes[m] = r
+        if net is not None:
+            net.close()
+        print(f"# {m}: step {r['t_step']:.4f}s setup {r['t_setup']:.4f}s solve "
+              f"{r['t_solve']:.4f}s its sum {sum(r['its'])} max {max(r['its'])}; LOBPCG "
+              f"{r['lobpcg_launch_s']:.4f}s (MB {r['lobpcg_block']}, max its "
+              f"{r['lobpcg_iters_max']}), per-problem setup rest {r['coarse_setup_s']:.4f}s "
+              f"{r['stage_sums']}",
+              file=sys.stderr, flush=True)
+    head = "learned" if "learned" in modes else ("fp32" if "fp32" in modes else list(modes)[0])
+    hr = modes[head]
+    # Roofline of the dominant solve kernel: a kernel-timed solve of the
+    # largest member (its launches bracketed by HIP events on the library
+    # stream).
+    big = max(probs, key=lambda p: p.ncell)
+    prec = 64 if head == "fp64" else 32
+    r_big = run_mode(big, prec, 1, 0, "-eig_autotune 1")
+    per_kernel, roofline = solve_roofline(big, r_big, 8 if head == "fp64" else 4,
+                                          "batch-largest", head)
+    cpu = None
+    if not args.no_cpu_baseline:
+        total, parts = batch_cpu_baseline(probs)
+        cpu = dict(value=round(total, 2), unit="s", cores=os.cpu_count(), kind="port",
+                   sample=(f"numpy/SciPy FP64 oracle: the smallest member ({parts['mesh']}^3) "
+                           "set up and solved in full, scaled by cell count to the batch "
+                           "(lower bound)"),
+                   parts=parts)
+    out = {
+        "metric": "batched setup+solve s: 64 uneven problems, one LOBPCG launch, PCG rtol 1e-8",
+        "value": round(hr["t_step"], 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["t_step"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32 preconditioner / f64 CG",
+        "data": "synthetic (seeded lognormal fields, sigma U[1,3], corr 0.1)",
+        "config": {"workload": f"{len(probs)} independent problems, cubes of "
+                               "{32,48,64,80,96}^3 cells (seed 5000+i), 8^3-cell subdomains, "
+                               f"6 eigenvectors; {ncell} cells, {nsub} subdomains in total",
+                   "headline_mode": head},
+        "modes": {m: {k: v for k, v in r.items() if k != "its"} | {"its_sum": sum(r["its"])}
+                  for m, r in modes.items()},
+        "its": hr["its"],
+        "warm_start_training": training,
+        "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in per_kernel.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def solve_roofline(prob, r, s, workload, head):
+    """Per-kernel durations of a kernel-timed solve and the roofline of the
+    dominant kernel (most total time among those with a byte model)."""
+    from premixfem_amd._lib import TIMED_KERNELS
+
+    N = int(np.prod(prob.mesh))
+    nc = r["stats"]["size_coarse"]
+    per_kernel = {}
+    for i, name in enumerate(TIMED_KERNELS):
+        cnt = float(r["kernel_count"][i])
+        if cnt <= 0:
+            continue
+        avg_ms = r["kernel_ms"][i] / cnt
+        by = kernel_bytes(name, N, prob.en_lv1, s, nc)
+        per_kernel[name] = dict(launches_per_step=float(cnt), avg_us=avg_ms * 1e3,
+                                bytes_per_launch=by,
+                                gbs=(by / (avg_ms * 1e-3) / 1e9) if by else None)
+    dom = max((k for k in per_kernel if per_kernel[k]["bytes_per_launch"]),
+              key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
+    dk = per_kernel[dom]
+    roofline = dict(kernel=dom, bound="hbm", achieved=round(dk["gbs"], 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(dk["gbs"] / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(workload, head, dom))
+    return per_kernel, roofline
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,9 +526,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="finish with an elementwise kernel that reads and writes 256 MiB "
                          "(calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--workload", default="setup_solve", choices=("setup_solve", "impes"),
+    ap.add_argument("--workload", default="setup_solve", choices=("setup_solve", "impes", "batch"),
                     help="impes: the spe10-fork IMPES loop on the 60x220x85 reservoir grid "
-                         "(a separate JSON line; --steps IMPES steps)")
+                         "(a separate JSON line; --steps IMPES steps); batch: 64 independent "
+                         "problems of {32..96}^3 cells set up by one batched call")
+    ap.add_argument("--batch-count", type=int, default=64, help="batch: number of problems")
     ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -348,6 +540,9 @@ def main():
 
     if args.workload == "impes":
         impes_main(args)
+        return
+    if args.workload == "batch":
+        batch_main(args)
         return
 
     from premixfem_amd import problems
@@ -374,29 +569,10 @@ def main():
               f"{r['stats']['eig_iters_mean']:.1f}", file=sys.stderr, flush=True)
     head = "learned" if "learned" in modes else ("fp32" if "fp32" in modes else list(modes)[0])
     hr = modes[head]
-    N = int(np.prod(prob.mesh))
-    nc = hr["stats"]["size_coarse"]
     s = 8 if head == "fp64" else 4
-    from premixfem_amd._lib import TIMED_KERNELS
-    per_kernel = {}
-    solve_bytes = 0.0
-    for i, name in enumerate(TIMED_KERNELS):
-        cnt = float(hr["kernel_count"][i])
-        if cnt <= 0:
-            continue
-        avg_ms = hr["kernel_ms"][i] / cnt
-        by = kernel_bytes(name, N, prob.en_lv1, s, nc)
-        if by is not None:
-            solve_bytes += by * cnt
-        per_kernel[name] = dict(launches_per_step=float(cnt), avg_us=avg_ms * 1e3,
-                                bytes_per_launch=by,
-                                gbs=(by / (avg_ms * 1e-3) / 1e9) if by else None)
-    dom = max((k for k in per_kernel if per_kernel[k]["bytes_per_launch"]),
-              key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches_per_step"])
-    dk = per_kernel[dom]
-    roofline = dict(kernel=dom, bound="hbm", achieved=round(dk["gbs"], 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(dk["gbs"] / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(prob.name, head, dom))
+    per_kernel, roofline = solve_roofline(prob, hr, s, prob.name, head)
+    solve_bytes = sum(v["bytes_per_launch"] * v["launches_per_step"] for v in per_kernel.values()
+                      if v["bytes_per_launch"])
     solve_gbs = solve_bytes / hr["t_solve"] / 1e9 if hr["t_solve"] > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

@@ -157,6 +157,19 @@ int pmx_create_A(pmx_ctx *ctx, double *diag, double *off);
  * LOBPCG, Galerkin coarse matrix and its FP64 factorisation) and level 3. */
 int pmx_setup(pmx_ctx *ctx);
 
+/* PC_setup of nprob independent problems (distinct contexts, each with its
+ * own mesh, kappa and options) in one call: level 1 and the coarse levels
+ * are built per context as pmx_setup does, while the subdomain eigenproblems
+ * of ALL problems are solved by ONE batched LOBPCG launch (one workgroup per
+ * subdomain of any problem).  A brick's random start depends only on its
+ * index inside its own problem, so every context ends up with the basis its
+ * own pmx_setup computes.  Requires equal en_lv1, -eig_* options and
+ * warm-start mode; under -eig_autotune the block size is tuned on the
+ * problem with the most subdomains and used for all.  The reference sets up
+ * one problem per PC_setup call (src/PreMixFEM_3D.c:1826-1838); a caller
+ * with many problems calls it once per problem. */
+int pmx_setup_batch(pmx_ctx *const *ctxs, int nprob);
+
 /* PC_setup with the level-2 eigenvectors supplied by the caller (host,
  * natural order, en_lv1 columns of M*N*P doubles each, column j of every
  * subdomain in vector j; eigen_num per subdomain, may be NULL = en_lv1).

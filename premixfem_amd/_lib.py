@@ -34,7 +34,7 @@ EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_s
            "pmx_velocity_device", "pmx_spe10_create", "pmx_spe10_assemble",
            "pmx_spe10_get_system", "pmx_spe10_solve", "pmx_spe10_transport",
            "pmx_spe10_saturation_range", "pmx_spe10_get_state", "pmx_spe10_set_state",
-           "pmx_spe10_run", "pmx_spe10_destroy")
+           "pmx_spe10_run", "pmx_spe10_destroy", "pmx_setup_batch")
 TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
                  "k_cg_update", "k_gemv", "coarse_small")
 
@@ -103,6 +103,7 @@ def lib():
     L.pmx_create_A.argtypes = [vp, _dp, _dp]
     L.pmx_setup.argtypes = [vp]
     L.pmx_setup_from_basis.argtypes = [vp, _dp, C.c_void_p, C.c_void_p]
+    L.pmx_setup_batch.argtypes = [C.POINTER(vp), C.c_int]
     L.pmx_apply.argtypes = [vp, _dp, _dp]
     L.pmx_solve.argtypes = [vp, _dp, _dp, C.c_int, C.c_double, C.c_int,
                             C.POINTER(C.c_int), C.POINTER(C.c_double)]

@@ -219,6 +219,7 @@ static int build_geometry(pmx_ctx* c) {
         b.nb[4] = sz > 0 ? s - sdx * sdy : -1;
         b.nb[5] = sz < sdz - 1 ? s + sdx * sdy : -1;
         b.nplanes = b.nx + b.ny + b.nz - 2;
+        b.lid = s;
         b.dof = 0;
         b.neig = 0;
         const auto key = std::make_tuple(b.nx, b.ny, b.nz);
@@ -979,6 +980,57 @@ static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
 int pmx_setup(pmx_ctx* c) {
   if (!c) return PMX_ERR_ARG_NULL;
   return setup_common(c, nullptr, nullptr, nullptr);
+}
+
+int pmx_setup_batch(pmx_ctx* const* ctxs, int nprob) {
+  if (!ctxs || nprob <= 0) {
+    set_error("pmx_setup_batch needs at least one context");
+    return PMX_ERR_ARG_NULL;
+  }
+  std::vector<pmx_ctx*> cs(ctxs, ctxs + nprob);
+  for (int i = 0; i < nprob; ++i) {
+    if (!cs[i]) {
+      set_error("pmx_setup_batch: context %d is null", i);
+      return PMX_ERR_ARG_NULL;
+    }
+    for (int j = 0; j < i; ++j)
+      if (cs[j] == cs[i]) {
+        set_error("pmx_setup_batch: context %d is listed twice", i);
+        return PMX_ERR_ARG_WRONG;
+      }
+  }
+  std::vector<double> t0(nprob);
+  for (pmx_ctx* c : cs) {
+    std::fill(c->t_stages, c->t_stages + PMX_MAX_LOG_STAGES, 0.0);
+    ++c->setup_gen;
+    c->setup_done = 0;
+    c->h_eigen_num_given.clear();
+  }
+  for (int i = 0; i < nprob; ++i) {
+    t0[i] = now();
+    PMX_TRY(setup_fine(cs[i]));
+  }
+  const double t1 = now();
+  // One launch has one block size: under -eig_autotune it is tuned on the
+  // problem with the most subdomains and used for all of them.
+  int tuner = -1;
+  for (int i = 0; i < nprob; ++i)
+    if (cs[i]->eig_autotune && (tuner < 0 || cs[i]->nsub > cs[tuner]->nsub)) tuner = i;
+  if (tuner >= 0) {
+    PMX_TRY(lobpcg_autotune(cs[tuner]));
+    for (pmx_ctx* c : cs) c->eig_guard = cs[tuner]->eig_guard;
+  }
+  PMX_TRY(lobpcg_batch(cs.data(), nprob));
+  const double t_eig = now() - t1;  // one shared launch: charged to every problem
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = cs[i];
+    const double tc = now();
+    c->t_stages[PMX_STAGE_SU_LV2_1] += t_eig;
+    PMX_TRY(setup_coarse(c));
+    c->t_stages[PMX_STAGE_SU] += (t1 - t0[i]) + t_eig + (now() - tc);
+    c->setup_done = 1;
+  }
+  return PMX_SUCCESS;
 }
 
 int pmx_setup_from_basis(pmx_ctx* c, const double* phi, const int* eigen_num,

@@ -56,6 +56,7 @@ struct Brick {
   int neig;         // eigen count of this subdomain
   int plane_off;    // offset of this shape in the plane-position tables
   int nplanes;      // nx+ny+nz-2
+  int lid;          // subdomain index inside its problem (seeds its random LOBPCG start)
 };
 
 // Per-brick extents and neighbour offsets, precomputed on the host so a
@@ -233,6 +234,27 @@ template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  // Owns its allocation: freed on scope exit (hipFree waits for the device,
+  // so kernels still reading a local buffer finish first).  Never freed
+  // inside a stream capture: only setup-time code holds local buffers.
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
   int alloc(size_t count) {
     if (count == n && p) return PMX_SUCCESS;
     release();

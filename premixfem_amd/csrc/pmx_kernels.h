@@ -62,6 +62,7 @@ int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol);
 int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows);
 int lobpcg_block_size(int en_lv1, int guard);
 int lobpcg_autotune(pmx_ctx* ctx);
+int lobpcg_batch(pmx_ctx** ctxs, int nprob);
 
 // learned warm start (pmx_cnn.hip): x0 rows [cell][mb] in brick order
 int cnn_warm_start(pmx_ctx* ctx, int mb, double* x0);

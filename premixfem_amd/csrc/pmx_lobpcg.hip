@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
         const int j = e / n, r = e % n;
         X[e] = (a.x0 && attempt == 0)
                    ? a.x0[(size_t)(v.off + r) * MB + j]
-                   : hash_uniform(a.seed ^ ((uint64_t)sidx << 40) ^ ((uint64_t)j << 32) ^
+                   : hash_uniform(a.seed ^ ((uint64_t)b.lid << 40) ^ ((uint64_t)j << 32) ^
                                   (uint64_t)r ^ ((uint64_t)attempt << 60));
       }
       __syncthreads();
@@ -982,6 +982,232 @@ int lobpcg_autotune(pmx_ctx* ctx) {
     it = tune_cache().emplace(key, best).first;
   }
   ctx->eig_guard = it->second - (k - 1);
+  return PMX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Batched setup of several independent problems: the subdomain
+// eigenproblems of all of them in ONE persistent LOBPCG launch (one
+// workgroup per subdomain of any problem).  The problems' bricks, plane
+// tables and fields are concatenated into a batch geometry; a brick's random
+// start is seeded by its index inside its own problem, so each problem gets
+// bit for bit the basis its own pmx_setup would compute.  Requires equal
+// en_lv1, eigensolver options and warm-start mode (none, or the learned
+// start for every problem).
+// ---------------------------------------------------------------------------
+int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
+  pmx_ctx* c0 = ctxs[0];
+  const int k = c0->en_lv1;
+  const int MB = lobpcg_block_size(k, c0->eig_guard);
+  if (MB == 0) {
+    set_error("en_lv1=%d exceeds the LOBPCG block limit", k);
+    return PMX_ERR_SUP;
+  }
+  std::vector<size_t> coff(nprob + 1, 0);
+  std::vector<int> boff(nprob + 1, 0);
+  bool any_wd = false, all_cnn = true;
+  int maxn = 0, minn = 1 << 30;
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = ctxs[i];
+    if (c->en_lv1 != k || c->eig_guard != c0->eig_guard || c->eig_tol != c0->eig_tol ||
+        c->eig_maxit != c0->eig_maxit || c->eig_sigma != c0->eig_sigma ||
+        c->seed != c0->seed || c->eig_stag != c0->eig_stag ||
+        (c->warm_start == 1) != (c0->warm_start == 1)) {
+      set_error("batched setup: problem %d has other en_lv1 / eigensolver options", i);
+      return PMX_ERR_ARG_WRONG;
+    }
+    coff[i + 1] = coff[i] + c->ncell;
+    boff[i + 1] = boff[i] + c->nsub;
+    any_wd = any_wd || c->have_wd;
+    all_cnn = all_cnn && c->warm_start == 1;
+    for (const Brick& b : c->h_bricks) {
+      maxn = std::max(maxn, b.nx * b.ny * b.nz);
+      minn = std::min(minn, b.nx * b.ny * b.nz);
+    }
+  }
+  if (MB > minn - 1 || (any_wd && MB < k + 1)) {
+    set_error("batched setup: block size %d does not fit the subdomains", MB);
+    return PMX_ERR_ARG_WRONG;
+  }
+  const size_t N = coff[nprob];
+  const int S = boff[nprob];
+  // Batch geometry: bricks and views shifted to the concatenated cell space,
+  // plane tables appended per problem.
+  std::vector<Brick> hb;
+  std::vector<BrickView> hv;
+  std::vector<int> pstart, pcell, spstart, scoff;
+  hb.reserve(S);
+  hv.reserve(S);
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = ctxs[i];
+    const int sh0 = (int)spstart.size();
+    for (int q = 0; q < c->nshape; ++q) {
+      spstart.push_back(c->h_shape_pstart[q] + (int)pstart.size());
+      scoff.push_back(c->h_shape_coff[q] + (int)pcell.size());
+    }
+    pstart.insert(pstart.end(), c->h_plane_start.begin(), c->h_plane_start.end());
+    pcell.insert(pcell.end(), c->h_plane_cell.begin(), c->h_plane_cell.end());
+    for (const Brick& b0 : c->h_bricks) {
+      Brick b = b0;
+      b.off += (int)coff[i];
+      b.shape += sh0;
+      b.plane_off = scoff[b.shape];
+      for (int d = 0; d < 6; ++d) b.nb[d] = -1;
+      hb.push_back(b);
+      BrickView v{};
+      v.off = b.off;
+      v.nx = b.nx;
+      v.ny = b.ny;
+      v.nz = b.nz;
+      v.nxny = b.nx * b.ny;
+      v.n = v.nxny * b.nz;
+      for (int d = 0; d < 6; ++d) {
+        v.noff[d] = -1;
+        v.next[d] = 0;
+      }
+      hv.push_back(v);
+    }
+  }
+  hipStream_t st = c0->stream;
+  DBuf<Brick> d_b;
+  DBuf<BrickView> d_v;
+  DBuf<int> d_ps, d_pc, d_sps, d_sco;
+  PMX_TRY(d_b.alloc(S));
+  PMX_TRY(d_v.alloc(S));
+  PMX_TRY(d_ps.alloc(pstart.size()));
+  PMX_TRY(d_pc.alloc(pcell.size()));
+  PMX_TRY(d_sps.alloc(spstart.size()));
+  PMX_TRY(d_sco.alloc(scoff.size()));
+  PMX_HIP(hipMemcpy(d_b.p, hb.data(), S * sizeof(Brick), hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(d_v.p, hv.data(), S * sizeof(BrickView), hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(d_ps.p, pstart.data(), pstart.size() * sizeof(int), hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(d_pc.p, pcell.data(), pcell.size() * sizeof(int), hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(d_sps.p, spstart.data(), spstart.size() * sizeof(int), hipMemcpyHostToDevice));
+  PMX_HIP(hipMemcpy(d_sco.p, scoff.data(), scoff.size() * sizeof(int), hipMemcpyHostToDevice));
+  Geo g = c0->geo;
+  g.nsub = S;
+  g.ncell = (int)N;
+  g.bricks = d_b.p;
+  g.views = d_v.p;
+  g.order = nullptr;
+  g.plane_start = d_ps.p;
+  g.plane_cell = d_pc.p;
+  g.shape_pstart = d_sps.p;
+  g.shape_coff = d_sco.p;
+  g.pin = -1;
+  g.stop = nullptr;
+  // Fields: face coefficients, eigen mass, well term, the FP32 LOBPCG
+  // preconditioner factors (built per problem into its slice), warm start.
+  DBuf<double> T[3], mass, wd, x0;
+  DBuf<float> pf, pb;
+  for (int a = 0; a < 3; ++a) PMX_TRY(T[a].alloc(N));
+  PMX_TRY(mass.alloc(N));
+  PMX_TRY(pf.alloc(N * 4));
+  PMX_TRY(pb.alloc(N * 4));
+  if (any_wd) {
+    PMX_TRY(wd.alloc(N));
+    PMX_HIP(hipMemsetAsync(wd.p, 0, N * sizeof(double), st));
+  }
+  if (all_cnn) PMX_TRY(x0.alloc(N * MB));
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = ctxs[i];
+    const size_t o = coff[i], nb = (size_t)c->ncell * sizeof(double);
+    PMX_TRY(launch_dic_factor(c, 1, c->eig_sigma, pf.p + o * 4, pb.p + o * 4, true));
+    if (all_cnn) PMX_TRY(cnn_warm_start(c, MB, x0.p + o * MB));
+    PMX_HIP(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a)
+      PMX_HIP(hipMemcpyAsync(T[a].p + o, c->T[a].p, nb, hipMemcpyDeviceToDevice, st));
+    PMX_HIP(hipMemcpyAsync(mass.p + o, c->mass.p, nb, hipMemcpyDeviceToDevice, st));
+    if (c->have_wd)
+      PMX_HIP(hipMemcpyAsync(wd.p + o, c->wd.p, nb, hipMemcpyDeviceToDevice, st));
+  }
+  g.wd = any_wd ? wd.p : nullptr;
+  g.wd32 = nullptr;
+  DBuf<double> lob, phi, eig;
+  DBuf<int> its, status, fallback, stag;
+  const size_t blk = N * MB;
+  PMX_TRY(lob.alloc(6 * blk));
+  PMX_HIP(hipMemsetAsync(lob.p, 0, 6 * blk * sizeof(double), st));
+  PMX_TRY(phi.alloc(N * k));
+  PMX_TRY(eig.alloc((size_t)S * k));
+  PMX_TRY(its.alloc(S));
+  PMX_TRY(status.alloc(S));
+  PMX_TRY(fallback.alloc(S));
+  PMX_TRY(stag.alloc(S));
+  LobArgs a;
+  a.stride = 1;
+  a.kwant = k - 1;
+  a.maxit = c0->eig_maxit;
+  a.kmax = k;
+  a.tol = c0->eig_tol;
+  a.seed = c0->seed;
+  a.X = lob.p;
+  a.AX = lob.p + blk;
+  a.W = lob.p + 2 * blk;
+  a.AW = lob.p + 3 * blk;
+  a.P = lob.p + 4 * blk;
+  a.AP = lob.p + 5 * blk;
+  a.pf = (const float4*)pf.p;
+  a.pb = (const float4*)pb.p;
+  a.Tx = T[0].p;
+  a.Ty = T[1].p;
+  a.Tz = T[2].p;
+  a.mass = mass.p;
+  a.x0 = all_cnn ? x0.p : nullptr;
+  a.phi = phi.p;
+  a.eig = eig.p;
+  a.its = its.p;
+  a.status = status.p;
+  a.fallback = fallback.p;
+  a.stagnated = stag.p;
+  a.stag_factor = c0->eig_stag;
+  a.debug = -1;
+  switch (MB) {
+    case 4:
+      hipLaunchKernelGGL(k_lobpcg<4>, dim3(S), dim3(kBlock), lobpcg_lds<4>(maxn), st, g, a);
+      break;
+    case 6:
+      hipLaunchKernelGGL(k_lobpcg<6>, dim3(S), dim3(kBlock), lobpcg_lds<6>(maxn), st, g, a);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_lobpcg<8>, dim3(S), dim3(kBlock), lobpcg_lds<8>(maxn), st, g, a);
+      break;
+    default:
+      hipLaunchKernelGGL(k_lobpcg<10>, dim3(S), dim3(kBlock), lobpcg_lds<10>(maxn), st, g, a);
+      break;
+  }
+  PMX_HIP(hipGetLastError());
+  std::vector<int> hits(S), hst(S), hfb(S), hsg(S);
+  std::vector<double> heig((size_t)S * k);
+  PMX_HIP(hipMemcpyAsync(hits.data(), its.p, S * sizeof(int), hipMemcpyDeviceToHost, st));
+  PMX_HIP(hipMemcpyAsync(hst.data(), status.p, S * sizeof(int), hipMemcpyDeviceToHost, st));
+  PMX_HIP(hipMemcpyAsync(hfb.data(), fallback.p, S * sizeof(int), hipMemcpyDeviceToHost, st));
+  PMX_HIP(hipMemcpyAsync(hsg.data(), stag.p, S * sizeof(int), hipMemcpyDeviceToHost, st));
+  PMX_HIP(hipMemcpyAsync(heig.data(), eig.p, heig.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = ctxs[i];
+    PMX_HIP(hipMemcpyAsync(c->phi.p, phi.p + coff[i] * k, (size_t)c->ncell * k * sizeof(double),
+                           hipMemcpyDeviceToDevice, st));
+  }
+  PMX_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < nprob; ++i) {
+    pmx_ctx* c = ctxs[i];
+    c->eig_block = MB;
+    c->h_eig_lv1.assign(heig.begin() + (size_t)boff[i] * k, heig.begin() + (size_t)boff[i + 1] * k);
+    c->eig_iters_max = c->eig_iters_sum = c->eig_fallbacks = c->eig_stagnated = 0;
+    for (int s = boff[i]; s < boff[i + 1]; ++s) {
+      c->eig_iters_max = std::max(c->eig_iters_max, hits[s]);
+      c->eig_iters_sum += hits[s];
+      c->eig_fallbacks += hfb[s];
+      c->eig_stagnated += hsg[s];
+      if (hst[s] != 0) {
+        set_error("batched setup: problem %d, subdomain %d: eigensolver status %d", i,
+                  s - boff[i], hst[s]);
+        return PMX_ERR_USER;
+      }
+    }
+  }
   return PMX_SUCCESS;
 }
 

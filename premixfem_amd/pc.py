@@ -103,6 +103,17 @@ def PC_setup(pc: PCShell) -> None:
     check(lib().pmx_setup(pc.ctx._require()))
 
 
+def PC_setup_batch(pcs) -> None:
+    """PC_setup of several independent problems at once: each context is set
+    up as PC_setup does (src/PreMixFEM_3D.c:1826-1838), with the subdomain
+    eigenproblems of all of them solved by one batched LOBPCG launch."""
+    pcs = list(pcs)
+    for pc in pcs:
+        _push_kappa(pc.ctx)
+    arr = (C.c_void_p * len(pcs))(*[pc.ctx._require() for pc in pcs])
+    check(lib().pmx_setup_batch(arr, len(pcs)))
+
+
 def PC_setup_from_basis(pc: PCShell, phi, eigen_num=None, eigvals=None) -> None:
     """PC_setup with the level-2 eigenvectors given (shape (en_lv1, ncell))."""
     _push_kappa(pc.ctx)

@@ -241,14 +241,47 @@ def config(name: str) -> Problem:
     raise KeyError(name)
 
 
+BATCH_SIZES = (32, 48, 64, 80, 96)
+
+
+def batch_member(i: int, sizes=BATCH_SIZES, brick: int = 8, en_lv1: int = 6) -> Problem:
+    """Member i of the batched workload of independent Darcy problems: a cube
+    of n^3 cells with n drawn uniformly from ``sizes``, a lognormal
+    permeability exp(g) with sigma drawn uniformly from [1, 3] (correlation
+    length 0.1 of the domain), seed 5000 + i, brick^3-cell subdomains,
+    en_lv1 eigenvectors, corner source/sink and the pin at (n-1, 0, 0)."""
+    rng = np.random.default_rng(5000 + i)
+    n = int(rng.choice(np.asarray(sizes)))
+    sigma = float(rng.uniform(1.0, 3.0))
+    mesh = (n, n, n)
+    dom = (1.0, 1.0, 1.0)
+    kap = grf_kappa(mesh, sigma, 0.1, 5000 + i)
+    sd = (n // brick,) * 3
+    return Problem(f"batch-{i}", dom, mesh, kap, corner_rhs(mesh, dom), sd=sd,
+                   en_lv1=en_lv1, en_lv2=4, pin_cell=(n - 1, 0, 0))
+
+
+def batch_config(count: int = 64, sizes=BATCH_SIZES) -> list:
+    """The batched workload: ``count`` members (64: sum of cells ~ 23 M)."""
+    return [batch_member(i, sizes) for i in range(count)]
+
+
 def training_problem(kind: str, seed: int, mesh=(64, 64, 64)) -> Problem:
     """A field of the same family as workload ``kind`` ("c2": binary box
-    medium, "c4": lognormal + fractures) on ``mesh`` with 16^3 subdomains,
+    medium, "c4": lognormal + fractures, both with 16^3 subdomains; "batch":
+    the batched members' lognormal family with 8^3 subdomains) on ``mesh``,
     for generating warm-start training samples (seeds disjoint from the
     workloads')."""
     dom = (1.0, 1.0, 1.0)
     vol = mesh[0] * mesh[1] * mesh[2]
     sd = tuple(m // 16 for m in mesh)
+    if kind == "batch":
+        # lognormal, sigma drawn from [1, 3] as in batch_member, 8^3 bricks
+        sigma = float(np.random.default_rng(seed).uniform(1.0, 3.0))
+        kap = grf_kappa(mesh, sigma, 0.1, seed)
+        return Problem(f"batch-train-{seed}", dom, mesh, kap, corner_rhs(mesh, dom),
+                       sd=tuple(m // 8 for m in mesh), en_lv1=6, en_lv2=4,
+                       pin_cell=(mesh[0] - 1, 0, 0))
     if kind == "c2":
         kap = box_inclusions_kappa(mesh, max(1, round(3000 * vol / 128 ** 3)), 2, 6, 1.0e5, seed)
         en = 6

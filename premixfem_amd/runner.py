@@ -51,3 +51,41 @@ def run(prob: Problem, precision: int = 64, extra: str = "", basis=None, rtol=No
     finally:
         if not keep_ctx:
             P.PC_final(ctx)
+
+
+def run_batch(probs, precision: int = 64, extra: str = "", net=None, rtol=None,
+              keep_ctx=False, max_it=10000):
+    """Set up several independent problems with one batched setup call
+    (PC_setup_batch: one LOBPCG launch for every subdomain of every problem),
+    then solve each.  Returns (list of per-problem dicts as ``run``, setup
+    seconds of the whole batch)."""
+    ctxs, shells = [], []
+    try:
+        for prob in probs:
+            ctx = P.PCCtx()
+            ctxs.append(ctx)
+            P.PC_init(ctx, prob.dom, prob.mesh, options_for(prob, precision, extra))
+            ctx.kappa = prob.kappa
+            if net is not None:
+                net.attach(ctx)
+            shells.append(P.PCShell(ctx))
+        t0 = time.perf_counter()
+        P.PC_setup_batch(shells)
+        t_setup = time.perf_counter() - t0
+        outs = []
+        for prob, ctx, shell in zip(probs, ctxs, shells):
+            t1 = time.perf_counter()
+            x, its, rnorm, reason = P.KSPSolve(shell, prob.rhs.ravel(),
+                                               rtol=prob.rtol if rtol is None else rtol,
+                                               max_it=max_it)
+            out = dict(x=x, its=its, rnorm=rnorm, reason=reason,
+                       t_solve=time.perf_counter() - t1, stats=P.PC_get_stats(ctx))
+            if keep_ctx:
+                out["ctx"] = ctx
+                out["shell"] = shell
+            outs.append(out)
+        return outs, t_setup
+    finally:
+        if not keep_ctx:
+            for ctx in ctxs:
+                P.PC_final(ctx)
