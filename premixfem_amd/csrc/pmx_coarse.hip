@@ -6,6 +6,7 @@
 
 #include "pmx_dense.h"
 #include "pmx_kernels.h"
+#include "pmx_mfma.h"
 
 namespace pmx {
 
@@ -180,36 +181,47 @@ int launch_galerkin(pmx_ctx* ctx) {
 // ---------------------------------------------------------------------------
 constexpr int kSw = 64;
 
-__global__ void k_sweep_pivot(const double* __restrict__ A, int n, int k0, int bs,
-                              double* __restrict__ D) {
+// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to 64):
+// thread t holds row t/4, columns 16 (t%4) .. +15 in registers; one barrier
+// per elimination step (the row / column broadcast buffers alternate).
+__global__ void __launch_bounds__(256) k_sweep_pivot(const double* __restrict__ A, int n, int k0,
+                                                     int bs, double* __restrict__ D) {
+  __shared__ double rowv[2][kSw], colv[2][kSw];
   __shared__ double M[kSw][kSw + 1];
-  __shared__ double rowt[kSw], fcol[kSw];
-  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
-    const int i = e / bs, j = e % bs;
-    M[i][j] = A[(size_t)(k0 + i) * n + k0 + j];
+  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
+  double m[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) {
+    const int j = q * 16 + c;
+    m[c] = (r < bs && j < bs) ? A[(size_t)(k0 + r) * n + k0 + j] : (r == j ? 1.0 : 0.0);
   }
+  for (int tq = 0; tq < 4; ++tq) {
+#pragma unroll
+    for (int tc = 0; tc < 16; ++tc) {
+      const int t = tq * 16 + tc, par = t & 1;
+      if (q == tq) colv[par][r] = m[tc];
+      if (r == t) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rowv[par][q * 16 + c] = m[c];
+      }
+      __syncthreads();
+      const double piv = 1.0 / rowv[par][t];
+      const double f = colv[par][r];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const int j = q * 16 + c;
+        const double rt = (j == t ? 1.0 : rowv[par][j]) * piv;
+        m[c] = r == t ? rt : (j == t ? 0.0 : m[c]) - f * rt;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 16; ++c) M[r][q * 16 + c] = m[c];
   __syncthreads();
-  for (int t = 0; t < bs; ++t) {
-    const double piv = 1.0 / M[t][t];
-    if (threadIdx.x < bs) {
-      const int j = threadIdx.x;
-      rowt[j] = (j == t ? 1.0 : M[t][j]) * piv;
-      fcol[j] = M[j][t];
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
-      const int i = e / bs, j = e % bs;
-      if (i == t)
-        M[i][j] = rowt[j];
-      else
-        M[i][j] = (j == t ? 0.0 : M[i][j]) - fcol[i] * rowt[j];
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) {
-    const int i = e / bs, j = e % bs;
+  for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
+    const int i = e / kSw, j = e % kSw;
     // Symmetrise the pivot inverse.
-    D[i * kSw + j] = 0.5 * (M[i][j] + M[j][i]);
+    D[e] = (i < bs && j < bs) ? 0.5 * (M[i][j] + M[j][i]) : 0.0;
   }
 }
 
@@ -233,16 +245,18 @@ __global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int b
   }
 }
 
-// Trailing update on 64x64 tiles, 4x4 outputs per thread.
-__global__ void k_sweep_update(double* __restrict__ A, int n, int k0, int bs,
-                               const double* __restrict__ P, const double* __restrict__ U,
-                               const double* __restrict__ D) {
-  __shared__ double Us[kSw][kSw + 1];  // Us[t][i]
-  __shared__ double Ps[kSw][kSw + 1];  // Ps[t][j]
+// Trailing update A_ij -= U_i P_j^T on 64x64 tiles (FP64 MFMA); the tiles of
+// the pivot block row / column take U, U^T and -D.
+__global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, int n, int k0,
+                                                      int bs, const double* __restrict__ P,
+                                                      const double* __restrict__ U,
+                                                      const double* __restrict__ D) {
+  using TL = MfmaTile<2, 2, 2, 2>;
+  static_assert(TL::TM == kSw && TL::TN == kSw, "sweep tiles are pivot blocks");
+  __shared__ double lds[TL::LDS_DOUBLES];
   const int I0 = blockIdx.y * kSw, J0 = blockIdx.x * kSw;
   const int kb = k0 / kSw;
   const int ti = blockIdx.y, tj = blockIdx.x;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
   if (ti == kb || tj == kb) {
     for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
       const int i = I0 + e / kSw, j = J0 + e % kSw;
@@ -258,38 +272,24 @@ __global__ void k_sweep_update(double* __restrict__ A, int n, int k0, int bs,
     }
     return;
   }
-  for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
-    const int rI = e / kSw, t = e % kSw;
-    Us[t][rI] = (I0 + rI < n && t < bs) ? U[(size_t)(I0 + rI) * kSw + t] : 0.0;
-    Ps[t][rI] = (J0 + rI < n && t < bs) ? P[(size_t)(J0 + rI) * kSw + t] : 0.0;
-  }
-  __syncthreads();
-  double acc[4][4];
+  mfma_d4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int t = 0; t < bs; ++t) {
-    double u[4], p[4];
+    for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+  // op(A) = U (rows I0..), op(B) = P^T (P row j = column j of op(B)).
+  mfma_tile_acc<2, 2, 2, 2, true>(U, kSw, P, kSw, n, n, bs, I0, J0, lds, acc);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) u[a] = Us[t][ty + 16 * a];
+  for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) p[c] = Ps[t][tx + 16 * c];
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[a][c] += u[a] * p[c];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = I0 + ty + 16 * a;
-    if (i >= n) continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = J0 + tx + 16 * c;
-      if (j < n) A[(size_t)i * n + j] -= acc[a][c];
-    }
-  }
+      for (int v = 0; v < 4; ++v) {
+        int r, c;
+        mfma_tile_pos<2, 2, 2, 2>(rb, cb, v, r, c);
+        const int i = I0 + r, j = J0 + c;
+        if (i < n && j < n) A[(size_t)i * n + j] -= acc[rb][cb][v];
+      }
 }
 
 __global__ void k_copy_shift(const double* __restrict__ A, double* __restrict__ B, int n,
@@ -858,54 +858,6 @@ int bicc_setup(pmx_ctx* ctx) {
 // dense matrices:  T = I, U = 0;  per stage D = G T, U += D, T -= A_c D.
 // ---------------------------------------------------------------------------
 constexpr int kGt = 64;   // GEMM tile
-constexpr int kGk = 16;   // GEMM k-chunk
-
-// C = A B (row-major n x n), 64x64 tiles, 4x4 outputs per thread.
-__global__ void __launch_bounds__(256) k_dgemm(const double* __restrict__ A,
-                                               const double* __restrict__ B,
-                                               double* __restrict__ C, int n) {
-  __shared__ double As[kGk][kGt + 1];  // As[k][i]
-  __shared__ double Bs[kGk][kGt + 1];  // Bs[k][j]
-  const int I0 = blockIdx.y * kGt, J0 = blockIdx.x * kGt;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
-  for (int k0 = 0; k0 < n; k0 += kGk) {
-    for (int e = threadIdx.x; e < kGt * kGk; e += blockDim.x) {
-      const int r = e / kGk, k = e % kGk;
-      As[k][r] = (I0 + r < n && k0 + k < n) ? A[(size_t)(I0 + r) * n + k0 + k] : 0.0;
-      const int kb = e / kGt, cb = e % kGt;
-      Bs[kb][cb] = (k0 + kb < n && J0 + cb < n) ? B[(size_t)(k0 + kb) * n + J0 + cb] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kGk; ++k) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[k][ty + 16 * a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = Bs[k][tx + 16 * c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] += av[a] * bv[c];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = I0 + ty + 16 * a;
-    if (i >= n) continue;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int j = J0 + tx + 16 * c;
-      if (j < n) C[(size_t)i * n + j] = acc[a][c];
-    }
-  }
-}
 
 // S = B^{-1} - [z/dg, h] Cap^{-1} [z/dg, h]^T  (the Woodbury-corrected A_c^{-1}).
 __global__ void k_form_S(const double* __restrict__ Binv, const double* __restrict__ z,
@@ -1024,7 +976,8 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
       if (T_is_I) {
         Dp = S.p;
       } else {
-        hipLaunchKernelGGL(k_dgemm, dim3(nt, nt), dim3(256), 0, ctx->stream, S.p, T.p, D.p, n);
+        hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 2, false>), dim3(nt, nt), dim3(256), 0,
+                           ctx->stream, n, n, n, 1.0, S.p, n, T.p, n, 0.0, D.p, n);
         Dp = D.p;
       }
     } else {
@@ -1162,99 +1115,210 @@ int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d) {
   return PMX_SUCCESS;
 }
 
+// Multi-workgroup block algebra of the level-3 eigensolvers: column-major
+// n x m blocks; Gram matrices summed by many workgroups (partials in fixed
+// order), the small Cholesky / eigenproblems in one workgroup.
+constexpr int kGramRows = 2048;   // rows per workgroup of a partial Gram
+constexpr int kLv3Max = 32;       // largest LOBPCG basis [X W P] (3 m)
+
+// part[(g*ma + i)*mb + j] = sum over rows of chunk g of A[:, i] . B[:, j].
+__global__ void __launch_bounds__(256) k_gram_part(const double* __restrict__ A, int ma,
+                                                   const double* __restrict__ B, int mb, int n,
+                                                   double* __restrict__ part) {
+  __shared__ double red[4][kLv3Max];
+  const int g = blockIdx.x, i = blockIdx.y;
+  const int r0 = g * kGramRows, r1 = min(n, r0 + kGramRows);
+  double acc[kLv3Max];
+#pragma unroll
+  for (int j = 0; j < kLv3Max; ++j) acc[j] = 0.0;
+  const double* a = A + (size_t)i * n;
+  for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    const double av = a[r];
+#pragma unroll
+    for (int j = 0; j < kLv3Max; ++j)
+      if (j < mb) acc[j] += av * B[(size_t)j * n + r];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kLv3Max; ++j) {
+    if (j >= mb) break;
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wv][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < mb) {
+    double v = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
+    part[((size_t)g * ma + i) * mb + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_gram_sum(const double* __restrict__ part, int G, int ma, int mb,
+                           double* __restrict__ out) {
+  for (int e = threadIdx.x; e < ma * mb; e += blockDim.x) {
+    double v = 0.0;
+    for (int g = 0; g < G; ++g) v += part[(size_t)g * ma * mb + e];
+    out[e] = v;
+  }
+}
+
+// mode 0: T = L^{-T} for the Cholesky factor L of G (ns x ns, row-major);
+// *flag = 1 if G is not numerically positive definite.  mode 1: G
+// (symmetrised) = Q diag(w) Q^T, T = Q, ascending w.
+__global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
+                            double* __restrict__ T, double* __restrict__ wout,
+                            int* __restrict__ flag) {
+  constexpr int L = kLv3Max;
+  __shared__ double A[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
+  __shared__ int fl;
+  // mode 0 works on the column-scaled Gram D G D (D = diag(G)^{-1/2}), so
+  // columns of very different norms (small residual directions) stay
+  // well-conditioned; T = D L^{-T}.
+  if (threadIdx.x < ns) w[threadIdx.x] = Gm[threadIdx.x * ns + threadIdx.x] > 0.0
+                                             ? 1.0 / sqrt(Gm[threadIdx.x * ns + threadIdx.x])
+                                             : 1.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+    const int i = e / ns, j = e % ns;
+    A[i * L + j] = mode == 0 ? Gm[e] * w[i] * w[j] : 0.5 * (Gm[e] + Gm[j * ns + i]);
+  }
+  __syncthreads();
+  if (mode == 0) {
+    const bool ok = wg_cholesky(A, ns, L, &fl);
+    if (threadIdx.x == 0 && !ok) *flag = 1;   // sticky: cleared by the caller
+    wg_trinv_lower(A, tmp, ns, L);  // A = L^{-1}
+    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
+      const int k = e / ns, c = e % ns;
+      T[e] = w[k] * A[c * L + k];
+    }
+    return;
+  }
+  wg_syev(A, Q, tmp, w, rot, ns, L);
+  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) T[e] = Q[(e / ns) * L + e % ns];
+  if (threadIdx.x < ns) wout[threadIdx.x] = w[threadIdx.x];
+}
+
+// out[:, c] = beta out[:, c] + alpha sum_{k < kin} S[:, k] T[(r0 + k) * tld + c0 + c]
+// for c < cout.
+__global__ void __launch_bounds__(256) k_xform(const double* __restrict__ S, int n, int kin,
+                                               const double* __restrict__ T, int tld, int r0,
+                                               int c0, int cout, double* __restrict__ out,
+                                               double alpha, double beta) {
+  __shared__ double Ts[kLv3Max * kLv3Max];
+  for (int e = threadIdx.x; e < kin * cout; e += blockDim.x)
+    Ts[e] = T[(size_t)(r0 + e / cout) * tld + c0 + e % cout];
+  __syncthreads();
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    double sv[kLv3Max];
+#pragma unroll
+    for (int k = 0; k < kLv3Max; ++k) sv[k] = k < kin ? S[(size_t)k * n + r] : 0.0;
+    for (int c = 0; c < cout; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < kLv3Max; ++k)
+        if (k < kin) v += sv[k] * Ts[k * cout + c];
+      out[(size_t)c * n + r] = beta == 0.0 ? alpha * v : beta * out[(size_t)c * n + r] + alpha * v;
+    }
+  }
+}
+
+// y += alpha x.
+__global__ void k_axpy(const double* __restrict__ x, double alpha, double* __restrict__ y,
+                       size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
+// res[j] = ||X_j|| (one workgroup per column).
+__global__ void k_col_norm(const double* __restrict__ X, int n, double* __restrict__ res) {
+  __shared__ double red[8];
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double e = X[(size_t)j * n + r];
+    s += e * e;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) res[j] = sqrt(s);
+}
+
+// R = AX - X diag(w), res[j] = ||R_j|| (one workgroup per column).
+__global__ void k_col_resid(const double* __restrict__ AX, const double* __restrict__ X,
+                            const double* __restrict__ w, double* __restrict__ R, int n,
+                            double* __restrict__ res) {
+  __shared__ double red[8];
+  const int j = blockIdx.x;
+  const double wj = w[j];
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double e = AX[(size_t)j * n + r] - wj * X[(size_t)j * n + r];
+    R[(size_t)j * n + r] = e;
+    s += e * e;
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) res[j] = sqrt(s);
+}
+
+namespace {
+
+struct Lv3Work {
+  pmx_ctx* ctx;
+  int n, ngram;
+  DBuf<double> part, G, T, wall;
+  DBuf<int> flag;
+
+  int gram(const double* A, int ma, const double* B, int mb) {
+    hipLaunchKernelGGL(k_gram_part, dim3(ngram, ma), dim3(256), 0, ctx->stream, A, ma, B, mb, n,
+                       part.p);
+    hipLaunchKernelGGL(k_gram_sum, dim3(1), dim3(256), 0, ctx->stream, part.p, ngram, ma, mb,
+                       G.p);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
+  int xform(const double* S, int kin, int tld, int r0, int c0, int cout, double* out,
+            const double* Tm = nullptr, double alpha = 1.0, double beta = 0.0) {
+    const int nb = std::min((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_xform, dim3(nb), dim3(256), 0, ctx->stream, S, n, kin, Tm ? Tm : T.p, tld,
+                       r0, c0, cout, out, alpha, beta);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
+  // S (ns columns) <- S L^{-T}: orthonormal; returns 1 in *bad if the Gram
+  // matrix is not positive definite.
+  int orthonormalize(double* S, int ns, double* tmp, int* bad) {
+    PMX_TRY(gram(S, ns, S, ns));
+    PMX_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, G.p, ns, 0, T.p, wall.p,
+                       flag.p);
+    PMX_HIP(hipGetLastError());
+    int h = 0;
+    PMX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));
+    *bad = h;
+    if (h) return PMX_SUCCESS;
+    PMX_TRY(xform(S, ns, ns, 0, 0, ns, tmp));
+    PMX_HIP(hipMemcpyAsync(S, tmp, (size_t)ns * n * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    return PMX_SUCCESS;
+  }
+};
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Level 3 (src/PreMixFEM_3D.c:918-1390): smallest en_lv2 eigenpairs of the
 // coarse matrix by block inverse (subspace) iteration on the dense inverse
 // with Rayleigh-Ritz, then Phi_cc = Phi V and A_cc = V^T A_c V + NIL I with
 // an LU factorisation.
 // ---------------------------------------------------------------------------
-// Y[:, c] = Ainv X[:, c] for m columns; one wavefront per row.
-__global__ void k_gemm_tall(const double* __restrict__ A, const double* __restrict__ X,
-                            double* __restrict__ Y, int n, int m) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= n) return;
-  const double* a = A + (size_t)row * n;
-  double s[2 * kMaxEig];
-  for (int c = 0; c < m; ++c) s[c] = 0.0;
-  for (int j = lane; j < n; j += 64) {
-    const double av = a[j];
-    for (int c = 0; c < m; ++c) s[c] += av * X[(size_t)c * n + j];
-  }
-  for (int c = 0; c < m; ++c) {
-    double v = s[c];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) Y[(size_t)c * n + row] = v;
-  }
-}
-
-// One-workgroup Rayleigh-Ritz step of the subspace iteration on column-major
-// blocks of n rows.  mode 0: G = Y^T Y, Y <- Y L^{-T} (Cholesky QR).
-// mode 1: H = Y^T Z, H = Q diag(w) Q^T, X = Y Q, Z <- Z Q, res[c] =
-// ||Z q_c - w_c X q_c||.
-__global__ void k_lv3_rr(double* __restrict__ Y, double* __restrict__ Z, double* __restrict__ X,
-                         int n, int m, int mode, double* __restrict__ w_out,
-                         double* __restrict__ res, int* __restrict__ fail) {
-  constexpr int L = 2 * kMaxEig;
-  __shared__ double G[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
-  __shared__ double red[8];
-  __shared__ int flag;
-  for (int e = 0; e < m * m; ++e) {
-    const int i = e / m, j = e % m;
-    if (j < i) continue;
-    const double* a = Y + (size_t)i * n;
-    const double* bb = (mode == 0 ? Y : Z) + (size_t)j * n;
-    double s = 0.0;
-    for (int r = threadIdx.x; r < n; r += blockDim.x) s += a[r] * bb[r];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) {
-      G[i * L + j] = s;
-      G[j * L + i] = s;
-    }
-  }
-  __syncthreads();
-  if (mode == 0) {
-    const bool ok = wg_cholesky(G, m, L, &flag);
-    if (!ok && threadIdx.x == 0) *fail = 1;
-    wg_trinv_lower(G, tmp, m, L);  // G = L^{-1}
-    // Y <- Y L^{-T}: new column c = sum_k Y[:,k] * Linv[c][k]
-    for (int r = threadIdx.x; r < n; r += blockDim.x) {
-      double v[L];
-      for (int k = 0; k < m; ++k) v[k] = Y[(size_t)k * n + r];
-      for (int c = 0; c < m; ++c) {
-        double s = 0.0;
-        for (int k = 0; k <= c; ++k) s += v[k] * G[c * L + k];
-        Y[(size_t)c * n + r] = s;
-      }
-    }
-    return;
-  }
-  wg_syev(G, Q, tmp, w, rot, m, L);
-  if (threadIdx.x < m) w_out[threadIdx.x] = w[threadIdx.x];
-  double rs[L];
-  for (int c = 0; c < m; ++c) rs[c] = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double yv[L], zv[L];
-    for (int k = 0; k < m; ++k) {
-      yv[k] = Y[(size_t)k * n + r];
-      zv[k] = Z[(size_t)k * n + r];
-    }
-    for (int c = 0; c < m; ++c) {
-      double xs = 0.0, zs = 0.0;
-      for (int k = 0; k < m; ++k) {
-        xs += yv[k] * Q[k * L + c];
-        zs += zv[k] * Q[k * L + c];
-      }
-      X[(size_t)c * n + r] = xs;
-      Z[(size_t)c * n + r] = zs;
-      const double e = zs - w[c] * xs;
-      rs[c] += e * e;
-    }
-  }
-  for (int c = 0; c < m; ++c) {
-    const double s = block_sum(rs[c], red);
-    if (threadIdx.x == 0) res[c] = sqrt(s);
-  }
+// Y[:, c] = Ainv X[:, c] for the m columns of column-major n x m blocks:
+// as row-major m x n blocks, Y = X Ainv^T (FP64 MFMA, 32 x 64 tiles).
+static void launch_gemm_tall(hipStream_t st, const double* Ainv, const double* X, double* Y, int n,
+                             int m) {
+  hipLaunchKernelGGL((k_gemm_mfma<1, 4, 2, 1, true>), dim3((n + 63) / 64, (m + 31) / 32),
+                     dim3(256), 0, st, m, n, n, 1.0, X, n, Ainv, n, 0.0, Y, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1468,6 +1532,15 @@ int level3_setup(pmx_ctx* ctx) {
   PMX_TRY(w.alloc(m));
   PMX_TRY(res.alloc(m));
   PMX_TRY(fail.alloc(1));
+  Lv3Work wk;
+  wk.ctx = ctx;
+  wk.n = n;
+  wk.ngram = (n + kGramRows - 1) / kGramRows;
+  PMX_TRY(wk.part.alloc((size_t)wk.ngram * kLv3Max * kLv3Max));
+  PMX_TRY(wk.G.alloc(kLv3Max * kLv3Max));
+  PMX_TRY(wk.T.alloc(kLv3Max * kLv3Max));
+  PMX_TRY(wk.wall.alloc(kLv3Max));
+  PMX_TRY(wk.flag.alloc(1));
   std::vector<double> h((size_t)n * m);
   std::mt19937_64 rng(ctx->seed + 7);
   std::normal_distribution<double> nd;
@@ -1475,7 +1548,6 @@ int level3_setup(pmx_ctx* ctx) {
   PMX_HIP(hipMemcpyAsync(X.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
   PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
-  const int wpb = kBlock / 64;
   const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
   std::vector<double> hw(m), hres(m), hist;
   int it = 0;
@@ -1483,8 +1555,7 @@ int level3_setup(pmx_ctx* ctx) {
   ctx->lv3_floor = 0.0;
   for (it = 1; it <= maxit; ++it) {
     if (it == 1) {
-      hipLaunchKernelGGL(k_gemm_tall, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream,
-                         ctx->Ainv.p, X.p, W.p, n, m);
+      launch_gemm_tall(ctx->stream, ctx->Ainv.p, X.p, W.p, n, m);
       hipLaunchKernelGGL(k_coarse_fix, dim3(m), dim3(kBlock), 0, ctx->stream, W.p, X.p,
                          ctx->c_z.p, ctx->c_h.p, n, ctx->c_cs.p, Y.p);
     } else {
@@ -1495,8 +1566,7 @@ int level3_setup(pmx_ctx* ctx) {
       // target ~0), computed on the small correction.
       hipLaunchKernelGGL(k_lv3_refine, dim3(std::min((n * m + kBlock - 1) / kBlock, 4096)),
                          dim3(kBlock), 0, ctx->stream, Z.p, X.p, w.p, R.p, n, m);
-      hipLaunchKernelGGL(k_gemm_tall, dim3((n + wpb - 1) / wpb), dim3(kBlock), 0, ctx->stream,
-                         ctx->Ainv.p, R.p, W.p, n, m);
+      launch_gemm_tall(ctx->stream, ctx->Ainv.p, R.p, W.p, n, m);
       hipLaunchKernelGGL(k_coarse_fix, dim3(m), dim3(kBlock), 0, ctx->stream, W.p, R.p,
                          ctx->c_z.p, ctx->c_h.p, n, ctx->c_cs.p, Y.p);
       hipLaunchKernelGGL(k_lv3_update, dim3(m), dim3(kBlock), 0, ctx->stream, X.p, Y.p, n,
@@ -1533,12 +1603,21 @@ int level3_setup(pmx_ctx* ctx) {
       }
       if (conv) break;
     }
-    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 0,
-                       w.p, res.p, fail.p);
+    // Rayleigh-Ritz: Y <- Y L^{-T} (Cholesky QR of Y^T Y), Z = A_c Y,
+    // Y^T Z = Q diag(w) Q^T, X = Y Q, Z <- Z Q.  W is free here (scratch).
+    PMX_TRY(wk.gram(Y.p, m, Y.p, m));
+    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, wk.G.p, m, 0, wk.T.p,
+                       wk.wall.p, fail.p);
+    PMX_TRY(wk.xform(Y.p, m, m, 0, 0, m, W.p));
+    std::swap(Y, W);
     hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
                        ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n, m);
-    hipLaunchKernelGGL(k_lv3_rr, dim3(1), dim3(kBlock), 0, ctx->stream, Y.p, Z.p, X.p, n, m, 1,
-                       w.p, res.p, fail.p);
+    PMX_TRY(wk.gram(Y.p, m, Z.p, m));
+    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, wk.G.p, m, 1, wk.T.p,
+                       w.p, fail.p);
+    PMX_TRY(wk.xform(Y.p, m, m, 0, 0, m, X.p));
+    PMX_TRY(wk.xform(Z.p, m, m, 0, 0, m, W.p));
+    std::swap(Z, W);
     PMX_HIP(hipGetLastError());
   }
   PMX_HIP(hipMemcpyAsync(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1575,193 +1654,6 @@ int level3_setup(pmx_ctx* ctx) {
 // n x m; the Gram matrices are summed by many workgroups (partials in fixed
 // order), the ns x ns Cholesky / eigenproblems run in one workgroup.
 // ---------------------------------------------------------------------------
-constexpr int kGramRows = 2048;   // rows per workgroup of a partial Gram
-constexpr int kLv3Max = 32;       // largest LOBPCG basis [X W P] (3 m)
-
-// part[(g*ma + i)*mb + j] = sum over rows of chunk g of A[:, i] . B[:, j].
-__global__ void __launch_bounds__(256) k_gram_part(const double* __restrict__ A, int ma,
-                                                   const double* __restrict__ B, int mb, int n,
-                                                   double* __restrict__ part) {
-  __shared__ double red[4][kLv3Max];
-  const int g = blockIdx.x, i = blockIdx.y;
-  const int r0 = g * kGramRows, r1 = min(n, r0 + kGramRows);
-  double acc[kLv3Max];
-#pragma unroll
-  for (int j = 0; j < kLv3Max; ++j) acc[j] = 0.0;
-  const double* a = A + (size_t)i * n;
-  for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    const double av = a[r];
-#pragma unroll
-    for (int j = 0; j < kLv3Max; ++j)
-      if (j < mb) acc[j] += av * B[(size_t)j * n + r];
-  }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < kLv3Max; ++j) {
-    if (j >= mb) break;
-    double v = acc[j];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wv][j] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < mb) {
-    double v = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
-    part[((size_t)g * ma + i) * mb + threadIdx.x] = v;
-  }
-}
-
-__global__ void k_gram_sum(const double* __restrict__ part, int G, int ma, int mb,
-                           double* __restrict__ out) {
-  for (int e = threadIdx.x; e < ma * mb; e += blockDim.x) {
-    double v = 0.0;
-    for (int g = 0; g < G; ++g) v += part[(size_t)g * ma * mb + e];
-    out[e] = v;
-  }
-}
-
-// mode 0: T = L^{-T} for the Cholesky factor L of G (ns x ns, row-major);
-// *flag = 1 if G is not numerically positive definite.  mode 1: G
-// (symmetrised) = Q diag(w) Q^T, T = Q, ascending w.
-__global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
-                            double* __restrict__ T, double* __restrict__ wout,
-                            int* __restrict__ flag) {
-  constexpr int L = kLv3Max;
-  __shared__ double A[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
-  __shared__ int fl;
-  // mode 0 works on the column-scaled Gram D G D (D = diag(G)^{-1/2}), so
-  // columns of very different norms (small residual directions) stay
-  // well-conditioned; T = D L^{-T}.
-  if (threadIdx.x < ns) w[threadIdx.x] = Gm[threadIdx.x * ns + threadIdx.x] > 0.0
-                                             ? 1.0 / sqrt(Gm[threadIdx.x * ns + threadIdx.x])
-                                             : 1.0;
-  __syncthreads();
-  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
-    const int i = e / ns, j = e % ns;
-    A[i * L + j] = mode == 0 ? Gm[e] * w[i] * w[j] : 0.5 * (Gm[e] + Gm[j * ns + i]);
-  }
-  __syncthreads();
-  if (mode == 0) {
-    const bool ok = wg_cholesky(A, ns, L, &fl);
-    if (threadIdx.x == 0) *flag = ok ? 0 : 1;
-    wg_trinv_lower(A, tmp, ns, L);  // A = L^{-1}
-    for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) {
-      const int k = e / ns, c = e % ns;
-      T[e] = w[k] * A[c * L + k];
-    }
-    return;
-  }
-  wg_syev(A, Q, tmp, w, rot, ns, L);
-  for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) T[e] = Q[(e / ns) * L + e % ns];
-  if (threadIdx.x < ns) wout[threadIdx.x] = w[threadIdx.x];
-}
-
-// out[:, c] = beta out[:, c] + alpha sum_{k < kin} S[:, k] T[(r0 + k) * tld + c0 + c]
-// for c < cout.
-__global__ void __launch_bounds__(256) k_xform(const double* __restrict__ S, int n, int kin,
-                                               const double* __restrict__ T, int tld, int r0,
-                                               int c0, int cout, double* __restrict__ out,
-                                               double alpha, double beta) {
-  __shared__ double Ts[kLv3Max * kLv3Max];
-  for (int e = threadIdx.x; e < kin * cout; e += blockDim.x)
-    Ts[e] = T[(size_t)(r0 + e / cout) * tld + c0 + e % cout];
-  __syncthreads();
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    double sv[kLv3Max];
-#pragma unroll
-    for (int k = 0; k < kLv3Max; ++k) sv[k] = k < kin ? S[(size_t)k * n + r] : 0.0;
-    for (int c = 0; c < cout; ++c) {
-      double v = 0.0;
-#pragma unroll
-      for (int k = 0; k < kLv3Max; ++k)
-        if (k < kin) v += sv[k] * Ts[k * cout + c];
-      out[(size_t)c * n + r] = beta == 0.0 ? alpha * v : beta * out[(size_t)c * n + r] + alpha * v;
-    }
-  }
-}
-
-// y += alpha x.
-__global__ void k_axpy(const double* __restrict__ x, double alpha, double* __restrict__ y,
-                       size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    y[i] += alpha * x[i];
-}
-
-// res[j] = ||X_j|| (one workgroup per column).
-__global__ void k_col_norm(const double* __restrict__ X, int n, double* __restrict__ res) {
-  __shared__ double red[8];
-  const int j = blockIdx.x;
-  double s = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    const double e = X[(size_t)j * n + r];
-    s += e * e;
-  }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) res[j] = sqrt(s);
-}
-
-// R = AX - X diag(w), res[j] = ||R_j|| (one workgroup per column).
-__global__ void k_col_resid(const double* __restrict__ AX, const double* __restrict__ X,
-                            const double* __restrict__ w, double* __restrict__ R, int n,
-                            double* __restrict__ res) {
-  __shared__ double red[8];
-  const int j = blockIdx.x;
-  const double wj = w[j];
-  double s = 0.0;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    const double e = AX[(size_t)j * n + r] - wj * X[(size_t)j * n + r];
-    R[(size_t)j * n + r] = e;
-    s += e * e;
-  }
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) res[j] = sqrt(s);
-}
-
-namespace {
-
-struct Lv3Work {
-  pmx_ctx* ctx;
-  int n, ngram;
-  DBuf<double> part, G, T, wall;
-  DBuf<int> flag;
-
-  int gram(const double* A, int ma, const double* B, int mb) {
-    hipLaunchKernelGGL(k_gram_part, dim3(ngram, ma), dim3(256), 0, ctx->stream, A, ma, B, mb, n,
-                       part.p);
-    hipLaunchKernelGGL(k_gram_sum, dim3(1), dim3(256), 0, ctx->stream, part.p, ngram, ma, mb,
-                       G.p);
-    PMX_HIP(hipGetLastError());
-    return PMX_SUCCESS;
-  }
-  int xform(const double* S, int kin, int tld, int r0, int c0, int cout, double* out,
-            const double* Tm = nullptr, double alpha = 1.0, double beta = 0.0) {
-    const int nb = std::min((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_xform, dim3(nb), dim3(256), 0, ctx->stream, S, n, kin, Tm ? Tm : T.p, tld,
-                       r0, c0, cout, out, alpha, beta);
-    PMX_HIP(hipGetLastError());
-    return PMX_SUCCESS;
-  }
-  // S (ns columns) <- S L^{-T}: orthonormal; returns 1 in *bad if the Gram
-  // matrix is not positive definite.
-  int orthonormalize(double* S, int ns, double* tmp, int* bad) {
-    PMX_TRY(gram(S, ns, S, ns));
-    hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, G.p, ns, 0, T.p, wall.p,
-                       flag.p);
-    PMX_HIP(hipGetLastError());
-    int h = 0;
-    PMX_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    PMX_HIP(hipStreamSynchronize(ctx->stream));
-    *bad = h;
-    if (h) return PMX_SUCCESS;
-    PMX_TRY(xform(S, ns, ns, 0, 0, ns, tmp));
-    PMX_HIP(hipMemcpyAsync(S, tmp, (size_t)ns * n * sizeof(double), hipMemcpyDeviceToDevice,
-                           ctx->stream));
-    return PMX_SUCCESS;
-  }
-};
-
-}  // namespace
 
 // Block LOBPCG (Knyazev 2001) on A_c with the ICC(0) preconditioner:
 // S = [X, M^{-1}(A_c X - X diag(w)), P], S orthonormalised by Cholesky QR
