@@ -906,16 +906,14 @@ __global__ void k_stage_update(Geo g, const double* __restrict__ Acb, int kmax,
 }
 
 // Y = A_cc^{-1} (V^T T), k2 x n (column j per thread).
-__global__ void k_lv3_left(const double* __restrict__ V, int k2, const double* __restrict__ LU,
-                           const double* __restrict__ T, double* __restrict__ Y, int n) {
+// Y[:, j] = A_cc^{-1} Y[:, j] (k2 x k2 LU with partial pivoting, row
+// interchanges in LU[k2*k2 ..]) for every column j of the k2 x n block Y.
+__global__ void k_lv3_lu_cols(int k2, const double* __restrict__ LU, double* __restrict__ Y,
+                              int n) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double b[kMaxEig];
-  for (int q = 0; q < k2; ++q) b[q] = 0.0;
-  for (int r = 0; r < n; ++r) {
-    const double tv = T[(size_t)r * n + j];
-    for (int q = 0; q < k2; ++q) b[q] += V[(size_t)q * n + r] * tv;
-  }
+  for (int q = 0; q < k2; ++q) b[q] = Y[(size_t)q * n + j];
   const double* piv = LU + k2 * k2;
   for (int i = 0; i < k2; ++i) {
     const int p = (int)piv[i];
@@ -976,13 +974,18 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
       if (T_is_I) {
         Dp = S.p;
       } else {
-        hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 2, false>), dim3(nt, nt), dim3(256), 0,
-                           ctx->stream, n, n, n, 1.0, S.p, n, T.p, n, 0.0, D.p, n);
+        hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 4, false>), dim3((n + 127) / 128, (n + 63) / 64),
+                           dim3(256), 0, ctx->stream, n, n, n, 1.0, S.p, n, T.p, n, 0.0, D.p, n);
         Dp = D.p;
       }
     } else {
-      hipLaunchKernelGGL(k_lv3_left, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
-                         ctx->V2.p, k2, ctx->Acc_lu.p, T.p, Y.p, n);
+      // Y = A_cc^{-1} V^T T: V^T T on MFMA (V column-major = V^T row-major),
+      // then the k2 x k2 solve per column.
+      hipLaunchKernelGGL((k_gemm_mfma<1, 4, 1, 1, false>), dim3((n + 63) / 64, (k2 + 15) / 16),
+                         dim3(256), 0, ctx->stream, k2, n, n, 1.0, ctx->V2.p, n, T.p, n, 0.0, Y.p,
+                         n);
+      hipLaunchKernelGGL(k_lv3_lu_cols, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                         ctx->stream, k2, ctx->Acc_lu.p, Y.p, n);
       hipLaunchKernelGGL(k_lv3_right, dim3(eb), dim3(kBlock), 0, ctx->stream, ctx->V2.p, k2, Y.p,
                          D.p, n);
       Dp = D.p;
@@ -1314,11 +1317,17 @@ struct Lv3Work {
 // an LU factorisation.
 // ---------------------------------------------------------------------------
 // Y[:, c] = Ainv X[:, c] for the m columns of column-major n x m blocks:
-// as row-major m x n blocks, Y = X Ainv^T (FP64 MFMA, 32 x 64 tiles).
+// as row-major m x n blocks, Y = X Ainv^T (FP64 MFMA, 32 x 32 tiles, K split
+// kTallSplit ways into ws: kTallSplit * m * n doubles).
+constexpr int kTallSplit = 8;
 static void launch_gemm_tall(hipStream_t st, const double* Ainv, const double* X, double* Y, int n,
-                             int m) {
-  hipLaunchKernelGGL((k_gemm_mfma<1, 4, 2, 1, true>), dim3((n + 63) / 64, (m + 31) / 32),
-                     dim3(256), 0, st, m, n, n, 1.0, X, n, Ainv, n, 0.0, Y, n);
+                             int m, double* ws) {
+  const int kc = ((n + kTallSplit - 1) / kTallSplit + kMk - 1) / kMk * kMk;
+  const int nz = (n + kc - 1) / kc;
+  hipLaunchKernelGGL((k_gemm_mfma_part<2, 2, 1, 1, true>), dim3((n + 31) / 32, (m + 31) / 32, nz),
+                     dim3(256), 0, st, m, n, n, kc, X, n, Ainv, n, ws);
+  hipLaunchKernelGGL(k_gemm_sum, dim3(std::min((m * n + 255) / 256, 2048)), dim3(256), 0, st, ws,
+                     nz, m, n, 1.0, 0.0, Y, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -1532,6 +1541,8 @@ int level3_setup(pmx_ctx* ctx) {
   PMX_TRY(w.alloc(m));
   PMX_TRY(res.alloc(m));
   PMX_TRY(fail.alloc(1));
+  DBuf<double> ws;
+  PMX_TRY(ws.alloc((size_t)kTallSplit * n * m));
   Lv3Work wk;
   wk.ctx = ctx;
   wk.n = n;
@@ -1555,7 +1566,7 @@ int level3_setup(pmx_ctx* ctx) {
   ctx->lv3_floor = 0.0;
   for (it = 1; it <= maxit; ++it) {
     if (it == 1) {
-      launch_gemm_tall(ctx->stream, ctx->Ainv.p, X.p, W.p, n, m);
+      launch_gemm_tall(ctx->stream, ctx->Ainv.p, X.p, W.p, n, m, ws.p);
       hipLaunchKernelGGL(k_coarse_fix, dim3(m), dim3(kBlock), 0, ctx->stream, W.p, X.p,
                          ctx->c_z.p, ctx->c_h.p, n, ctx->c_cs.p, Y.p);
     } else {
@@ -1566,7 +1577,7 @@ int level3_setup(pmx_ctx* ctx) {
       // target ~0), computed on the small correction.
       hipLaunchKernelGGL(k_lv3_refine, dim3(std::min((n * m + kBlock - 1) / kBlock, 4096)),
                          dim3(kBlock), 0, ctx->stream, Z.p, X.p, w.p, R.p, n, m);
-      launch_gemm_tall(ctx->stream, ctx->Ainv.p, R.p, W.p, n, m);
+      launch_gemm_tall(ctx->stream, ctx->Ainv.p, R.p, W.p, n, m, ws.p);
       hipLaunchKernelGGL(k_coarse_fix, dim3(m), dim3(kBlock), 0, ctx->stream, W.p, R.p,
                          ctx->c_z.p, ctx->c_h.p, n, ctx->c_cs.p, Y.p);
       hipLaunchKernelGGL(k_lv3_update, dim3(m), dim3(kBlock), 0, ctx->stream, X.p, Y.p, n,
