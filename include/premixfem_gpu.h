@@ -108,7 +108,8 @@ typedef struct {
  * -coarse_composite 0|1 (dense path: compose the coarse stages into one
  * operator, default, or apply them one by one), -coarse_deflate 0|1 (dense
  * path: invert A_c deflated with a Woodbury correction, default, or
- * directly).  Allocates the
+ * directly), -quad 0|1 (quad-cell stream kernels where every brick's
+ * x-extent is a multiple of 4, default 1).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 

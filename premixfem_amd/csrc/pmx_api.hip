@@ -111,6 +111,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->lv2_icc = 1;
     } else if (k == "-eig_autotune") {
       PMX_TRY(geti(c->eig_autotune));
+    } else if (k == "-quad") {
+      PMX_TRY(geti(c->quad_ok));
     } else if (k == "-coarse_composite") {
       PMX_TRY(geti(c->coarse_composite));
     } else if (k == "-coarse_band") {

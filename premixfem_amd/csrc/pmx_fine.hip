@@ -442,14 +442,13 @@ __device__ __forceinline__ void cg_finish(const CgCtl& c, int its, int reason, d
 //   beta checks (beta = 0 -> CONVERGED_ATOL, sign change -> INDEFINITE_PC);
 //   p = z + (beta/betaold) p (p = z when i = 0), w = A p, partial p'w.
 // p_old is read at neighbours, so p goes to a second buffer.
-__global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
-    Geo g, const double* __restrict__ z, const double* __restrict__ pold,
-    double* __restrict__ pnew, double* __restrict__ w, const double* __restrict__ Tx,
-    const double* __restrict__ Ty, const double* __restrict__ Tz, CgCtl cg) {
-  PMX_STOP_CHECK(g.stop);
-  __shared__ double sh[4];
-  __shared__ double red[16];
-  const int i = (int)cg.scal[kSIter];
+// Scalar part of k_cg_p_spmv: the convergence tests of iteration i on the
+// previous update's ||r||^2 and the beta checks.  Returns false when the
+// solve ends here (every block takes the same decision); otherwise bc =
+// beta / betaold (0 at i = 0), first = (i == 0), i = the iteration.
+__device__ __forceinline__ bool cg_p_begin(const CgCtl& cg, double* sh, double& bc, bool& first,
+                                           int& i) {
+  i = (int)cg.scal[kSIter];
   if (threadIdx.x < 64) {  // wave 0 (a block may have only one wave)
     double s0, s1, s2;
     wave_sum3(cg.zr + (size_t)(i & 1) * cg.nparts,
@@ -477,7 +476,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
       reason = PMX_DIVERGED_ITS;
     if (reason) {
       cg_finish(cg, i, reason, rr);
-      return;
+      return false;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       cg.scal[kSIts] = i;
@@ -486,14 +485,28 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
   }
   if (beta == 0.0) {
     cg_finish(cg, i + 1, PMX_CONVERGED_ATOL, -1.0);
-    return;
+    return false;
   }
   if (i > 0 && beta * sh[1] < 0.0) {
     cg_finish(cg, i + 1, PMX_DIVERGED_INDEFINITE_PC, -1.0);
-    return;
+    return false;
   }
-  const bool first = i == 0;
-  const double bc = first ? 0.0 : beta / sh[1];
+  first = i == 0;
+  bc = first ? 0.0 : beta / sh[1];
+  return true;
+}
+
+__global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
+    Geo g, const double* __restrict__ z, const double* __restrict__ pold,
+    double* __restrict__ pnew, double* __restrict__ w, const double* __restrict__ Tx,
+    const double* __restrict__ Ty, const double* __restrict__ Tz, CgCtl cg) {
+  PMX_STOP_CHECK(g.stop);
+  __shared__ double sh[4];
+  __shared__ double red[16];
+  double bc;
+  bool first;
+  int i;
+  if (!cg_p_begin(cg, sh, bc, first, i)) return;
   const BrickView v = load_view(g, block_brick(g));
   double acc = 0.0;
   const double* TT3[3] = {Tx, Ty, Tz};
@@ -519,6 +532,151 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv(
     w[c] = wc;
     acc += pc * wc;
   });
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) cg.pw[(size_t)(i & 1) * cg.nparts + blockIdx.x] = acc;
+}
+
+// The same iteration step with four consecutive x-cells per thread, for
+// bricks whose x-extent is a multiple of 4 (so every brick offset and row is
+// 32-byte aligned): p of the brick is staged in LDS, the interior
+// neighbours come from there, the face neighbours from the adjacent bricks;
+// z, p, w and the face coefficients move as double4.  Each cell's p and w
+// are computed by the same operations in the same order as in k_cg_p_spmv;
+// only the block's p'w partial is summed in another order.
+// Requires n <= 4 * blockDim.x; LDS: n doubles.
+__global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv_q(
+    Geo g, const double* __restrict__ z, const double* __restrict__ pold,
+    double* __restrict__ pnew, double* __restrict__ w, const double* __restrict__ Tx,
+    const double* __restrict__ Ty, const double* __restrict__ Tz, CgCtl cg) {
+  PMX_STOP_CHECK(g.stop);
+  extern __shared__ double psh[];
+  __shared__ double sh[4];
+  __shared__ double red[16];
+  double bc;
+  bool first;
+  int i;
+  if (!cg_p_begin(cg, sh, bc, first, i)) return;
+  const BrickView v = load_view(g, block_brick(g));
+  const int q0 = 4 * (int)threadIdx.x;
+  const bool act = q0 < v.n;
+  const int c0 = v.off + q0;
+  auto ld4 = [](const double* a, int c) { return *reinterpret_cast<const double4*>(a + c); };
+  auto comb = [&](double a, double b2) { return first ? a : a + bc * b2; };
+  auto pv4 = [&](int c) {
+    const double4 zz = ld4(z, c);
+    if (first) return zz;
+    const double4 po = ld4(pold, c);
+    return double4{zz.x + bc * po.x, zz.y + bc * po.y, zz.z + bc * po.z, zz.w + bc * po.w};
+  };
+  const double4 zero{0.0, 0.0, 0.0, 0.0};
+  double4 pc = zero;
+  if (act) {
+    pc = pv4(c0);
+    *reinterpret_cast<double4*>(pnew + c0) = pc;
+    *reinterpret_cast<double4*>(psh + q0) = pc;
+  }
+  __syncthreads();
+  double acc = 0.0;
+  if (act) {
+    const int lx0 = q0 % v.nx, l1 = q0 / v.nx;
+    const int ly = l1 % v.ny, lz = l1 / v.ny;
+    const double P[4] = {pc.x, pc.y, pc.z, pc.w};
+    // Per cell: diag and the off-diagonal sum accumulate direction by
+    // direction (x, y, z) in the order of k_cg_p_spmv; a missing neighbour
+    // has T = 0 and p = 0, so its terms add +0.
+    double diag[4], off[4];
+    {
+      double pxl = 0.0, pxr = 0.0, txr = 0.0;
+      if (lx0 > 0) {
+        pxl = psh[q0 - 1];
+      } else if (v.noff[0] >= 0) {
+        const int h = v.noff[0] + (lz * v.ny + ly) * v.next[0] + v.next[0] - 1;
+        pxl = comb(z[h], pold[h]);
+      }
+      if (lx0 + 4 < v.nx) {
+        pxr = psh[q0 + 4];
+        txr = Tx[c0 + 4];
+      } else if (v.noff[1] >= 0) {
+        const int h = v.noff[1] + (lz * v.ny + ly) * v.next[1];
+        pxr = comb(z[h], pold[h]);
+        txr = Tx[h];
+      }
+      const double4 t4 = ld4(Tx, c0);
+      const double T5[5] = {t4.x, t4.y, t4.z, t4.w, txr};
+      const double P6[6] = {pxl, P[0], P[1], P[2], P[3], pxr};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        diag[j] = 0.0;
+        off[j] = 0.0;
+        diag[j] += T5[j] + T5[j + 1];
+        off[j] += T5[j] * P6[j];
+        off[j] += T5[j + 1] * P6[j + 2];
+      }
+    }
+    {
+      double4 pl = zero, ph = zero, th = zero;
+      if (ly > 0) {
+        pl = *reinterpret_cast<const double4*>(psh + q0 - v.nx);
+      } else if (v.noff[2] >= 0) {
+        pl = pv4(v.noff[2] + (lz * v.next[2] + v.next[2] - 1) * v.nx + lx0);
+      }
+      if (ly < v.ny - 1) {
+        ph = *reinterpret_cast<const double4*>(psh + q0 + v.nx);
+        th = ld4(Ty, c0 + v.nx);
+      } else if (v.noff[3] >= 0) {
+        const int h = v.noff[3] + lz * v.next[3] * v.nx + lx0;
+        ph = pv4(h);
+        th = ld4(Ty, h);
+      }
+      const double4 tl = ld4(Ty, c0);
+      const double TL[4] = {tl.x, tl.y, tl.z, tl.w}, TH[4] = {th.x, th.y, th.z, th.w};
+      const double PL[4] = {pl.x, pl.y, pl.z, pl.w}, PH[4] = {ph.x, ph.y, ph.z, ph.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        diag[j] += TL[j] + TH[j];
+        off[j] += TL[j] * PL[j];
+        off[j] += TH[j] * PH[j];
+      }
+    }
+    {
+      double4 pl = zero, ph = zero, th = zero;
+      if (lz > 0) {
+        pl = *reinterpret_cast<const double4*>(psh + q0 - v.nxny);
+      } else if (v.noff[4] >= 0) {
+        pl = pv4(v.noff[4] + ((v.next[4] - 1) * v.ny + ly) * v.nx + lx0);
+      }
+      if (lz < v.nz - 1) {
+        ph = *reinterpret_cast<const double4*>(psh + q0 + v.nxny);
+        th = ld4(Tz, c0 + v.nxny);
+      } else if (v.noff[5] >= 0) {
+        const int h = v.noff[5] + ly * v.nx + lx0;
+        ph = pv4(h);
+        th = ld4(Tz, h);
+      }
+      const double4 tl = ld4(Tz, c0);
+      const double TL[4] = {tl.x, tl.y, tl.z, tl.w}, TH[4] = {th.x, th.y, th.z, th.w};
+      const double PL[4] = {pl.x, pl.y, pl.z, pl.w}, PH[4] = {ph.x, ph.y, ph.z, ph.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        diag[j] += TL[j] + TH[j];
+        off[j] += TL[j] * PL[j];
+        off[j] += TH[j] * PH[j];
+      }
+    }
+    double4 wd = zero;
+    if (g.wd) wd = ld4(g.wd, c0);
+    const double WD[4] = {wd.x, wd.y, wd.z, wd.w};
+    double wv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (c0 + j == g.pin) diag[j] += g.tau;
+      if (g.wd) diag[j] += WD[j];
+      wv[j] = diag[j] * P[j] - off[j];
+    }
+    *reinterpret_cast<double4*>(w + c0) = double4{wv[0], wv[1], wv[2], wv[3]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc += P[j] * wv[j];
+  }
   acc = block_sum(acc, red);
   if (threadIdx.x == 0) cg.pw[(size_t)(i & 1) * cg.nparts + blockIdx.x] = acc;
 }
@@ -695,6 +853,221 @@ __global__ void __launch_bounds__(kStreamBlock) k_prolong(Geo g, S* __restrict__
       if (j < ke) acc += ph[j] * (S)cs[j];
     y[c] = y[c] + acc;
   }
+}
+
+template <typename S>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+  using type = float4;
+};
+template <>
+struct Vec4T<double> {
+  using type = double4;
+};
+
+template <typename S>
+__device__ __forceinline__ void ld4v(const S* p, S (&o)[4]) {
+  const typename Vec4T<S>::type v = *reinterpret_cast<const typename Vec4T<S>::type*>(p);
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+  o[3] = v.w;
+}
+
+template <typename S>
+__device__ __forceinline__ void st4v(S* p, const S (&o)[4]) {
+  *reinterpret_cast<typename Vec4T<S>::type*>(p) = typename Vec4T<S>::type{o[0], o[1], o[2], o[3]};
+}
+
+// Quad-cell k_resid_restrict (bricks with x-extent a multiple of 4, n <= 4
+// blockDim.x): thread t owns cells 4t..4t+3 of its brick; y is staged in LDS
+// for the interior neighbours; all global loads (x, y, faces, the KM basis
+// rows of the quad) are issued before the LDS barrier.  Each cell's r is
+// computed exactly as stencil_fast does; the restriction sums run in another
+// order than in the per-cell kernel.  LDS: n values of S.
+template <typename S, typename TT, int KM>
+__global__ void __launch_bounds__(kStreamBlock) k_resid_restrict_q(
+    Geo g, const double* __restrict__ x, const S* __restrict__ y, const TT* __restrict__ Tx,
+    const TT* __restrict__ Ty, const TT* __restrict__ Tz, const S* __restrict__ phi,
+    double* __restrict__ cvec) {
+  PMX_STOP_CHECK(g.stop);
+  extern __shared__ char shraw[];
+  S* ysh = reinterpret_cast<S*>(shraw);
+  __shared__ double red[16 * kMaxEig];
+  const int sb = block_brick(g);
+  const Brick b = g.bricks[sb];
+  const BrickView v = load_view(g, sb);
+  const int ke = b.neig;
+  const int q0 = 4 * (int)threadIdx.x;
+  const bool act = q0 < v.n;
+  const int c0 = v.off + q0;
+  const int lx0 = q0 % v.nx, l1 = q0 / v.nx;
+  const int ly = l1 % v.ny, lz = l1 / v.ny;
+  double xr[4] = {0.0, 0.0, 0.0, 0.0};
+  S yo[4] = {}, tx[4] = {}, ty[4] = {}, tz[4] = {}, tyh[4] = {}, tzh[4] = {}, wd[4] = {};
+  S hq[4][4] = {};
+  S hxl = S(0), hxr = S(0), txr = S(0);
+  S ph[4 * KM];
+  if (act) {
+    {
+      const double4 v4 = *reinterpret_cast<const double4*>(x + c0);
+      xr[0] = v4.x;
+      xr[1] = v4.y;
+      xr[2] = v4.z;
+      xr[3] = v4.w;
+    }
+    const typename Vec4T<S>::type* p4 =
+        reinterpret_cast<const typename Vec4T<S>::type*>(phi + (size_t)c0 * KM);
+#pragma unroll
+    for (int u = 0; u < KM; ++u) {
+      const typename Vec4T<S>::type t = p4[u];
+      ph[4 * u] = t.x;
+      ph[4 * u + 1] = t.y;
+      ph[4 * u + 2] = t.z;
+      ph[4 * u + 3] = t.w;
+    }
+    if (y) {
+      ld4v(y + c0, yo);
+      ld4v(Tx + c0, tx);
+      ld4v(Ty + c0, ty);
+      ld4v(Tz + c0, tz);
+      const TT* W = well_term<TT>(g);
+      if (W) ld4v(W + c0, wd);
+      if (lx0 == 0 && v.noff[0] >= 0)
+        hxl = y[v.noff[0] + (lz * v.ny + ly) * v.next[0] + v.next[0] - 1];
+      if (lx0 + 4 < v.nx) {
+        txr = Tx[c0 + 4];
+      } else if (v.noff[1] >= 0) {
+        const int h = v.noff[1] + (lz * v.ny + ly) * v.next[1];
+        hxr = y[h];
+        txr = Tx[h];
+      }
+      if (ly == 0 && v.noff[2] >= 0)
+        ld4v(y + v.noff[2] + (lz * v.next[2] + v.next[2] - 1) * v.nx + lx0, hq[0]);
+      if (ly < v.ny - 1) {
+        ld4v(Ty + c0 + v.nx, tyh);
+      } else if (v.noff[3] >= 0) {
+        const int h = v.noff[3] + lz * v.next[3] * v.nx + lx0;
+        ld4v(y + h, hq[1]);
+        ld4v(Ty + h, tyh);
+      }
+      if (lz == 0 && v.noff[4] >= 0)
+        ld4v(y + v.noff[4] + ((v.next[4] - 1) * v.ny + ly) * v.nx + lx0, hq[2]);
+      if (lz < v.nz - 1) {
+        ld4v(Tz + c0 + v.nxny, tzh);
+      } else if (v.noff[5] >= 0) {
+        const int h = v.noff[5] + ly * v.nx + lx0;
+        ld4v(y + h, hq[3]);
+        ld4v(Tz + h, tzh);
+      }
+      st4v(ysh + q0, yo);
+    }
+  }
+  if (y) __syncthreads();
+  double acc[kMaxEig];
+#pragma unroll
+  for (int j = 0; j < kMaxEig; ++j) acc[j] = 0.0;
+  if (act) {
+    double r[4] = {xr[0], xr[1], xr[2], xr[3]};
+    if (y) {
+      // neighbour y: a missing one has T = 0; its value is then the cell's
+      // own, as in stencil_fast
+      S yl[4], yh[4], zl[4], zh[4];
+      if (ly > 0) ld4v(ysh + q0 - v.nx, yl);
+      if (ly < v.ny - 1) ld4v(ysh + q0 + v.nx, yh);
+      if (lz > 0) ld4v(ysh + q0 - v.nxny, zl);
+      if (lz < v.nz - 1) ld4v(ysh + q0 + v.nxny, zh);
+      const bool hl = lx0 > 0 || v.noff[0] >= 0, hr = lx0 + 4 < v.nx || v.noff[1] >= 0;
+      const S xl = lx0 > 0 ? ysh[q0 - 1] : hxl;
+      const S xh = lx0 + 4 < v.nx ? ysh[q0 + 4] : hxr;
+      const bool byl = ly > 0 || v.noff[2] >= 0, byh = ly < v.ny - 1 || v.noff[3] >= 0;
+      const bool bzl = lz > 0 || v.noff[4] >= 0, bzh = lz < v.nz - 1 || v.noff[5] >= 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const S xc = yo[j];
+        const S vxl = j > 0 ? yo[j - 1] : (hl ? xl : xc);
+        const S vxh = j < 3 ? yo[j + 1] : (hr ? xh : xc);
+        const S vyl = byl ? (ly > 0 ? yl[j] : hq[0][j]) : xc;
+        const S vyh = byh ? (ly < v.ny - 1 ? yh[j] : hq[1][j]) : xc;
+        const S vzl = bzl ? (lz > 0 ? zl[j] : hq[2][j]) : xc;
+        const S vzh = bzh ? (lz < v.nz - 1 ? zh[j] : hq[3][j]) : xc;
+        const S thx = j < 3 ? tx[j + 1] : txr;
+        S diag = S(0), a = S(0);
+        diag += tx[j] + thx;
+        a += tx[j] * vxl + thx * vxh;
+        diag += ty[j] + tyh[j];
+        a += ty[j] * vyl + tyh[j] * vyh;
+        diag += tz[j] + tzh[j];
+        a += tz[j] * vzl + tzh[j] * vzh;
+        if (c0 + j == g.pin) diag += (S)g.tau;
+        if (well_term<TT>(g)) diag += wd[j];
+        r[j] -= (double)(diag * xc - a);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < KM; ++j)
+        if (j < ke) acc[j] += (double)ph[q * KM + j] * r[q];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    if (j < ke) {
+      double sum = acc[j];
+      for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+      if (lane == 0) red[wv * kMaxEig + j] = sum;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ke) {
+    double sum = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) sum += red[q * kMaxEig + threadIdx.x];
+    cvec[b.dof + threadIdx.x] = sum;
+  }
+}
+
+// Quad-cell k_prolong: y += phi c for cells 4t..4t+3 (same arithmetic per
+// cell as k_prolong).
+template <typename S, int KM>
+__global__ void __launch_bounds__(kStreamBlock) k_prolong_q(Geo g, S* __restrict__ y,
+                                                            const S* __restrict__ phi,
+                                                            const double* __restrict__ cvec,
+                                                            double* __restrict__ iter) {
+  PMX_STOP_CHECK(g.stop);
+  const Brick b = g.bricks[block_brick(g)];
+  if (iter && blockIdx.x == 0 && threadIdx.x == 0) *iter += 1.0;
+  const int q0 = 4 * (int)threadIdx.x;
+  const int n = b.nx * b.ny * b.nz;
+  if (q0 >= n) return;
+  const int c0 = b.off + q0;
+  const int ke = b.neig;
+  S cs[KM];
+#pragma unroll
+  for (int j = 0; j < KM; ++j) cs[j] = j < ke ? (S)cvec[b.dof + j] : S(0);
+  S ph[4 * KM];
+  const typename Vec4T<S>::type* p4 =
+      reinterpret_cast<const typename Vec4T<S>::type*>(phi + (size_t)c0 * KM);
+#pragma unroll
+  for (int u = 0; u < KM; ++u) {
+    const typename Vec4T<S>::type t = p4[u];
+    ph[4 * u] = t.x;
+    ph[4 * u + 1] = t.y;
+    ph[4 * u + 2] = t.z;
+    ph[4 * u + 3] = t.w;
+  }
+  S yv[4];
+  ld4v(y + c0, yv);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    S acc = (S)0;
+#pragma unroll
+    for (int j = 0; j < KM; ++j)
+      if (j < ke) acc += ph[q * KM + j] * cs[j];
+    yv[q] = yv[q] + acc;
+  }
+  st4v(y + c0, yv);
 }
 
 // r = b - A x, partial r'r; with an initial guess (x != null) also the
@@ -948,8 +1321,45 @@ int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool 
   return PMX_SUCCESS;
 }
 
+template <typename S, typename TT>
+static bool launch_resid_restrict_q(pmx_ctx* ctx, const double* x, const S* y, const TT* Tx,
+                                    const TT* Ty, const TT* Tz, const S* phi, double* cvec) {
+  const int qt = ctx->quad_ok ? quad_threads(ctx) : 0;
+  if (!qt) return false;
+  size_t lds = 0;
+  for (const Brick& b : ctx->h_bricks) lds = std::max(lds, (size_t)b.nx * b.ny * b.nz);
+  lds *= sizeof(S);
+  switch (ctx->en_lv1) {
+    case 4:
+      hipLaunchKernelGGL((k_resid_restrict_q<S, TT, 4>), dim3(ctx->nsub), dim3(qt), lds,
+                         ctx->stream, ctx->geo, x, y, Tx, Ty, Tz, phi, cvec);
+      return true;
+    case 6:
+      hipLaunchKernelGGL((k_resid_restrict_q<S, TT, 6>), dim3(ctx->nsub), dim3(qt), lds,
+                         ctx->stream, ctx->geo, x, y, Tx, Ty, Tz, phi, cvec);
+      return true;
+    case 8:
+      hipLaunchKernelGGL((k_resid_restrict_q<S, TT, 8>), dim3(ctx->nsub), dim3(qt), lds,
+                         ctx->stream, ctx->geo, x, y, Tx, Ty, Tz, phi, cvec);
+      return true;
+    default:
+      return false;
+  }
+}
+
 int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec) {
   const int nt = stream_threads(ctx);
+  // (FP32 only: the FP64 quad variant needs more registers than the
+  // occupancy of 1024-thread workgroups allows.)
+  const bool done =
+      ctx->pc_precision == 32 &&
+      launch_resid_restrict_q<float, float>(ctx, x, y_zero ? nullptr : ctx->w_y32.p,
+                                            ctx->T32[0].p, ctx->T32[1].p, ctx->T32[2].p,
+                                            ctx->phi32.p, cvec);
+  if (done) {
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
   if (ctx->pc_precision == 32)
     hipLaunchKernelGGL((k_resid_restrict<float, float>), dim3(ctx->nsub), dim3(nt), 0,
                        ctx->stream, ctx->geo, x, y_zero ? nullptr : ctx->w_y32.p, ctx->T32[0].p,
@@ -971,9 +1381,38 @@ int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec) {
   return PMX_SUCCESS;
 }
 
+template <typename S>
+static bool launch_prolong_q(pmx_ctx* ctx, S* y, const S* phi, const double* cvec, double* iter) {
+  const int qt = ctx->quad_ok ? quad_threads(ctx) : 0;
+  if (!qt) return false;
+  switch (ctx->en_lv1) {
+    case 4:
+      hipLaunchKernelGGL((k_prolong_q<S, 4>), dim3(ctx->nsub), dim3(qt), 0, ctx->stream, ctx->geo,
+                         y, phi, cvec, iter);
+      return true;
+    case 6:
+      hipLaunchKernelGGL((k_prolong_q<S, 6>), dim3(ctx->nsub), dim3(qt), 0, ctx->stream, ctx->geo,
+                         y, phi, cvec, iter);
+      return true;
+    case 8:
+      hipLaunchKernelGGL((k_prolong_q<S, 8>), dim3(ctx->nsub), dim3(qt), 0, ctx->stream, ctx->geo,
+                         y, phi, cvec, iter);
+      return true;
+    default:
+      return false;
+  }
+}
+
 int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter) {
   double* iter = count_iter ? ctx->scal.p + kSIter : nullptr;
   const int nt = stream_threads(ctx);
+  const bool done = ctx->pc_precision == 32
+                        ? launch_prolong_q<float>(ctx, ctx->w_y32.p, ctx->phi32.p, cvec, iter)
+                        : launch_prolong_q<double>(ctx, ctx->w_y.p, ctx->phi.p, cvec, iter);
+  if (done) {
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
   if (ctx->pc_precision == 32)
     hipLaunchKernelGGL(k_prolong<float>, dim3(ctx->nsub), dim3(nt), 0, ctx->stream, ctx->geo,
                        ctx->w_y32.p, ctx->phi32.p, ctx->en_lv1, cvec, iter);
@@ -984,10 +1423,32 @@ int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter) {
   return PMX_SUCCESS;
 }
 
+// Quad-cell stream kernels apply when every brick's x-extent is a multiple
+// of 4 and a brick fits in 4 * kStreamBlock cells; returns their block size
+// (0: use the per-cell kernels).
+int quad_threads(const pmx_ctx* ctx) {
+  int mx = 0;
+  for (const Brick& b : ctx->h_bricks) {
+    if (b.nx % 4 != 0) return 0;
+    mx = std::max(mx, b.nx * b.ny * b.nz);
+  }
+  if (mx > 4 * kStreamBlock) return 0;
+  return std::max(64, (mx / 4 + 63) / 64 * 64);
+}
+
 int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* pnew, double* w) {
-  hipLaunchKernelGGL(k_cg_p_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
-                     ctx->geo, z, pold, pnew, w, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
-                     make_cg(ctx));
+  const int qt = ctx->quad_ok ? quad_threads(ctx) : 0;
+  if (qt) {
+    size_t lds = 0;
+    for (const Brick& b : ctx->h_bricks) lds = std::max(lds, (size_t)b.nx * b.ny * b.nz);
+    hipLaunchKernelGGL(k_cg_p_spmv_q, dim3(ctx->nsub), dim3(qt), lds * sizeof(double),
+                       ctx->stream, ctx->geo, z, pold, pnew, w, ctx->T[0].p, ctx->T[1].p,
+                       ctx->T[2].p, make_cg(ctx));
+  } else {
+    hipLaunchKernelGGL(k_cg_p_spmv, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
+                       ctx->geo, z, pold, pnew, w, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                       make_cg(ctx));
+  }
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

@@ -292,6 +292,7 @@ struct pmx_ctx {
   double eig_tol = 1.0e-8;      // LOBPCG relative residual tolerance
   int eig_maxit = 400;
   int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1
+  int quad_ok = 1;              // quad-cell stream kernels where the bricks allow (-quad 0: off)
   int eig_autotune = 0;         // time the LOBPCG block sizes on a sample and keep the fastest
   int eig_block = 0;            // block size of the last LOBPCG solve
   double eig_sigma = 1.0e-6;    // shift of the LOBPCG preconditioner A^N + sigma M

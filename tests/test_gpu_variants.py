@@ -119,3 +119,30 @@ def test_lobpcg_autotuner_keeps_the_basis():
                                                          rel=1e-6)
     err = np.linalg.norm(tuned["x"] - base["x"]) / np.linalg.norm(base["x"])
     assert err <= 1e-8
+
+
+def _uneven_prob():
+    """Bricks with x-extents 12 and uneven z-extents (6, 5, 5): the quad-cell
+    kernels' halo rows come from neighbours of other shapes."""
+    mesh = (24, 20, 16)
+    kap = problems.grf_kappa(mesh, 1.5, 0.2, 77)
+    return problems.Problem("uneven", (1.0, 1.0, 1.0), mesh, kap,
+                            problems.corner_rhs(mesh, (1.0, 1.0, 1.0)), sd=(2, 2, 3), en_lv1=4,
+                            en_lv2=3, pin_cell=(mesh[0] - 1, 0, 0))
+
+
+@pytest.mark.parametrize("name", ["small", "c2_small", "uneven"])
+def test_quad_stream_kernels_match_per_cell(name):
+    """The quad-cell stream kernels (x-extents multiple of 4) against the
+    per-cell ones: the per-cell arithmetic is the same, the block partial sums
+    run in another order, so the iterations agree within 1 and the solutions
+    within the solver tolerance."""
+    prob = _uneven_prob() if name == "uneven" else problems.config(name)
+    print("quad vs per-cell", name, prob.mesh, prob.sd, flush=True)
+    r1 = runner.run(prob, extra="-quad 1")
+    r0 = runner.run(prob, extra="-quad 0")
+    err = np.linalg.norm(r1["x"] - r0["x"]) / np.linalg.norm(r0["x"])
+    print(f"{name}: its {r1['its']} vs {r0['its']}, x rel diff {err:.2e}", flush=True)
+    assert r1["reason"] == "CONVERGED_RTOL" and r0["reason"] == "CONVERGED_RTOL"
+    assert abs(r1["its"] - r0["its"]) <= 1
+    assert err <= 1e-8
