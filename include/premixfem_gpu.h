@@ -318,6 +318,13 @@ int pmx_spe10_set_state(pmx_spe10 *sp, const double *S_w, const double *P);
 int pmx_spe10_run(pmx_spe10 *sp, pmx_ctx *ctx, int max_steps, const double *simu_time_stages,
                   const double *DS_max_stages, const int *steps_stages, double rtol, int max_it,
                   pmx_spe10_log *log, int *steps_done);
+/* Per-step output hook of pmx_spe10_run: called after the pressure solve
+ * and the bounds check, before the transport of each step (where the
+ * reference writes its log row and the S_w / P snapshots, :246-271), with
+ * the step's log row.  A nonzero return stops the loop with that code.
+ * cb == NULL removes the hook. */
+typedef int (*pmx_spe10_step_fn)(void *user, const pmx_spe10_log *row);
+int pmx_spe10_set_step_hook(pmx_spe10 *sp, pmx_spe10_step_fn cb, void *user);
 int pmx_spe10_destroy(pmx_spe10 *sp);
 
 /* Last error message of this thread (empty if none). */

@@ -34,7 +34,7 @@ EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_s
            "pmx_velocity_device", "pmx_spe10_create", "pmx_spe10_assemble",
            "pmx_spe10_get_system", "pmx_spe10_solve", "pmx_spe10_transport",
            "pmx_spe10_saturation_range", "pmx_spe10_get_state", "pmx_spe10_set_state",
-           "pmx_spe10_run", "pmx_spe10_destroy", "pmx_setup_batch")
+           "pmx_spe10_run", "pmx_spe10_destroy", "pmx_setup_batch", "pmx_spe10_set_step_hook")
 TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
                  "k_cg_update", "k_gemv", "coarse_small")
 
@@ -82,6 +82,9 @@ class Spe10Log(C.Structure):
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
+
+# Per-step hook of pmx_spe10_run: int (*)(void *user, const pmx_spe10_log *row).
+STEP_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(Spe10Log))
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -144,6 +147,7 @@ def lib():
     L.pmx_spe10_run.argtypes = [vp, vp, C.c_int, _dp, _dp, _ip, C.c_double, C.c_int,
                                 C.c_void_p, C.POINTER(C.c_int)]
     L.pmx_spe10_destroy.argtypes = [vp]
+    L.pmx_spe10_set_step_hook.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.pmx_last_error.restype = C.c_char_p
     for name in EXPORTS:
         if name != "pmx_last_error":

@@ -211,6 +211,8 @@ struct pmx_spe10 {
   hipStream_t stream = nullptr;
   DBuf<double> perm[3], poro, sw, p, kappa[3], wd, rhs, rate, bmax, scal, part;
   std::vector<double> hpart;
+  pmx_spe10_step_fn hook = nullptr;
+  void* hook_user = nullptr;
 };
 
 namespace {
@@ -429,6 +431,16 @@ int pmx_spe10_set_state(pmx_spe10* sp, const double* S_w, const double* P) {
   return PMX_SUCCESS;
 }
 
+int pmx_spe10_set_step_hook(pmx_spe10* sp, pmx_spe10_step_fn cb, void* user) {
+  if (!sp) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  sp->hook = cb;
+  sp->hook_user = user;
+  return PMX_SUCCESS;
+}
+
 int pmx_spe10_run(pmx_spe10* sp, pmx_ctx* ctx, int max_steps, const double* simu_time_stages,
                   const double* DS_max_stages, const int* steps_stages, double rtol, int max_it,
                   pmx_spe10_log* log, int* steps_done) {
@@ -472,6 +484,16 @@ int pmx_spe10_run(pmx_spe10* sp, pmx_ctx* ctx, int max_steps, const double* simu
       if (log) log[ind] = row;
       if (steps_done) *steps_done = ind + 1;
       return PMX_ERR_USER;
+    }
+    if (sp->hook) {
+      PMX_HIP(hipStreamSynchronize(sp->stream));
+      const int hr = sp->hook(sp->hook_user, &row);
+      if (hr != 0) {
+        set_error("IMPES step hook returned %d at step %d", hr, ind);
+        if (log) log[ind] = row;
+        if (steps_done) *steps_done = ind + 1;
+        return hr;
+      }
     }
     if (simu_time > simu_time_stages[kStages]) {
       if (log) log[ind] = row;

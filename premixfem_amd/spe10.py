@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import pc as P
-from ._lib import Spe10Log, check, lib
+from ._lib import STEP_HOOK, Spe10Log, check, lib
 
 # Stage tables of main (src/spe10.c:117-119).
 SIMU_TIME_STAGES = (0.0, 99.99, 199.99, 399.99, 899.99, 1599.99, 1999.99)
@@ -93,16 +93,58 @@ class Reservoir:
                                         None if b is None else b.ctypes.data_as(C.c_void_p)))
 
     def run(self, max_steps=20, rtol=1e-8, max_it=10000, simu_time_stages=SIMU_TIME_STAGES,
-            DS_max_stages=DS_MAX_STAGES, steps_stages=S_W_STEPS_STAGES):
+            DS_max_stages=DS_MAX_STAGES, steps_stages=S_W_STEPS_STAGES, output_dir=None,
+            save_files_freq=1, output_P=False, output_realtime=False):
         """The IMPES loop of main (src/spe10.c:197-274); returns the per-step
-        log rows as dicts."""
+        log rows as dicts.  With ``output_dir`` the driver's files are
+        written there as the loop runs (``P_solver.csv``, ``S_w<step>.vtr``
+        every ``save_files_freq`` steps, ``P<step>.vtr`` with ``output_P``,
+        the ``.vtr.series`` indexes); ``output_realtime`` prints each step's
+        line as the reference does (:251-253)."""
+        from . import io_vtk
+
         logs = (Spe10Log * max(1, max_steps))()
         done = C.c_int(0)
-        rc = lib().pmx_spe10_run(self.handle, self.ctx.handle, int(max_steps),
-                                 np.array(simu_time_stages, dtype=np.float64),
-                                 np.array(DS_max_stages, dtype=np.float64),
-                                 np.array(steps_stages, dtype=np.int32), rtol, max_it,
-                                 C.cast(logs, C.c_void_p), C.byref(done))
+        out = None
+        hook = None
+        if output_dir is not None or output_realtime:
+            if output_dir is not None:
+                out = io_vtk.ImpesOutput(output_dir, self.mesh, self.H, save_files_freq, output_P)
+            errors = []
+
+            def on_step(_user, row_p):
+                try:
+                    row = row_p.contents.as_dict()
+                    if output_realtime:
+                        wall = row["t_assemble"] + row["t_setup"] + row["t_solve"]
+                        print(f"{row['step']}, simu_time={row['simu_time']:.5E}, "
+                              f"stage_ind={row['stage']}, DS_max={row['DS_max']:.5f}, "
+                              f"S_w_steps={row['S_w_steps']}, wall_time={wall:.5f}, "
+                              f"iter_count={row['its']}, S_w=[{row['S_w_min']:.5E}, "
+                              f"{row['S_w_max']:.5E}].", flush=True)
+                    if out is not None:
+                        sw, p = self.state()
+                        out.step(row, sw, p)
+                    return 0
+                except Exception as e:  # surfaced after the loop
+                    errors.append(e)
+                    return 1
+
+            hook = STEP_HOOK(on_step)
+            check(lib().pmx_spe10_set_step_hook(self.handle, C.cast(hook, C.c_void_p), None))
+        try:
+            rc = lib().pmx_spe10_run(self.handle, self.ctx.handle, int(max_steps),
+                                     np.array(simu_time_stages, dtype=np.float64),
+                                     np.array(DS_max_stages, dtype=np.float64),
+                                     np.array(steps_stages, dtype=np.int32), rtol, max_it,
+                                     C.cast(logs, C.c_void_p), C.byref(done))
+        finally:
+            if hook is not None:
+                lib().pmx_spe10_set_step_hook(self.handle, None, None)
+            if out is not None:
+                out.close()
+        if hook is not None and errors:
+            raise errors[0]
         rows = [logs[i].as_dict() for i in range(done.value)]
         check(rc)
         return rows

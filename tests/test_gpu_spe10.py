@@ -214,3 +214,33 @@ def test_impes_loop_matches_oracle():
         t += dt
     assert np.linalg.norm(p - x) / np.linalg.norm(x) <= 1e-8
     np.testing.assert_allclose(sw, Sw.ravel(), rtol=1e-7, atol=1e-9)
+
+
+def test_impes_outputs_written_by_the_loop(tmp_path):
+    """The driver's files (src/spe10.c:203-274) from the device loop: the
+    CSV log has one row per step, the snapshots are taken after the solve
+    and before the transport of their step."""
+    from premixfem_amd import io_vtk
+
+    perm, poro = _fields()
+    print("impes outputs", flush=True)
+    res = spe10.Reservoir(MESH, perm, poro, SD, precision=32)
+    try:
+        rows = res.run(max_steps=3, rtol=1e-8, output_dir=str(tmp_path), save_files_freq=2,
+                       output_P=True)
+        _, p_final = res.state()
+    finally:
+        res.close()
+    assert len(rows) == 3
+    lines = open(tmp_path / "P_solver.csv").read().splitlines()
+    assert lines[0] == io_vtk.CSV_HEADER.strip() and len(lines) == 4
+    assert [int(l.split(",")[6]) for l in lines[1:]] == [r["its"] for r in rows]
+    s0, _, name = io_vtk.read_vtr(tmp_path / "S_w0.vtr")
+    assert name == "S_w" and s0.shape == (MESH[2], MESH[1], MESH[0])
+    np.testing.assert_array_equal(s0, 0.2)               # initial saturation
+    p2, _, _ = io_vtk.read_vtr(tmp_path / "P2.vtr")
+    np.testing.assert_array_equal(p2.ravel(), p_final)  # the last step's pressure
+    series = io_vtk.read_series(tmp_path / "S_w.vtr.series")
+    assert [f["name"] for f in series["files"]] == ["S_w0.vtr", "S_w2.vtr"]
+    # written with %.9f, as the reference does
+    assert series["files"][1]["time"] == pytest.approx(rows[2]["simu_time"], abs=1e-9)
