@@ -787,6 +787,197 @@ __global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __rest
   }
 }
 
+template <typename S>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+  using type = float4;
+};
+template <>
+struct Vec4T<double> {
+  using type = double4;
+};
+
+template <typename S>
+__device__ __forceinline__ void ld4v(const S* p, S (&o)[4]) {
+  const typename Vec4T<S>::type v = *reinterpret_cast<const typename Vec4T<S>::type*>(p);
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+  o[3] = v.w;
+}
+
+template <typename S>
+__device__ __forceinline__ void st4v(S* p, const S (&o)[4]) {
+  *reinterpret_cast<typename Vec4T<S>::type*>(p) = typename Vec4T<S>::type{o[0], o[1], o[2], o[3]};
+}
+
+// k_lv1_last for bricks of quads (x-extent a multiple of 4, at most 4096
+// cells, four quads per thread): the residual r = x - A y and the output
+// z = y + w work on quads of cells with y staged in LDS (interior neighbours
+// from there, face neighbours from the adjacent bricks) and x kept in
+// registers from the residual to the z'x partial; the hyperplane sweeps are
+// those of k_lv1_last.  Each cell's arithmetic is that of k_lv1_last; the
+// z'x partial is summed in another order.  LDS: 2 n values of S.
+template <typename S, typename TT, typename F4T>
+__global__ void __launch_bounds__(kBlock) k_lv1_last_q(Geo g, const double* __restrict__ x,
+                                                       const S* __restrict__ y,
+                                                       const TT* __restrict__ Tx,
+                                                       const TT* __restrict__ Ty,
+                                                       const TT* __restrict__ Tz,
+                                                       const F4T* __restrict__ fac_f,
+                                                       const F4T* __restrict__ fac_b,
+                                                       double* __restrict__ z,
+                                                       S* __restrict__ yout, CgCtl cg) {
+  PMX_STOP_CHECK(g.stop);
+  constexpr int QPT = 4;  // quads per thread (4096-cell bricks, 256 threads)
+  static_assert(QPT * 4 * kBlock >= 4 * kStreamBlock, "a quad brick must fit one pass");
+  extern __shared__ char shraw[];
+  const int sb = block_brick(g);
+  const Brick b = g.bricks[sb];
+  const BrickView v = load_view(g, sb);
+  S* lv = reinterpret_cast<S*>(shraw);
+  S* ysh = lv + v.n;
+  __shared__ int sps[kMaxPlanes + 1];
+  __shared__ double red[16];
+  load_planes(g, b, sps);
+  double xr[QPT][4];
+#pragma unroll
+  for (int u = 0; u < QPT; ++u) {
+    const int q0 = 4 * ((int)threadIdx.x + u * kBlock);
+    if (q0 < v.n) {
+      const double4 x4 = *reinterpret_cast<const double4*>(x + v.off + q0);
+      xr[u][0] = x4.x;
+      xr[u][1] = x4.y;
+      xr[u][2] = x4.z;
+      xr[u][3] = x4.w;
+      S yo[4];
+      ld4v(y + v.off + q0, yo);
+      st4v(ysh + q0, yo);
+    }
+  }
+  __syncthreads();
+  const TT* W = well_term<TT>(g);
+#pragma unroll
+  for (int u = 0; u < QPT; ++u) {
+    const int q0 = 4 * ((int)threadIdx.x + u * kBlock);
+    if (q0 >= v.n) continue;
+    const int c0 = v.off + q0;
+    const int lx0 = q0 % v.nx, l1 = q0 / v.nx;
+    const int ly = l1 % v.ny, lz = l1 / v.ny;
+    S yo[4], tx[4], ty[4], tz[4], tyh[4] = {}, tzh[4] = {}, wd[4] = {};
+    S yl[4] = {}, yh[4] = {}, zl[4] = {}, zh[4] = {};
+    ld4v(ysh + q0, yo);
+    ld4v(Tx + c0, tx);
+    ld4v(Ty + c0, ty);
+    ld4v(Tz + c0, tz);
+    if (W) ld4v(W + c0, wd);
+    // neighbour y: a missing one has T = 0 and takes the cell's own value,
+    // as in stencil_fast
+    S xl = yo[0], xh = yo[3], txr = S(0);
+    if (lx0 > 0) {
+      xl = ysh[q0 - 1];
+    } else if (v.noff[0] >= 0) {
+      xl = y[v.noff[0] + (lz * v.ny + ly) * v.next[0] + v.next[0] - 1];
+    }
+    if (lx0 + 4 < v.nx) {
+      xh = ysh[q0 + 4];
+      txr = Tx[c0 + 4];
+    } else if (v.noff[1] >= 0) {
+      const int h = v.noff[1] + (lz * v.ny + ly) * v.next[1];
+      xh = y[h];
+      txr = Tx[h];
+    }
+    bool hyl = true, hyh = true, hzl = true, hzh = true;
+    if (ly > 0)
+      ld4v(ysh + q0 - v.nx, yl);
+    else if (v.noff[2] >= 0)
+      ld4v(y + v.noff[2] + (lz * v.next[2] + v.next[2] - 1) * v.nx + lx0, yl);
+    else
+      hyl = false;
+    if (ly < v.ny - 1) {
+      ld4v(ysh + q0 + v.nx, yh);
+      ld4v(Ty + c0 + v.nx, tyh);
+    } else if (v.noff[3] >= 0) {
+      const int h = v.noff[3] + lz * v.next[3] * v.nx + lx0;
+      ld4v(y + h, yh);
+      ld4v(Ty + h, tyh);
+    } else {
+      hyh = false;
+    }
+    if (lz > 0)
+      ld4v(ysh + q0 - v.nxny, zl);
+    else if (v.noff[4] >= 0)
+      ld4v(y + v.noff[4] + ((v.next[4] - 1) * v.ny + ly) * v.nx + lx0, zl);
+    else
+      hzl = false;
+    if (lz < v.nz - 1) {
+      ld4v(ysh + q0 + v.nxny, zh);
+      ld4v(Tz + c0 + v.nxny, tzh);
+    } else if (v.noff[5] >= 0) {
+      const int h = v.noff[5] + ly * v.nx + lx0;
+      ld4v(y + h, zh);
+      ld4v(Tz + h, tzh);
+    } else {
+      hzh = false;
+    }
+    S r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const S xc = yo[j];
+      const S vxl = j > 0 ? yo[j - 1] : xl;
+      const S vxh = j < 3 ? yo[j + 1] : xh;
+      const S vyl = hyl ? yl[j] : xc, vyh = hyh ? yh[j] : xc;
+      const S vzl = hzl ? zl[j] : xc, vzh = hzh ? zh[j] : xc;
+      const S thx = j < 3 ? tx[j + 1] : txr;
+      S diag = S(0), a = S(0);
+      diag += tx[j] + thx;
+      a += tx[j] * vxl + thx * vxh;
+      diag += ty[j] + tyh[j];
+      a += ty[j] * vyl + tyh[j] * vyh;
+      diag += tz[j] + tzh[j];
+      a += tz[j] * vzl + tzh[j] * vzh;
+      if (c0 + j == g.pin) diag += (S)g.tau;
+      if (W) diag += wd[j];
+      const S ay = diag * xc - a;
+      r[j] = (S)xr[u][j] - ay;
+    }
+    st4v(lv + q0, r);
+  }
+  __syncthreads();
+  dic_sweeps<S, F4T, false>(g, b, v, sps, fac_f, fac_b, lv);
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < QPT; ++u) {
+    const int q0 = 4 * ((int)threadIdx.x + u * kBlock);
+    if (q0 >= v.n) continue;
+    const int c0 = v.off + q0;
+    S yo[4], w4[4], yn[4];
+    ld4v(ysh + q0, yo);
+    ld4v(lv + q0, w4);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) yn[j] = yo[j] + w4[j];
+    if (z) {
+      double4 z4;
+      z4.x = (double)yn[0];
+      z4.y = (double)yn[1];
+      z4.z = (double)yn[2];
+      z4.w = (double)yn[3];
+      *reinterpret_cast<double4*>(z + c0) = z4;
+      acc += z4.x * xr[u][0];
+      acc += z4.y * xr[u][1];
+      acc += z4.z * xr[u][2];
+      acc += z4.w * xr[u][3];
+    }
+    if (yout) st4v(yout + c0, yn);
+  }
+  if (cg.zr) {  // z'r partials into the slot of the iteration the counter names
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0)
+      cg.zr[(size_t)((int)cg.scal[kSIter] & 1) * cg.nparts + blockIdx.x] = acc;
+  }
+}
+
 // r = x - A y and c[dof+j] = sum_cells phi[cell][j] r[cell]  (the VecMDot of
 // _PC_apply_vec_lv2, src/PreMixFEM_3D.c:1564).  y == nullptr means y = 0.
 template <typename S, typename TT>
@@ -853,31 +1044,6 @@ __global__ void __launch_bounds__(kStreamBlock) k_prolong(Geo g, S* __restrict__
       if (j < ke) acc += ph[j] * (S)cs[j];
     y[c] = y[c] + acc;
   }
-}
-
-template <typename S>
-struct Vec4T;
-template <>
-struct Vec4T<float> {
-  using type = float4;
-};
-template <>
-struct Vec4T<double> {
-  using type = double4;
-};
-
-template <typename S>
-__device__ __forceinline__ void ld4v(const S* p, S (&o)[4]) {
-  const typename Vec4T<S>::type v = *reinterpret_cast<const typename Vec4T<S>::type*>(p);
-  o[0] = v.x;
-  o[1] = v.y;
-  o[2] = v.z;
-  o[3] = v.w;
-}
-
-template <typename S>
-__device__ __forceinline__ void st4v(S* p, const S (&o)[4]) {
-  *reinterpret_cast<typename Vec4T<S>::type*>(p) = typename Vec4T<S>::type{o[0], o[1], o[2], o[3]};
 }
 
 // Quad-cell k_resid_restrict (bricks with x-extent a multiple of 4, n <= 4
@@ -1301,6 +1467,17 @@ int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool 
   CgCtl cg = make_cg(ctx);
   if (!cg_partial) cg.zr = nullptr;
   const bool big = ctx->max_plane > kBlock;
+  if (ctx->pc_precision == 32 && !big && ctx->quad_ok && quad_threads(ctx) != 0) {
+    // quad bricks hold at most 4096 cells: four quads per thread
+    hipLaunchKernelGGL((k_lv1_last_q<float, float, float4>), dim3(ctx->nsub), dim3(kBlock),
+                       2 * brick_lds_bytes(ctx, sizeof(float)), ctx->stream, ctx->geo, x,
+                       ctx->w_y32.p, ctx->T32[0].p, ctx->T32[1].p, ctx->T32[2].p,
+                       (const float4*)ctx->fac32_f.p, (const float4*)ctx->fac32_b.p, z,
+                       keep_y ? ctx->w_y32b.p : nullptr, cg);
+    if (keep_y) std::swap(ctx->w_y32, ctx->w_y32b);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
   if (ctx->pc_precision == 32) {
     auto k = big ? k_lv1_last<float, float, float4, true> : k_lv1_last<float, float, float4, false>;
     hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), brick_lds_bytes(ctx, sizeof(float)),
