@@ -28,6 +28,14 @@
 namespace pmx {
 
 constexpr int kLobCh = 4;      // preconditioner columns per sweep pair
+// Threads per subdomain workgroup.  Large subdomains (>= kLobBigN cells) use
+// 512 threads: 8 waves share the CU's register budget (one workgroup per
+// CU), so a slow subdomain left running alone streams its blocks with twice
+// the waves (MB = 8 at 16^3: 23 % less time per iteration; MB = 10 spills a
+// few registers at this size and is still faster than with 256 threads).
+// Small subdomains are better served by more workgroups per CU.
+constexpr int kLobWaves = 8;   // most waves per workgroup (LDS reduction slots)
+constexpr int kLobBigN = 2048;
 constexpr int kStagIts = 40;   // iterations without progress before the stagnation test
 
 struct LobArgs {
@@ -389,8 +397,8 @@ __device__ inline int wg_svqb(double* G, double* G1, double* Z, double* Q1, doub
   return keep;
 }
 
-template <int MB>
-__global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
+template <int MB, int BS>
+__global__ void __launch_bounds__(BS) k_lobpcg(Geo g, LobArgs a) {
   constexpr int LD = 3 * MB;
   extern __shared__ double lds[];
   const int sidx = blockIdx.x * a.stride;
@@ -416,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
   double* TM = Q + LD * LD;
   double* w = TM + LD * LD;
   double* rot = w + LD;
-  double* red = rot + 3 * LD;  // 4 waves x MB*MB
+  double* red = rot + 3 * LD;  // kLobWaves x MB*MB
   float* pv = reinterpret_cast<float*>(lds);
   __shared__ double lam[MB], nrm[MB], msum_s;
   __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag;
@@ -777,15 +785,25 @@ __global__ void __launch_bounds__(kBlock) k_lobpcg(Geo g, LobArgs a) {
 template <int MB>
 static size_t lobpcg_lds(int maxn) {
   constexpr int LD = 3 * MB;
-  const size_t small = (5 * LD * LD + LD + 3 * LD + 4 * MB * MB + 64) * sizeof(double);
+  const size_t small = (5 * LD * LD + LD + 3 * LD + kLobWaves * MB * MB + 64) * sizeof(double);
   const size_t prec = (size_t)maxn * kLobCh * sizeof(float);
   return std::max(small, prec);
 }
 
 template <int MB>
+static void launch_lobpcg_geo(const Geo& g, int nblocks, hipStream_t st, const LobArgs& a,
+                              int maxn) {
+  if (maxn >= kLobBigN)
+    hipLaunchKernelGGL((k_lobpcg<MB, 512>), dim3(nblocks), dim3(512), lobpcg_lds<MB>(maxn), st, g,
+                       a);
+  else
+    hipLaunchKernelGGL((k_lobpcg<MB, 256>), dim3(nblocks), dim3(256), lobpcg_lds<MB>(maxn), st, g,
+                       a);
+}
+
+template <int MB>
 static void launch_lobpcg_mb(pmx_ctx* ctx, const LobArgs& a, int maxn) {
-  hipLaunchKernelGGL(k_lobpcg<MB>, dim3((ctx->nsub + a.stride - 1) / a.stride), dim3(kBlock),
-                     lobpcg_lds<MB>(maxn), ctx->stream, ctx->geo, a);
+  launch_lobpcg_geo<MB>(ctx->geo, (ctx->nsub + a.stride - 1) / a.stride, ctx->stream, a, maxn);
 }
 
 int lobpcg_block_size(int en_lv1, int guard) {
@@ -936,17 +954,20 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 // Block-size autotuner of the batched LOBPCG (-eig_autotune 1).  The block
 // size MB (wanted pairs + guard vectors) trades work per iteration against
 // iterations; the best value depends on the subdomain size, the eigen count
-// and the medium.  Every admissible MB in {4, 6, 8, 10} is timed from a
-// random start on the whole batch (a strided sample of 1024 subdomains on
+// and the medium.  Every admissible MB in {4, 6, 8, 10} is timed from the
+// start the setup will use (random, or the learned warm start) on the whole
+// batch (a strided sample of 1024 subdomains on
 // larger problems: the launch is timed at the occupancy it runs at); the
 // fastest sets -eig_guard, cached per (largest subdomain, en_lv1, well
-// term) for the process.  Tuning only changes how fast LOBPCG reaches the
+// term, warm start) for the process.  Tuning only changes how fast LOBPCG reaches the
 // tolerance.
 namespace {
 struct TuneKey {
-  int maxn, k, well;
+  int maxn, k, well, warm;
   bool operator<(const TuneKey& o) const {
-    return maxn != o.maxn ? maxn < o.maxn : (k != o.k ? k < o.k : well < o.well);
+    if (maxn != o.maxn) return maxn < o.maxn;
+    if (k != o.k) return k < o.k;
+    return well != o.well ? well < o.well : warm < o.warm;
   }
 };
 std::map<TuneKey, int>& tune_cache() {
@@ -962,7 +983,8 @@ int lobpcg_autotune(pmx_ctx* ctx) {
     minn = std::min(minn, b.nx * b.ny * b.nz);
   }
   const int k = ctx->en_lv1;
-  const TuneKey key{maxn, k, ctx->have_wd};
+  const bool warm = ctx->warm_start == 1;
+  const TuneKey key{maxn, k, ctx->have_wd, warm ? 1 : 0};
   auto it = tune_cache().find(key);
   if (it == tune_cache().end()) {
     const int need = ctx->have_wd ? k + 1 : k;   // at least one guard vector
@@ -972,7 +994,13 @@ int lobpcg_autotune(pmx_ctx* ctx) {
     for (int mb : {4, 6, 8, 10}) {
       if (mb < need || mb > minn - 1) continue;
       float ms = 0.0f;
-      PMX_TRY(lobpcg_run(ctx, nullptr, mb, stride, &ms));
+      // with the learned warm start the trials start from it, as the setup will
+      DBuf<double> x0;
+      if (warm) {
+        PMX_TRY(x0.alloc((size_t)ctx->ncell * mb));
+        PMX_TRY(cnn_warm_start(ctx, mb, x0.p));
+      }
+      PMX_TRY(lobpcg_run(ctx, warm ? x0.p : nullptr, mb, stride, &ms));
       if (!best || ms < best_ms) {
         best = mb;
         best_ms = ms;
@@ -1164,16 +1192,16 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
   a.debug = -1;
   switch (MB) {
     case 4:
-      hipLaunchKernelGGL(k_lobpcg<4>, dim3(S), dim3(kBlock), lobpcg_lds<4>(maxn), st, g, a);
+      launch_lobpcg_geo<4>(g, S, st, a, maxn);
       break;
     case 6:
-      hipLaunchKernelGGL(k_lobpcg<6>, dim3(S), dim3(kBlock), lobpcg_lds<6>(maxn), st, g, a);
+      launch_lobpcg_geo<6>(g, S, st, a, maxn);
       break;
     case 8:
-      hipLaunchKernelGGL(k_lobpcg<8>, dim3(S), dim3(kBlock), lobpcg_lds<8>(maxn), st, g, a);
+      launch_lobpcg_geo<8>(g, S, st, a, maxn);
       break;
     default:
-      hipLaunchKernelGGL(k_lobpcg<10>, dim3(S), dim3(kBlock), lobpcg_lds<10>(maxn), st, g, a);
+      launch_lobpcg_geo<10>(g, S, st, a, maxn);
       break;
   }
   PMX_HIP(hipGetLastError());
