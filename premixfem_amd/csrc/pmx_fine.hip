@@ -311,13 +311,22 @@ __device__ __forceinline__ void dic_sweep(const BrickView& v, int nplanes, const
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int k = k0 + d;
+      // The bounds of plane k + D are read from LDS before the cell update,
+      // so their latency overlaps that of the update's neighbour reads.
+      const int kn = k + D;
+      const int kk = kn < nplanes ? kn : nplanes - 1;
+      const int sn = FWD ? kk : nplanes - 1 - kk;
+      const int lo = sps[sn], hi = sps[sn + 1];
       if (ok[d]) dic_cell<S, F4T, FWD>(v, pk[d], f[d], lv);
       if (BIG && k < nplanes) {
         const int s = FWD ? k : nplanes - 1 - k;
         for (int pos = sps[s] + t + (int)blockDim.x; pos < sps[s + 1]; pos += blockDim.x)
           dic_cell<S, F4T, FWD>(v, pc[pos], fac[v.off + pos], lv);
       }
-      fetch(k + D, f[d], pk[d], ok[d]);
+      ok[d] = kn < nplanes && lo + t < hi;
+      const int pos = min(lo + t, hi - 1);
+      f[d] = fac[v.off + pos];
+      pk[d] = pc[pos];
       __syncthreads();
     }
   }
