@@ -12,6 +12,8 @@ and the outer PETSc CG that ``src/Expe_scal.c:100-119`` drives:
 * ``assemble_A``           -- ``PC_create_A`` (:1757-1824) (+ optional pin)
 * ``dic_factor/dic_solve`` -- level-1 block ICC(0) of ``_PC_setup_lv1_J`` (:80-222),
                               applied by ``_PC_apply_vec_lv1`` (:1429-1479)
+* ``ExactBlocks``          -- the same blocks factored exactly
+                              (``-use_full_Cholesky_lv1``, :205-207)
 * ``mass_diag``            -- the eigen mass matrices of ``_PC_setup_lv2_eigen`` (:265-343)
 * ``ghep``                 -- the per-subdomain generalized eigenproblems (:345-390)
 * ``Oracle.setup``         -- ``_PC_setup`` (:1392-1427): lv2 Galerkin (:737-916),
@@ -295,6 +297,32 @@ class DIC:
         return y
 
 
+class ExactBlocks:
+    """Level-1 blocks A[Omega_i, Omega_i] solved exactly
+    (``-use_full_Cholesky_lv1``: PCCHOLESKY with CHOLMOD on every block,
+    src/PreMixFEM_3D.c:205-207).  The blocks are those of :class:`DIC`: the
+    block diagonal ``A_diag`` and the couplings inside each subdomain; the
+    block-diagonal matrix is factored once by a sparse direct solver."""
+
+    def __init__(self, lay: Layout, T, A_diag):
+        dic = DIC(lay, T, A_diag)
+        n = len(A_diag)
+        idx = np.arange(n)
+        rows, cols, vals = [idx], [idx], [np.asarray(A_diag, dtype=np.float64)]
+        for c, s in ((dic.cxl, dic.sx), (dic.cyl, dic.sy), (dic.czl, dic.sz)):
+            m = c != 0.0
+            i = idx[m]
+            rows += [i, i - s]
+            cols += [i - s, i]
+            vals += [-c[m], -c[m]]
+        self.B = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                               shape=(n, n))
+        self._solve = spla.factorized(self.B)
+
+    def solve(self, r):
+        return self._solve(np.asarray(r, dtype=np.float64).ravel())
+
+
 # --------------------------------------------------------------------------
 # Level 2: local eigenproblems  (src/PreMixFEM_3D.c:224-421)
 # --------------------------------------------------------------------------
@@ -504,7 +532,7 @@ class Oracle:
     def __init__(self, dom, mesh, kappa, sd=2, en_lv1=4, en_lv2=4, eg_op=EIG_OP_MOD,
                  pin_cell=None, si_lv1=1, si_lv2=1, eb_lv1=-1.0, eb_lv2=-1.0,
                  use_W_cycle=False, no_shift_A_cc=False, use_2level=False, wd=None,
-                 lv2_icc=False):
+                 lv2_icc=False, use_full_Cholesky_lv1=False):
         if min(dom) <= 0.0:
             raise ValueError("dom must be positive (:1653)")
         if min(mesh) <= 0:
@@ -521,6 +549,7 @@ class Oracle:
         self.eb_lv1 = 1.0 / NIL if eb_lv1 < 0 else eb_lv1
         self.eb_lv2 = 1.0 / NIL if eb_lv2 < 0 else eb_lv2
         self.use_W_cycle, self.no_shift_A_cc, self.use_2level = use_W_cycle, no_shift_A_cc, use_2level
+        self.use_full_Cholesky_lv1 = use_full_Cholesky_lv1
         self.t = dict.fromkeys(STAGES, 0.0)
         self.T = transmissibilities(self.lay, self.kappa)
         self.pin = None
@@ -589,7 +618,7 @@ class Oracle:
         a_diag = lv1_block_diag(lay, self.T, self.pin)
         if self.wd is not None:
             a_diag = a_diag + self.wd
-        self.dic = DIC(lay, self.T, a_diag)
+        self.dic = (ExactBlocks if self.use_full_Cholesky_lv1 else DIC)(lay, self.T, a_diag)
         self.t["su_lv1"] += time.perf_counter() - ts
         # Level 2 eigen.
         ts = time.perf_counter()

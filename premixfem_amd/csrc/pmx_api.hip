@@ -487,6 +487,9 @@ static int setup_fine(pmx_ctx* c) {
   PMX_TRY(c->fac_f.alloc((size_t)c->ncell * 4));
   PMX_TRY(c->fac_b.alloc((size_t)c->ncell * 4));
   PMX_TRY(launch_dic_factor(c, 0, 0.0, c->fac_f.p, c->fac_b.p, false));
+  c->geo.l1L = c->geo.l1U = nullptr;
+  c->geo.l1off = nullptr;
+  if (c->use_full_Cholesky_lv1) PMX_TRY(chol_lv1_setup(c));
   if (c->pc_precision == 32) {
     PMX_TRY(c->fac32_f.alloc((size_t)c->ncell * 4));
     PMX_TRY(c->fac32_b.alloc((size_t)c->ncell * 4));
@@ -655,8 +658,8 @@ int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* opti
     set_error("-pc_precision must be 32 or 64");
     return fail(PMX_ERR_ARG_WRONG);
   }
-  if (c->use_full_Cholesky_lv1) {
-    set_error("-use_full_Cholesky_lv1 (exact level-1 blocks) is not implemented");
+  if (c->use_full_Cholesky_lv1 && c->pc_precision != 64) {
+    set_error("-use_full_Cholesky_lv1 needs -pc_precision 64");
     return fail(PMX_ERR_SUP);
   }
   c->si_lv1 = std::max(1, c->si_lv1);

@@ -153,6 +153,26 @@ __global__ void k_mass(Geo g, int op, const double* __restrict__ kx, const doubl
 // term) + sigma * mass (the LOBPCG preconditioner).
 // Writes plane-ordered {dinv, cx, cy, cz} for the lower and upper couplings.
 // ---------------------------------------------------------------------------
+// Diagonal of the level-1 block at cell c in the accumulation order of
+// _PC_setup_lv1_J (src/PreMixFEM_3D.c:93-197): lower faces and the upper
+// faces on the brick boundary in x, y, z order, then the interior upper
+// faces, then the pin and well terms.
+__device__ __forceinline__ double lv1_diag_order(const Geo& g, int c, const double lo[3],
+                                                 const double hi[3], const bool in_hi[3]) {
+  double a = lo[0];
+  if (!in_hi[0]) a = a + hi[0];
+  a = a + lo[1];
+  if (!in_hi[1]) a = a + hi[1];
+  a = a + lo[2];
+  if (!in_hi[2]) a = a + hi[2];
+  if (in_hi[0]) a = a + hi[0];
+  if (in_hi[1]) a = a + hi[1];
+  if (in_hi[2]) a = a + hi[2];
+  if (c == g.pin) a = a + g.tau;
+  if (g.wd) a = a + g.wd[c];
+  return a;
+}
+
 template <typename F4T>
 __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
                              const double* __restrict__ Ty, const double* __restrict__ Tz,
@@ -182,17 +202,7 @@ __global__ void k_dic_factor(Geo g, int mode, const double* __restrict__ Tx,
       }
       double a;
       if (mode == 0) {
-        a = lo[0];
-        if (!in_hi[0]) a = a + hi[0];
-        a = a + lo[1];
-        if (!in_hi[1]) a = a + hi[1];
-        a = a + lo[2];
-        if (!in_hi[2]) a = a + hi[2];
-        if (in_hi[0]) a = a + hi[0];
-        if (in_hi[1]) a = a + hi[1];
-        if (in_hi[2]) a = a + hi[2];
-        if (c == g.pin) a = a + g.tau;
-        if (g.wd) a = a + g.wd[c];
+        a = lv1_diag_order(g, c, lo, hi, in_hi);
       } else {
         a = 0.0;
         if (lx > 0) a += lo[0];
@@ -345,6 +355,90 @@ __device__ __forceinline__ void dic_sweeps(const Geo& g, const Brick& b, const B
   const int* pc = g.plane_cell + g.shape_coff[b.shape];
   dic_sweep<S, F4T, true, BIG>(v, b.nplanes, sps, pc, fac_f, lv);
   dic_sweep<S, F4T, false, BIG>(v, b.nplanes, sps, pc, fac_b, lv);
+}
+
+// ---------------------------------------------------------------------------
+// Exact level-1 blocks (-use_full_Cholesky_lv1, src/PreMixFEM_3D.c:205-207:
+// PCCHOLESKY instead of PCICC on every block A[Omega_i, Omega_i]).  In the
+// brick-local x-fastest order a block is banded with half-bandwidth
+// bw = nx*ny, and its Cholesky factor fills exactly that band.  The solve is
+// column-oriented: one workgroup barrier per column, the factor column read
+// one column ahead.
+// ---------------------------------------------------------------------------
+constexpr int kCholMaxBw = 1023;   // nx*ny of a brick (4 band entries per thread)
+
+// L y = r then L^T z = y on the brick vector lv (FP64); tmp: n doubles of LDS.
+// Every thread of the workgroup calls it; the result replaces lv.
+__device__ __forceinline__ void chol_solve_brick(const Geo& g, int sb, const BrickView& v,
+                                                 double* lv, double* tmp) {
+  const int n = v.n, bw = v.nxny, W = bw + 1;
+  const double* L = g.l1L + g.l1off[sb];
+  const double* U = g.l1U + g.l1off[sb];
+  const int t = threadIdx.x;
+  // forward: tmp[j] = y_j, lv[j+d] -= L(j+d, j) y_j
+  double lnext[4], pnext = L[0];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int d = 1 + t + q * (int)blockDim.x;
+    lnext[q] = d <= bw && d < n ? L[d] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    double lc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lc[q] = lnext[q];
+    const double piv = pnext;
+    if (j + 1 < n) {
+      const double* col = L + (size_t)(j + 1) * W;
+      pnext = col[0];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int d = 1 + t + q * (int)blockDim.x;
+        lnext[q] = d <= bw && j + 1 + d < n ? col[d] : 0.0;
+      }
+    }
+    const double y = lv[j] / piv;
+    if (t == 0) tmp[j] = y;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int d = 1 + t + q * (int)blockDim.x;
+      if (d <= bw && j + d < n) lv[j + d] -= lc[q] * y;
+    }
+    __syncthreads();
+  }
+  // backward: lv[j] = z_j, tmp[j-d] -= L(j, j-d) z_j
+  {
+    const double* row = U + (size_t)(n - 1) * W;
+    pnext = row[0];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int d = 1 + t + q * (int)blockDim.x;
+      lnext[q] = d <= bw && n - 1 - d >= 0 ? row[d] : 0.0;
+    }
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    double lc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lc[q] = lnext[q];
+    const double piv = pnext;
+    if (j > 0) {
+      const double* row = U + (size_t)(j - 1) * W;
+      pnext = row[0];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int d = 1 + t + q * (int)blockDim.x;
+        lnext[q] = d <= bw && j - 1 - d >= 0 ? row[d] : 0.0;
+      }
+    }
+    const double z = tmp[j] / piv;
+    if (t == 0) lv[j] = z;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int d = 1 + t + q * (int)blockDim.x;
+      if (d <= bw && j - d >= 0) tmp[j - d] -= lc[q] * z;
+    }
+    __syncthreads();
+  }
 }
 
 // (A x)_c for one cell: diag*x_c - sum_faces T x_nb, diag = sum of the
@@ -694,7 +788,7 @@ __global__ void __launch_bounds__(kStreamBlock) k_cg_p_spmv_q(
 // UPD: first fuses part 2 of PCG iteration i: dpi = p'w (sign-change check ->
 // INDEFINITE_MAT), a = beta/dpi, x += a p, r -= a w (r is the PC input x),
 // partial ||r||^2.
-template <typename S, typename F4T, bool UPD, bool BIG>
+template <typename S, typename F4T, bool UPD, bool BIG, bool CHOL = false>
 __global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, double* __restrict__ x,
                                                       S* __restrict__ y,
                                                       const F4T* __restrict__ fac_f,
@@ -747,13 +841,18 @@ __global__ void __launch_bounds__(kBlock) k_lv1_first(Geo g, double* __restrict_
     for (int li = threadIdx.x; li < v.n; li += blockDim.x) lv[li] = (S)x[v.off + li];
     __syncthreads();
   }
-  dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  if constexpr (CHOL) {
+    chol_solve_brick(g, sb, v, reinterpret_cast<double*>(lv),
+                     reinterpret_cast<double*>(lv) + v.n);
+  } else {
+    dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  }
   for (int li = threadIdx.x; li < v.n; li += blockDim.x) y[v.off + li] = lv[li];
 }
 
 // Final lv1 stage: r = x - A y, z = y + B1 r, and partial sums of z.x
 // (the PCG's beta = z'r, the PC input x being the PCG residual).
-template <typename S, typename TT, typename F4T, bool BIG>
+template <typename S, typename TT, typename F4T, bool BIG, bool CHOL = false>
 __global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __restrict__ x,
                                                      const S* __restrict__ y,
                                                      const TT* __restrict__ Tx,
@@ -777,7 +876,12 @@ __global__ void __launch_bounds__(kBlock) k_lv1_last(Geo g, const double* __rest
     lv[c - v.off] = (S)x[c] - ay;
   });
   __syncthreads();
-  dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  if constexpr (CHOL) {
+    chol_solve_brick(g, sb, v, reinterpret_cast<double*>(lv),
+                     reinterpret_cast<double*>(lv) + v.n);
+  } else {
+    dic_sweeps<S, F4T, BIG>(g, b, v, sps, fac_f, fac_b, lv);
+  }
   double acc = 0.0;
   for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
     const int c = v.off + li;
@@ -1427,6 +1531,133 @@ int launch_mass(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
+// Band Cholesky of the level-1 block of brick blockIdx.x, left-looking by
+// columns: column j of L from A(j:j+bw, j) minus the products with the
+// previous bw columns (row j of L staged in LDS), then scaled by the pivot.
+// Entries of A: the block diagonal (lv1_diag_order) and -T of the faces
+// inside the brick.  *fail = 1 + j on a non-positive pivot.
+__global__ void __launch_bounds__(kBlock) k_chol_factor(Geo g, const double* __restrict__ Tx,
+                                                        const double* __restrict__ Ty,
+                                                        const double* __restrict__ Tz,
+                                                        double* __restrict__ Lb,
+                                                        int* __restrict__ fail) {
+  __shared__ double lrow[kCholMaxBw + 1];
+  __shared__ double s_piv;
+  const int sb = blockIdx.x;
+  const Brick b = g.bricks[sb];
+  const int n = b.nx * b.ny * b.nz, bw = b.nx * b.ny, W = bw + 1;
+  double* L = Lb + g.l1off[sb];
+  const double* TT[3] = {Tx, Ty, Tz};
+  for (int j = 0; j < n; ++j) {
+    // row j of L: lrow[q] = L(j, j - bw + q), 0 before the first column
+    for (int q = threadIdx.x; q < bw; q += blockDim.x) {
+      const int k = j - bw + q;
+      lrow[q] = k >= 0 ? L[(size_t)k * W + (j - k)] : 0.0;
+    }
+    __syncthreads();
+    const int lx = j % b.nx, ly = (j / b.nx) % b.ny, lz = j / bw;
+    double val[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int d = threadIdx.x + q * (int)blockDim.x;
+      val[q] = 0.0;
+      if (d > bw || j + d >= n) continue;
+      double a = 0.0;
+      if (d == 0) {
+        const int c = b.off + j;
+        double lo[3], hi[3];
+        bool in_hi[3];
+        const int lc[3] = {lx, ly, lz};
+        const int ln[3] = {b.nx, b.ny, b.nz};
+        for (int e = 0; e < 3; ++e) {
+          lo[e] = TT[e][c];
+          const int nh = nbr_index(g, b, c, lx, ly, lz, 2 * e + 1);
+          hi[e] = nh >= 0 ? TT[e][nh] : 0.0;
+          in_hi[e] = lc[e] < ln[e] - 1;
+        }
+        a = lv1_diag_order(g, c, lo, hi, in_hi);
+      } else {
+        // the upper neighbours of cell j inside the brick (the lower face of
+        // the neighbour is their shared face)
+        if (d == 1 && lx + 1 < b.nx) a -= Tx[b.off + j + 1];
+        if (d == b.nx && ly + 1 < b.ny) a -= Ty[b.off + j + b.nx];
+        if (d == bw && lz + 1 < b.nz) a -= Tz[b.off + j + bw];
+      }
+      const int i = j + d;
+      const int k0 = max(0, i - bw);
+      double sacc = 0.0;
+      for (int k = k0; k < j; ++k) sacc += L[(size_t)k * W + (i - k)] * lrow[k - (j - bw)];
+      val[q] = a - sacc;
+    }
+    if (threadIdx.x == 0) {
+      const double p = val[0];
+      if (!(p > 0.0) && *fail == 0) *fail = 1 + j;
+      s_piv = p > 0.0 ? sqrt(p) : 1.0;
+    }
+    __syncthreads();
+    const double piv = s_piv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int d = threadIdx.x + q * (int)blockDim.x;
+      if (d > bw) continue;
+      L[(size_t)j * W + d] = j + d >= n ? 0.0 : (d == 0 ? piv : val[q] / piv);
+    }
+    __syncthreads();
+  }
+}
+
+// Row layout of the factor: U[j*W + d] = L(j, j - d) = L[(j-d)*W + d].
+__global__ void k_chol_rows(Geo g, const double* __restrict__ Lb, double* __restrict__ Ub) {
+  const int sb = blockIdx.x;
+  const Brick b = g.bricks[sb];
+  const int n = b.nx * b.ny * b.nz, bw = b.nx * b.ny, W = bw + 1;
+  const double* L = Lb + g.l1off[sb];
+  double* U = Ub + g.l1off[sb];
+  for (size_t e = threadIdx.x; e < (size_t)n * W; e += blockDim.x) {
+    const int j = (int)(e / W), d = (int)(e % W);
+    U[e] = j - d >= 0 ? L[(size_t)(j - d) * W + d] : 0.0;
+  }
+}
+
+int chol_lv1_setup(pmx_ctx* ctx) {
+  std::vector<long long> off(ctx->nsub + 1, 0);
+  for (int s = 0; s < ctx->nsub; ++s) {
+    const Brick& b = ctx->h_bricks[s];
+    const long long bw = (long long)b.nx * b.ny;
+    if (bw > kCholMaxBw) {
+      set_error("-use_full_Cholesky_lv1: subdomain x-y faces of %lld cells exceed %d", bw,
+                kCholMaxBw);
+      return PMX_ERR_SUP;
+    }
+    off[s + 1] = off[s] + (long long)b.nx * b.ny * b.nz * (bw + 1);
+  }
+  PMX_TRY(ctx->l1off.alloc(ctx->nsub + 1));
+  PMX_HIP(hipMemcpy(ctx->l1off.p, off.data(), off.size() * sizeof(long long),
+                    hipMemcpyHostToDevice));
+  PMX_TRY(ctx->l1L.alloc((size_t)off[ctx->nsub]));
+  PMX_TRY(ctx->l1U.alloc((size_t)off[ctx->nsub]));
+  ctx->geo.l1off = ctx->l1off.p;
+  ctx->geo.l1L = ctx->l1L.p;
+  ctx->geo.l1U = ctx->l1U.p;
+  DBuf<int> fail;
+  PMX_TRY(fail.alloc(1));
+  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_chol_factor, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                     ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->l1L.p, fail.p);
+  hipLaunchKernelGGL(k_chol_rows, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                     ctx->l1L.p, ctx->l1U.p);
+  PMX_HIP(hipGetLastError());
+  int h = 0;
+  PMX_HIP(hipMemcpyAsync(&h, fail.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  if (h) {
+    set_error("-use_full_Cholesky_lv1: a level-1 block is not positive definite (column %d)",
+              h - 1);
+    return PMX_ERR_USER;
+  }
+  return PMX_SUCCESS;
+}
+
 int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* fac_b,
                       bool f32) {
   const size_t lds = brick_lds_bytes(ctx, sizeof(double));
@@ -1448,6 +1679,15 @@ int launch_dic_factor(pmx_ctx* ctx, int mode, double sigma, void* fac_f, void* f
 int launch_lv1_first(pmx_ctx* ctx, double* x, bool upd, double* xs, const double* p,
                      const double* w) {
   const CgCtl cg = make_cg(ctx);
+  if (ctx->use_full_Cholesky_lv1) {
+    auto k = upd ? k_lv1_first<double, double4, true, false, true>
+                 : k_lv1_first<double, double4, false, false, true>;
+    hipLaunchKernelGGL(k, dim3(ctx->nsub), dim3(kBlock), 2 * brick_lds_bytes(ctx, sizeof(double)),
+                       ctx->stream, ctx->geo, x, ctx->w_y.p, (const double4*)ctx->fac_f.p,
+                       (const double4*)ctx->fac_b.p, xs, p, w, cg);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
   const bool big = ctx->max_plane > kBlock;
   if (ctx->pc_precision == 32) {
     auto k = upd ? (big ? k_lv1_first<float, float4, true, true>
@@ -1476,6 +1716,16 @@ int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool 
   CgCtl cg = make_cg(ctx);
   if (!cg_partial) cg.zr = nullptr;
   const bool big = ctx->max_plane > kBlock;
+  if (ctx->use_full_Cholesky_lv1) {
+    hipLaunchKernelGGL((k_lv1_last<double, double, double4, false, true>), dim3(ctx->nsub),
+                       dim3(kBlock), 2 * brick_lds_bytes(ctx, sizeof(double)), ctx->stream,
+                       ctx->geo, x, ctx->w_y.p, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p,
+                       (const double4*)ctx->fac_f.p, (const double4*)ctx->fac_b.p, z,
+                       keep_y ? ctx->w_yb.p : nullptr, cg);
+    if (keep_y) std::swap(ctx->w_y, ctx->w_yb);
+    PMX_HIP(hipGetLastError());
+    return PMX_SUCCESS;
+  }
   if (ctx->pc_precision == 32 && !big && ctx->quad_ok && quad_threads(ctx) != 0) {
     // quad bricks hold at most 4096 cells: four quads per thread
     hipLaunchKernelGGL((k_lv1_last_q<float, float, float4>), dim3(ctx->nsub), dim3(kBlock),

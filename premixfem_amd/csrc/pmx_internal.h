@@ -87,6 +87,12 @@ struct Geo {
   const double* stop;     // PCG "done" flag: kernels return at once when set (nullptr: none)
   const double* wd;       // per-cell diagonal (well) term, brick order; nullptr: none
   const float* wd32;      // its FP32 copy (FP32 preconditioner)
+  // Exact level-1 factors (-use_full_Cholesky_lv1): per brick the band
+  // Cholesky factor by columns (l1L[off + j*(bw+1) + d] = L(j+d, j)) and by
+  // rows (l1U[off + j*(bw+1) + d] = L(j, j-d)), bw = nx*ny; nullptr: none.
+  const double* l1L;
+  const double* l1U;
+  const long long* l1off;
 };
 
 // The well term of the operator diagonal in the precision of the face
@@ -337,6 +343,8 @@ struct pmx_ctx {
 
   // ---- level 1 ----
   pmx::DBuf<double> fac_f, fac_b;   // plane-major {dinv, cx, cy, cz} lower / upper couplings
+  pmx::DBuf<double> l1L, l1U;       // exact level-1 band factors (-use_full_Cholesky_lv1)
+  pmx::DBuf<long long> l1off;       // per brick offset into l1L / l1U
   pmx::DBuf<float> fac32_f, fac32_b;
 
   // ---- level 2 ----

@@ -23,6 +23,7 @@ int launch_lv1_first(pmx_ctx* ctx, double* x, bool upd, double* xs, const double
                      const double* w);
 int stream_threads(const pmx_ctx* ctx);
 int quad_threads(const pmx_ctx* ctx);
+int chol_lv1_setup(pmx_ctx* ctx);
 int launch_lv1_last(pmx_ctx* ctx, const double* x, double* z, bool keep_y, bool cg_partial);
 int launch_resid_restrict(pmx_ctx* ctx, const double* x, bool y_zero, double* cvec);
 int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter);

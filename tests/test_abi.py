@@ -52,7 +52,7 @@ def _init(dom, mesh, options):
     ((1.0, 1.0, 1.0), (8, 8, 8), "-eg_op 3", 62),    # :1685
     ((1.0, 1.0, 1.0), (8, 8, 8), "-sd 9", 62),       # more subdomains than cells
     ((1.0, 1.0, 1.0), (8, 8, 8), "-pc_precision 16", 62),
-    ((1.0, 1.0, 1.0), (8, 8, 8), "-use_full_Cholesky_lv1", 56),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-use_full_Cholesky_lv1 -pc_precision 32", 56),  # FP64 only
     ((1.0, 1.0, 1.0), (8, 8, 8), "-pin 8,0,0", 63),
     ((1.0, 1.0, 1.0), (8, 8, 8), "-no_such_option 1", 62),
 ])

@@ -28,6 +28,10 @@ VARIANTS = [
     # exact level-2 factor in band storage (the large-coarse-space path)
     ("band", dict(), "-coarse_band 1"),
     ("band-W", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle"),
+    # exact level-1 blocks (band Cholesky per brick)
+    ("full-Cholesky-lv1", dict(use_full_Cholesky_lv1=True), "-use_full_Cholesky_lv1"),
+    ("full-Cholesky-lv1-W", dict(use_full_Cholesky_lv1=True, use_W_cycle=True),
+     "-use_full_Cholesky_lv1 -use_W_cycle"),
 ]
 
 

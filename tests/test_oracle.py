@@ -265,3 +265,33 @@ def test_bad_arguments():
         O.Oracle((1, 1, 1), (4, 4, 4), np.ones((3, 4, 4, 4)), en_lv1=0)
     with pytest.raises(ValueError):
         O.Oracle((1, 1, 1), (4, 4, 4), np.ones((3, 4, 4, 4)), eg_op=7)
+
+
+def test_exact_level1_blocks_are_the_principal_submatrices():
+    """-use_full_Cholesky_lv1 (src/PreMixFEM_3D.c:205-207): the oracle's
+    exact level-1 matrix is A with every coupling between different
+    subdomains removed (diagonal kept), and its solve is exact."""
+    prob = problems.config("small")
+    orc = O.Oracle(prob.dom, prob.mesh, prob.kappa, sd=prob.sd, en_lv1=prob.en_lv1,
+                   en_lv2=prob.en_lv2, pin_cell=prob.pin_cell, use_full_Cholesky_lv1=True)
+    orc.setup()
+    B = orc.dic.B.toarray()
+    A = orc.A.toarray()
+    lay = orc.lay
+    sub = lay.cell_sub().ravel()
+    same = sub[:, None] == sub[None, :]
+    off = same & ~np.eye(B.shape[0], dtype=bool)
+    np.testing.assert_array_equal(B[off], A[off])
+    # the diagonal in the reference's accumulation order (lv1_block_diag)
+    np.testing.assert_allclose(np.diag(B), np.diag(A), rtol=1e-14)
+    assert not B[~same].any()
+    r = np.random.default_rng(0).standard_normal(B.shape[0])
+    y = orc.dic.solve(r)
+    assert np.linalg.norm(B @ y - r) <= 1e-12 * np.linalg.norm(r)
+    # exact blocks precondition at least as well as ICC(0)
+    ref = O.Oracle(prob.dom, prob.mesh, prob.kappa, sd=prob.sd, en_lv1=prob.en_lv1,
+                   en_lv2=prob.en_lv2, pin_cell=prob.pin_cell).setup()
+    b = prob.rhs.ravel()
+    _, its_exact, _, reason = O.pcg(orc.matvec, b, orc.apply, rtol=1e-8)
+    _, its_icc, _, _ = O.pcg(ref.matvec, b, ref.apply, rtol=1e-8)
+    assert reason == "CONVERGED_RTOL" and its_exact <= its_icc
