@@ -489,11 +489,15 @@ def batch_main(args):
 
 def solve_roofline(prob, r, s, workload, head):
     """Per-kernel durations of a kernel-timed solve and the roofline of the
-    dominant kernel (most total time among those with a byte model)."""
+    dominant kernel (most total time among those with a byte model).  The
+    coarse slot ("k_gemv") has a byte model only on the dense composite
+    path (one GEMV); on the band / stage-wise / ICC(0) paths it times the
+    whole coarse program and is reported without one."""
     from premixfem_amd._lib import TIMED_KERNELS
 
     N = int(np.prod(prob.mesh))
     nc = r["stats"]["size_coarse"]
+    composite = r["stats"].get("coarse_mode", 0) == 0
     per_kernel = {}
     for i, name in enumerate(TIMED_KERNELS):
         cnt = float(r["kernel_count"][i])
@@ -501,6 +505,9 @@ def solve_roofline(prob, r, s, workload, head):
             continue
         avg_ms = r["kernel_ms"][i] / cnt
         by = kernel_bytes(name, N, prob.en_lv1, s, nc)
+        if name == "k_gemv" and not composite:
+            by = None
+            name = "coarse_program"
         per_kernel[name] = dict(launches_per_step=float(cnt), avg_us=avg_ms * 1e3,
                                 bytes_per_launch=by,
                                 gbs=(by / (avg_ms * 1e-3) / 1e9) if by else None)

@@ -89,6 +89,9 @@ typedef struct {
   int eig_stagnated;         /* subdomains accepted at the eigensolver's stagnation test
                                 (worst residual within -eig_stag x -eig_tol) */
   int eig_block;             /* LOBPCG block size used (chosen by -eig_autotune 1) */
+  int coarse_mode;           /* level-2/3 path of the apply: 0 dense composite operator
+                                (one GEMV), 1 dense stage by stage, 2 band Cholesky,
+                                3 ICC(0) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),

@@ -58,7 +58,7 @@ class Stats(C.Structure):
         ("eig_iters_mean", C.c_double), ("eig_fallbacks", C.c_int), ("lv3_iters", C.c_int),
         ("pc_precision", C.c_int), ("its", C.c_int), ("rnorm", C.c_double),
         ("rnorm0", C.c_double), ("reason", C.c_int), ("t_solve", C.c_double),
-        ("eig_stagnated", C.c_int), ("eig_block", C.c_int),
+        ("eig_stagnated", C.c_int), ("eig_block", C.c_int), ("coarse_mode", C.c_int),
     ]
 
     def as_dict(self):

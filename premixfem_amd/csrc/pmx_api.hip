@@ -1277,6 +1277,7 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_fallbacks = c->eig_fallbacks;
   st->eig_stagnated = c->eig_stagnated;
   st->eig_block = c->eig_block;
+  st->coarse_mode = c->lv2_icc ? 3 : (c->use_band ? 2 : (c->coarse_composite ? 0 : 1));
   st->lv3_iters = c->lv3_iters;
   st->pc_precision = c->pc_precision;
   st->its = c->last_its;
