@@ -538,6 +538,9 @@ def main():
                          "(a separate JSON line; --steps IMPES steps); batch: 64 independent "
                          "problems of {32..96}^3 cells set up by one batched call")
     ap.add_argument("--batch-count", type=int, default=64, help="batch: number of problems")
+    ap.add_argument("--train-fields", type=int, default=4,
+                    help="learned mode: seeded training fields of the workload's family")
+    ap.add_argument("--train-epochs", type=int, default=30, help="learned mode: training epochs")
     ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -564,7 +567,8 @@ def main():
         net = None
         extra = "-eig_autotune 1"
         if m == "learned":
-            net, training = train_warm_start(prob)
+            net, training = train_warm_start(prob, nfields=args.train_fields,
+                                             epochs=args.train_epochs)
             print(f"# warm-start network: {training}", file=sys.stderr, flush=True)
             extra += " -warm_start cnn"
         r = run_mode(prob, prec, args.steps, args.warmup, extra, net)
