@@ -33,9 +33,12 @@ constexpr int kLobCh = 4;      // preconditioner columns per sweep pair
 // CU), so a slow subdomain left running alone streams its blocks with twice
 // the waves (MB = 8 at 16^3: 23 % less time per iteration; MB = 10 spills a
 // few registers at this size and is still faster than with 256 threads).
-// Small subdomains are better served by more workgroups per CU.
+// Small subdomains (<= kLobSmallN cells) use 128 threads: twice the
+// subdomains in flight per CU (8^3 batch: 38 % less time; 64 threads were
+// slower again).
 constexpr int kLobWaves = 8;   // most waves per workgroup (LDS reduction slots)
 constexpr int kLobBigN = 2048;
+constexpr int kLobSmallN = 512;   // at most this many cells: 128-thread workgroups
 constexpr int kStagIts = 40;   // iterations without progress before the stagnation test
 
 struct LobArgs {
@@ -795,6 +798,9 @@ static void launch_lobpcg_geo(const Geo& g, int nblocks, hipStream_t st, const L
                               int maxn) {
   if (maxn >= kLobBigN)
     hipLaunchKernelGGL((k_lobpcg<MB, 512>), dim3(nblocks), dim3(512), lobpcg_lds<MB>(maxn), st, g,
+                       a);
+  else if (maxn <= kLobSmallN)
+    hipLaunchKernelGGL((k_lobpcg<MB, 128>), dim3(nblocks), dim3(128), lobpcg_lds<MB>(maxn), st, g,
                        a);
   else
     hipLaunchKernelGGL((k_lobpcg<MB, 256>), dim3(nblocks), dim3(256), lobpcg_lds<MB>(maxn), st, g,
