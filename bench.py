@@ -370,7 +370,7 @@ def batch_main(args):
     for m in args.modes.split(","):
         prec = 64 if m == "fp64" else 32
         net = None
-        extra = "-eig_autotune 1"
+        extra = "-eig_autotune 1 " + args.batch_extra
         if m == "learned":
             net, training = train_warm_start(problems.training_problem("batch", 0),
                                              kind="batch")
@@ -474,7 +474,7 @@ def batch_main(args):
         "config": {"workload": f"{len(probs)} independent problems, cubes of "
                                "{32,48,64,80,96}^3 cells (seed 5000+i), 8^3-cell subdomains, "
                                f"6 eigenvectors; {ncell} cells, {nsub} subdomains in total",
-                   "headline_mode": head},
+                   "headline_mode": head, "member_options": args.batch_extra},
         "modes": {m: {k: v for k, v in r.items() if k != "its"} | {"its_sum": sum(r["its"])}
                   for m, r in modes.items()},
         "its": hr["its"],
@@ -538,6 +538,10 @@ def main():
                          "(a separate JSON line; --steps IMPES steps); batch: 64 independent "
                          "problems of {32..96}^3 cells set up by one batched call")
     ap.add_argument("--batch-count", type=int, default=64, help="batch: number of problems")
+    ap.add_argument("--batch-extra", default="-coarse_composite 0",
+                    help="batch: extra preconditioner options for every member (default: the "
+                         "coarse stages applied one by one; composing them costs 2 nc^3 flops "
+                         "per member, which members solved in <= 20 iterations never recover)")
     ap.add_argument("--train-fields", type=int, default=4,
                     help="learned mode: seeded training fields of the workload's family")
     ap.add_argument("--train-epochs", type=int, default=30, help="learned mode: training epochs")
