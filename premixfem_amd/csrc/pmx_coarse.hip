@@ -174,97 +174,93 @@ int launch_galerkin(pmx_ctx* ctx) {
 }
 
 // ---------------------------------------------------------------------------
-// Dense SPD inversion by the block sweep operator: for each 64-wide pivot
+// Dense SPD inversion by the block sweep operator: for each 128-wide pivot
 // block k with D = A_kk^{-1}:  A_ij -= A_ik D A_kj (i,j != k),
 // A_ik <- A_ik D, A_kj <- D A_kj, A_kk <- -D.  After all blocks A = -A^{-1}.
 // Pivots are Schur complements of an SPD matrix, so no pivoting is needed.
 // ---------------------------------------------------------------------------
-constexpr int kSw = 64;
+constexpr int kSw = 128;   // pivot block
+constexpr int kSwT = 64;   // trailing-update tile (pivot blocks are whole tiles)
 
-// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to 64):
-// thread t holds row t/4, columns 16 (t%4) .. +15 in registers; one barrier
-// per elimination step (the row / column broadcast buffers alternate).
-__global__ void __launch_bounds__(256) k_sweep_pivot(const double* __restrict__ A, int n, int k0,
-                                                     int bs, double* __restrict__ D) {
+// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to 128):
+// 512 threads, thread t holds row t/4, columns 32 (t%4) .. +31 in registers;
+// one barrier per elimination step (the row / column broadcast buffers
+// alternate).  The unsymmetrised inverse goes through Dt (global scratch) to
+// be symmetrised.
+constexpr int kPivThreads = 512;
+__global__ void __launch_bounds__(kPivThreads) k_sweep_pivot(const double* __restrict__ A, int n,
+                                                             int k0, int bs,
+                                                             double* __restrict__ D,
+                                                             double* __restrict__ Dt) {
   __shared__ double rowv[2][kSw], colv[2][kSw];
-  __shared__ double M[kSw][kSw + 1];
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  double m[16];
+  double m[32];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) {
-    const int j = q * 16 + c;
+  for (int c = 0; c < 32; ++c) {
+    const int j = q * 32 + c;
     m[c] = (r < bs && j < bs) ? A[(size_t)(k0 + r) * n + k0 + j] : (r == j ? 1.0 : 0.0);
   }
   for (int tq = 0; tq < 4; ++tq) {
 #pragma unroll
-    for (int tc = 0; tc < 16; ++tc) {
-      const int t = tq * 16 + tc, par = t & 1;
+    for (int tc = 0; tc < 32; ++tc) {
+      const int t = tq * 32 + tc, par = t & 1;
       if (q == tq) colv[par][r] = m[tc];
       if (r == t) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) rowv[par][q * 16 + c] = m[c];
+        for (int c = 0; c < 32; ++c) rowv[par][q * 32 + c] = m[c];
       }
       __syncthreads();
       const double piv = 1.0 / rowv[par][t];
       const double f = colv[par][r];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        const int j = q * 16 + c;
+      for (int c = 0; c < 32; ++c) {
+        const int j = q * 32 + c;
         const double rt = (j == t ? 1.0 : rowv[par][j]) * piv;
         m[c] = r == t ? rt : (j == t ? 0.0 : m[c]) - f * rt;
       }
     }
   }
 #pragma unroll
-  for (int c = 0; c < 16; ++c) M[r][q * 16 + c] = m[c];
+  for (int c = 0; c < 32; ++c) Dt[r * kSw + q * 32 + c] = m[c];
   __syncthreads();
   for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
     const int i = e / kSw, j = e % kSw;
     // Symmetrise the pivot inverse.
-    D[e] = (i < bs && j < bs) ? 0.5 * (M[i][j] + M[j][i]) : 0.0;
+    D[e] = (i < bs && j < bs) ? 0.5 * (Dt[i * kSw + j] + Dt[j * kSw + i]) : 0.0;
   }
 }
 
-// P[r][t] = A[r][k0+t]; U[r][t] = sum_s A[r][k0+s] D[s][t].  4 rows per block.
+// P[r][t] = A[r][k0+t] for t < bs, 0 beyond.
 __global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int bs,
-                              const double* __restrict__ D, double* __restrict__ P,
-                              double* __restrict__ U) {
-  __shared__ double Ds[kSw][kSw + 1];
-  __shared__ double Ar[4][kSw];
-  for (int e = threadIdx.x; e < bs * bs; e += blockDim.x) Ds[e / bs][e % bs] = D[(e / bs) * kSw + e % bs];
-  const int r0 = blockIdx.x * 4;
-  const int rr = threadIdx.x / kSw, t = threadIdx.x % kSw;
-  const int r = r0 + rr;
-  if (r < n && t < bs) Ar[rr][t] = A[(size_t)r * n + k0 + t];
-  __syncthreads();
-  if (r < n && t < bs) {
-    double s = 0.0;
-    for (int q = 0; q < bs; ++q) s += Ar[rr][q] * Ds[q][t];
-    P[(size_t)r * kSw + t] = Ar[rr][t];
-    U[(size_t)r * kSw + t] = s;
+                              double* __restrict__ P) {
+  const size_t tot = (size_t)n * kSw;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / kSw), t = (int)(e % kSw);
+    P[e] = t < bs ? A[(size_t)r * n + k0 + t] : 0.0;
   }
 }
 
-// Trailing update A_ij -= U_i P_j^T on 64x64 tiles (FP64 MFMA); the tiles of
-// the pivot block row / column take U, U^T and -D.
+// Trailing update A_ij -= U_i P_j^T on 64x64 tiles (FP64 MFMA, K = bs); the
+// tiles of the pivot block rows / columns take U, U^T and -D.
 __global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, int n, int k0,
                                                       int bs, const double* __restrict__ P,
                                                       const double* __restrict__ U,
                                                       const double* __restrict__ D) {
   using TL = MfmaTile<2, 2, 2, 2>;
-  static_assert(TL::TM == kSw && TL::TN == kSw, "sweep tiles are pivot blocks");
+  static_assert(TL::TM == kSwT && TL::TN == kSwT && kSw % kSwT == 0,
+                "pivot blocks are whole tiles");
   __shared__ double lds[TL::LDS_DOUBLES];
-  const int I0 = blockIdx.y * kSw, J0 = blockIdx.x * kSw;
-  const int kb = k0 / kSw;
-  const int ti = blockIdx.y, tj = blockIdx.x;
-  if (ti == kb || tj == kb) {
-    for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
-      const int i = I0 + e / kSw, j = J0 + e % kSw;
+  const int I0 = blockIdx.y * kSwT, J0 = blockIdx.x * kSwT;
+  const bool prow = I0 >= k0 && I0 < k0 + kSw, pcol = J0 >= k0 && J0 < k0 + kSw;
+  if (prow || pcol) {
+    for (int e = threadIdx.x; e < kSwT * kSwT; e += blockDim.x) {
+      const int i = I0 + e / kSwT, j = J0 + e % kSwT;
       if (i >= n || j >= n) continue;
       double v;
-      if (ti == kb && tj == kb)
+      if (prow && pcol)
         v = -D[(i - k0) * kSw + (j - k0)];
-      else if (ti == kb)
+      else if (prow)
         v = U[(size_t)j * kSw + (i - k0)];
       else
         v = U[(size_t)i * kSw + (j - k0)];
@@ -305,32 +301,33 @@ __global__ void k_copy_shift(const double* __restrict__ A, double* __restrict__ 
 
 int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift) {
   const int n = ctx->nc;
-  DBuf<double> D, P, U;
+  DBuf<double> D, Dt, P, U;
   PMX_TRY(D.alloc(kSw * kSw));
+  PMX_TRY(Dt.alloc(kSw * kSw));
   PMX_TRY(P.alloc((size_t)n * kSw));
   PMX_TRY(U.alloc((size_t)n * kSw));
   const int nblk = 1024;
   hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, A, Ainv, n, shift,
                      1.0);
   PMX_HIP(hipGetLastError());
-  const int nt = (n + kSw - 1) / kSw;
+  const int nt = (n + kSwT - 1) / kSwT;
+  const int npb = std::min((n * kSw + 255) / 256, 2048);
   for (int k0 = 0; k0 < n; k0 += kSw) {
     const int bs = std::min(kSw, n - k0);
-    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kBlock), 0, ctx->stream, Ainv, n, k0, bs,
-                       D.p);
-    hipLaunchKernelGGL(k_sweep_panel, dim3((n + 3) / 4), dim3(kBlock), 0, ctx->stream, Ainv, n,
-                       k0, bs, D.p, P.p, U.p);
-    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(kBlock), 0, ctx->stream, Ainv, n, k0,
-                       bs, P.p, U.p, D.p);
+    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kPivThreads), 0, ctx->stream, Ainv, n, k0,
+                       bs, D.p, Dt.p);
+    hipLaunchKernelGGL(k_sweep_panel, dim3(npb), dim3(256), 0, ctx->stream, Ainv, n, k0, bs, P.p);
+    // U = P D (the pivot columns times the pivot inverse)
+    hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 4, false>), dim3(1, (n + 63) / 64), dim3(256), 0,
+                       ctx->stream, n, kSw, kSw, 1.0, P.p, kSw, D.p, kSw, 0.0, U.p, kSw);
+    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(256), 0, ctx->stream, Ainv, n, k0, bs,
+                       P.p, U.p, D.p);
     PMX_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, Ainv, Ainv, n, 0.0,
                      -1.0);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipStreamSynchronize(ctx->stream));
-  D.release();
-  P.release();
-  U.release();
   return PMX_SUCCESS;
 }
 
