@@ -447,7 +447,7 @@ def batch_main(args):
     # stream).
     big = max(probs, key=lambda p: p.ncell)
     prec = 64 if head == "fp64" else 32
-    r_big = run_mode(big, prec, 1, 0, "-eig_autotune 1")
+    r_big = run_mode(big, prec, 1, 0, "-eig_autotune 1 " + args.batch_extra)
     per_kernel, roofline = solve_roofline(big, r_big, 8 if head == "fp64" else 4,
                                           "batch-largest", head)
     cpu = None

@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(kPivThreads) k_sweep_pivot(const double* __res
                                                              int k0, int bs,
                                                              double* __restrict__ D,
                                                              double* __restrict__ Dt) {
-  __shared__ double rowv[2][kSw], colv[2][kSw];
+  __shared__ double2 rowv2[2][kSw / 2];
+  __shared__ double colv[2][kSw];
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
   double m[32];
 #pragma unroll
@@ -207,16 +208,20 @@ __global__ void __launch_bounds__(kPivThreads) k_sweep_pivot(const double* __res
       if (q == tq) colv[par][r] = m[tc];
       if (r == t) {
 #pragma unroll
-        for (int c = 0; c < 32; ++c) rowv[par][q * 32 + c] = m[c];
+        for (int c = 0; c < 32; c += 2) rowv2[par][q * 16 + c / 2] = double2{m[c], m[c + 1]};
       }
       __syncthreads();
-      const double piv = 1.0 / rowv[par][t];
+      const double* rowv = reinterpret_cast<const double*>(rowv2[par]);
+      const double piv = 1.0 / rowv[t];
       const double f = colv[par][r];
 #pragma unroll
-      for (int c = 0; c < 32; ++c) {
+      for (int c = 0; c < 32; c += 2) {
+        const double2 rv = rowv2[par][q * 16 + c / 2];
         const int j = q * 32 + c;
-        const double rt = (j == t ? 1.0 : rowv[par][j]) * piv;
-        m[c] = r == t ? rt : (j == t ? 0.0 : m[c]) - f * rt;
+        const double rt0 = (j == t ? 1.0 : rv.x) * piv;
+        const double rt1 = (j + 1 == t ? 1.0 : rv.y) * piv;
+        m[c] = r == t ? rt0 : (j == t ? 0.0 : m[c]) - f * rt0;
+        m[c + 1] = r == t ? rt1 : (j + 1 == t ? 0.0 : m[c + 1]) - f * rt1;
       }
     }
   }
