@@ -112,7 +112,13 @@ typedef struct {
  * operator, default, or apply them one by one), -coarse_deflate 0|1 (dense
  * path: invert A_c deflated with a Woodbury correction, default, or
  * directly), -quad 0|1 (quad-cell stream kernels where every brick's
- * x-extent is a multiple of 4, default 1).  Allocates the
+ * x-extent is a multiple of 4, default 1), -procs_virtual px,py,pz (the
+ * preconditioner of px*py*pz PETSc processes on one GPU: process boxes by
+ * PETSc's default DMDA ownership (src/PreMixFEM_3D.c:1711), -sd subdomains
+ * per axis inside every box (:58-72), level 2 coupled inside a box only
+ * (:806-888), en_lv2 level-3 vectors per box from its box-local Neumann
+ * coarse matrix (:918-1175), A_cc over all boxes (:1177-1390); dense coarse
+ * path, at most 64 level-3 vectors in total).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 

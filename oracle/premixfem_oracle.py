@@ -19,6 +19,10 @@ and the outer PETSc CG that ``src/Expe_scal.c:100-119`` drives:
 * ``Oracle.setup``         -- ``_PC_setup`` (:1392-1427): lv2 Galerkin (:737-916),
                               lv3 eigen (:918-1175), lv3 coarse-coarse (:1177-1390)
 * ``Oracle.apply``         -- ``PC_apply_vec`` (:1840-1936)
+* ``Layout(procs=...)``    -- process boxes: DMDA ownership (:1711), subdomains per
+                              box (:58-72); ``Oracle._lv3_boxes`` and
+                              ``energy_gram(inner=True)`` -- the per-process
+                              level 2 (:806-888) and level 3 (:918-1390)
 * ``pcg``                  -- PETSc ``KSPSolve_CG`` with ``KSP_NORM_UNPRECONDITIONED``
                               (``src/Expe_scal.c:114``) and the default convergence test.
 
@@ -70,24 +74,52 @@ def dof_idx(eigen_num):
     return out
 
 
-class Layout:
-    """Grid geometry, partition and per-cell local coordinates."""
+def dmda_ownership(n: int, p: int):
+    """PETSc's default DMDA ownership of n cells over p processes along one
+    axis (DMSetUp_DA_3D: l_j = n/p + (n%p > j)), as (starts, lens)."""
+    return partition(n, p)
 
-    def __init__(self, dom, mesh, sd):
+
+class Layout:
+    """Grid geometry, partition and per-cell local coordinates.
+
+    ``procs`` = (px, py, pz) process boxes (the DMDA decomposition of
+    ``src/PreMixFEM_3D.c:1711``): each box is cut into ``sd`` subdomains per
+    axis by ``_PC_setup_lv1_partition`` (:47-78) from its own corners, so the
+    global subdomain grid has px*sdx x py*sdy x pz*sdz entries.  ``self.sd``
+    holds those global counts; ``self.sd_box`` the per-box ones."""
+
+    def __init__(self, dom, mesh, sd, procs=(1, 1, 1)):
         self.dom = tuple(float(v) for v in dom)
         self.mesh = tuple(int(v) for v in mesh)
         if isinstance(sd, int):
             sd = (sd, sd, sd)
         # The reference resets sd == 1 to 2 (:1718-1721).
-        self.sd = tuple(2 if s == 1 else int(s) for s in sd)
+        sd_box = tuple(2 if s == 1 else int(s) for s in sd)
+        self.procs = tuple(int(p) for p in procs)
+        self.sd_box = sd_box
+        self.sd = tuple(p * s for p, s in zip(self.procs, sd_box))
         M, N, P = self.mesh
         self.Hx, self.Hy, self.Hz = (self.dom[0] / M, self.dom[1] / N, self.dom[2] / P)
         self.meas_elem = self.Hx * self.Hy * self.Hz
         self.meas_face_yz = self.Hy * self.Hz
         self.meas_face_zx = self.Hz * self.Hx
         self.meas_face_xy = self.Hx * self.Hy
-        self.parts = [partition(n, s) for n, s in zip(self.mesh, self.sd)]
+        self.parts = []
+        self.box_parts = []
+        for n, p, s in zip(self.mesh, self.procs, sd_box):
+            bst, bln = dmda_ownership(n, p)
+            if min(bln) < s:
+                raise ValueError("a process box has fewer cells than subdomains")
+            starts, lens = [], []
+            for b0, bl in zip(bst, bln):
+                st, ln = partition(bl, s)
+                starts += [b0 + v for v in st]
+                lens += ln
+            self.parts.append((starts, lens))
+            self.box_parts.append((bst, bln))
         self.nsub = self.sd[0] * self.sd[1] * self.sd[2]
+        self.nbox = self.procs[0] * self.procs[1] * self.procs[2]
         # Per-axis subdomain index and local coordinate of each cell.
         self.ax_sub, self.ax_loc = [], []
         for (starts, lens), n in zip(self.parts, self.mesh):
@@ -110,6 +142,17 @@ class Layout:
         """Subdomain id of every cell, shape (P, N, M)."""
         Z, Y, X = np.meshgrid(self.ax_sub[2], self.ax_sub[1], self.ax_sub[0], indexing="ij")
         return X + self.sd[0] * (Y + self.sd[1] * Z)
+
+    def sub_box(self, s):
+        """Process box (x-fastest rank) holding subdomain s."""
+        sdx, sdy, _ = self.sd
+        sx, sy, sz = s % sdx, (s // sdx) % sdy, s // (sdx * sdy)
+        bx, by, bz = sx // self.sd_box[0], sy // self.sd_box[1], sz // self.sd_box[2]
+        return bx + self.procs[0] * (by + self.procs[1] * bz)
+
+    def axis_box(self, a):
+        """Process-box index of every coordinate along axis a."""
+        return self.ax_sub[a] // self.sd_box[a]
 
 
 # --------------------------------------------------------------------------
@@ -532,7 +575,7 @@ class Oracle:
     def __init__(self, dom, mesh, kappa, sd=2, en_lv1=4, en_lv2=4, eg_op=EIG_OP_MOD,
                  pin_cell=None, si_lv1=1, si_lv2=1, eb_lv1=-1.0, eb_lv2=-1.0,
                  use_W_cycle=False, no_shift_A_cc=False, use_2level=False, wd=None,
-                 lv2_icc=False, use_full_Cholesky_lv1=False):
+                 lv2_icc=False, use_full_Cholesky_lv1=False, procs=(1, 1, 1)):
         if min(dom) <= 0.0:
             raise ValueError("dom must be positive (:1653)")
         if min(mesh) <= 0:
@@ -541,7 +584,9 @@ class Oracle:
             raise ValueError("eigen counts must be >= 1 (:1681,:1685)")
         if eg_op not in (EIG_OP_MOD, EIG_OP_MEAN, EIG_OP_DIAG):
             raise ValueError("eg_op (:1690)")
-        self.lay = Layout(dom, mesh, sd)
+        self.lay = Layout(dom, mesh, sd, procs)
+        if self.lay.nbox > 1 and (use_2level or lv2_icc):
+            raise ValueError("process boxes: the 2-level and ICC(0) variants are not covered")
         self.kappa = np.asarray(kappa, dtype=np.float64)
         self.en_lv1, self.en_lv2, self.eg_op = int(en_lv1), int(en_lv2), int(eg_op)
         self.pin_cell = pin_cell
@@ -566,28 +611,39 @@ class Oracle:
             self.A = (self.A + sp.diags(self.wd)).tocsr()
         self.lv2_icc = bool(lv2_icc)
 
-    def energy_gram(self, Phi):
+    def energy_gram(self, Phi, inner=False):
         """Phi^T A Phi in the reference's energy form: every face contributes
         T (phi_L - phi_R)(phi_L - phi_R)^T, computed from the differences of
         the basis values (:737-890, :1177-1360), never from A Phi.  For a face
         between two subdomains the difference row holds phi_L and -phi_R in
         different columns, which gives the reference's T phi phi^T diagonal
         and -T phi phi'^T neighbour blocks.  The pin and well terms add
-        d phi phi^T.  Avoids the cancellation of A Phi near the null space."""
+        d phi phi^T.  Avoids the cancellation of A Phi near the null space.
+
+        inner=True drops the faces between process boxes altogether: the
+        box-local Neumann matrix A_i_inner of ``_PC_setup_lv3_eigen``
+        (``ex != proc_startx`` ..., :939-1073)."""
         M, N, P = self.lay.mesh
         n = M * N * P
         idx = np.arange(n).reshape(P, N, M)
+        bx, by, bz = (self.lay.axis_box(a) for a in range(3))
+        keep = (np.broadcast_to(bx[:-1] == bx[1:], (P, N, M - 1)),
+                np.broadcast_to((by[:-1] == by[1:])[:, None], (P, N - 1, M)),
+                np.broadcast_to((bz[:-1] == bz[1:])[:, None, None], (P - 1, N, M)))
         rows, cols, vals, tf = [], [], [], []
         nf = 0
-        for Tf, a, b in ((self.T[0], idx[:, :, :-1], idx[:, :, 1:]),
-                         (self.T[1], idx[:, :-1, :], idx[:, 1:, :]),
-                         (self.T[2], idx[:-1, :, :], idx[1:, :, :])):
-            a, b = a.ravel(), b.ravel()
+        for Tf, a, b, kf in ((self.T[0], idx[:, :, :-1], idx[:, :, 1:], keep[0]),
+                             (self.T[1], idx[:, :-1, :], idx[:, 1:, :], keep[1]),
+                             (self.T[2], idx[:-1, :, :], idx[1:, :, :], keep[2])):
+            a, b, Tf = a.ravel(), b.ravel(), Tf.ravel()
+            if inner:
+                kf = kf.ravel()
+                a, b, Tf = a[kf], b[kf], Tf[kf]
             f = nf + np.arange(a.size)
             rows += [f, f]
             cols += [a, b]
             vals += [np.ones(a.size), -np.ones(a.size)]
-            tf.append(Tf.ravel())
+            tf.append(Tf)
             nf += a.size
         inc = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                             shape=(nf, n))
@@ -662,6 +718,14 @@ class Oracle:
         self.Phi = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                                  shape=(M * N * P, nc))
         A_c = self.energy_gram(self.Phi).tocsr()
+        self.dof_box = np.repeat([lay.sub_box(s) for s in range(lay.nsub)], self.eigen_num_lv1)
+        if lay.nbox > 1:
+            # _PC_setup_lv2_J (:806-888) couples only the subdomains of one
+            # process: the blocks between boxes are dropped, the faces
+            # between boxes stay on the diagonal blocks.
+            A_c = A_c.tocoo()
+            same = self.dof_box[A_c.row] == self.dof_box[A_c.col]
+            A_c = sp.csr_matrix((A_c.data[same], (A_c.row[same], A_c.col[same])), shape=A_c.shape)
         self.A_c_sp = (0.5 * (A_c + A_c.T)).tocsr()
         self.A_c = self.A_c_sp.toarray() if nc <= DENSE_COARSE_MAX else None
         if self.lv2_icc:
@@ -684,6 +748,17 @@ class Oracle:
             self.t["su_lv2_2"] += time.perf_counter() - ts
         # Level 3 eigen (:918-1175): smallest en_lv2 of the coarse matrix.
         ts = time.perf_counter()
+        if lay.nbox > 1:
+            self._lv3_boxes(nc)
+            self.t["su_lv3_1"] += time.perf_counter() - ts
+            ts = time.perf_counter()
+            self.A_cc = self.energy_gram(sp.csr_matrix(self.Phi_cc)).toarray()
+            if not self.no_shift_A_cc:
+                self.A_cc = self.A_cc + NIL * np.eye(self.A_cc.shape[0])
+            self.lu_cc = sla.lu_factor(self.A_cc)
+            self.t["su_lv3_2"] += time.perf_counter() - ts
+            self.t["su"] += time.perf_counter() - t0
+            return self
         k2 = min(self.en_lv2, nc)
         if self.A_c is not None:
             lam2, V2 = sla.eigh(self.A_c, subset_by_index=[0, k2 - 1])
@@ -712,6 +787,38 @@ class Oracle:
         self.t["su_lv3_2"] += time.perf_counter() - ts
         self.t["su"] += time.perf_counter() - t0
         return self
+
+    def _lv3_boxes(self, nc):
+        """Level 3 with process boxes (:918-1175): every box b takes the
+        smallest en_lv2 eigenpairs of its box-local Neumann matrix A_i_inner
+        (faces between boxes dropped), counted against eb_lv2 per box
+        (:1100-1115); its Phi_cc vectors Phi_b v_j live on box b only.  The
+        columns are ordered by box (x-fastest rank), as the ranks own the
+        rows of A_cc (:1196-1199)."""
+        A_in = self.energy_gram(self.Phi, inner=True).toarray()
+        self.A_inner = 0.5 * (A_in + A_in.T)
+        cols, self.eig_lv2_box, self.eigen_num_lv2_box = [], [], []
+        for b in range(self.lay.nbox):
+            d = np.flatnonzero(self.dof_box == b)
+            k2 = min(self.en_lv2, d.size)
+            lam, V = sla.eigh(self.A_inner[np.ix_(d, d)], subset_by_index=[0, k2 - 1])
+            num = k2
+            for j in range(k2):
+                if lam[j] >= self.eb_lv2:
+                    num = j + 1
+                    break
+            full = np.zeros((nc, num))
+            full[d, :] = V[:, :num]
+            cols.append(full)
+            self.eig_lv2_box.append(lam)
+            self.eigen_num_lv2_box.append(num)
+        self.V2 = sign_normalize(np.hstack(cols))
+        self.eigen_num_lv2 = self.V2.shape[1]
+        self.eigen_min_lv2 = min(lam[0] for lam in self.eig_lv2_box)
+        self.eigen_max_lv2 = max(lam[n - 1] for lam, n in zip(self.eig_lv2_box,
+                                                              self.eigen_num_lv2_box))
+        self.eig_lv2 = np.concatenate(self.eig_lv2_box)
+        self.Phi_cc = np.asarray(self.Phi @ self.V2)
 
     # ---- apply ----------------------------------------------------------
     def apply_lv1(self, r):

@@ -38,6 +38,45 @@ static void partition(int n, int sd, std::vector<int>& st, std::vector<int>& ln)
   for (int i = 1; i < sd; ++i) st[i] = st[i - 1] + ln[i - 1];
 }
 
+// With p process boxes along an axis: PETSc's default DMDA ownership
+// (l_j = n/p + (n%p > j)) and, inside every box, the partition above from
+// the box's own corner (src/PreMixFEM_3D.c:58-72), concatenated.
+static void partition_boxes(int n, int p, int sd, std::vector<int>& st, std::vector<int>& ln) {
+  std::vector<int> bst, bln, s1, l1;
+  partition(n, p, bst, bln);
+  st.clear();
+  ln.clear();
+  for (int j = 0; j < p; ++j) {
+    partition(bln[j], sd, s1, l1);
+    for (int i = 0; i < sd; ++i) {
+      st.push_back(bst[j] + s1[i]);
+      ln.push_back(l1[i]);
+    }
+  }
+}
+
+// "-procs_virtual px,py,pz" (or one count for all axes).
+static int parse_procs(const std::string& v, int* pv) {
+  std::vector<std::string> items;
+  size_t pos = 0;
+  for (;;) {
+    const size_t comma = v.find(',', pos);
+    items.push_back(v.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos));
+    if (comma == std::string::npos) break;
+    pos = comma + 1;
+  }
+  if (items.size() != 1 && items.size() != 3) return PMX_ERR_ARG_WRONG;
+  int vals[3];
+  for (size_t q = 0; q < items.size(); ++q) {
+    char* end = nullptr;
+    const long x = strtol(items[q].c_str(), &end, 10);
+    if (items[q].empty() || *end) return PMX_ERR_ARG_WRONG;
+    vals[q] = (int)x;
+  }
+  for (int a = 0; a < 3; ++a) pv[a] = items.size() == 1 ? vals[0] : vals[a];
+  return PMX_SUCCESS;
+}
+
 static int parse_options(pmx_ctx* c, const char* options) {
   if (!options) return PMX_SUCCESS;
   std::istringstream is(options);
@@ -109,6 +148,12 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_2level = 1;
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
+    } else if (k == "-procs_virtual") {
+      if (!need(i) || parse_procs(tok[i + 1], c->pv)) {
+        set_error("-procs_virtual needs px,py,pz");
+        return PMX_ERR_ARG_WRONG;
+      }
+      ++i;
     } else if (k == "-eig_autotune") {
       PMX_TRY(geti(c->eig_autotune));
     } else if (k == "-quad") {
@@ -183,7 +228,7 @@ static int build_geometry(pmx_ctx* c) {
   const int M = c->mesh[0], N = c->mesh[1], P = c->mesh[2];
   c->ncell = M * N * P;
   for (int a = 0; a < 3; ++a) {
-    partition(c->mesh[a], c->sd[a], c->h_starts[a], c->h_lens[a]);
+    partition_boxes(c->mesh[a], c->pv[a], c->sd_box[a], c->h_starts[a], c->h_lens[a]);
     c->h_ax_sub[a].assign(c->mesh[a], 0);
     c->h_ax_loc[a].assign(c->mesh[a], 0);
     for (int i = 0; i < c->sd[a]; ++i)
@@ -222,6 +267,7 @@ static int build_geometry(pmx_ctx* c) {
         b.nb[5] = sz < sdz - 1 ? s + sdx * sdy : -1;
         b.nplanes = b.nx + b.ny + b.nz - 2;
         b.lid = s;
+        b.box = sx / c->sd_box[0] + c->pv[0] * (sy / c->sd_box[1] + c->pv[1] * (sz / c->sd_box[2]));
         b.dof = 0;
         b.neig = 0;
         const auto key = std::make_tuple(b.nx, b.ny, b.nz);
@@ -388,8 +434,12 @@ static int setup_coarse(pmx_ctx* c) {
   // level-2 factor: the dense inverse, or the block ICC(0) of the spe10 fork.
   PMX_TRY(c->Acb.alloc((size_t)c->nsub * 7 * c->en_lv1 * c->en_lv1));
   for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
-  c->use_band = !c->lv2_icc && !c->use_2level &&
+  c->use_band = !c->lv2_icc && !c->use_2level && c->nbox == 1 &&
                  (c->coarse_band == 1 || (c->coarse_band < 0 && c->nc > kBandAutoNc));
+  // With process boxes A_c keeps only the couplings inside a box, and every
+  // box block is nonsingular through its faces to the other boxes: no
+  // near-null direction to deflate.
+  if (c->nbox > 1) c->coarse_deflate = 0;
   if (c->lv2_icc || c->use_band) {
     for (DBuf<double>* b : {&c->Ac, &c->Ainv, &c->Fop}) b->release();
     PMX_TRY(launch_galerkin(c));
@@ -427,7 +477,8 @@ static int setup_coarse(pmx_ctx* c) {
   double t1 = now();
   c->t_stages[PMX_STAGE_SU_LV2_2] += t1 - t0;
   if (!c->use_2level) {
-    PMX_TRY((c->lv2_icc || c->use_band) ? level3_setup_icc(c) : level3_setup(c));
+    PMX_TRY((c->lv2_icc || c->use_band) ? level3_setup_icc(c)
+                                        : (c->nbox > 1 ? level3_setup_boxes(c) : level3_setup(c)));
     double t2 = now();
     c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
   }
@@ -633,10 +684,25 @@ int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* opti
       return fail(PMX_ERR_ARG_WRONG);
     }
     if (c->sd[a] == 1) c->sd[a] = 2;  // src/PreMixFEM_3D.c:1718-1721
-    if (c->sd[a] > c->mesh[a]) {
-      set_error("sub_domains=%d exceeds the %d cells of axis %d", c->sd[a], c->mesh[a], a);
+    if (c->pv[a] < 1 || c->pv[a] > c->mesh[a]) {
+      set_error("-procs_virtual: %d process boxes on the %d cells of axis %d", c->pv[a],
+                c->mesh[a], a);
       return fail(PMX_ERR_ARG_WRONG);
     }
+    // sub_domains counts per process box; the smallest box has n/p cells.
+    if (c->sd[a] > c->mesh[a] / c->pv[a]) {
+      set_error("sub_domains=%d exceeds the %d cells of axis %d", c->sd[a],
+                c->mesh[a] / c->pv[a], a);
+      return fail(PMX_ERR_ARG_WRONG);
+    }
+    c->sd_box[a] = c->sd[a];
+    c->sd[a] *= c->pv[a];
+  }
+  c->nbox = c->pv[0] * c->pv[1] * c->pv[2];
+  if (c->nbox > 1 && (c->use_2level || c->lv2_icc || c->coarse_band == 1)) {
+    set_error("-procs_virtual: the 2-level, ICC(0) level-2 and band variants are not built for "
+              "process boxes");
+    return fail(PMX_ERR_SUP);
   }
   if (c->en_lv1 < 1) {
     set_error("Error in max_eigen_num_lv1=%d for the level-1 problem.", c->en_lv1);
@@ -1295,10 +1361,18 @@ int pmx_print_stat(const pmx_ctx* c) {
          "\teigen_num_lv1_range=[%d, %d].\n",
          s.eigen_max_lv1_range[0], s.eigen_max_lv1_range[1], s.eigen_min_lv1_range[0],
          s.eigen_min_lv1_range[1], s.eigen_num_lv1_range[0], s.eigen_num_lv1_range[1]);
+  // With process boxes the count is per box (PC_get_range over the ranks).
+  int num_lo = s.eigen_num_lv2, num_hi = s.eigen_num_lv2;
+  if (!c->h_eigen_num_lv2_box.empty() && c->nbox > 1) {
+    const auto mm = std::minmax_element(c->h_eigen_num_lv2_box.begin(),
+                                        c->h_eigen_num_lv2_box.end());
+    num_lo = *mm.first;
+    num_hi = *mm.second;
+  }
   printf("eigen_max_lv2_range=[%.5E, %.5E], \teigen_min_lv2_range=[%.5E, %.5E], "
          "\teigen_num_lv2_range=[%d, %d], \tsize_mat_lv3=%d.\n",
-         s.eigen_max_lv2, s.eigen_max_lv2, s.eigen_min_lv2, s.eigen_min_lv2, s.eigen_num_lv2,
-         s.eigen_num_lv2, s.size_mat_lv3);
+         s.eigen_max_lv2, s.eigen_max_lv2, s.eigen_min_lv2, s.eigen_min_lv2, num_lo, num_hi,
+         s.size_mat_lv3);
   const double* t = s.t_stages;
   printf("su_lv1=[%.5f, %.5f], \tsu_lv2_1=[%.5f, %.5f], \tsu_lv2_2=[%.5f, %.5f], "
          "\tsu_lv3_1=[%.5f, %.5f], \tsu_lv3_2=[%.5f, %.5f], \tsu=[%.5f, %.5f].\n",

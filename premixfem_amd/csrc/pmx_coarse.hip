@@ -21,17 +21,25 @@ namespace pmx {
 // Output: dense nc x nc (row-major) and block rows Acb[s][slot][kmax][kmax]
 // with slot 0 = self, 1..6 = -x,+x,-y,+y,-z,+z.
 // ---------------------------------------------------------------------------
+//
+// Process boxes (_PC_setup_lv2_J, :806-888): the dense A_c, which level 2
+// inverts, keeps only the couplings between subdomains of one box, while the
+// block rows Acb keep all of them -- they are Phi^T A Phi, which the coarse
+// residual updates between stages stand for.  inner = 1 builds the box-local
+// Neumann matrix A_i_inner of level 3 (:939-1073) into Ac instead: faces
+// between boxes are dropped from the diagonal blocks too, and Acb is not
+// written (pass nullptr).
 __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
                            const double* __restrict__ Tx, const double* __restrict__ Ty,
                            const double* __restrict__ Tz, int nc, double* __restrict__ Ac,
-                           double* __restrict__ Acb) {
+                           double* __restrict__ Acb, int inner) {
   __shared__ double red[8];
   const int s = blockIdx.x;
   const Brick b = g.bricks[s];
   const int n = b.nx * b.ny * b.nz;
   const int ke = b.neig;
   const double* TT3[3] = {Tx, Ty, Tz};
-  double* blk = Acb + (size_t)s * 7 * kmax * kmax;
+  double* blk = Acb ? Acb + (size_t)s * 7 * kmax * kmax : nullptr;
   for (int i0 = 0; i0 < ke; i0 += 4) {
     double acc[4][kMaxEig];
 #pragma unroll
@@ -57,7 +65,7 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         const int lext = (d >> 1) == 0 ? b.nx : ((d >> 1) == 1 ? b.ny : b.nz);
         const bool inside = (d & 1) ? (lcoord < lext - 1) : (lcoord > 0);
         if (!inside) {
-          sdiag += t;
+          if (!inner || g.bricks[b.nb[d]].box == b.box) sdiag += t;
         } else if ((d & 1) == 0) {
           // Interior face with the lower neighbour: T (phi_c - phi_nb)(...)^T
           // from the differences of the basis values, as the reference
@@ -95,8 +103,10 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
             Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
             Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
           }
-          blk[i * kmax + j] = v;
-          blk[j * kmax + i] = v;
+          if (Acb) {
+            blk[i * kmax + j] = v;
+            blk[j * kmax + i] = v;
+          }
         }
       }
     }
@@ -149,12 +159,14 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         for (int j = 0; j < o.neig; ++j) {
           const double v = block_sum(acc[a][j], red);
           if (threadIdx.x == 0) {
-            if (Ac) {
+            if (Ac && o.box == b.box) {
               Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
               Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
             }
-            blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
-            Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
+            if (Acb) {
+              blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
+              Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
+            }
           }
         }
       }
@@ -168,7 +180,7 @@ int launch_galerkin(pmx_ctx* ctx) {
   PMX_HIP(hipMemsetAsync(ctx->Acb.p, 0, sizeof(double) * ctx->Acb.n, ctx->stream));
   hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
                      ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->nc,
-                     ctx->Ac.p, ctx->Acb.p);
+                     ctx->Ac.p, ctx->Acb.p, 0);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -910,11 +922,13 @@ __global__ void k_stage_update(Geo g, const double* __restrict__ Acb, int kmax,
 // Y = A_cc^{-1} (V^T T), k2 x n (column j per thread).
 // Y[:, j] = A_cc^{-1} Y[:, j] (k2 x k2 LU with partial pivoting, row
 // interchanges in LU[k2*k2 ..]) for every column j of the k2 x n block Y.
+// KM: capacity of the level-3 space (kMaxEig, or kMaxCC with process boxes).
+template <int KM>
 __global__ void k_lv3_lu_cols(int k2, const double* __restrict__ LU, double* __restrict__ Y,
                               int n) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  double b[kMaxEig];
+  double b[KM];
   for (int q = 0; q < k2; ++q) b[q] = Y[(size_t)q * n + j];
   const double* piv = LU + k2 * k2;
   for (int i = 0; i < k2; ++i) {
@@ -986,8 +1000,9 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
       hipLaunchKernelGGL((k_gemm_mfma<1, 4, 1, 1, false>), dim3((n + 63) / 64, (k2 + 15) / 16),
                          dim3(256), 0, ctx->stream, k2, n, n, 1.0, ctx->V2.p, n, T.p, n, 0.0, Y.p,
                          n);
-      hipLaunchKernelGGL(k_lv3_lu_cols, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                         ctx->stream, k2, ctx->Acc_lu.p, Y.p, n);
+      hipLaunchKernelGGL(k2 <= kMaxEig ? k_lv3_lu_cols<kMaxEig> : k_lv3_lu_cols<kMaxCC>,
+                         dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, k2,
+                         ctx->Acc_lu.p, Y.p, n);
       hipLaunchKernelGGL(k_lv3_right, dim3(eb), dim3(kBlock), 0, ctx->stream, ctx->V2.p, k2, Y.p,
                          D.p, n);
       Dp = D.p;
@@ -1074,11 +1089,12 @@ __global__ void k_coarse_spmm(Geo g, const double* __restrict__ Acb, int kmax,
 // d = V (A_cc^{-1} (V^T t)) with the LU of A_cc (level-3 apply,
 // _PC_apply_vec_lv3, src/PreMixFEM_3D.c:1597-1648, written in the coarse
 // space since Phi_cc = Phi V).  One workgroup.
+template <int KM>
 __global__ void k_lv3_apply(const double* __restrict__ V, int nc, int k2,
                             const double* __restrict__ LU, const double* __restrict__ t,
                             double* __restrict__ d) {
   __shared__ double red[8];
-  __shared__ double c[kMaxEig];
+  __shared__ double c[KM];
   for (int j = 0; j < k2; ++j) {
     double s = 0.0;
     for (int r = threadIdx.x; r < nc; r += blockDim.x) s += V[(size_t)j * nc + r] * t[r];
@@ -1089,7 +1105,7 @@ __global__ void k_lv3_apply(const double* __restrict__ V, int nc, int k2,
   if (threadIdx.x == 0) {
     // LU with row pivots stored after the k2 x k2 factors.
     const double* piv = LU + k2 * k2;
-    double b[kMaxEig];
+    double b[KM];
     for (int i = 0; i < k2; ++i) b[i] = c[i];
     for (int i = 0; i < k2; ++i) {
       const int p = (int)piv[i];
@@ -1114,7 +1130,8 @@ __global__ void k_lv3_apply(const double* __restrict__ V, int nc, int k2,
 }
 
 int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d) {
-  hipLaunchKernelGGL(k_lv3_apply, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, ctx->nc,
+  hipLaunchKernelGGL(ctx->eigen_num_lv2 <= kMaxEig ? k_lv3_apply<kMaxEig> : k_lv3_apply<kMaxCC>,
+                     dim3(1), dim3(kBlock), 0, ctx->stream, ctx->V2.p, ctx->nc,
                      ctx->eigen_num_lv2, ctx->Acc_lu.p, t, d);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
@@ -1341,7 +1358,7 @@ static void launch_gemm_tall(hipStream_t st, const double* Ainv, const double* X
 // Sign of every column of V: positive inner product with 1 + frac(0.618..(r+1)).
 __global__ void k_lv3_sign(double* __restrict__ V, int n, int k2) {
   __shared__ double red[8];
-  __shared__ double sg[kMaxEig];
+  __shared__ double sg[kMaxCC];
   for (int j = 0; j < k2; ++j) {
     double s = 0.0;
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
@@ -1373,18 +1390,19 @@ __global__ void k_phicc(Geo g, const double* __restrict__ phi, int kmax,
 
 // Row a = blockIdx.y of brick s = blockIdx.x's share of A_cc: the lower
 // faces of its cells (to any brick) and the diagonal terms.
+template <int KM>
 __global__ void __launch_bounds__(256) k_acc_part(Geo g, const double* __restrict__ phicc, int k2,
                                                   const double* __restrict__ Tx,
                                                   const double* __restrict__ Ty,
                                                   const double* __restrict__ Tz,
                                                   double* __restrict__ part) {
-  __shared__ double red[4][kMaxEig];
+  __shared__ double red[4][KM];
   const int s = blockIdx.x, a = blockIdx.y;
   const BrickView v = load_view(g, s);
   const double* TT3[3] = {Tx, Ty, Tz};
-  double acc[kMaxEig];
+  double acc[KM];
 #pragma unroll
-  for (int q = 0; q < kMaxEig; ++q) acc[q] = 0.0;
+  for (int q = 0; q < KM; ++q) acc[q] = 0.0;
   for (int li = threadIdx.x; li < v.n; li += blockDim.x) {
     const int lx = li % v.nx, ly = (li / v.nx) % v.ny, lz = li / v.nxny;
     const int c = v.off + li;
@@ -1399,7 +1417,7 @@ __global__ void __launch_bounds__(256) k_acc_part(Geo g, const double* __restric
       const double* po = phicc + (size_t)o * k2;
       const double da = t * (pc[a] - po[a]);
 #pragma unroll
-      for (int q = 0; q < kMaxEig; ++q)
+      for (int q = 0; q < KM; ++q)
         if (q < k2) acc[q] += da * (pc[q] - po[q]);
     }
     double dg = 0.0;
@@ -1408,13 +1426,13 @@ __global__ void __launch_bounds__(256) k_acc_part(Geo g, const double* __restric
     if (dg != 0.0) {
       const double pa = dg * pc[a];
 #pragma unroll
-      for (int q = 0; q < kMaxEig; ++q)
+      for (int q = 0; q < KM; ++q)
         if (q < k2) acc[q] += pa * pc[q];
     }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < kMaxEig; ++q) {
+  for (int q = 0; q < KM; ++q) {
     if (q >= k2) break;
     double x = acc[q];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -1430,43 +1448,44 @@ __global__ void __launch_bounds__(256) k_acc_part(Geo g, const double* __restric
 
 // A_cc = sum of the brick partials (fixed order), symmetrised, + shift I, and
 // its LU with partial pivoting (row-major factors, pivots after them).
+template <int KM>
 __global__ void k_acc_lu(const double* __restrict__ part, int nsub, int k2, double shift,
                          double* __restrict__ LU) {
-  __shared__ double A[kMaxEig * kMaxEig];
+  __shared__ double A[KM * KM];
   for (int e = threadIdx.x; e < k2 * k2; e += blockDim.x) {
     double x = 0.0;
     for (int s = 0; s < nsub; ++s) x += part[(size_t)s * k2 * k2 + e];
-    A[(e / k2) * kMaxEig + e % k2] = x;
+    A[(e / k2) * KM + e % k2] = x;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   for (int i = 0; i < k2; ++i)
     for (int j = i + 1; j < k2; ++j) {
-      const double m = 0.5 * (A[i * kMaxEig + j] + A[j * kMaxEig + i]);
-      A[i * kMaxEig + j] = m;
-      A[j * kMaxEig + i] = m;
+      const double m = 0.5 * (A[i * KM + j] + A[j * KM + i]);
+      A[i * KM + j] = m;
+      A[j * KM + i] = m;
     }
-  for (int i = 0; i < k2; ++i) A[i * kMaxEig + i] += shift;
-  double piv[kMaxEig];
+  for (int i = 0; i < k2; ++i) A[i * KM + i] += shift;
+  double piv[KM];
   for (int k = 0; k < k2; ++k) {
     int p = k;
     for (int i = k + 1; i < k2; ++i)
-      if (fabs(A[i * kMaxEig + k]) > fabs(A[p * kMaxEig + k])) p = i;
+      if (fabs(A[i * KM + k]) > fabs(A[p * KM + k])) p = i;
     piv[k] = p;
     if (p != k)
       for (int j = 0; j < k2; ++j) {
-        const double t = A[k * kMaxEig + j];
-        A[k * kMaxEig + j] = A[p * kMaxEig + j];
-        A[p * kMaxEig + j] = t;
+        const double t = A[k * KM + j];
+        A[k * KM + j] = A[p * KM + j];
+        A[p * KM + j] = t;
       }
     for (int i = k + 1; i < k2; ++i) {
-      const double f = A[i * kMaxEig + k] / A[k * kMaxEig + k];
-      A[i * kMaxEig + k] = f;
-      for (int j = k + 1; j < k2; ++j) A[i * kMaxEig + j] -= f * A[k * kMaxEig + j];
+      const double f = A[i * KM + k] / A[k * KM + k];
+      A[i * KM + k] = f;
+      for (int j = k + 1; j < k2; ++j) A[i * KM + j] -= f * A[k * KM + j];
     }
   }
   for (int i = 0; i < k2; ++i)
-    for (int j = 0; j < k2; ++j) LU[i * k2 + j] = A[i * kMaxEig + j];
+    for (int j = 0; j < k2; ++j) LU[i * k2 + j] = A[i * KM + j];
   for (int i = 0; i < k2; ++i) LU[k2 * k2 + i] = piv[i];
 }
 
@@ -1479,14 +1498,141 @@ static int lv3_finish_energy(pmx_ctx* ctx, int num) {
   PMX_TRY(part.alloc((size_t)ctx->nsub * num * num));
   hipLaunchKernelGGL(k_phicc, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo, ctx->phi.p,
                      ctx->en_lv1, ctx->V2.p, n, num, phicc.p);
-  hipLaunchKernelGGL(k_acc_part, dim3(ctx->nsub, num), dim3(256), 0, ctx->stream, ctx->geo,
-                     phicc.p, num, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, part.p);
+  const bool small = num <= kMaxEig;
+  hipLaunchKernelGGL(small ? k_acc_part<kMaxEig> : k_acc_part<kMaxCC>, dim3(ctx->nsub, num),
+                     dim3(256), 0, ctx->stream, ctx->geo, phicc.p, num, ctx->T[0].p, ctx->T[1].p,
+                     ctx->T[2].p, part.p);
   PMX_TRY(ctx->Acc_lu.alloc((size_t)num * num + num));
-  hipLaunchKernelGGL(k_acc_lu, dim3(1), dim3(kBlock), 0, ctx->stream, part.p, ctx->nsub, num,
+  hipLaunchKernelGGL(small ? k_acc_lu<kMaxEig> : k_acc_lu<kMaxCC>, dim3(1), dim3(kBlock), 0,
+                     ctx->stream, part.p, ctx->nsub, num,
                      ctx->no_shift_A_cc ? 0.0 : kNil, ctx->Acc_lu.p);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipStreamSynchronize(ctx->stream));
   return PMX_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Level 3 with process boxes (_PC_setup_lv3_eigen, src/PreMixFEM_3D.c:918-1175,
+// for every box at once): box b takes the smallest en_lv2 eigenpairs of its
+// box-local Neumann coarse matrix A_i_inner.  The boxes are independent
+// dense symmetric eigenproblems of a few hundred unknowns: one workgroup per
+// box runs the parallel cyclic Jacobi of wg_syev on its block in global
+// memory (all eigenpairs, accurate for the zero eigenvalue of an unpinned
+// box).  Column j < k2 of box b goes to Vt[(b*k2max + j)*nc + dof] with the
+// block's row dofs; wout[b*k2max + j] = lambda_j.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_box_syev(const double* __restrict__ Ain, int nc,
+                                                  const int* __restrict__ dofs,
+                                                  const int* __restrict__ boff,
+                                                  const long long* __restrict__ woff,
+                                                  double* __restrict__ work, int k2max,
+                                                  double* __restrict__ Vt,
+                                                  double* __restrict__ wout) {
+  const int bx = blockIdx.x;
+  const int m = boff[bx + 1] - boff[bx];
+  const int* d = dofs + boff[bx];
+  double* A = work + woff[bx];
+  double* Q = A + (size_t)m * m;
+  double* tmp = Q + (size_t)m * m;
+  double* w = tmp + (size_t)m * m;
+  double* rot = w + m;   // 4 m doubles
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+    const int i = e / m, j = e % m;
+    A[e] = 0.5 * (Ain[(size_t)d[i] * nc + d[j]] + Ain[(size_t)d[j] * nc + d[i]]);
+  }
+  __syncthreads();
+  wg_syev(A, Q, tmp, w, rot, m, m);
+  const int k2 = min(k2max, m);
+  for (int e = threadIdx.x; e < m * k2; e += blockDim.x) {
+    const int i = e / k2, j = e % k2;
+    Vt[(size_t)(bx * k2max + j) * nc + d[i]] = Q[(size_t)i * m + j];
+  }
+  if (threadIdx.x < k2) wout[bx * k2max + threadIdx.x] = w[threadIdx.x];
+}
+
+int level3_setup_boxes(pmx_ctx* ctx) {
+  const int n = ctx->nc, nbox = ctx->nbox, k2max = ctx->en_lv2;
+  // Coarse dofs of every box, in subdomain order.
+  std::vector<std::vector<int>> lists(nbox);
+  for (const Brick& b : ctx->h_bricks)
+    for (int j = 0; j < b.neig; ++j) lists[b.box].push_back(b.dof + j);
+  std::vector<int> boff(nbox + 1, 0), dofs;
+  std::vector<long long> woff(nbox + 1, 0);
+  for (int q = 0; q < nbox; ++q) {
+    const long long m = (long long)lists[q].size();
+    boff[q + 1] = boff[q] + (int)m;
+    dofs.insert(dofs.end(), lists[q].begin(), lists[q].end());
+    woff[q + 1] = woff[q] + 3 * m * m + 5 * m;
+  }
+  DBuf<double> Ain, work, Vt, wv;
+  DBuf<int> d_dofs, d_boff;
+  DBuf<long long> d_woff;
+  PMX_TRY(Ain.alloc((size_t)n * n));
+  PMX_TRY(work.alloc((size_t)woff[nbox]));
+  PMX_TRY(Vt.alloc((size_t)n * nbox * k2max));
+  PMX_TRY(wv.alloc((size_t)nbox * k2max));
+  PMX_TRY(d_dofs.alloc(dofs.size()));
+  PMX_TRY(d_boff.alloc(boff.size()));
+  PMX_TRY(d_woff.alloc(woff.size()));
+  PMX_HIP(hipMemcpyAsync(d_dofs.p, dofs.data(), dofs.size() * sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream));
+  PMX_HIP(hipMemcpyAsync(d_boff.p, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream));
+  PMX_HIP(hipMemcpyAsync(d_woff.p, woff.data(), woff.size() * sizeof(long long),
+                         hipMemcpyHostToDevice, ctx->stream));
+  PMX_HIP(hipMemsetAsync(Ain.p, 0, sizeof(double) * (size_t)n * n, ctx->stream));
+  PMX_HIP(hipMemsetAsync(Vt.p, 0, sizeof(double) * (size_t)n * nbox * k2max, ctx->stream));
+  PMX_HIP(hipMemsetAsync(wv.p, 0, sizeof(double) * (size_t)nbox * k2max, ctx->stream));
+  hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                     ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, n, Ain.p,
+                     (double*)nullptr, 1);
+  hipLaunchKernelGGL(k_box_syev, dim3(nbox), dim3(256), 0, ctx->stream, Ain.p, n, d_dofs.p,
+                     d_boff.p, d_woff.p, work.p, k2max, Vt.p, wv.p);
+  PMX_HIP(hipGetLastError());
+  std::vector<double> hw((size_t)nbox * k2max);
+  PMX_HIP(hipMemcpyAsync(hw.data(), wv.p, hw.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  // Per-box count against eb_lv2 (:1100-1115); the columns of A_cc are the
+  // boxes' vectors in rank order.
+  ctx->h_eigen_num_lv2_box.assign(nbox, 0);
+  ctx->h_eig_lv2.clear();
+  int K = 0;
+  double emin = 0.0, emax = 0.0;
+  for (int q = 0; q < nbox; ++q) {
+    const int k2 = std::min(k2max, boff[q + 1] - boff[q]);
+    const double* lam = hw.data() + (size_t)q * k2max;
+    int num = k2;
+    for (int j = 0; j < k2; ++j)
+      if (lam[j] >= ctx->eb_lv2) {
+        num = j + 1;
+        break;
+      }
+    ctx->h_eigen_num_lv2_box[q] = num;
+    emin = q == 0 ? lam[0] : std::min(emin, lam[0]);
+    emax = q == 0 ? lam[num - 1] : std::max(emax, lam[num - 1]);
+    ctx->h_eig_lv2.insert(ctx->h_eig_lv2.end(), lam, lam + num);
+    K += num;
+  }
+  if (K > kMaxCC) {
+    set_error("-procs_virtual: %d level-3 vectors over all boxes exceed %d", K, kMaxCC);
+    return PMX_ERR_SUP;
+  }
+  PMX_TRY(ctx->V2.alloc((size_t)n * K));
+  int col = 0;
+  for (int q = 0; q < nbox; ++q) {
+    const int num = ctx->h_eigen_num_lv2_box[q];
+    PMX_HIP(hipMemcpyAsync(ctx->V2.p + (size_t)col * n, Vt.p + (size_t)q * k2max * n,
+                           (size_t)num * n * sizeof(double), hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    col += num;
+  }
+  ctx->eigen_num_lv2 = K;
+  ctx->eigen_min_lv2 = emin;
+  ctx->eigen_max_lv2 = emax;
+  ctx->lv3_iters = 1;
+  ctx->lv3_floor = 0.0;
+  return lv3_finish_energy(ctx, K);
 }
 
 // Refined shift-invert step, part 1: R = A_c X - X diag(w) (Z = A_c X).

@@ -24,6 +24,7 @@ constexpr int kStreamBlock = 1024; // largest workgroup of the streaming per-bri
 constexpr int kMaxPlanes = 768;    // hyperplanes of a brick (3 * 256 - 2 at most)
 constexpr int kMaxShapes = 8;      // uneven partitions give at most 2 lengths per axis
 constexpr int kMaxEig = 16;        // largest en_lv1 / en_lv2 supported on device
+constexpr int kMaxCC = 64;         // largest level-3 space (all process boxes together)
 constexpr double kNil = 1.0e-12;   // NIL of src/PreMixFEM_3D.h:14
 constexpr int kBandAutoNc = 12288; // coarse size above which level 2 uses the band factor
 
@@ -57,6 +58,7 @@ struct Brick {
   int plane_off;    // offset of this shape in the plane-position tables
   int nplanes;      // nx+ny+nz-2
   int lid;          // subdomain index inside its problem (seeds its random LOBPCG start)
+  int box;          // process box (x-fastest rank of the -procs_virtual decomposition)
 };
 
 // Per-brick extents and neighbour offsets, precomputed on the host so a
@@ -285,7 +287,10 @@ struct pmx_ctx {
   // ---- options (src/PreMixFEM_3D.c:1662-1705) ----
   double dom[3];
   int mesh[3];
-  int sd[3];
+  int sd[3];                        // subdomains per axis over the whole grid
+  int pv[3] = {1, 1, 1};            // process boxes per axis (-procs_virtual)
+  int sd_box[3] = {2, 2, 2};        // subdomains per axis inside one box
+  int nbox = 1;
   int si_lv1 = 1, si_lv2 = 1;
   int en_lv1 = 4, en_lv2 = 4;
   int eg_op = PMX_EIG_OP_MOD;
@@ -391,6 +396,7 @@ struct pmx_ctx {
   int eigen_num_lv2 = 0;
   double eigen_min_lv2 = 0.0, eigen_max_lv2 = 0.0;
   std::vector<double> h_eig_lv2;
+  std::vector<int> h_eigen_num_lv2_box;  // level-3 count of every process box
   pmx::DBuf<double> V2;             // [nc][en_lv2]
   pmx::DBuf<double> Acc_lu;         // k2 x k2 LU (row-major) + pivots packed after
   int lv3_iters = 0;

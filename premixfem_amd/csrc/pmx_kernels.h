@@ -46,6 +46,7 @@ int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t,
                           int (*kt_begin)(pmx_ctx*, int), void (*kt_end)(pmx_ctx*, int));
 int launch_resid_restrict_f64(pmx_ctx* ctx, const double* x, double* cvec);
 int level3_setup(pmx_ctx* ctx);
+int level3_setup_boxes(pmx_ctx* ctx);
 int launch_gemv(pmx_ctx* ctx, const double* A, const double* x, double* y, int n);
 int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, int first,
                          int update_t);

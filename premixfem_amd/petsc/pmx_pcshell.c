@@ -30,7 +30,7 @@ static void opt_append(char *buf, size_t cap, const char *fmt, const char *name,
 /* Options the GPU core understands beyond the reference's own set. */
 static const char *kValueOptions[] = {"-sdx",      "-sdy",       "-sdz",    "-pc_precision",
                                       "-pin",      "-eig_tol",   "-eig_maxit", "-eig_sigma",
-                                      "-seed",     "-warm_start", "-cg_chunk"};
+                                      "-seed",     "-warm_start", "-cg_chunk", "-procs_virtual"};
 static const char *kFlagOptions[] = {"-kernel_timing", "-log_stages"};
 
 /* src/PreMixFEM_3D.c:1650-1741: option values with the reference's defaults,

@@ -55,6 +55,12 @@ def _init(dom, mesh, options):
     ((1.0, 1.0, 1.0), (8, 8, 8), "-use_full_Cholesky_lv1 -pc_precision 32", 56),  # FP64 only
     ((1.0, 1.0, 1.0), (8, 8, 8), "-pin 8,0,0", 63),
     ((1.0, 1.0, 1.0), (8, 8, 8), "-no_such_option 1", 62),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2,2", 62),         # px,py,pz or one count
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2,x,1", 62),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 0,1,1", 62),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 3,1,1 -sd 3", 62),  # boxes of 2 cells
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2 -use_2level", 56),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2 -lv2_icc", 56),
 ])
 def test_init_argument_errors(dom, mesh, options, code):
     rc, msg = _init(dom, mesh, options)
