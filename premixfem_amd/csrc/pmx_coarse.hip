@@ -1,6 +1,7 @@
 // Coarse levels: Galerkin assembly of Phi^T A Phi, its dense FP64 SPD
 // inversion (block Gauss-Jordan sweep), the coarse-space operators of the
 // V-cycle, and the level-3 eigenproblem / coarse-coarse matrix.
+#include <algorithm>
 #include <cmath>
 #include <random>
 
@@ -1514,40 +1515,124 @@ static int lv3_finish_energy(pmx_ctx* ctx, int num) {
 // ---------------------------------------------------------------------------
 // Level 3 with process boxes (_PC_setup_lv3_eigen, src/PreMixFEM_3D.c:918-1175,
 // for every box at once): box b takes the smallest en_lv2 eigenpairs of its
-// box-local Neumann coarse matrix A_i_inner.  The boxes are independent
-// dense symmetric eigenproblems of a few hundred unknowns: one workgroup per
-// box runs the parallel cyclic Jacobi of wg_syev on its block in global
-// memory (all eigenpairs, accurate for the zero eigenvalue of an unpinned
-// box).  Column j < k2 of box b goes to Vt[(b*k2max + j)*nc + dof] with the
-// block's row dofs; wout[b*k2max + j] = lambda_j.
+// box-local Neumann coarse matrix A_i_inner, a dense symmetric block of a few
+// hundred unknowns.  All eigenpairs by parallel cyclic Jacobi (round-robin
+// ordering, m/2 disjoint rotations per round; threshold rotations, so the
+// zero eigenvalue of an unpinned box stays accurate), every box at once:
+// one launch per round, A' = J^T A J and Q' = Q J over the whole grid into
+// ping-pong buffers, each workgroup recomputing the round's rotations in
+// LDS.  Blocks are packed row-major at moff[b], m_b = boff[b+1] - boff[b].
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_box_syev(const double* __restrict__ Ain, int nc,
-                                                  const int* __restrict__ dofs,
-                                                  const int* __restrict__ boff,
-                                                  const long long* __restrict__ woff,
-                                                  double* __restrict__ work, int k2max,
-                                                  double* __restrict__ Vt,
-                                                  double* __restrict__ wout) {
-  const int bx = blockIdx.x;
-  const int m = boff[bx + 1] - boff[bx];
-  const int* d = dofs + boff[bx];
-  double* A = work + woff[bx];
-  double* Q = A + (size_t)m * m;
-  double* tmp = Q + (size_t)m * m;
-  double* w = tmp + (size_t)m * m;
-  double* rot = w + m;   // 4 m doubles
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+constexpr int kJacMaxM = 2048;   // largest box block (LDS of the rotation table)
+constexpr int kJacTile = 2048;   // matrix entries per workgroup and round
+
+__global__ void k_box_gather(const double* __restrict__ Ain, int nc, const int* __restrict__ dofs,
+                             const int* __restrict__ boff, const long long* __restrict__ moff,
+                             double* __restrict__ A, double* __restrict__ Q) {
+  const int b = blockIdx.y;
+  const int m = boff[b + 1] - boff[b];
+  const int* d = dofs + boff[b];
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m * m; e += gridDim.x * blockDim.x) {
     const int i = e / m, j = e % m;
-    A[e] = 0.5 * (Ain[(size_t)d[i] * nc + d[j]] + Ain[(size_t)d[j] * nc + d[i]]);
+    A[moff[b] + e] = 0.5 * (Ain[(size_t)d[i] * nc + d[j]] + Ain[(size_t)d[j] * nc + d[i]]);
+    Q[moff[b] + e] = i == j ? 1.0 : 0.0;
+  }
+}
+
+// Round r of the sweep for box blockIdx.y: rows/columns i and partner(i) =
+// (2r - i) mod (mp - 1) (index mp - 1 pairs with r; an odd m has a dummy
+// index m), rotation (c, s) of the pair from the pre-round A.
+__global__ void __launch_bounds__(256) k_jacobi_round(const double* __restrict__ A,
+                                                      double* __restrict__ A2,
+                                                      const double* __restrict__ Q,
+                                                      double* __restrict__ Q2,
+                                                      const int* __restrict__ boff,
+                                                      const long long* __restrict__ moff, int r,
+                                                      int* __restrict__ rotated) {
+  __shared__ double cc[kJacMaxM], ss[kJacMaxM];
+  __shared__ int part[kJacMaxM];
+  const int b = blockIdx.y;
+  const int m = boff[b + 1] - boff[b];
+  const int e0 = blockIdx.x * kJacTile;
+  if (e0 >= m * m) return;
+  const int mp = (m + 1) & ~1;
+  const double* Ab = A + moff[b];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    int pi;
+    if (r >= mp - 1)
+      pi = i;  // a smaller box than the largest: identity round
+    else if (i == mp - 1)
+      pi = r;
+    else if (i == r)
+      pi = mp - 1;
+    else
+      pi = ((2 * r - i) % (mp - 1) + (mp - 1)) % (mp - 1);
+    double c = 1.0, sv = 0.0;
+    if (pi < m && pi != i) {
+      const int p = min(i, pi), q = max(i, pi);
+      const double app = Ab[(size_t)p * m + p], aqq = Ab[(size_t)q * m + q];
+      const double apq = Ab[(size_t)p * m + q];
+      if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
+        const double th = (aqq - app) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        c = 1.0 / sqrt(t * t + 1.0);
+        sv = i == p ? -t * c : t * c;
+        if (i == p && blockIdx.x == 0) rotated[b] = 1;
+      }
+    } else {
+      pi = i;
+    }
+    cc[i] = c;
+    ss[i] = sv;
+    part[i] = pi;
   }
   __syncthreads();
-  wg_syev(A, Q, tmp, w, rot, m, m);
-  const int k2 = min(k2max, m);
-  for (int e = threadIdx.x; e < m * k2; e += blockDim.x) {
-    const int i = e / k2, j = e % k2;
-    Vt[(size_t)(bx * k2max + j) * nc + d[i]] = Q[(size_t)i * m + j];
+  const double* Qb = Q + moff[b];
+  double* A2b = A2 + moff[b];
+  double* Q2b = Q2 + moff[b];
+  const int e1 = min(e0 + kJacTile, m * m);
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int i = e / m, j = e % m;
+    const int ip = part[i], jp = part[j];
+    const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
+    double v;
+    if (ip == j && ip != i) {
+      v = 0.0;  // the annihilated pair
+    } else {
+      v = ci * cj * Ab[(size_t)i * m + j] + ci * sj * Ab[(size_t)i * m + jp] +
+          si * cj * Ab[(size_t)ip * m + j] + si * sj * Ab[(size_t)ip * m + jp];
+    }
+    A2b[e] = v;
+    Q2b[e] = cj * Qb[e] + sj * Qb[(size_t)i * m + jp];
   }
-  if (threadIdx.x < k2) wout[bx * k2max + threadIdx.x] = w[threadIdx.x];
+}
+
+// Eigenvalues = the diagonal, ranked ascending (ties by index); the k2
+// smallest columns of Q go to Vt[(b*k2max + rank)*nc + dof], lambda to
+// wout[b*k2max + rank].
+__global__ void k_box_pick(const double* __restrict__ A, const double* __restrict__ Q, int nc,
+                           const int* __restrict__ dofs, const int* __restrict__ boff,
+                           const long long* __restrict__ moff, int k2max,
+                           double* __restrict__ Vt, double* __restrict__ wout) {
+  const int b = blockIdx.x;
+  const int m = boff[b + 1] - boff[b];
+  const int* d = dofs + boff[b];
+  const double* Ab = A + moff[b];
+  const double* Qb = Q + moff[b];
+  const int k2 = min(k2max, m);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double wi = Ab[(size_t)i * m + i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double wj = Ab[(size_t)j * m + j];
+      rank += (wj < wi || (wj == wi && j < i)) ? 1 : 0;
+    }
+    if (rank < k2) {
+      wout[b * k2max + rank] = wi;
+      double* col = Vt + (size_t)(b * k2max + rank) * nc;
+      for (int q = 0; q < m; ++q) col[d[q]] = Qb[(size_t)q * m + i];
+    }
+  }
 }
 
 int level3_setup_boxes(pmx_ctx* ctx) {
@@ -1557,28 +1642,35 @@ int level3_setup_boxes(pmx_ctx* ctx) {
   for (const Brick& b : ctx->h_bricks)
     for (int j = 0; j < b.neig; ++j) lists[b.box].push_back(b.dof + j);
   std::vector<int> boff(nbox + 1, 0), dofs;
-  std::vector<long long> woff(nbox + 1, 0);
+  std::vector<long long> moff(nbox + 1, 0);
+  int mmax = 0;
   for (int q = 0; q < nbox; ++q) {
     const long long m = (long long)lists[q].size();
     boff[q + 1] = boff[q] + (int)m;
     dofs.insert(dofs.end(), lists[q].begin(), lists[q].end());
-    woff[q + 1] = woff[q] + 3 * m * m + 5 * m;
+    moff[q + 1] = moff[q] + m * m;
+    mmax = std::max(mmax, (int)m);
   }
-  DBuf<double> Ain, work, Vt, wv;
-  DBuf<int> d_dofs, d_boff;
-  DBuf<long long> d_woff;
+  if (mmax > kJacMaxM) {
+    set_error("-procs_virtual: a box with %d coarse unknowns (at most %d)", mmax, kJacMaxM);
+    return PMX_ERR_SUP;
+  }
+  DBuf<double> Ain, Aj, Aj2, Qj, Qj2, Vt, wv;
+  DBuf<int> d_dofs, d_boff, rot;
+  DBuf<long long> d_moff;
   PMX_TRY(Ain.alloc((size_t)n * n));
-  PMX_TRY(work.alloc((size_t)woff[nbox]));
+  for (DBuf<double>* w : {&Aj, &Aj2, &Qj, &Qj2}) PMX_TRY(w->alloc((size_t)moff[nbox]));
+  PMX_TRY(rot.alloc(nbox));
   PMX_TRY(Vt.alloc((size_t)n * nbox * k2max));
   PMX_TRY(wv.alloc((size_t)nbox * k2max));
   PMX_TRY(d_dofs.alloc(dofs.size()));
   PMX_TRY(d_boff.alloc(boff.size()));
-  PMX_TRY(d_woff.alloc(woff.size()));
+  PMX_TRY(d_moff.alloc(moff.size()));
   PMX_HIP(hipMemcpyAsync(d_dofs.p, dofs.data(), dofs.size() * sizeof(int), hipMemcpyHostToDevice,
                          ctx->stream));
   PMX_HIP(hipMemcpyAsync(d_boff.p, boff.data(), boff.size() * sizeof(int), hipMemcpyHostToDevice,
                          ctx->stream));
-  PMX_HIP(hipMemcpyAsync(d_woff.p, woff.data(), woff.size() * sizeof(long long),
+  PMX_HIP(hipMemcpyAsync(d_moff.p, moff.data(), moff.size() * sizeof(long long),
                          hipMemcpyHostToDevice, ctx->stream));
   PMX_HIP(hipMemsetAsync(Ain.p, 0, sizeof(double) * (size_t)n * n, ctx->stream));
   PMX_HIP(hipMemsetAsync(Vt.p, 0, sizeof(double) * (size_t)n * nbox * k2max, ctx->stream));
@@ -1586,8 +1678,30 @@ int level3_setup_boxes(pmx_ctx* ctx) {
   hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
                      ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, n, Ain.p,
                      (double*)nullptr, 1);
-  hipLaunchKernelGGL(k_box_syev, dim3(nbox), dim3(256), 0, ctx->stream, Ain.p, n, d_dofs.p,
-                     d_boff.p, d_woff.p, work.p, k2max, Vt.p, wv.p);
+  const int ntile = (int)(((long long)mmax * mmax + kJacTile - 1) / kJacTile);
+  hipLaunchKernelGGL(k_box_gather, dim3(std::min(ntile, 256), nbox), dim3(256), 0, ctx->stream,
+                     Ain.p, n, d_dofs.p, d_boff.p, d_moff.p, Aj.p, Qj.p);
+  PMX_HIP(hipGetLastError());
+  // Sweeps until one leaves every box without a rotation (at most 40).
+  const int rounds = ((mmax + 1) & ~1) - 1;
+  int sweeps = 0;
+  std::vector<int> hrot(nbox);
+  for (; sweeps < 40; ++sweeps) {
+    PMX_HIP(hipMemsetAsync(rot.p, 0, nbox * sizeof(int), ctx->stream));
+    for (int r = 0; r < rounds; ++r) {
+      hipLaunchKernelGGL(k_jacobi_round, dim3(ntile, nbox), dim3(256), 0, ctx->stream, Aj.p,
+                         Aj2.p, Qj.p, Qj2.p, d_boff.p, d_moff.p, r, rot.p);
+      std::swap(Aj, Aj2);
+      std::swap(Qj, Qj2);
+    }
+    PMX_HIP(hipGetLastError());
+    PMX_HIP(hipMemcpyAsync(hrot.data(), rot.p, nbox * sizeof(int), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (std::all_of(hrot.begin(), hrot.end(), [](int v) { return v == 0; })) break;
+  }
+  hipLaunchKernelGGL(k_box_pick, dim3(nbox), dim3(256), 0, ctx->stream, Aj.p, Qj.p, n, d_dofs.p,
+                     d_boff.p, d_moff.p, k2max, Vt.p, wv.p);
   PMX_HIP(hipGetLastError());
   std::vector<double> hw((size_t)nbox * k2max);
   PMX_HIP(hipMemcpyAsync(hw.data(), wv.p, hw.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -1630,7 +1744,7 @@ int level3_setup_boxes(pmx_ctx* ctx) {
   ctx->eigen_num_lv2 = K;
   ctx->eigen_min_lv2 = emin;
   ctx->eigen_max_lv2 = emax;
-  ctx->lv3_iters = 1;
+  ctx->lv3_iters = sweeps + 1;
   ctx->lv3_floor = 0.0;
   return lv3_finish_energy(ctx, K);
 }
