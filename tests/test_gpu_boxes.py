@@ -104,3 +104,28 @@ def test_one_box_is_the_default_path():
     b = runner.run(prob)
     assert a["its"] == b["its"]
     assert np.array_equal(a["x"], b["x"])
+
+
+def test_boxes_in_a_batched_setup():
+    """Two problems with process boxes set up by one pmx_setup_batch call end
+    up as their own setups (iterations equal, solution within 1e-12), and
+    the FP32 learned warm start leaves the box result unchanged."""
+    from premixfem_amd.cnn import WarmStartNet
+
+    probs = [_prob((13, 10, 9), 3, 3, 31), _prob((12, 15, 8), 3, 3, 32)]
+    opts = _opts((2, 1, 2), "")
+    outs, _ = runner.run_batch(probs, precision=64, extra=opts)
+    for prob, o in zip(probs, outs):
+        ref = runner.run(prob, precision=64, extra=opts)
+        e = np.linalg.norm(o["x"] - ref["x"]) / np.linalg.norm(ref["x"])
+        print(f"batch member {prob.mesh}: its {o['its']} vs {ref['its']}, x {e:.1e}", flush=True)
+        assert o["its"] == ref["its"] and e <= 1e-12
+    net = WarmStartNet(probs[0].en_lv1 - 1, seed=5)
+    try:
+        warm = runner.run(probs[0], precision=32, extra=opts + " -warm_start cnn", net=net)
+    finally:
+        net.close()
+    cold = runner.run(probs[0], precision=32, extra=opts)
+    e = np.linalg.norm(warm["x"] - cold["x"]) / np.linalg.norm(cold["x"])
+    print(f"warm start: its {warm['its']} vs {cold['its']}, x {e:.1e}", flush=True)
+    assert abs(warm["its"] - cold["its"]) <= 1 and e <= 1e-6
