@@ -175,9 +175,12 @@ int pmx_setup(pmx_ctx *ctx);
  * index inside its own problem, so every context ends up with the basis its
  * own pmx_setup computes.  Requires equal en_lv1, -eig_* options and
  * warm-start mode; under -eig_autotune the block size is tuned on the
- * problem with the most subdomains and used for all.  The reference sets up
- * one problem per PC_setup call (src/PreMixFEM_3D.c:1826-1838); a caller
- * with many problems calls it once per problem. */
+ * problem with the most subdomains and used for all.  The coarse levels of
+ * the problems are then built concurrently by host threads (the first
+ * context's -batch_threads, default 4), each on its own context's stream.
+ * The reference sets up one problem per PC_setup call
+ * (src/PreMixFEM_3D.c:1826-1838); a caller with many problems calls it once
+ * per problem. */
 int pmx_setup_batch(pmx_ctx *const *ctxs, int nprob);
 
 /* PC_setup with the level-2 eigenvectors supplied by the caller (host,

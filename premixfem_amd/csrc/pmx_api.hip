@@ -2,6 +2,7 @@
 // brick tables, setup orchestration (src/PreMixFEM_3D.c:_PC_setup), the
 // V-cycle (PC_apply_vec) and the GPU-resident PCG (PETSc KSPSolve_CG).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -9,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <sstream>
+#include <thread>
 #include <tuple>
 
 #include "pmx_kernels.h"
@@ -146,6 +148,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       c->use_full_Cholesky_lv1 = 1;
     } else if (k == "-use_2level") {
       c->use_2level = 1;
+    } else if (k == "-batch_threads") {
+      PMX_TRY(geti(c->batch_threads));
     } else if (k == "-lv2_icc") {
       c->lv2_icc = 1;
     } else if (k == "-procs_virtual") {
@@ -1093,14 +1097,53 @@ int pmx_setup_batch(pmx_ctx* const* ctxs, int nprob) {
   }
   PMX_TRY(lobpcg_batch(cs.data(), nprob));
   const double t_eig = now() - t1;  // one shared launch: charged to every problem
-  for (int i = 0; i < nprob; ++i) {
-    pmx_ctx* c = cs[i];
-    const double tc = now();
-    c->t_stages[PMX_STAGE_SU_LV2_1] += t_eig;
-    PMX_TRY(setup_coarse(c));
-    c->t_stages[PMX_STAGE_SU] += (t1 - t0[i]) + t_eig + (now() - tc);
-    c->setup_done = 1;
+  // The coarse levels of the members are independent: host threads take the
+  // members largest first, each working on its own context's stream, so the
+  // one-workgroup kernels of one member (pivot inverses, small
+  // eigenproblems) overlap the trailing updates of another.
+  std::vector<int> order(nprob);
+  for (int i = 0; i < nprob; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return cs[x]->ncell > cs[y]->ncell; });
+  std::vector<int> rc(nprob, PMX_SUCCESS);
+  std::vector<std::string> msg(nprob);
+  std::atomic<int> next{0};
+  int dev = 0;
+  PMX_HIP(hipGetDevice(&dev));
+  auto worker = [&]() {
+    if (hipSetDevice(dev) != hipSuccess) return;
+    for (;;) {
+      const int k = next.fetch_add(1);
+      if (k >= nprob) break;
+      pmx_ctx* c = cs[order[k]];
+      const double tc = now();
+      c->t_stages[PMX_STAGE_SU_LV2_1] += t_eig;
+      rc[order[k]] = setup_coarse(c);
+      if (rc[order[k]]) {
+        msg[order[k]] = g_err;
+        continue;
+      }
+      c->t_stages[PMX_STAGE_SU] += (t1 - t0[order[k]]) + t_eig + (now() - tc);
+      c->setup_done = 1;
+    }
+  };
+  const int nth = std::max(1, std::min(cs[0]->batch_threads, nprob));
+  if (nth == 1) {
+    worker();
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t) pool.emplace_back(worker);
+    for (std::thread& th : pool) th.join();
   }
+  if (next.load() < nprob) {
+    set_error("pmx_setup_batch: hipSetDevice failed in a worker thread");
+    return PMX_ERR_LIB;
+  }
+  for (int i = 0; i < nprob; ++i)
+    if (rc[i]) {
+      set_error("problem %d: %s", i, msg[i].c_str());
+      return rc[i];
+    }
   return PMX_SUCCESS;
 }
 
