@@ -177,7 +177,7 @@ int pmx_setup(pmx_ctx *ctx);
  * warm-start mode; under -eig_autotune the block size is tuned on the
  * problem with the most subdomains and used for all.  The coarse levels of
  * the problems are then built concurrently by host threads (the first
- * context's -batch_threads, default 4), each on its own context's stream.
+ * context's -batch_threads, default 8), each on its own context's stream.
  * The reference sets up one problem per PC_setup call
  * (src/PreMixFEM_3D.c:1826-1838); a caller with many problems calls it once
  * per problem. */

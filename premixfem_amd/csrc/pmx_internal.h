@@ -310,7 +310,7 @@ struct pmx_ctx {
   int log_stages = 0;           // synchronise and time every apply stage
   int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels
   int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
-  int batch_threads = 4;        // pmx_setup_batch: host threads for the members' coarse levels
+  int batch_threads = 8;        // pmx_setup_batch: host threads for the members' coarse levels
   uint64_t seed = 12345;
 
   // ---- derived geometry ----
