@@ -310,6 +310,9 @@ struct pmx_ctx {
   int log_stages = 0;           // synchronise and time every apply stage
   int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels
   int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
+  int eig_order = 1;            // launch LOBPCG hardest-first from the previous setup's counts
+  std::vector<int> h_lob_its;   // LOBPCG iterations per subdomain of the last setup
+  pmx::DBuf<int> d_lob_order;
   int batch_threads = 8;        // pmx_setup_batch: host threads for the members' coarse levels
   uint64_t seed = 12345;
 

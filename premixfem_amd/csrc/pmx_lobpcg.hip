@@ -19,6 +19,7 @@
 //   the quantity SLEPc's relative test bounds under STSINVERT (default 1e-8).
 // * Output: M-orthonormal eigenvectors with a fixed sign convention, written
 //   into the basis rows, eigenvalues ascending with lambda_0 = 0.
+#include <algorithm>
 #include <cmath>
 #include <map>
 
@@ -67,6 +68,7 @@ struct LobArgs {
   int* stagnated;     // output [nsub] 1 = accepted at the stagnation test
   double stag_factor; // stagnation acceptance: worst residual <= stag_factor * tol
   int debug;          // subdomain whose iterations are traced with printf (-1: none)
+  const int* order;   // workgroup b solves subdomain order[b] (nullptr: b * stride)
 };
 
 __device__ __forceinline__ double hash_uniform(uint64_t a) {
@@ -404,7 +406,7 @@ template <int MB, int BS>
 __global__ void __launch_bounds__(BS) k_lobpcg(Geo g, LobArgs a) {
   constexpr int LD = 3 * MB;
   extern __shared__ double lds[];
-  const int sidx = blockIdx.x * a.stride;
+  const int sidx = a.order ? a.order[blockIdx.x] : blockIdx.x * a.stride;
   const Brick b = g.bricks[sidx];
   const BrickView v = load_view(g, sidx);
   const int n = v.n;
@@ -884,6 +886,24 @@ static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, f
   a.stagnated = stag.p;
   a.stag_factor = ctx->eig_stag;
   a.debug = ctx->eig_debug;
+  // Launch order: one 512-thread workgroup fills a CU, so subdomains beyond
+  // the first wave of workgroups start late, and a hard one started late
+  // sets the launch time.  A context that was set up before launches its
+  // subdomains by decreasing iteration count of that setup (the IMPES loop
+  // re-sets up every step on a slowly changing kappa); the first setup runs
+  // in index order.  Workgroups are independent, so the results do not
+  // depend on the order.
+  a.order = nullptr;
+  if (!ms && stride == 1 && ctx->eig_order && (int)ctx->h_lob_its.size() == ctx->nsub) {
+    std::vector<int> ord(ctx->nsub);
+    for (int s = 0; s < ctx->nsub; ++s) ord[s] = s;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int x, int y) { return ctx->h_lob_its[x] > ctx->h_lob_its[y]; });
+    PMX_TRY(ctx->d_lob_order.alloc(ctx->nsub));
+    PMX_HIP(hipMemcpyAsync(ctx->d_lob_order.p, ord.data(), ctx->nsub * sizeof(int),
+                           hipMemcpyHostToDevice, ctx->stream));
+    a.order = ctx->d_lob_order.p;
+  }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms) {
     PMX_HIP(hipEventCreate(&e0));
@@ -927,6 +947,7 @@ static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, f
   PMX_HIP(hipMemcpyAsync(ctx->h_eig_lv1.data(), eig.p, ctx->h_eig_lv1.size() * sizeof(double),
                          hipMemcpyDeviceToHost, ctx->stream));
   PMX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->h_lob_its = hits;
   int bad = 0, first_bad = -1;
   ctx->eig_iters_max = 0;
   ctx->eig_iters_sum = 0;
@@ -1196,6 +1217,7 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
   a.stagnated = stag.p;
   a.stag_factor = c0->eig_stag;
   a.debug = -1;
+  a.order = nullptr;
   switch (MB) {
     case 4:
       launch_lobpcg_geo<4>(g, S, st, a, maxn);
