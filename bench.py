@@ -549,6 +549,8 @@ def main():
                     help="learned mode: seeded training fields of the workload's family")
     ap.add_argument("--train-epochs", type=int, default=30, help="learned mode: training epochs")
     ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
+    ap.add_argument("--extra", default="",
+                    help="setup_solve: options appended to every mode's (recorded in the JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 "
@@ -572,7 +574,7 @@ def main():
     for m in args.modes.split(","):
         prec = 64 if m == "fp64" else 32
         net = None
-        extra = "-eig_autotune 1"
+        extra = ("-eig_autotune 1 " + args.extra).strip()
         if m == "learned":
             net, training = train_warm_start(prob, nfields=args.train_fields,
                                              epochs=args.train_epochs)
@@ -624,6 +626,7 @@ def main():
             # launch runs the subdomains hardest first by the previous
             # setup's iteration counts (results do not depend on the order)
             "lobpcg_launch_order": "previous setup's iteration counts",
+            "extra_options": args.extra or None,
         },
         "modes": {m: dict(t_step=round(r["t_step"], 5), t_setup=round(r["t_setup"], 5),
                           t_solve=round(r["t_solve"], 5), its=r["its"],
