@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <sstream>
+#include <stdexcept>
 #include <thread>
 #include <tuple>
 
@@ -1120,7 +1121,12 @@ int pmx_setup_batch(pmx_ctx* const* ctxs, int nprob) {
       pmx_ctx* c = cs[order[k]];
       const double tc = now();
       c->t_stages[PMX_STAGE_SU_LV2_1] += t_eig;
-      rc[order[k]] = setup_coarse(c);
+      try {
+        rc[order[k]] = setup_coarse(c);
+      } catch (const std::exception& e) {  // host allocation inside a worker thread
+        set_error("coarse setup: %s", e.what());
+        rc[order[k]] = PMX_ERR_MEM;
+      }
       if (rc[order[k]]) {
         msg[order[k]] = g_err;
         continue;
