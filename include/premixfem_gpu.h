@@ -118,7 +118,10 @@ typedef struct {
  * per axis inside every box (:58-72), level 2 coupled inside a box only
  * (:806-888), en_lv2 level-3 vectors per box from its box-local Neumann
  * coarse matrix (:918-1175), A_cc over all boxes (:1177-1390); dense coarse
- * path, at most 64 level-3 vectors in total).  Allocates the
+ * path, at most 64 level-3 vectors in total), -eig_order 0|1 (launch the
+ * subdomain eigenproblems of a repeated setup hardest first by the previous
+ * setup's iteration counts, default 1; results do not depend on it),
+ * -batch_threads t (pmx_setup_batch host threads, default 8).  Allocates the
  * context and selects the current HIP device. */
 int pmx_init(pmx_ctx **ctx, const double *dom, const int *mesh, const char *options);
 
