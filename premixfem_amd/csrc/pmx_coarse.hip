@@ -1193,7 +1193,7 @@ __global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
                             double* __restrict__ T, double* __restrict__ wout,
                             int* __restrict__ flag) {
   constexpr int L = kLv3Max;
-  __shared__ double A[L * L], Q[L * L], tmp[L * L], w[L], rot[4 * L];
+  __shared__ double A[L * L], Q[L * L], tmp[L * L], Q1[L * L], w[L], rot[4 * L];
   __shared__ int fl;
   // mode 0 works on the column-scaled Gram D G D (D = diag(G)^{-1/2}), so
   // columns of very different norms (small residual directions) stay
@@ -1217,7 +1217,7 @@ __global__ void k_lv3_small(const double* __restrict__ Gm, int ns, int mode,
     }
     return;
   }
-  wg_syev(A, Q, tmp, w, rot, ns, L);
+  wg_syev2(A, tmp, Q, Q1, w, rot, ns, L);
   for (int e = threadIdx.x; e < ns * ns; e += blockDim.x) T[e] = Q[(e / ns) * L + e % ns];
   if (threadIdx.x < ns) wout[threadIdx.x] = w[threadIdx.x];
 }

@@ -64,23 +64,27 @@ __device__ inline void wg_matmul(const double* A, const double* B, double* C, in
   __syncthreads();
 }
 
-// Symmetric eigendecomposition A = Q diag(w) Q^T by parallel cyclic Jacobi
-// (round-robin ordering, m/2 disjoint rotations per round), eigenvalues in
-// ascending order.  A (destroyed), Q and tmp are m x m in LDS; w has m
-// entries; rot is scratch of 4*m doubles.  m <= 64.
-__device__ inline void wg_syev(double* A, double* Q, double* tmp, double* w, double* rot, int m,
-                               int ld, int max_sweeps = 40) {
+// Symmetric eigendecomposition by parallel cyclic Jacobi with two barriers
+// per round (ping-pong buffers).  Input in A0 (destroyed); on return the
+// eigenvectors are in Q0 (columns, ascending eigenvalues w).  A1, Q1 are
+// scratch; rot holds 3*ld doubles.  Rotations are applied only where the
+// coupling is significant relative to the diagonal (threshold Jacobi, which
+// keeps small eigenvalues relatively accurate); a sweep without rotations
+// ends the iteration.
+__device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
+                                double* rot, int m, int ld, int max_sweeps = 30) {
+  __shared__ int s_rot[2];
+  double* cc = rot;
+  double* ss = rot + ld;
+  int* part = reinterpret_cast<int*>(rot + 2 * ld);
   for (int e = threadIdx.x; e < m * m; e += blockDim.x)
-    Q[(e / m) * ld + e % m] = (e / m == e % m) ? 1.0 : 0.0;
-  __shared__ int s_rot;
+    Q0[(e / m) * ld + e % m] = (e / m == e % m) ? 1.0 : 0.0;
+  if (threadIdx.x == 0) s_rot[0] = s_rot[1] = 0;
   __syncthreads();
-  const int mp = (m + 1) & ~1;  // even number of players; index m is a dummy
-  double* cc = rot;             // per index: c
-  double* ss = rot + m;         // per index: signed s
-  int* part = reinterpret_cast<int*>(rot + 2 * m);  // per index: partner
+  const int mp = (m + 1) & ~1;
+  double *Ac = A0, *An = A1, *Qc = Q0, *Qn = Q1;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (threadIdx.x == 0) s_rot = 0;
-    __syncthreads();
+    const int fl = sweep & 1;
     for (int r = 0; r < mp - 1; ++r) {
       for (int k = threadIdx.x; k < mp / 2; k += blockDim.x) {
         int p, q;
@@ -97,13 +101,10 @@ __device__ inline void wg_syev(double* A, double* Q, double* tmp, double* w, dou
           q = t;
         }
         if (q < m) {
-          const double app = A[p * ld + p], aqq = A[q * ld + q], apq = A[p * ld + q];
+          const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
           double c = 1.0, s = 0.0;
-          // Rotate only when the coupling is significant relative to the
-          // diagonal (threshold Jacobi: high relative accuracy of small
-          // eigenvalues); a sweep without rotations ends the iteration.
           if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
-            s_rot = 1;
+            s_rot[fl] = 1;
             const double th = (aqq - app) / (2.0 * apq);
             const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
             c = 1.0 / sqrt(t * t + 1.0);
@@ -122,37 +123,35 @@ __device__ inline void wg_syev(double* A, double* Q, double* tmp, double* w, dou
         }
       }
       __syncthreads();
+      // every reader of the previous sweep's flag is past this point
+      if (r == 0 && threadIdx.x == 0) s_rot[fl ^ 1] = 0;
       for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
         const int i = e / m, j = e % m;
         const int ip = part[i], jp = part[j];
         const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
-        double v;
-        if (ip == j && ip != i) {
-          v = 0.0;  // the annihilated pair
-        } else {
-          v = ci * cj * A[i * ld + j] + ci * sj * A[i * ld + jp] + si * cj * A[ip * ld + j] +
-              si * sj * A[ip * ld + jp];
-        }
-        tmp[i * ld + j] = v;
+        double x;
+        if (ip == j && ip != i)
+          x = 0.0;
+        else
+          x = ci * cj * Ac[i * ld + j] + ci * sj * Ac[i * ld + jp] + si * cj * Ac[ip * ld + j] +
+              si * sj * Ac[ip * ld + jp];
+        An[i * ld + j] = x;
+        Qn[i * ld + j] = cj * Qc[i * ld + j] + sj * Qc[i * ld + jp];
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int i = e / m, j = e % m;
-        A[i * ld + j] = tmp[i * ld + j];
-        tmp[i * ld + j] = cc[j] * Q[i * ld + j] + ss[j] * Q[i * ld + part[j]];
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) Q[(e / m) * ld + e % m] = tmp[(e / m) * ld + e % m];
-      __syncthreads();
+      double* t = Ac;
+      Ac = An;
+      An = t;
+      t = Qc;
+      Qc = Qn;
+      Qn = t;
     }
-    if (s_rot == 0) break;
-    __syncthreads();
+    if (s_rot[fl] == 0) break;
   }
-  // Sort ascending (selection on thread 0), then permute Q's columns.
   int* perm = reinterpret_cast<int*>(rot);
   if (threadIdx.x == 0) {
     for (int i = 0; i < m; ++i) {
-      w[i] = A[i * ld + i];
+      w[i] = Ac[i * ld + i];
       perm[i] = i;
     }
     for (int i = 0; i < m; ++i) {
@@ -170,10 +169,11 @@ __device__ inline void wg_syev(double* A, double* Q, double* tmp, double* w, dou
   __syncthreads();
   for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
     const int i = e / m, j = e % m;
-    tmp[i * ld + j] = Q[i * ld + perm[j]];
+    An[i * ld + j] = Qc[i * ld + perm[j]];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x) Q[(e / m) * ld + e % m] = tmp[(e / m) * ld + e % m];
+  for (int e = threadIdx.x; e < m * m; e += blockDim.x)
+    Q0[(e / m) * ld + e % m] = An[(e / m) * ld + e % m];
   __syncthreads();
 }
 

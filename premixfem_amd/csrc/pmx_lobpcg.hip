@@ -245,118 +245,6 @@ __device__ inline void wg_transform(double* B, int n, const double* C, int ldc, 
   __syncthreads();
 }
 
-// Symmetric eigendecomposition by parallel cyclic Jacobi with two barriers
-// per round (ping-pong buffers).  Input in A0 (destroyed); on return the
-// eigenvectors are in Q0 (columns, ascending eigenvalues w).  A1, Q1 are
-// scratch; rot holds 3*ld doubles.  Rotations are applied only where the
-// coupling is significant relative to the diagonal (threshold Jacobi, which
-// keeps small eigenvalues relatively accurate); a sweep without rotations
-// ends the iteration.
-__device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
-                                double* rot, int m, int ld, int max_sweeps = 30) {
-  __shared__ int s_rot[2];
-  double* cc = rot;
-  double* ss = rot + ld;
-  int* part = reinterpret_cast<int*>(rot + 2 * ld);
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x)
-    Q0[(e / m) * ld + e % m] = (e / m == e % m) ? 1.0 : 0.0;
-  if (threadIdx.x == 0) s_rot[0] = s_rot[1] = 0;
-  __syncthreads();
-  const int mp = (m + 1) & ~1;
-  double *Ac = A0, *An = A1, *Qc = Q0, *Qn = Q1;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    const int fl = sweep & 1;
-    for (int r = 0; r < mp - 1; ++r) {
-      for (int k = threadIdx.x; k < mp / 2; k += blockDim.x) {
-        int p, q;
-        if (k == 0) {
-          p = r;
-          q = mp - 1;
-        } else {
-          p = (r + k) % (mp - 1);
-          q = (r - k + (mp - 1)) % (mp - 1);
-        }
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        if (q < m) {
-          const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
-          double c = 1.0, s = 0.0;
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
-            s_rot[fl] = 1;
-            const double th = (aqq - app) / (2.0 * apq);
-            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
-          }
-          cc[p] = c;
-          cc[q] = c;
-          ss[p] = -s;
-          ss[q] = s;
-          part[p] = q;
-          part[q] = p;
-        } else if (p < m) {
-          cc[p] = 1.0;
-          ss[p] = 0.0;
-          part[p] = p;
-        }
-      }
-      __syncthreads();
-      // every reader of the previous sweep's flag is past this point
-      if (r == 0 && threadIdx.x == 0) s_rot[fl ^ 1] = 0;
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int i = e / m, j = e % m;
-        const int ip = part[i], jp = part[j];
-        const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
-        double x;
-        if (ip == j && ip != i)
-          x = 0.0;
-        else
-          x = ci * cj * Ac[i * ld + j] + ci * sj * Ac[i * ld + jp] + si * cj * Ac[ip * ld + j] +
-              si * sj * Ac[ip * ld + jp];
-        An[i * ld + j] = x;
-        Qn[i * ld + j] = cj * Qc[i * ld + j] + sj * Qc[i * ld + jp];
-      }
-      __syncthreads();
-      double* t = Ac;
-      Ac = An;
-      An = t;
-      t = Qc;
-      Qc = Qn;
-      Qn = t;
-    }
-    if (s_rot[fl] == 0) break;
-  }
-  int* perm = reinterpret_cast<int*>(rot);
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < m; ++i) {
-      w[i] = Ac[i * ld + i];
-      perm[i] = i;
-    }
-    for (int i = 0; i < m; ++i) {
-      int best = i;
-      for (int j = i + 1; j < m; ++j)
-        if (w[j] < w[best]) best = j;
-      const double tw = w[i];
-      w[i] = w[best];
-      w[best] = tw;
-      const int tp = perm[i];
-      perm[i] = perm[best];
-      perm[best] = tp;
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-    const int i = e / m, j = e % m;
-    An[i * ld + j] = Qc[i * ld + perm[j]];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < m * m; e += blockDim.x)
-    Q0[(e / m) * ld + e % m] = An[(e / m) * ld + e % m];
-  __syncthreads();
-}
 
 // SVQB of an nb x nb M-Gram G (destroyed): Z = D U_kept theta^{-1/2},
 // D = diag(G)^{-1/2}; returns the number of kept columns.  G1 and Q1 are
