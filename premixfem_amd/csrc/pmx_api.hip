@@ -181,6 +181,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(getd(c->eig_sigma));
     } else if (k == "-eig_guard") {
       PMX_TRY(geti(c->eig_guard));
+    } else if (k == "-bicc_prefetch") {
+      PMX_TRY(geti(c->bicc_prefetch));
     } else if (k == "-eig_order") {
       PMX_TRY(geti(c->eig_order));
     } else if (k == "-eig_debug") {

@@ -769,6 +769,136 @@ __global__ void __launch_bounds__(1024) k_bicc_solve(const double* __restrict__ 
   }
 }
 
+// The same sweeps with the next round's row data loaded ahead: every round
+// of the flattened (direction, plane, row group) schedule first loads its
+// slot, D^{-1} row, neighbour coupling rows and right-hand side, which do
+// not depend on the sweep, so those loads overlap the previous round's
+// barriers instead of starting after them.  Only the backward sweep's own
+// value of its first plane (written by the last forward round) is read at
+// use time.  KP = lanes per subdomain = en_lv1 (8); the arithmetic and its
+// order are those of k_bicc_solve.
+template <int KP>
+__global__ void __launch_bounds__(1024) k_bicc_solve_pf(const double* __restrict__ Acb, int kmax,
+                                                        BiccArgs bi,
+                                                        const double* __restrict__ tcol,
+                                                        double* __restrict__ dcol, int nc) {
+  extern __shared__ double ring[];   // [2][maxcnt * KP]
+  __shared__ double vsh[2][1024];
+  __shared__ int pl[kBiccMaxPlanes + 1];
+  const double* t = tcol + (size_t)blockIdx.x * nc;
+  double* d = dcol + (size_t)blockIdx.x * nc;
+  for (int e = threadIdx.x; e <= bi.nplanes; e += blockDim.x) pl[e] = bi.plane[e];
+  __syncthreads();
+  const size_t kk = (size_t)kmax * kmax;
+  const int grp = threadIdx.x / KP, i = threadIdx.x % KP;
+  const int epr = blockDim.x / KP;
+  const int rs = bi.maxcnt * KP;
+  const int np = bi.nplanes;
+  // Row data of the current round (c*) and of the next one (n*).
+  bool cact = false, nact = false;
+  int cdof = 0, cneig = 0, ndof = 0, nneig = 0;
+  int cop[3], cko[3], nop[3], nko[3];
+  double cdi[KP], cB[3 * KP], cv0 = 0.0, ndi[KP], nB[3 * KP], nv0 = 0.0;
+#define PMX_BICC_FETCH(DIRN, PP, R0)                                                        \
+  {                                                                                        \
+    const bool ffwd = (DIRN) == 0;                                                         \
+    const int fp = ffwd ? (PP) : np - 1 - (PP);                                            \
+    const int fq = (R0) + grp;                                                             \
+    nact = false;                                                                          \
+    if (fq < pl[fp + 1] - pl[fp]) {                                                        \
+      const BiccSlot* sl = bi.slot + pl[fp] + fq;                                          \
+      nneig = sl->neig;                                                                    \
+      nact = i < nneig;                                                                    \
+      if (nact) {                                                                          \
+        const int fs = sl->s;                                                              \
+        ndof = sl->dof;                                                                    \
+        const double* Di = bi.dinv + (size_t)fs * kk + (size_t)i * kmax;                   \
+        _Pragma("unroll") for (int j = 0; j < KP; ++j) ndi[j] = j < nneig ? Di[j] : 0.0;   \
+        nv0 = ffwd ? t[ndof + i] : ((PP) == 0 ? 0.0 : d[ndof + i]);                        \
+        const double* blk = Acb + (size_t)fs * 7 * kk + (size_t)i * kmax;                  \
+        _Pragma("unroll") for (int a = 0; a < 3; ++a) {                                    \
+          nop[a] = ffwd ? sl->lpos[a] : sl->hpos[a];                                       \
+          nko[a] = ffwd ? sl->lk[a] : sl->hk[a];                                           \
+          const double* B = blk + (size_t)(2 * a + (ffwd ? 1 : 2)) * kk;                   \
+          _Pragma("unroll") for (int j = 0; j < KP; ++j) nB[a * KP + j] =                  \
+              (nop[a] >= 0 && j < nko[a]) ? B[j] : 0.0;                                    \
+        }                                                                                  \
+      }                                                                                    \
+    }                                                                                      \
+  }
+  int dirn = 0, pp = 0, r0 = 0, round = 0;
+  PMX_BICC_FETCH(0, 0, 0)
+  for (;;) {
+    cact = nact;
+    cdof = ndof;
+    cneig = nneig;
+    cv0 = nv0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      cop[a] = nop[a];
+      cko[a] = nko[a];
+    }
+#pragma unroll
+    for (int j = 0; j < KP; ++j) cdi[j] = ndi[j];
+#pragma unroll
+    for (int j = 0; j < 3 * KP; ++j) cB[j] = nB[j];
+    const bool fwd = dirn == 0;
+    const int p = fwd ? pp : np - 1 - pp;
+    double* cur = ring + (p & 1) * rs;
+    const double* oth = ring + ((p + 1) & 1) * rs;  // plane p-1 (fwd) or p+1 (bwd)
+    const int cnt = pl[p + 1] - pl[p];
+    double* vb = vsh[round & 1];
+    const int q = r0 + grp;
+    double v = 0.0;
+    if (cact) {
+      v = fwd ? cv0 : 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (cop[a] < 0) continue;
+        const double* xo = oth + cop[a] * KP;
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < KP; ++j)
+          if (j < cko[a]) acc += cB[a * KP + j] * xo[j];
+        v = fwd ? v - acc : v + acc;
+      }
+    }
+    vb[threadIdx.x] = v;
+    // Next round of the schedule, and its loads.
+    int nd = dirn, npp = pp, nr0 = r0 + epr;
+    const bool plane_end = nr0 >= cnt;
+    if (plane_end) {
+      nr0 = 0;
+      if (++npp == np) {
+        npp = 0;
+        ++nd;
+      }
+    }
+    const bool more = nd < 2;
+    nact = false;
+    if (more) PMX_BICC_FETCH(nd, npp, nr0)
+    __syncthreads();
+    if (cact) {
+      const double own = fwd ? 0.0 : (pp == 0 ? d[cdof + i] : cv0);
+      const double* vg = vb + grp * KP;
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < KP; ++j)
+        if (j < cneig) acc += cdi[j] * vg[j];
+      const double r = fwd ? acc : own - acc;
+      cur[q * KP + i] = r;
+      d[cdof + i] = r;
+    }
+    if (plane_end) __syncthreads();  // this plane's ring slot is complete before the next reads it
+    if (!more) break;
+    dirn = nd;
+    pp = npp;
+    r0 = nr0;
+    ++round;
+  }
+#undef PMX_BICC_FETCH
+}
+
 static int bicc_lanes(int kmax) {
   int kp = 1;
   while (kp < kmax) kp *= 2;
@@ -784,8 +914,14 @@ static BiccArgs bicc_args(const pmx_ctx* ctx) {
 int launch_bicc_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
   const BiccArgs bi = bicc_args(ctx);
   const size_t lds = 2 * (size_t)bi.maxcnt * bi.kp * sizeof(double);
-  hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->Acb.p,
-                     ctx->en_lv1, bi, t, d, ctx->nc);
+  // The prefetching variant reads whole KP-wide rows: only where KP = en_lv1
+  // (8, the spe10 fork's count; checked bit-identical to k_bicc_solve).
+  if (bi.kp == 8 && ctx->en_lv1 == 8 && ctx->bicc_prefetch)
+    hipLaunchKernelGGL(k_bicc_solve_pf<8>, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->Acb.p,
+                       ctx->en_lv1, bi, t, d, ctx->nc);
+  else
+    hipLaunchKernelGGL(k_bicc_solve, dim3(ncol), dim3(1024), lds, ctx->stream, ctx->Acb.p,
+                       ctx->en_lv1, bi, t, d, ctx->nc);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }

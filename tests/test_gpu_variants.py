@@ -150,3 +150,17 @@ def test_quad_stream_kernels_match_per_cell(name):
     assert r1["reason"] == "CONVERGED_RTOL" and r0["reason"] == "CONVERGED_RTOL"
     assert abs(r1["its"] - r0["its"]) <= 1
     assert err <= 1e-8
+
+
+def test_bicc_prefetch_solve_is_bit_identical():
+    """The block ICC(0) level-2 solve that loads its next round's rows ahead
+    (en_lv1 = 8) gives bit for bit the plain kernel's PCG solution."""
+    import dataclasses
+
+    prob = dataclasses.replace(_prob(), en_lv1=8, en_lv2=4)
+    opts = "-lv2_icc -eg_op 2 -no_shift_A_cc -bicc_prefetch "
+    a = runner.run(prob, precision=64, extra=opts + "1")
+    b = runner.run(prob, precision=64, extra=opts + "0")
+    print(f"bicc prefetch: its {a['its']} vs {b['its']}", flush=True)
+    assert a["its"] == b["its"]
+    assert np.array_equal(a["x"], b["x"])
