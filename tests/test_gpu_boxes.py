@@ -30,6 +30,8 @@ CASES = [
     ("2x1x2-W", (13, 10, 9), 3, 3, (2, 1, 2), dict(use_W_cycle=True), "-use_W_cycle"),
     ("2x1x2-eb_lv2", (13, 10, 9), 3, 3, (2, 1, 2), dict(eb_lv2=1e-9), "-eb_lv2 1e-9"),
     ("1x3x1-noshift", (12, 15, 8), 4, 2, (1, 3, 1), dict(no_shift_A_cc=True), "-no_shift_A_cc"),
+    ("2x1x2-si2", (13, 10, 9), 3, 3, (2, 1, 2), dict(si_lv1=2, si_lv2=2), "-si_lv1 2 -si_lv2 2"),
+    ("2x2x1-uneven", (11, 13, 7), 3, 2, (2, 2, 1), {}, ""),
     # 8 boxes x 4 vectors: a level-3 space above 16
     ("2x2x2", (16, 16, 16), 4, 4, (2, 2, 2), {}, ""),
     ("2x2x2-stagewise", (16, 16, 16), 4, 4, (2, 2, 2), {}, "-coarse_composite 0"),
