@@ -92,6 +92,11 @@ typedef struct {
   int coarse_mode;           /* level-2/3 path of the apply: 0 dense composite operator
                                 (one GEMV), 1 dense stage by stage, 2 band Cholesky,
                                 3 ICC(0) */
+  int coarse_stages;         /* coarse stages run per apply: the level-2/3 stages of the
+                                cycle, or 1 when -coarse_collapse cut them to the first
+                                level-2 stage (one process, exact level-2 factor) */
+  double lv3_floor;          /* level-3 eigensolver: the residual it stagnated at and
+                                accepted (0 when it met -eig_tol) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),
@@ -108,6 +113,9 @@ typedef struct {
  * -eig_guard g (LOBPCG guard vectors), -eig_autotune 0|1 (time the LOBPCG
  * block sizes on a sample of subdomains and keep the fastest, cached per
  * subdomain size and eigen count for the process),
+ * -coarse_collapse 0|1 (one process with an exact level-2 factor: apply only
+ * the first level-2 stage of the coarse cycle, since the later level-3 and
+ * level-2 stages add exactly zero in exact arithmetic; default 1),
  * -coarse_composite 0|1 (dense path: compose the coarse stages into one
  * operator, default, or apply them one by one), -coarse_deflate 0|1 (dense
  * path: invert A_c deflated with a Woodbury correction, default, or

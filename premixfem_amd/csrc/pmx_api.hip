@@ -165,6 +165,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(geti(c->quad_ok));
     } else if (k == "-coarse_composite") {
       PMX_TRY(geti(c->coarse_composite));
+    } else if (k == "-coarse_collapse") {
+      PMX_TRY(geti(c->coarse_collapse));
     } else if (k == "-coarse_band") {
       PMX_TRY(geti(c->coarse_band));
     } else if (k == "-coarse_deflate") {
@@ -476,6 +478,23 @@ static int setup_coarse(pmx_ctx* c) {
     for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
     prog.push_back(3);
     for (int i = 0; i < c->si_lv2; ++i) prog.push_back(2);
+  }
+  // One process, exact level-2 factor (dense inverse or band Cholesky of the
+  // full Galerkin matrix A_c = Phi^T A Phi): after the first level-2 stage
+  // the coarse residual Phi^T (x - A y) = t - A_c A_c^{-1} t is zero, so the
+  // level-3 stage and every later level-2 stage of the V- or W-cycle
+  // (src/PreMixFEM_3D.c:1857-1929) add exactly zero.  (Unpinned, A_c is
+  // singular with the coarse constant as null vector; the pseudo-inverse
+  // leaves t's component along it, which is 1^T r = 0 for the consistent
+  // residuals of a zero-sum right-hand side.)  Computed in floating point,
+  // those stages add only rounding, amplified by up to 1/lambda_min(A_c)
+  // (~1e11 with one pinned cell), so the program is cut to the first stage.
+  // Not with process boxes (A_c keeps only the couplings inside a box) nor
+  // with the inexact ICC(0) level-2 factor.
+  c->coarse_collapsed = 0;
+  if (c->coarse_collapse && c->nbox == 1 && !c->lv2_icc && !c->use_2level && prog.size() > 1) {
+    prog.assign(1, 2);
+    c->coarse_collapsed = 1;
   }
   if (c->pc_precision == 32) {
     const size_t nphi = (size_t)c->ncell * c->en_lv1;
@@ -1397,6 +1416,8 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_stagnated = c->eig_stagnated;
   st->eig_block = c->eig_block;
   st->coarse_mode = c->lv2_icc ? 3 : (c->use_band ? 2 : (c->coarse_composite ? 0 : 1));
+  st->coarse_stages = (int)c->coarse_prog.size();
+  st->lv3_floor = c->lv3_floor;
   st->lv3_iters = c->lv3_iters;
   st->pc_precision = c->pc_precision;
   st->its = c->last_its;

@@ -1836,6 +1836,13 @@ int level3_setup_boxes(pmx_ctx* ctx) {
     PMX_HIP(hipStreamSynchronize(ctx->stream));
     if (std::all_of(hrot.begin(), hrot.end(), [](int v) { return v == 0; })) break;
   }
+  if (sweeps == 40) {
+    int b = 0;
+    while (hrot[b] == 0) ++b;
+    set_error("level-3 box eigenproblems: the Jacobi sweeps still rotate in box %d after 40 "
+              "sweeps", b);
+    return PMX_ERR_NOT_CONVERGED;
+  }
   hipLaunchKernelGGL(k_box_pick, dim3(nbox), dim3(256), 0, ctx->stream, Aj.p, Qj.p, n, d_dofs.p,
                      d_boff.p, d_moff.p, k2max, Vt.p, wv.p);
   PMX_HIP(hipGetLastError());

@@ -376,6 +376,11 @@ struct pmx_ctx {
                                     // coarse stages one by one with their residual updates (0)
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
+  // One process with an exact level-2 factor: the coarse stages after the
+  // first level-2 stage vanish in exact arithmetic, and the program is cut
+  // to that stage (-coarse_collapse 1, default); 0 keeps every stage.
+  int coarse_collapse = 1;
+  int coarse_collapsed = 0;         // the last setup cut the program
   // Level-2 factor of the spe10 fork (src/spe10.c:509-516): ICC(0) of A_c,
   // i.e. a block incomplete Cholesky over the subdomains (-lv2_icc).
   int lv2_icc = 0;

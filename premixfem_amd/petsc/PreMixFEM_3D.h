@@ -48,6 +48,8 @@ typedef struct preconditioner_context {
   PetscBool use_W_cycle, no_shift_A_cc, use_full_Cholesky_lv1, use_2level;
   PetscInt M, N, P;
   pmx_ctx *gpu;      /* the GPU preconditioner */
+  Mat op;            /* the PC operator PC_setup checked against kappa ... */
+  PetscObjectState op_state;  /* ... and its state then */
 } PCCtx;
 
 PetscErrorCode PC_init(PCCtx *s_ctx, PetscScalar *dom, PetscInt *mesh);

@@ -174,25 +174,116 @@ PetscErrorCode PC_create_A(PCCtx *s_ctx, Mat *A) {
   PetscFunctionReturn(PETSC_SUCCESS);
 }
 
-/* src/PreMixFEM_3D.c:1826-1838 (PCShellSetSetUp callback). */
-PetscErrorCode PC_setup(PC pc) {
-  PCCtx *s_ctx;
+/* Pushes the caller's kappa vectors to the GPU context (the reference's
+ * _PC_setup reads s_ctx->kappa at every setup, src/PreMixFEM_3D.c:1398-1402). */
+static PetscErrorCode push_kappa(PCCtx *s_ctx) {
+  const PetscScalar *k[DIM];
+  int d, rc;
 
   PetscFunctionBeginUser;
-  PetscCall(PCShellGetContext(pc, &s_ctx));
-  PMX_CALL(pmx_setup(s_ctx->gpu));
+  for (d = 0; d < DIM; ++d) PetscCall(VecGetArrayRead(s_ctx->kappa[d], &k[d]));
+  rc = pmx_set_kappa(s_ctx->gpu, k[0], k[1], k[2]);
+  for (d = 0; d < DIM; ++d) PetscCall(VecRestoreArrayRead(s_ctx->kappa[d], &k[d]));
+  if (rc) SETERRQ(PETSC_COMM_SELF, rc, "%s", pmx_last_error());
   PetscFunctionReturn(PETSC_SUCCESS);
 }
 
-/* src/PreMixFEM_3D.c:1840-1936 (PCShellSetApply callback): y = B x. */
+/* The reference's apply takes A from PCGetOperators (src/PreMixFEM_3D.c:1852)
+ * for its residuals; the GPU builds the same operator from kappa.  Every row
+ * of A must equal the GPU's (PC_create_A's) within 1e-12 of its diagonal, or
+ * the setup is refused: a preconditioner of another operator would be built
+ * silently otherwise. */
+static PetscErrorCode check_operator(PCCtx *s_ctx, Mat A) {
+  static const int dx[6] = {-1, 1, 0, 0, 0, 0}, dy[6] = {0, 0, -1, 1, 0, 0},
+                   dz[6] = {0, 0, 0, 0, -1, 1};
+  const PetscInt M = s_ctx->M, N = s_ctx->N, P = s_ctx->P, n = M * N * P;
+  PetscScalar *diag, *off;
+  PetscInt row, bad = -1;
+
+  PetscFunctionBeginUser;
+  PetscCall(PetscMalloc2(n, &diag, 6 * n, &off));
+  {
+    const int rc = pmx_create_A(s_ctx->gpu, diag, off);
+    if (rc) {
+      PetscCall(PetscFree2(diag, off));
+      SETERRQ(PETSC_COMM_SELF, rc, "%s", pmx_last_error());
+    }
+  }
+  for (row = 0; row < n && bad < 0; ++row) {
+    const PetscInt ex = row % M, ey = (row / M) % N, ez = row / (M * N);
+    const PetscScalar tol = 1.0e-12 * (diag[row] > 0.0 ? diag[row] : -diag[row]);
+    PetscScalar want[7];
+    PetscInt want_col[7], nw = 0, ncols, j, d;
+    const PetscInt *cols;
+    const PetscScalar *vals;
+    want_col[nw] = row;
+    want[nw++] = diag[row];
+    for (d = 0; d < 6; ++d) {
+      const PetscInt x = ex + dx[d], y = ey + dy[d], z = ez + dz[d];
+      if (x < 0 || x >= M || y < 0 || y >= N || z < 0 || z >= P) continue;
+      want_col[nw] = (z * N + y) * M + x;
+      want[nw++] = off[6 * row + d];
+    }
+    PetscCall(MatGetRow(A, row, &ncols, &cols, &vals));
+    /* every stored entry of the row must be one of the 7-point entries with
+     * the GPU's value; entries the row does not store must be zero there */
+    for (j = 0; j < ncols && bad < 0; ++j) {
+      PetscInt w = 0;
+      while (w < nw && want_col[w] != cols[j]) ++w;
+      if (w == nw ? (vals[j] != 0.0) : (vals[j] - want[w] > tol || want[w] - vals[j] > tol))
+        bad = row;
+      else if (w < nw)
+        want_col[w] = -1 - want_col[w]; /* matched */
+    }
+    for (d = 0; d < nw && bad < 0; ++d)
+      if (want_col[d] >= 0 && (want[d] > tol || -want[d] > tol)) bad = row;
+    PetscCall(MatRestoreRow(A, row, &ncols, &cols, &vals));
+  }
+  PetscCall(PetscFree2(diag, off));
+  PetscCheck(bad < 0, PETSC_COMM_WORLD, PETSC_ERR_ARG_WRONG,
+             "PC_setup: the operator of the PC differs from the one built from s_ctx->kappa "
+             "(first at row %d); the GPU preconditioner supports PC_create_A's operator only",
+             (int)bad);
+  PetscFunctionReturn(PETSC_SUCCESS);
+}
+
+/* src/PreMixFEM_3D.c:1826-1838 (PCShellSetSetUp callback): kappa is read
+ * again (as _PC_setup does, :1398-1402), the PC's operator is checked
+ * against it, and the GPU preconditioner is built. */
+PetscErrorCode PC_setup(PC pc) {
+  PCCtx *s_ctx;
+  Mat A;
+
+  PetscFunctionBeginUser;
+  PetscCall(PCShellGetContext(pc, &s_ctx));
+  PetscCall(PCGetOperators(pc, &A, NULL));
+  PetscCheck(s_ctx->kappa[0] && s_ctx->kappa[1] && s_ctx->kappa[2], PETSC_COMM_WORLD,
+             PETSC_ERR_ARG_WRONG, "PC_setup: s_ctx->kappa has not been created");
+  PetscCall(push_kappa(s_ctx));
+  PetscCall(check_operator(s_ctx, A));
+  PMX_CALL(pmx_setup(s_ctx->gpu));
+  s_ctx->op = A;
+  PetscCall(PetscObjectStateGet((PetscObject)A, &s_ctx->op_state));
+  PetscFunctionReturn(PETSC_SUCCESS);
+}
+
+/* src/PreMixFEM_3D.c:1840-1936 (PCShellSetApply callback): y = B x.  The
+ * residuals of the cycle use the operator PC_setup checked; an operator
+ * that has been replaced or modified since is refused. */
 PetscErrorCode PC_apply_vec(PC pc, Vec x, Vec y) {
   PCCtx *s_ctx;
   const PetscScalar *xa;
   PetscScalar *ya;
+  Mat A;
+  PetscObjectState st;
   int rc;
 
   PetscFunctionBeginUser;
   PetscCall(PCShellGetContext(pc, &s_ctx));
+  PetscCall(PCGetOperators(pc, &A, NULL));
+  PetscCall(PetscObjectStateGet((PetscObject)A, &st));
+  PetscCheck(A == s_ctx->op && st == s_ctx->op_state, PETSC_COMM_WORLD, PETSC_ERR_ARG_WRONG,
+             "PC_apply_vec: the operator changed after PC_setup");
   PetscCall(VecGetArrayRead(x, &xa));
   PetscCall(VecGetArray(y, &ya));
   rc = pmx_apply(s_ctx->gpu, xa, ya);

@@ -6,7 +6,14 @@
  * system and reports the relative difference of the two solutions.
  *
  * Output: one line "RESULT its=<n> reason=<r> rnorm=<x> [gpu_its=<n>
- * gpu_reason=<r> rel_diff=<x>]".  Test infrastructure only. */
+ * gpu_reason=<r> rel_diff=<x>]".
+ *
+ * -kappa_rescale: after the first solve, kappa is doubled in place and the
+ * operator scaled by 2 (the operator of 2 kappa, exactly), and the system is
+ * solved again: PC_setup must read the new kappa.  Prints "RESULT2 its=<n>
+ * reason=<r> ratio=<|x1|/|x2|>" (2 when the second solve used 2 kappa).
+ * -operator_mismatch: the operator alone is scaled before the second solve,
+ * which PC_setup must refuse.  Test infrastructure only. */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -58,7 +65,7 @@ static PetscErrorCode create_rhs(PCCtx *s_ctx, Vec *rhs) {
 static PetscErrorCode run(void) {
   PetscScalar dom[3] = {1.0, 1.0, 1.0};
   PetscInt mesh[3] = {16, 16, 16}, its, n;
-  PetscBool compare = PETSC_FALSE;
+  PetscBool compare = PETSC_FALSE, rescale = PETSC_FALSE, mismatch = PETSC_FALSE;
   PCCtx s_ctx;
   Mat A;
   Vec rhs, u;
@@ -72,6 +79,8 @@ static PetscErrorCode run(void) {
   PetscCall(PetscOptionsGetInt(NULL, NULL, "-mesh_y", &mesh[1], NULL));
   PetscCall(PetscOptionsGetInt(NULL, NULL, "-mesh_z", &mesh[2], NULL));
   PetscCall(PetscOptionsHasName(NULL, NULL, "-compare_gpu_pcg", &compare));
+  PetscCall(PetscOptionsHasName(NULL, NULL, "-kappa_rescale", &rescale));
+  PetscCall(PetscOptionsHasName(NULL, NULL, "-operator_mismatch", &mismatch));
   PetscCall(PC_init(&s_ctx, &dom[0], &mesh[0]));
   PetscCall(PC_print_info(&s_ctx));
   PetscCall(create_kappa(&s_ctx));
@@ -121,6 +130,26 @@ static PetscErrorCode run(void) {
     free(xg);
   }
   printf("\n");
+  if (rescale || mismatch) {
+    PetscReal n1, n2;
+    PetscCall(VecNorm(u, NORM_2, &n1));
+    if (rescale)
+      for (int d = 0; d < DIM; ++d) {
+        PetscScalar *a;
+        PetscInt i, nk;
+        PetscCall(VecGetSize(s_ctx.kappa[d], &nk));
+        PetscCall(VecGetArray(s_ctx.kappa[d], &a));
+        for (i = 0; i < nk; ++i) a[i] *= 2.0;
+        PetscCall(VecRestoreArray(s_ctx.kappa[d], &a));
+      }
+    PetscCall(MatScale(A, 2.0));
+    PetscCall(VecSet(u, 0.0));
+    PetscCall(KSPSolve(ksp, rhs, u));
+    PetscCall(KSPGetIterationNumber(ksp, &its));
+    PetscCall(KSPGetConvergedReason(ksp, &reason));
+    PetscCall(VecNorm(u, NORM_2, &n2));
+    printf("RESULT2 its=%d reason=%d ratio=%.15e\n", (int)its, (int)reason, (double)(n1 / n2));
+  }
   PetscCall(PC_print_stat(&s_ctx));
   PetscCall(KSPDestroy(&ksp));
   PetscCall(VecDestroy(&rhs));

@@ -152,8 +152,11 @@ struct mock_vec {
   double *a;
 };
 struct mock_mat {
+  PetscObjectState state; /* first member: PetscObjectStateGet reads it */
   PetscInt M, N, P;
   double *v; /* [cell][7]: diag, -x, +x, -y, +y, -z, +z */
+  PetscInt row_cols[7];   /* MatGetRow's output of the current row */
+  PetscScalar row_vals[7];
 };
 
 PetscErrorCode DMDACreate3d(MPI_Comm comm, DMBoundaryType bx, DMBoundaryType by,
@@ -300,6 +303,56 @@ PetscErrorCode MatSetValuesStencil(Mat A, PetscInt m, const MatStencil *row, Pet
         A->v[(size_t)cell * 7 + slot] += v[r * n + c];
     }
   }
+  A->state++;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode MatScale(Mat A, PetscScalar a) {
+  size_t i, n = (size_t)A->M * A->N * A->P * 7;
+  for (i = 0; i < n; ++i) A->v[i] *= a;
+  A->state++;
+  return PETSC_SUCCESS;
+}
+
+/* The stored entries of one row in increasing column order (the 7-point
+ * star inside the grid). */
+PetscErrorCode MatGetRow(Mat A, PetscInt row, PetscInt *ncols, const PetscInt **cols,
+                         const PetscScalar **vals) {
+  const PetscInt M = A->M, N = A->N, P = A->P;
+  const PetscInt ex = row % M, ey = (row / M) % N, ez = row / (M * N);
+  const double *v = A->v + (size_t)row * 7;
+  PetscInt k = 0;
+  if (row < 0 || row >= M * N * P) return mock_error(PETSC_ERR_ARG_OUTOFRANGE, __FILE__, __LINE__, "row");
+#define MOCK_ROW_ENTRY(cond, col, slot) \
+  if (cond) {                          \
+    A->row_cols[k] = (col);            \
+    A->row_vals[k++] = v[slot];        \
+  }
+  MOCK_ROW_ENTRY(ez > 0, row - M * N, 5)
+  MOCK_ROW_ENTRY(ey > 0, row - M, 3)
+  MOCK_ROW_ENTRY(ex > 0, row - 1, 1)
+  MOCK_ROW_ENTRY(1, row, 0)
+  MOCK_ROW_ENTRY(ex < M - 1, row + 1, 2)
+  MOCK_ROW_ENTRY(ey < N - 1, row + M, 4)
+  MOCK_ROW_ENTRY(ez < P - 1, row + M * N, 6)
+#undef MOCK_ROW_ENTRY
+  *ncols = k;
+  *cols = A->row_cols;
+  *vals = A->row_vals;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode MatRestoreRow(Mat A, PetscInt row, PetscInt *ncols, const PetscInt **cols,
+                             const PetscScalar **vals) {
+  (void)A, (void)row;
+  *ncols = 0;
+  *cols = NULL;
+  *vals = NULL;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PetscObjectStateGet(PetscObject obj, PetscObjectState *state) {
+  *state = ((struct mock_mat *)obj)->state;
   return PETSC_SUCCESS;
 }
 
@@ -344,6 +397,8 @@ PetscErrorCode MatDestroy(Mat *A) {
 
 /* ---- KSP (CG) and PCSHELL ---- */
 struct mock_pc {
+  Mat A;                      /* operator (KSPSetOperators) */
+  PetscObjectState setup_state; /* its state at the last setup */
   void *ctx;
   PetscErrorCode (*setup)(PC);
   PetscErrorCode (*apply)(PC, Vec, Vec);
@@ -372,6 +427,14 @@ PetscErrorCode KSPCreate(MPI_Comm comm, KSP *ksp) {
 PetscErrorCode KSPSetOperators(KSP ksp, Mat A, Mat P) {
   (void)P;
   ksp->A = A;
+  ksp->pc.A = A;
+  ksp->pc.set_up = 0;
+  return PETSC_SUCCESS;
+}
+
+PetscErrorCode PCGetOperators(PC pc, Mat *Amat, Mat *Pmat) {
+  if (Amat) *Amat = pc->A;
+  if (Pmat) *Pmat = pc->A;
   return PETSC_SUCCESS;
 }
 
@@ -392,10 +455,14 @@ PetscErrorCode KSPSetFromOptions(KSP ksp) {
   return PETSC_SUCCESS;
 }
 
+/* PCSetUp semantics: the shell's setup runs again when the operator was
+ * set anew or its values changed since the last setup. */
 PetscErrorCode KSPSetUp(KSP ksp) {
-  if (ksp->pc.is_shell && ksp->pc.setup && !ksp->pc.set_up) {
+  if (ksp->pc.is_shell && ksp->pc.setup &&
+      (!ksp->pc.set_up || ksp->A->state != ksp->pc.setup_state)) {
     PetscCall(ksp->pc.setup(&ksp->pc));
     ksp->pc.set_up = 1;
+    ksp->pc.setup_state = ksp->A->state;
   }
   return PETSC_SUCCESS;
 }

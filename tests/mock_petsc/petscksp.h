@@ -16,6 +16,8 @@ typedef int PetscMPIInt;
 typedef double PetscScalar;
 typedef double PetscReal;
 typedef enum { PETSC_FALSE = 0, PETSC_TRUE = 1 } PetscBool;
+typedef long long PetscObjectState;
+typedef void *PetscObject;
 
 #define PETSC_SUCCESS 0
 #define PETSC_COMM_WORLD 0
@@ -108,7 +110,15 @@ PetscErrorCode MatSetValuesStencil(Mat A, PetscInt m, const MatStencil *row, Pet
 PetscErrorCode MatAssemblyBegin(Mat A, MatAssemblyType t);
 PetscErrorCode MatAssemblyEnd(Mat A, MatAssemblyType t);
 PetscErrorCode MatMult(Mat A, Vec x, Vec y);
+PetscErrorCode MatScale(Mat A, PetscScalar a);
+PetscErrorCode MatGetRow(Mat A, PetscInt row, PetscInt *ncols, const PetscInt **cols,
+                         const PetscScalar **vals);
+PetscErrorCode MatRestoreRow(Mat A, PetscInt row, PetscInt *ncols, const PetscInt **cols,
+                             const PetscScalar **vals);
 PetscErrorCode MatDestroy(Mat *A);
+/* Object state: increases with every change of the object's values (the
+ * mock knows one kind of object with a state, Mat). */
+PetscErrorCode PetscObjectStateGet(PetscObject obj, PetscObjectState *state);
 
 /* KSP / PC */
 typedef struct mock_ksp *KSP;
@@ -129,6 +139,7 @@ PetscErrorCode KSPGetConvergedReason(KSP ksp, KSPConvergedReason *r);
 PetscErrorCode KSPGetResidualNorm(KSP ksp, PetscReal *rn);
 PetscErrorCode KSPDestroy(KSP *ksp);
 PetscErrorCode PCSetType(PC pc, PCType t);
+PetscErrorCode PCGetOperators(PC pc, Mat *Amat, Mat *Pmat);
 PetscErrorCode PCShellSetContext(PC pc, void *ctx);
 PetscErrorCode PCShellGetContext(PC pc, void *ctx);
 PetscErrorCode PCShellSetSetUp(PC pc, PetscErrorCode (*f)(PC));

@@ -45,6 +45,9 @@ static double *dup(const double *a, size_t n) {
 
 int pmx_set_kappa(pmx_ctx *c, const double *kx, const double *ky, const double *kz) {
   const size_t n = (size_t)c->M * c->N * c->P;
+  free(c->kx);
+  free(c->ky);
+  free(c->kz);
   c->kx = dup(kx, n);
   c->ky = dup(ky, n);
   c->kz = dup(kz, n);
@@ -75,6 +78,7 @@ int pmx_create_A(pmx_ctx *c, double *diag, double *off) {
         diag[i] = 0.0;
         for (int d = 0; d < 6; ++d) diag[i] -= off[6 * i + d];
       }
+  free(c->diag);
   c->diag = dup(diag, n);
   return PMX_SUCCESS;
 }

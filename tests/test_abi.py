@@ -105,3 +105,31 @@ def test_pcshell_adapter_propagates_error_codes():
     r = subprocess.run([str(exe), "-en_lv1", "0"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 62
     assert "max_eigen_num_lv1=0" in r.stderr
+
+
+def test_pcshell_setup_reads_kappa_again():
+    """The reference's _PC_setup reads s_ctx->kappa at every setup
+    (src/PreMixFEM_3D.c:1398-1402): after kappa is doubled (and the operator
+    with it), the second KSPSolve uses the new kappa, so the solution halves."""
+    _make("stub")
+    exe = MOCK / "_build" / "driver_stub"
+    r = subprocess.run([str(exe), "-mesh_x", "12", "-mesh_y", "10", "-mesh_z", "8",
+                        "-ksp_rtol", "1e-10", "-kappa_rescale"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    res = next(l for l in r.stdout.splitlines() if l.startswith("RESULT2"))
+    fields = dict(kv.split("=") for kv in res.split()[1:])
+    assert fields["reason"] == "2"
+    assert float(fields["ratio"]) == pytest.approx(2.0, rel=1e-12)
+
+
+def test_pcshell_setup_refuses_an_operator_not_built_from_kappa():
+    """PC_apply_vec's residuals use the PC's operator (:1852); the GPU
+    preconditioner is built from kappa, so an operator that differs from
+    PC_create_A's is refused with PETSC_ERR_ARG_WRONG."""
+    _make("stub")
+    exe = MOCK / "_build" / "driver_stub"
+    r = subprocess.run([str(exe), "-mesh_x", "12", "-mesh_y", "10", "-mesh_z", "8",
+                        "-operator_mismatch"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 62, (r.returncode, r.stderr)
+    assert "differs from the one built from s_ctx->kappa" in r.stderr

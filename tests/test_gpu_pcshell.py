@@ -33,3 +33,29 @@ def test_host_cg_through_pcshell_matches_gpu_pcg(args):
     assert f["reason"] == "2" and f["gpu_reason"] == "2"
     assert abs(int(f["its"]) - int(f["gpu_its"])) <= 1
     assert float(f["rel_diff"]) <= 1e-8
+
+
+@pytest.mark.gpu
+def test_pcshell_setup_rereads_kappa_on_the_gpu():
+    """Doubling kappa (and the operator) between two KSPSolve calls: the
+    second setup reads the new kappa (src/PreMixFEM_3D.c:1398-1402), so the
+    GPU preconditioner matches the new operator and the solution halves; an
+    operator changed without kappa is refused (PETSC_ERR_ARG_WRONG)."""
+    assert EXE.exists(), f"{EXE} is not built: run __graft_entry__.build()"
+    base = ["-mesh_x", "16", "-mesh_y", "16", "-mesh_z", "16", "-sd", "2", "-en_lv1", "4",
+            "-pin", "15,0,0", "-ksp_rtol", "1e-10"]
+    r = subprocess.run([str(EXE), *base, "-kappa_rescale"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    first = next(l for l in r.stdout.splitlines() if l.startswith("RESULT "))
+    second = next(l for l in r.stdout.splitlines() if l.startswith("RESULT2"))
+    f1 = dict(kv.split("=") for kv in first.split()[1:])
+    f2 = dict(kv.split("=") for kv in second.split()[1:])
+    assert f1["reason"] == "2" and f2["reason"] == "2"
+    # the preconditioner of 2 kappa is half the one of kappa: same iterations
+    assert abs(int(f1["its"]) - int(f2["its"])) <= 1
+    assert float(f2["ratio"]) == pytest.approx(2.0, rel=1e-8)
+    r = subprocess.run([str(EXE), *base, "-operator_mismatch"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 62, (r.returncode, r.stderr[-2000:])

@@ -4,7 +4,9 @@ steps on levels 1 and 2, the MEAN and DIAG eigen mass operators, the
 no-shift coarse-coarse matrix and an adaptive eigen count (eb_lv1).  The
 oracle's level-2 basis is given to the GPU so that the comparison isolates
 the cycle; y = B x within 1e-10, PCG iterations within 1 and the pressure
-within 1e-8."""
+within 1e-8.  On one process the GPU applies only the first level-2 stage
+of the coarse cycle by default (the later stages add exactly zero in exact
+arithmetic); the "-full" variants run every stage as the oracle does."""
 import numpy as np
 import pytest
 
@@ -32,6 +34,13 @@ VARIANTS = [
     ("full-Cholesky-lv1", dict(use_full_Cholesky_lv1=True), "-use_full_Cholesky_lv1"),
     ("full-Cholesky-lv1-W", dict(use_full_Cholesky_lv1=True, use_W_cycle=True),
      "-use_full_Cholesky_lv1 -use_W_cycle"),
+    # every coarse stage of the cycle run (no cut to the first level-2
+    # stage): the composed operator, the stage-wise and the band programs
+    ("V-full", dict(), "-coarse_collapse 0"),
+    ("W-cycle-full", dict(use_W_cycle=True), "-use_W_cycle -coarse_collapse 0"),
+    ("W-si2-full", dict(use_W_cycle=True, si_lv2=2), "-use_W_cycle -si_lv2 2 -coarse_collapse 0"),
+    ("stagewise-full", dict(), "-coarse_composite 0 -coarse_collapse 0"),
+    ("band-W-full", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle -coarse_collapse 0"),
 ]
 
 
@@ -162,5 +171,30 @@ def test_bicc_prefetch_solve_is_bit_identical():
     a = runner.run(prob, precision=64, extra=opts + "1")
     b = runner.run(prob, precision=64, extra=opts + "0")
     print(f"bicc prefetch: its {a['its']} vs {b['its']}", flush=True)
+    assert a["its"] == b["its"]
+    assert np.array_equal(a["x"], b["x"])
+
+
+def test_bicc_prefetch_solve_is_bit_identical_with_multi_round_planes():
+    """The same on the reservoir grid's subdomain split (10 x 20 x 17
+    subdomains, here of 4^3 cells): up to 170 subdomains share a coarse
+    hyperplane, so a plane takes several rounds of the prefetching sweep
+    (its same-plane prefetch, the double-buffered right-hand side and the
+    backward reads of D^-1 rows)."""
+    mesh = (40, 80, 68)
+    dom = (1.0, 1.0, 1.0)
+    kap = problems.layered_kappa(mesh, 2.0, (10, 10, 2), 1e-3, 2e4, 31)
+    prob = problems.Problem("planes", dom, mesh, kap, problems.corner_rhs(mesh, dom),
+                            sd=(10, 20, 17), en_lv1=8, en_lv2=4, eg_op=2,
+                            pin_cell=(mesh[0] - 1, 0, 0))
+    sx, sy, sz = np.meshgrid(*(np.arange(n) for n in prob.sd), indexing="ij")
+    widest = np.bincount((sx + sy + sz).ravel()).max()
+    assert widest > 128, widest
+    opts = "-lv2_icc -no_shift_A_cc -bicc_prefetch "
+    print("bicc prefetch on", mesh, prob.sd, "widest plane", widest, flush=True)
+    a = runner.run(prob, precision=64, extra=opts + "1")
+    b = runner.run(prob, precision=64, extra=opts + "0")
+    print(f"bicc prefetch (multi-round planes): its {a['its']} vs {b['its']}", flush=True)
+    assert a["reason"] == "CONVERGED_RTOL"
     assert a["its"] == b["its"]
     assert np.array_equal(a["x"], b["x"])

@@ -192,3 +192,46 @@ def test_face_fluxes_divergence_is_operator():
     Fx1, Fy1, Fz1 = O.face_fluxes(lay, T1, xs)
     np.testing.assert_allclose(Fx1, -lay.meas_face_yz ** 2 / lay.meas_elem, rtol=1e-14)
     assert np.abs(Fy1).max() == 0.0 and np.abs(Fz1).max() == 0.0
+
+
+def _upwind_case(sign):
+    """A 6 x 6 x 5 reservoir whose pressure varies along z only, so the x and
+    y faces carry no flux; column (1, 1) has no well."""
+    mesh = (6, 6, 5)
+    prm = S.Spe10Params()
+    lay = S.layout(mesh, (2, 2, 1), prm.H)
+    M, N, P = mesh
+    rng = np.random.default_rng(4)
+    perm = [np.exp(rng.standard_normal((P, N, M))) * 1e-13 for _ in range(3)]
+    Sw = 0.2 + 0.6 * rng.random((P, N, M))
+    kappa = S.kappa_from(prm, perm, Sw)
+    poro = 0.1 + 0.2 * rng.random((P, N, M))
+    p = np.broadcast_to((sign * (P - np.arange(P)) * 1.0e3)[:, None, None], (P, N, M)).copy()
+    return prm, lay, perm, kappa, poro, p, Sw
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0], ids=["flow+z", "flow-z"])
+def test_saturation_rate_takes_the_upwind_fractional_flow(sign):
+    """get_next_S_w_multisteps (src/spe10.c:1985-2032) weights each face flux
+    by the fractional flow of the cell the flow comes from (the cell with
+    the higher pressure).  Cell (1, 1, z) of a column whose pressure falls
+    (flow +z) or rises (flow -z) along z, computed by hand per face: a
+    downwind weighting gives a different rate."""
+    prm, lay, perm, kappa, poro, p, Sw = _upwind_case(sign)
+    rate = S.saturation_rate(prm, lay, perm, kappa, poro, p, Sw)
+    lw, _, lt = S.mobility(prm, Sw)
+    fw = lw / lt
+    hz2 = lay.Hz * lay.Hz
+    x = y = 1
+    for z in (1, 2, 3):
+        up, down = 0.0, 0.0
+        for nz in (z - 1, z + 1):
+            kb = 2.0 / (1.0 / kappa[2][nz, y, x] + 1.0 / kappa[2][z, y, x])
+            flux_in = kb * (p[nz, y, x] - p[z, y, x]) / hz2
+            src = nz if p[nz, y, x] > p[z, y, x] else z
+            dst = z if src == nz else nz
+            up += fw[src, y, x] * flux_in
+            down += fw[dst, y, x] * flux_in
+        want = up / poro[z, y, x]
+        assert rate[z, y, x] == pytest.approx(want, rel=1e-12)
+        assert abs(rate[z, y, x] - down / poro[z, y, x]) > 1e-6 * abs(want)
