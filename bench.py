@@ -493,9 +493,11 @@ def batch_main(args):
 def solve_roofline(prob, r, s, workload, head):
     """Per-kernel durations of a kernel-timed solve and the roofline of the
     dominant kernel (most total time among those with a byte model).  The
-    coarse slot ("k_gemv") has a byte model only on the dense composite
-    path (one GEMV); on the band / stage-wise / ICC(0) paths it times the
-    whole coarse program and is reported without one."""
+    coarse slot ("k_gemv") times one level-2 coarse solve: the GEMV with the
+    composite operator on the dense path ("k_gemv"), the plane / band /
+    ICC(0) sweeps on the sparse paths ("coarse_solve"); its bytes are the
+    factor bytes that solve reads (the library's coarse_solve_bytes) plus
+    the coarse vectors."""
     from premixfem_amd._lib import TIMED_KERNELS
 
     N = int(np.prod(prob.mesh))
@@ -508,9 +510,10 @@ def solve_roofline(prob, r, s, workload, head):
             continue
         avg_ms = r["kernel_ms"][i] / cnt
         by = kernel_bytes(name, N, prob.en_lv1, s, nc)
-        if name == "k_gemv" and not composite:
-            by = None
-            name = "coarse_program"
+        if name == "k_gemv":
+            by = int(r["stats"]["coarse_solve_bytes"]) + 16 * nc
+            if not composite:
+                name = "coarse_solve"
         per_kernel[name] = dict(launches_per_step=float(cnt), avg_us=avg_ms * 1e3,
                                 bytes_per_launch=by,
                                 gbs=(by / (avg_ms * 1e-3) / 1e9) if by else None)

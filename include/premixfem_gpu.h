@@ -97,6 +97,9 @@ typedef struct {
                                 level-2 stage (one process, exact level-2 factor) */
   double lv3_floor;          /* level-3 eigensolver: the residual it stagnated at and
                                 accepted (0 when it met -eig_tol) */
+  double coarse_factor_bytes; /* stored bytes of the level-2 factor (dense inverse or
+                                 composite operator, band tiles, plane inverses, ICC(0)) */
+  double coarse_solve_bytes;  /* bytes one level-2 solve reads from that factor */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),

@@ -572,6 +572,8 @@ class BlockICC:
 class Oracle:
     """``PCCtx`` + ``_PC_setup`` + ``PC_apply_vec`` on one process."""
 
+    GRAM_CHUNK = 1 << 23   # faces per chunk of the energy-form Galerkin sums
+
     def __init__(self, dom, mesh, kappa, sd=2, en_lv1=4, en_lv2=4, eg_op=EIG_OP_MOD,
                  pin_cell=None, si_lv1=1, si_lv2=1, eb_lv1=-1.0, eb_lv2=-1.0,
                  use_W_cycle=False, no_shift_A_cc=False, use_2level=False, wd=None,
@@ -630,8 +632,7 @@ class Oracle:
         keep = (np.broadcast_to(bx[:-1] == bx[1:], (P, N, M - 1)),
                 np.broadcast_to((by[:-1] == by[1:])[:, None], (P, N - 1, M)),
                 np.broadcast_to((bz[:-1] == bz[1:])[:, None, None], (P - 1, N, M)))
-        rows, cols, vals, tf = [], [], [], []
-        nf = 0
+        fa, fb, tf = [], [], []
         for Tf, a, b, kf in ((self.T[0], idx[:, :, :-1], idx[:, :, 1:], keep[0]),
                              (self.T[1], idx[:, :-1, :], idx[:, 1:, :], keep[1]),
                              (self.T[2], idx[:-1, :, :], idx[1:, :, :], keep[2])):
@@ -639,16 +640,22 @@ class Oracle:
             if inner:
                 kf = kf.ravel()
                 a, b, Tf = a[kf], b[kf], Tf[kf]
-            f = nf + np.arange(a.size)
-            rows += [f, f]
-            cols += [a, b]
-            vals += [np.ones(a.size), -np.ones(a.size)]
+            fa.append(a)
+            fb.append(b)
             tf.append(Tf)
-            nf += a.size
-        inc = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                            shape=(nf, n))
-        G = (inc @ Phi).tocsr()
-        out = G.T @ sp.diags(np.concatenate(tf)) @ G
+        fa, fb, tf = np.concatenate(fa), np.concatenate(fb), np.concatenate(tf)
+        # Faces in chunks (one chunk unless the grid is large): the sum over
+        # faces is split, the per-face terms are unchanged.
+        out = None
+        for c0 in range(0, fa.size, self.GRAM_CHUNK):
+            a, b = fa[c0:c0 + self.GRAM_CHUNK], fb[c0:c0 + self.GRAM_CHUNK]
+            f = np.arange(a.size)
+            inc = sp.csr_matrix((np.concatenate([np.ones(a.size), -np.ones(a.size)]),
+                                 (np.concatenate([f, f]), np.concatenate([a, b]))),
+                                shape=(a.size, n))
+            G = (inc @ Phi).tocsr()
+            part = G.T @ sp.diags(tf[c0:c0 + self.GRAM_CHUNK]) @ G
+            out = part if out is None else out + part
         d = np.zeros(n)
         if self.pin is not None:
             (px, py, pz), tau = self.pin
@@ -742,9 +749,14 @@ class Oracle:
             self.t["su"] += time.perf_counter() - t0
             return self
         if not self.lv2_icc:
+            # Exact level-2 factor (MUMPS Cholesky, :894-906): dense LAPACK
+            # Cholesky, or a sparse LU (SuperLU) of the block 7-point A_c
+            # when a dense matrix would be too large.
+            self.chol_c = self.lu_c_sp = None
             if self.A_c is None:
-                raise ValueError("dense coarse factorisation needs nc <= %d" % DENSE_COARSE_MAX)
-            self.chol_c = sla.cho_factor(self.A_c, lower=True)
+                self.lu_c_sp = spla.splu(self.A_c_sp.tocsc())
+            else:
+                self.chol_c = sla.cho_factor(self.A_c, lower=True)
             self.t["su_lv2_2"] += time.perf_counter() - ts
         # Level 3 eigen (:918-1175): smallest en_lv2 of the coarse matrix.
         ts = time.perf_counter()
@@ -834,6 +846,8 @@ class Oracle:
             cs = self.bicc.solve(c)
         elif self.use_2level:
             cs = sla.lu_solve(self.lu_c, c)
+        elif self.chol_c is None:
+            cs = self.lu_c_sp.solve(c)
         else:
             cs = sla.cho_solve(self.chol_c, c)
         y = self.Phi @ cs

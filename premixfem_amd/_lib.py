@@ -60,6 +60,7 @@ class Stats(C.Structure):
         ("rnorm0", C.c_double), ("reason", C.c_int), ("t_solve", C.c_double),
         ("eig_stagnated", C.c_int), ("eig_block", C.c_int), ("coarse_mode", C.c_int),
         ("coarse_stages", C.c_int), ("lv3_floor", C.c_double),
+        ("coarse_factor_bytes", C.c_double), ("coarse_solve_bytes", C.c_double),
     ]
 
     def as_dict(self):

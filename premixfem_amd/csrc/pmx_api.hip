@@ -167,6 +167,8 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(geti(c->coarse_composite));
     } else if (k == "-coarse_collapse") {
       PMX_TRY(geti(c->coarse_collapse));
+    } else if (k == "-coarse_planes") {
+      PMX_TRY(geti(c->coarse_planes));
     } else if (k == "-coarse_band") {
       PMX_TRY(geti(c->coarse_band));
     } else if (k == "-coarse_deflate") {
@@ -454,7 +456,14 @@ static int setup_coarse(pmx_ctx* c) {
   if (c->lv2_icc || c->use_band) {
     for (DBuf<double>* b : {&c->Ac, &c->Ainv, &c->Fop}) b->release();
     PMX_TRY(launch_galerkin(c));
-    PMX_TRY(c->lv2_icc ? bicc_setup(c) : band_setup(c));
+    c->use_planes = c->use_band && c->coarse_planes;
+    if (c->use_planes) {
+      c->band.release();
+      c->band_dinv.release();
+    } else {
+      c->pl_G.release();
+    }
+    PMX_TRY(c->lv2_icc ? bicc_setup(c) : (c->use_planes ? planes_setup(c) : band_setup(c)));
   } else {
     PMX_TRY(c->Ac.alloc((size_t)c->nc * c->nc));
     PMX_TRY(c->Ainv.alloc((size_t)c->nc * c->nc));
@@ -1415,9 +1424,31 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_fallbacks = c->eig_fallbacks;
   st->eig_stagnated = c->eig_stagnated;
   st->eig_block = c->eig_block;
-  st->coarse_mode = c->lv2_icc ? 3 : (c->use_band ? 2 : (c->coarse_composite ? 0 : 1));
+  st->coarse_mode = c->lv2_icc ? 3
+                    : (c->use_band ? (c->use_planes ? 4 : 2) : (c->coarse_composite ? 0 : 1));
   st->coarse_stages = (int)c->coarse_prog.size();
   st->lv3_floor = c->lv3_floor;
+  {
+    const double nc2 = (double)c->nc * c->nc * sizeof(double);
+    const double kk = (double)c->nsub * 7 * c->en_lv1 * c->en_lv1 * sizeof(double);
+    if (c->lv2_icc) {
+      // block rows and the inverse pivot blocks, read by both sweeps
+      st->coarse_factor_bytes = kk + (double)c->nsub * c->en_lv1 * c->en_lv1 * sizeof(double);
+      st->coarse_solve_bytes = 2.0 * st->coarse_factor_bytes;
+    } else if (c->use_band && c->use_planes) {
+      const int Z = (int)c->pl_dof.size() - 1;
+      double last = 0.0;
+      if (Z > 0) last = (double)(c->pl_goff[Z] - c->pl_goff[Z - 1]) * sizeof(double);
+      st->coarse_factor_bytes = (double)c->pl_bytes;
+      st->coarse_solve_bytes = 2.0 * c->pl_bytes - last;  // G_z twice, the last plane once
+    } else if (c->use_band) {
+      st->coarse_factor_bytes = (double)(c->band.n + c->band_dinv.n) * sizeof(double);
+      st->coarse_solve_bytes = 2.0 * st->coarse_factor_bytes;
+    } else {
+      st->coarse_factor_bytes = nc2;
+      st->coarse_solve_bytes = nc2;
+    }
+  }
   st->lv3_iters = c->lv3_iters;
   st->pc_precision = c->pc_precision;
   st->its = c->last_its;
@@ -1567,6 +1598,8 @@ int pmx_final(pmx_ctx* c) {
   c->band.release();
   c->band_dinv.release();
   c->band_tmp.release();
+  c->pl_G.release();
+  c->pl_w.release();
   for (DBuf<double>* b : {&c->fac_f, &c->fac_b, &c->mass, &c->phi, &c->Ac, &c->Acb, &c->Ainv,
                           &c->V2, &c->Acc_lu, &c->w_x, &c->w_y, &c->w_yb, &c->w_r, &c->w_b, &c->w_p,
                           &c->w_z, &c->w_q, &c->w_tmp, &c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e,

@@ -318,34 +318,37 @@ __global__ void k_copy_shift(const double* __restrict__ A, double* __restrict__ 
 }
 
 int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift) {
-  const int n = ctx->nc;
+  return spd_invert(ctx->stream, ctx->nc, A, Ainv, shift);
+}
+
+int spd_invert(hipStream_t stream, int n, const double* A, double* Ainv, double shift) {
   DBuf<double> D, Dt, P, U;
   PMX_TRY(D.alloc(kSw * kSw));
   PMX_TRY(Dt.alloc(kSw * kSw));
   PMX_TRY(P.alloc((size_t)n * kSw));
   PMX_TRY(U.alloc((size_t)n * kSw));
   const int nblk = 1024;
-  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, A, Ainv, n, shift,
+  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, stream, A, Ainv, n, shift,
                      1.0);
   PMX_HIP(hipGetLastError());
   const int nt = (n + kSwT - 1) / kSwT;
   const int npb = std::min((n * kSw + 255) / 256, 2048);
   for (int k0 = 0; k0 < n; k0 += kSw) {
     const int bs = std::min(kSw, n - k0);
-    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kPivThreads), 0, ctx->stream, Ainv, n, k0,
+    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kPivThreads), 0, stream, Ainv, n, k0,
                        bs, D.p, Dt.p);
-    hipLaunchKernelGGL(k_sweep_panel, dim3(npb), dim3(256), 0, ctx->stream, Ainv, n, k0, bs, P.p);
+    hipLaunchKernelGGL(k_sweep_panel, dim3(npb), dim3(256), 0, stream, Ainv, n, k0, bs, P.p);
     // U = P D (the pivot columns times the pivot inverse)
     hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 4, false>), dim3(1, (n + 63) / 64), dim3(256), 0,
-                       ctx->stream, n, kSw, kSw, 1.0, P.p, kSw, D.p, kSw, 0.0, U.p, kSw);
-    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(256), 0, ctx->stream, Ainv, n, k0, bs,
+                       stream, n, kSw, kSw, 1.0, P.p, kSw, D.p, kSw, 0.0, U.p, kSw);
+    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(256), 0, stream, Ainv, n, k0, bs,
                        P.p, U.p, D.p);
     PMX_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, ctx->stream, Ainv, Ainv, n, 0.0,
+  hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, stream, Ainv, Ainv, n, 0.0,
                      -1.0);
   PMX_HIP(hipGetLastError());
-  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  PMX_HIP(hipStreamSynchronize(stream));
   return PMX_SUCCESS;
 }
 
@@ -1157,7 +1160,10 @@ int coarse_operator_setup(pmx_ctx* ctx, const std::vector<int>& prog) {
 // total correction in u: GEMVs with B^{-1} on the whole device, everything
 // else fused per segment.
 int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol) {
-  return ctx->use_band ? launch_band_solve(ctx, t, d, ncol) : launch_bicc_solve(ctx, t, d, ncol);
+  if (ctx->use_band)
+    return ctx->use_planes ? launch_planes_solve(ctx, t, d, ncol)
+                           : launch_band_solve(ctx, t, d, ncol);
+  return launch_bicc_solve(ctx, t, d, ncol);
 }
 
 int launch_coarse_program(pmx_ctx* ctx, const std::vector<int>& prog, double* t, double* u,

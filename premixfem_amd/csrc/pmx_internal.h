@@ -391,6 +391,15 @@ struct pmx_ctx {
   int use_band = 0;                 // chosen at setup
   pmx::DBuf<double> band, band_dinv, band_tmp;
   int band_nt = 0, band_w = 0;
+  // The same exact factor by z-planes of subdomains (default on this path,
+  // -coarse_planes 0: the band factor): the inverse Schur complements G_z
+  // of the block-tridiagonal A_c, plane after plane, row-major.
+  int coarse_planes = 1;
+  int use_planes = 0;               // chosen at setup
+  pmx::DBuf<double> pl_G, pl_w;
+  std::vector<int> pl_dof;          // first dof of every plane (+ nc)
+  std::vector<size_t> pl_goff;      // offset of G_z in pl_G (+ total)
+  size_t pl_bytes = 0;
   pmx::DBuf<double> bicc_dinv;      // [nsub][kmax][kmax] inverse pivot blocks
   pmx::DBuf<int> bicc_plane;        // coarse hyperplane prefix counts (nplanes+1)
   pmx::DBuf<int> bicc_elem;         // subdomains in hyperplane order

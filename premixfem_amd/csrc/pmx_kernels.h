@@ -37,6 +37,8 @@ int launch_check_field(pmx_ctx* ctx, const double* a, int n, bool strict, int* b
 // coarse levels (pmx_coarse.hip)
 int launch_galerkin(pmx_ctx* ctx);
 int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift);
+// Inverse of a dense row-major SPD n x n matrix (+ shift I), on stream.
+int spd_invert(hipStream_t stream, int n, const double* A, double* Ainv, double shift);
 int coarse_solve_setup(pmx_ctx* ctx, double delta);
 int launch_coarse_solve(pmx_ctx* ctx, const double* t, double* d, double* ytmp, int ncol);
 int launch_coarse_fix(pmx_ctx* ctx, const double* y, const double* t, double* d);
@@ -58,7 +60,10 @@ int level3_setup_icc(pmx_ctx* ctx);
 // exact level-2 factor in band storage (pmx_band.hip)
 int band_setup(pmx_ctx* ctx);
 int launch_band_solve(pmx_ctx* ctx, const double* t, double* d, int ncol);
-// level-2 solve of the sparse coarse paths: ICC(0) sweeps or the band factor
+// exact level-2 factor by z-planes of subdomains (pmx_planes.hip)
+int planes_setup(pmx_ctx* ctx);
+int launch_planes_solve(pmx_ctx* ctx, const double* t, double* d, int ncol);
+// level-2 solve of the sparse coarse paths: ICC(0) sweeps, the band or the plane factor
 int launch_coarse_prec(pmx_ctx* ctx, const double* t, double* d, int ncol);
 
 // level-2 eigensolver (pmx_lobpcg.hip)

@@ -1,0 +1,264 @@
+// Exact level-2 coarse solve for large coarse spaces by z-planes of
+// subdomains (the reference factors A_c with MUMPS, src/PreMixFEM_3D.c:894-906).
+//
+// In the natural subdomain order (x fastest, src/PreMixFEM_3D.c:23-27) with
+// the eigen index inner (:40-45), the dofs of one z-plane of subdomains are
+// contiguous and A_c is block tridiagonal over the planes: diagonal blocks
+// D_z (the couplings inside plane z, block 5-point) and C_z = A_{z,z-1},
+// whose only non-zero k x k blocks couple a subdomain with the one below it.
+// Block LDL^T with explicit inverses of the Schur complements:
+//   S_0 = D_0,  S_z = D_z - C_z G_{z-1} C_z^T,  G_z = S_z^{-1},
+// and A_c u = t becomes
+//   forward   w_0 = t_0,  w_{z+1} = t_{z+1} - C_{z+1} (G_z w_z),
+//   backward  u_Z = G_Z w_Z,  u_z = G_z (w_z - C_{z+1}^T u_{z+1}).
+// A solve is 2Z-1 dense GEMVs of n_p x n_p (n_p = dofs of a plane), each
+// spread over the whole device with one workgroup per subdomain of the
+// plane, instead of one small launch per 64-wide tile column of a band
+// factor.  The C blocks are read from the block rows Acb (slots -z / +z).
+// At 256^3 cells with 16^3 subdomains of 16^3 cells and k = 8: 16 planes of
+// n_p = 2048, G_z 33.5 MB each (0.54 GB), 31 GEMVs per solve.
+#include <cmath>
+#include <vector>
+
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+constexpr int kPlThreads = 256;
+
+// One plane step for right-hand side blockIdx.y (vectors of stride nc):
+// out = G_z w_z for the rows of subdomain s = sub0 + blockIdx.x, then
+//   fwd:  w[s + z-neighbour] -= A_c(s_up, s) out           (slot -z of s_up)
+//   bwd:  d[s] = out;  w[s - z-neighbour] -= A_c(s_dn, s) out  (slot +z of s_dn)
+// Every subdomain of plane z has at most one neighbour in the next plane
+// and no two share it, so the in-place updates of w do not collide.
+template <int KR>
+__global__ void __launch_bounds__(kPlThreads) k_pl_step(Geo g, const double* __restrict__ Acb,
+                                                        int kmax, const double* __restrict__ G,
+                                                        int np, int dof0, int sub0, int fwd,
+                                                        double* __restrict__ w,
+                                                        double* __restrict__ d, int nc,
+                                                        const double* stop) {
+  PMX_STOP_CHECK(stop);
+  __shared__ double red[kPlThreads / 64][KR];
+  __shared__ double outv[KR];
+  const int s = sub0 + blockIdx.x;
+  const Brick b = g.bricks[s];
+  const int nr = b.neig, r0 = b.dof - dof0;
+  const size_t co = (size_t)blockIdx.y * nc;
+  const double* x = w + co + dof0;
+  double acc[KR];
+#pragma unroll
+  for (int i = 0; i < KR; ++i) acc[i] = 0.0;
+  const double* Gr = G + (size_t)r0 * np;
+  if (((np | dof0 | nc) & 1) == 0) {  // 16-byte aligned rows and segments
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    const int n2 = np >> 1;
+#pragma unroll 2
+    for (int j = threadIdx.x; j < n2; j += kPlThreads) {
+      const double2 xv = x2[j];
+#pragma unroll
+      for (int i = 0; i < KR; ++i) {
+        if (i < nr) {
+          const double2 gv = reinterpret_cast<const double2*>(Gr + (size_t)i * np)[j];
+          acc[i] += gv.x * xv.x + gv.y * xv.y;
+        }
+      }
+    }
+  } else {
+    for (int j = threadIdx.x; j < np; j += kPlThreads) {
+      const double xv = x[j];
+#pragma unroll
+      for (int i = 0; i < KR; ++i)
+        if (i < nr) acc[i] += Gr[(size_t)i * np + j] * xv;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < KR; ++i) {
+    double v = acc[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wv][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < KR) {
+    double v = 0.0;
+    for (int q = 0; q < kPlThreads / 64; ++q) v += red[q][threadIdx.x];
+    outv[threadIdx.x] = v;
+    if (!fwd && (int)threadIdx.x < nr) d[co + b.dof + threadIdx.x] = v;
+  }
+  __syncthreads();
+  const int nbi = b.nb[fwd ? 5 : 4];
+  if (nbi < 0) return;
+  const Brick o = g.bricks[nbi];
+  const int slot = fwd ? 5 : 6;  // the block of o that couples it with s
+  const double* blk = Acb + ((size_t)nbi * 7 + slot) * kmax * kmax;
+  if ((int)threadIdx.x < o.neig) {
+    double v = 0.0;
+    for (int j = 0; j < nr; ++j) v += blk[threadIdx.x * kmax + j] * outv[j];
+    w[co + o.dof + threadIdx.x] -= v;
+  }
+}
+
+// D_z of plane z into S (n_p x n_p row-major, zeroed before): the self block
+// and the blocks of the in-plane neighbours -x, +x, -y, +y of every
+// subdomain of the plane.
+__global__ void k_pl_gather(Geo g, const double* __restrict__ Acb, int kmax, int sub0, int dof0,
+                            int np, double* __restrict__ S) {
+  const int s = sub0 + blockIdx.x;
+  const Brick b = g.bricks[s];
+  const size_t kk = (size_t)kmax * kmax;
+  for (int e = threadIdx.x; e < 5 * kmax * kmax; e += blockDim.x) {
+    const int slot = e / (kmax * kmax), i = (e / kmax) % kmax, j = e % kmax;
+    const int o = slot == 0 ? s : b.nb[slot - 1];
+    if (o < 0) continue;
+    const Brick ob = g.bricks[o];
+    if (i >= b.neig || j >= ob.neig) continue;
+    S[(size_t)(b.dof - dof0 + i) * np + (ob.dof - dof0 + j)] =
+        Acb[(size_t)s * 7 * kk + slot * kk + i * kmax + j];
+  }
+}
+
+// H = C_z G_{z-1}: the rows of subdomain s of plane z are the rows of its
+// lower neighbour's block of G_{z-1} combined by A_c(s, s_dn).
+__global__ void k_pl_cg(Geo g, const double* __restrict__ Acb, int kmax, int sub0, int dof0,
+                        const double* __restrict__ Gp, int npp, int dof0p,
+                        double* __restrict__ H) {
+  const int s = sub0 + blockIdx.x;
+  const Brick b = g.bricks[s];
+  const int sd = b.nb[4];
+  if (sd < 0) return;
+  const Brick bd = g.bricks[sd];
+  const double* blk = Acb + ((size_t)s * 7 + 5) * kmax * kmax;
+  for (int col = threadIdx.x; col < npp; col += blockDim.x) {
+    double gv[kMaxEig];
+    for (int a = 0; a < bd.neig; ++a) gv[a] = Gp[(size_t)(bd.dof - dof0p + a) * npp + col];
+    for (int i = 0; i < b.neig; ++i) {
+      double v = 0.0;
+      for (int a = 0; a < bd.neig; ++a) v += blk[i * kmax + a] * gv[a];
+      H[(size_t)(b.dof - dof0 + i) * npp + col] = v;
+    }
+  }
+}
+
+// S_z -= H C_z^T: column j of subdomain s' takes sum_b H[i][s'_dn, b] A_c(s', s'_dn)[j][b].
+__global__ void k_pl_schur(Geo g, const double* __restrict__ Acb, int kmax, int sub0, int dof0,
+                           int np, const double* __restrict__ H, int npp, int dof0p,
+                           double* __restrict__ S) {
+  const int sp = sub0 + blockIdx.x;
+  const Brick b = g.bricks[sp];
+  const int sd = b.nb[4];
+  if (sd < 0) return;
+  const Brick bd = g.bricks[sd];
+  const double* blk = Acb + ((size_t)sp * 7 + 5) * kmax * kmax;
+  for (int i = blockIdx.y * blockDim.x + threadIdx.x; i < np; i += gridDim.y * blockDim.x) {
+    double hv[kMaxEig];
+    for (int q = 0; q < bd.neig; ++q) hv[q] = H[(size_t)i * npp + (bd.dof - dof0p + q)];
+    for (int j = 0; j < b.neig; ++j) {
+      double v = 0.0;
+      for (int q = 0; q < bd.neig; ++q) v += hv[q] * blk[j * kmax + q];
+      S[(size_t)i * np + (b.dof - dof0 + j)] -= v;
+    }
+  }
+}
+
+// A = (A + A^T) / 2 for a dense n x n matrix; fail if a diagonal entry is
+// not positive and finite (the SPD matrices and inverses of this path).
+__global__ void k_pl_symmetrize(double* __restrict__ A, int n, int* __restrict__ fail,
+                                int fail_id) {
+  const size_t nn = (size_t)n * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nn;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (j > i) {
+      const double v = 0.5 * (A[e] + A[(size_t)j * n + i]);
+      A[e] = v;
+      A[(size_t)j * n + i] = v;
+    } else if (i == j) {
+      const double v = A[e];
+      if (!(v > 0.0) || !(v < INFINITY)) atomicMax(fail, fail_id);
+    }
+  }
+}
+
+int planes_setup(pmx_ctx* ctx) {
+  const int L = ctx->sd[0] * ctx->sd[1], Z = ctx->sd[2];
+  ctx->pl_dof.assign(Z + 1, 0);
+  ctx->pl_goff.assign(Z + 1, 0);
+  for (int z = 0; z < Z; ++z) {
+    ctx->pl_dof[z + 1] = ctx->h_dof[(size_t)(z + 1) * L];
+    const size_t np = ctx->pl_dof[z + 1] - ctx->pl_dof[z];
+    ctx->pl_goff[z + 1] = ctx->pl_goff[z] + np * np;
+  }
+  int npmax = 0;
+  for (int z = 0; z < Z; ++z) npmax = std::max(npmax, ctx->pl_dof[z + 1] - ctx->pl_dof[z]);
+  ctx->pl_bytes = ctx->pl_goff[Z] * sizeof(double);
+  PMX_TRY(ctx->pl_G.alloc(ctx->pl_goff[Z]));
+  DBuf<double> S, H;
+  DBuf<int> fail;
+  PMX_TRY(S.alloc((size_t)npmax * npmax));
+  PMX_TRY(H.alloc((size_t)npmax * npmax));
+  PMX_TRY(fail.alloc(1));
+  PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
+  const int kmax = ctx->en_lv1;
+  for (int z = 0; z < Z; ++z) {
+    const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
+    PMX_HIP(hipMemsetAsync(S.p, 0, sizeof(double) * (size_t)np * np, ctx->stream));
+    hipLaunchKernelGGL(k_pl_gather, dim3(L), dim3(256), 0, ctx->stream, ctx->geo, ctx->Acb.p,
+                       kmax, z * L, dof0, np, S.p);
+    if (z > 0) {
+      const int dof0p = ctx->pl_dof[z - 1], npp = dof0 - dof0p;
+      const double* Gp = ctx->pl_G.p + ctx->pl_goff[z - 1];
+      hipLaunchKernelGGL(k_pl_cg, dim3(L), dim3(256), 0, ctx->stream, ctx->geo, ctx->Acb.p, kmax,
+                         z * L, dof0, Gp, npp, dof0p, H.p);
+      hipLaunchKernelGGL(k_pl_schur, dim3(L, (np + 255) / 256), dim3(256), 0, ctx->stream,
+                         ctx->geo, ctx->Acb.p, kmax, z * L, dof0, np, H.p, npp, dof0p, S.p);
+    }
+    hipLaunchKernelGGL(k_pl_symmetrize, dim3(1024), dim3(256), 0, ctx->stream, S.p, np, fail.p,
+                       z + 1);
+    PMX_HIP(hipGetLastError());
+    double* Gz = ctx->pl_G.p + ctx->pl_goff[z];
+    PMX_TRY(spd_invert(ctx->stream, np, S.p, Gz, 0.0));
+    hipLaunchKernelGGL(k_pl_symmetrize, dim3(1024), dim3(256), 0, ctx->stream, Gz, np, fail.p,
+                       Z + z + 1);
+    PMX_HIP(hipGetLastError());
+  }
+  int hf = 0;
+  PMX_HIP(hipMemcpyAsync(&hf, fail.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  PMX_HIP(hipStreamSynchronize(ctx->stream));
+  if (hf) {
+    const bool inv = hf > Z;
+    set_error("coarse solve by planes: %s of plane %d has a non-positive or non-finite diagonal",
+              inv ? "the inverse Schur complement" : "the Schur complement",
+              (inv ? hf - Z : hf) - 1);
+    return PMX_ERR_NOT_CONVERGED;
+  }
+  return PMX_SUCCESS;
+}
+
+int launch_planes_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
+  const int L = ctx->sd[0] * ctx->sd[1], Z = ctx->sd[2], nc = ctx->nc;
+  PMX_TRY(ctx->pl_w.alloc((size_t)nc * std::max(ncol, 32)));
+  double* w = ctx->pl_w.p;
+  PMX_HIP(hipMemcpyAsync(w, t, (size_t)nc * ncol * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  const bool wide = ctx->en_lv1 > 8;
+  auto step = [&](int z, int fwd) {
+    const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
+    const double* Gz = ctx->pl_G.p + ctx->pl_goff[z];
+    if (wide)
+      hipLaunchKernelGGL(k_pl_step<kMaxEig>, dim3(L, ncol), dim3(kPlThreads), 0, ctx->stream,
+                         ctx->geo, ctx->Acb.p, ctx->en_lv1, Gz, np, dof0, z * L, fwd, w, d, nc,
+                         ctx->geo.stop);
+    else
+      hipLaunchKernelGGL(k_pl_step<8>, dim3(L, ncol), dim3(kPlThreads), 0, ctx->stream,
+                         ctx->geo, ctx->Acb.p, ctx->en_lv1, Gz, np, dof0, z * L, fwd, w, d, nc,
+                         ctx->geo.stop);
+  };
+  for (int z = 0; z + 1 < Z; ++z) step(z, 1);
+  for (int z = Z - 1; z >= 0; --z) step(z, 0);
+  PMX_HIP(hipGetLastError());
+  return PMX_SUCCESS;
+}
+
+}  // namespace pmx

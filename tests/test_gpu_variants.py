@@ -27,9 +27,12 @@ VARIANTS = [
     # deflated inverse
     ("stagewise", dict(), "-coarse_composite 0"),
     ("stagewise-plain", dict(), "-coarse_composite 0 -coarse_deflate 0"),
-    # exact level-2 factor in band storage (the large-coarse-space path)
-    ("band", dict(), "-coarse_band 1"),
-    ("band-W", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle"),
+    # exact level-2 factor of the large-coarse-space path: by z-planes of
+    # subdomains (its default) and in band storage
+    ("planes", dict(), "-coarse_band 1"),
+    ("planes-W", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle"),
+    ("band", dict(), "-coarse_band 1 -coarse_planes 0"),
+    ("band-W", dict(use_W_cycle=True), "-coarse_band 1 -coarse_planes 0 -use_W_cycle"),
     # exact level-1 blocks (band Cholesky per brick)
     ("full-Cholesky-lv1", dict(use_full_Cholesky_lv1=True), "-use_full_Cholesky_lv1"),
     ("full-Cholesky-lv1-W", dict(use_full_Cholesky_lv1=True, use_W_cycle=True),
@@ -40,7 +43,9 @@ VARIANTS = [
     ("W-cycle-full", dict(use_W_cycle=True), "-use_W_cycle -coarse_collapse 0"),
     ("W-si2-full", dict(use_W_cycle=True, si_lv2=2), "-use_W_cycle -si_lv2 2 -coarse_collapse 0"),
     ("stagewise-full", dict(), "-coarse_composite 0 -coarse_collapse 0"),
-    ("band-W-full", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle -coarse_collapse 0"),
+    ("band-W-full", dict(use_W_cycle=True),
+     "-coarse_band 1 -coarse_planes 0 -use_W_cycle -coarse_collapse 0"),
+    ("planes-W-full", dict(use_W_cycle=True), "-coarse_band 1 -use_W_cycle -coarse_collapse 0"),
 ]
 
 
@@ -198,3 +203,27 @@ def test_bicc_prefetch_solve_is_bit_identical_with_multi_round_planes():
     assert a["reason"] == "CONVERGED_RTOL"
     assert a["its"] == b["its"]
     assert np.array_equal(a["x"], b["x"])
+
+
+@pytest.mark.parametrize("mesh,sd,k", [((32, 32, 32), (4, 4, 4), 4), ((30, 26, 33), (3, 2, 5), 5)],
+                         ids=["32^3-sd4", "uneven-sd3x2x5"])
+def test_plane_factor_matches_dense_inverse(mesh, sd, k):
+    """The coarse solve by z-planes (block LDL^T over the planes of
+    subdomains, explicit inverse Schur complements) against the dense
+    deflated inverse on the same setup: level-3 eigenvalues, iterations and
+    the pressure.  Uneven planes: 5 planes of 6 subdomains, odd eigen count."""
+    kap = problems.grf_kappa(mesh, 2.0, 0.1, 91)
+    prob = problems.Problem("planes", (1.0, 1.0, 1.0), mesh, kap,
+                            problems.corner_rhs(mesh, (1.0, 1.0, 1.0)), sd=sd, en_lv1=k,
+                            en_lv2=4, pin_cell=(mesh[0] - 1, 0, 0))
+    print("planes vs dense", mesh, sd, k, flush=True)
+    a = runner.run(prob, extra="-coarse_band 1")
+    b = runner.run(prob)
+    assert a["stats"]["coarse_mode"] == 4 and b["stats"]["coarse_mode"] == 0
+    err = np.linalg.norm(a["x"] - b["x"]) / np.linalg.norm(b["x"])
+    print(f"planes: its {a['its']} vs {b['its']}, x rel diff {err:.2e}, lv3 "
+          f"{a['stats']['eigen_min_lv2']:.6e} vs {b['stats']['eigen_min_lv2']:.6e}", flush=True)
+    assert a["reason"] == "CONVERGED_RTOL"
+    assert abs(a["its"] - b["its"]) <= 1
+    assert err <= 1e-9
+    assert a["stats"]["eigen_max_lv2"] == pytest.approx(b["stats"]["eigen_max_lv2"], rel=1e-6)
