@@ -221,6 +221,24 @@ int pmx_solve(pmx_ctx *ctx, const double *b, double *x, int x0_nonzero, double r
 int pmx_solve_device(pmx_ctx *ctx, const double *d_b, double *d_x, int x0_nonzero, double rtol,
                      int max_it, int *its, double *rnorm);
 
+/* pmx_solve_device without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The PCG's per-subdomain partial sums and device scalars live in a fixed
+ * device array of the library (PMX_NOALLOC_SCRATCH_DOUBLES doubles, 4 MiB,
+ * held on every GPU the library runs on until the process ends) instead of
+ * the context's buffers; the iterations are launched eagerly (no HIP graph,
+ * no -kernel_timing events).  The context must have been set up by
+ * pmx_setup, which allocates the preconditioner and the PCG vectors; this
+ * call then calls no hipMalloc / hipFree.  Limit: 6 * nsub + 16 doubles of
+ * scratch, i.e. at most 87378 subdomains (PMX_ERR_ARG_OUTOFRANGE beyond).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several threads or streams) race.  Results are those of
+ * pmx_solve_device. */
+#define PMX_NOALLOC_SCRATCH_DOUBLES 524288
+int pmx_solve_device_noalloc(pmx_ctx *ctx, const double *d_b, double *d_x, int x0_nonzero,
+                             double rtol, int max_it, int *its, double *rnorm);
+
 /* Per-kernel GPU time of the solve phase, measured with HIP events on the
  * context's stream when the context was created with -kernel_timing.
  * Kernels, in order: 0 level-1 first sweep, 1 level-1 last sweep (with the

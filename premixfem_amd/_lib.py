@@ -34,7 +34,8 @@ EXPORTS = ("pmx_init", "pmx_print_info", "pmx_set_kappa", "pmx_create_A", "pmx_s
            "pmx_velocity_device", "pmx_spe10_create", "pmx_spe10_assemble",
            "pmx_spe10_get_system", "pmx_spe10_solve", "pmx_spe10_transport",
            "pmx_spe10_saturation_range", "pmx_spe10_get_state", "pmx_spe10_set_state",
-           "pmx_spe10_run", "pmx_spe10_destroy", "pmx_setup_batch", "pmx_spe10_set_step_hook")
+           "pmx_spe10_run", "pmx_spe10_destroy", "pmx_setup_batch", "pmx_spe10_set_step_hook",
+           "pmx_solve_device_noalloc")
 TIMED_KERNELS = ("k_lv1_first", "k_lv1_last", "k_resid_restrict", "k_prolong", "k_cg_p_spmv",
                  "k_cg_update", "k_gemv", "coarse_small")
 
@@ -119,6 +120,8 @@ def lib():
     L.pmx_final.argtypes = [vp]
     L.pmx_solve_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                    C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.pmx_solve_device_noalloc.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                           C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.pmx_get_kernel_times.argtypes = [vp, _dp, _ip, C.c_int, C.c_int]
     L.pmx_cnn_create.argtypes = [C.POINTER(vp), C.c_int, C.c_void_p, C.c_ulonglong]
     L.pmx_cnn_get_params.argtypes = [vp, C.c_void_p, C.c_void_p]

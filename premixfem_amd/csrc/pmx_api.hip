@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
@@ -27,6 +28,37 @@ void set_error(const char* fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_err = buf;
+}
+
+// Fixed device array of pmx_solve_device_noalloc: the PCG's per-subdomain
+// partial sums (6 nsub doubles) and its device scalars (16 doubles).  Each
+// GPU holds its own copy until the process ends; host code never uses the
+// name `scratch` for its memory, only the address hipGetSymbolAddress gives
+// for the current device (scratch_address).
+constexpr int kScratchDoubles = PMX_NOALLOC_SCRATCH_DOUBLES;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// The array's address on device dev (the current device), looked up once.
+static int scratch_address(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index %d beyond the %d devices this library tracks", dev, kMaxDevices);
+    return PMX_ERR_SUP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress of the solve scratch array: %s", hipGetErrorString(e));
+      return PMX_ERR_LIB;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PMX_SUCCESS;
 }
 
 static double now() {
@@ -411,6 +443,8 @@ static int alloc_work(pmx_ctx* c) {
   PMX_TRY(c->partial.alloc(std::max<size_t>(6 * (size_t)c->nsub, 4096)));
   PMX_TRY(c->scal.alloc(16));
   PMX_HIP(hipMemset(c->scal.p, 0, 16 * sizeof(double)));
+  c->cg_part = c->partial.p;
+  c->cg_scal = c->scal.p;
   return PMX_SUCCESS;
 }
 
@@ -797,7 +831,8 @@ int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* opti
     set_error("hipStreamCreate failed");
     return fail(PMX_ERR_LIB);
   }
-  rc = build_geometry(c);
+  rc = scratch_address(dev, &c->scratch_dev);
+  if (!rc) rc = build_geometry(c);
   if (!rc) {
     // The level-1 kernels keep a brick in LDS (the factorisation in FP64):
     // bound the subdomain size by the device's LDS.
@@ -1269,14 +1304,14 @@ static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* 
     PMX_TRY(launch_residual(c, bb, nullptr, r));
   }
   double* hs = c->h_scal;
-  PMX_HIP(hipMemcpyAsync(hs, c->scal.p, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
+  PMX_HIP(hipMemcpyAsync(hs, c->cg_scal, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream));
   PMX_HIP(hipStreamSynchronize(c->stream));
   const double t_solve0 = now();
   // Iterations are enqueued in chunks; the convergence test runs on the
   // device, which marks the solve done so the remaining kernels of the chunk
   // return at once.  The host reads the scalars once per chunk.
-  c->geo.stop = c->scal.p + kSDone;
+  c->geo.stop = c->cg_scal + kSDone;
   int rc = PMX_SUCCESS;
   if (hs[kSDone] == 0.0) {
     // z = B r0; its z'r partials go to the slot of iteration 0.
@@ -1307,7 +1342,7 @@ static int solve_core(pmx_ctx* c, int x0_nonzero, double rtol, int max_it, int* 
         rc = enqueue_chunk(chunk);
         if (rc) break;
       }
-      if (hipMemcpyAsync(hs, c->scal.p, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
+      if (hipMemcpyAsync(hs, c->cg_scal, kSNumScal * sizeof(double), hipMemcpyDeviceToHost,
                          c->stream) != hipSuccess ||
           hipStreamSynchronize(c->stream) != hipSuccess) {
         set_error("PCG: reading device scalars failed");
@@ -1350,7 +1385,7 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
               int* its, double* rnorm) {
   PMX_TRY(solve_check(c, b, x));
   const size_t nb = c->ncell * sizeof(double);
-  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
+  PMX_HIP(hipMemsetAsync(c->cg_scal, 0, 16 * sizeof(double), c->stream));
   PMX_HIP(hipMemcpyAsync(c->nat_buf.p, b, nb, hipMemcpyHostToDevice, c->stream));
   PMX_TRY(launch_nat2brick(c, c->nat_buf.p, c->w_b.p, 1, c->ncell, c->ncell));
   if (x0_nonzero) {
@@ -1368,7 +1403,56 @@ int pmx_solve(pmx_ctx* c, const double* b, double* x, int x0_nonzero, double rto
 int pmx_solve_device(pmx_ctx* c, const double* d_b, double* d_x, int x0_nonzero, double rtol,
                      int max_it, int* its, double* rnorm) {
   PMX_TRY(solve_check(c, d_b, d_x));
-  PMX_HIP(hipMemsetAsync(c->scal.p, 0, 16 * sizeof(double), c->stream));
+  PMX_HIP(hipMemsetAsync(c->cg_scal, 0, 16 * sizeof(double), c->stream));
+  PMX_TRY(launch_nat2brick(c, d_b, c->w_b.p, 1, c->ncell, c->ncell));
+  if (x0_nonzero) PMX_TRY(launch_nat2brick(c, d_x, c->w_x.p, 1, c->ncell, c->ncell));
+  const int rc = solve_core(c, x0_nonzero, rtol, max_it, its, rnorm);
+  if (rc != PMX_SUCCESS && rc != PMX_ERR_NOT_CONVERGED) return rc;
+  PMX_TRY(launch_brick2nat(c, c->w_x.p, d_x, 1, c->ncell, c->ncell));
+  PMX_HIP(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+int pmx_solve_device_noalloc(pmx_ctx* c, const double* d_b, double* d_x, int x0_nonzero,
+                             double rtol, int max_it, int* its, double* rnorm) {
+  if (!c) {
+    set_error("null argument");
+    return PMX_ERR_ARG_NULL;
+  }
+  const size_t need = 6 * (size_t)c->nsub + kSNumScal;
+  if (need > (size_t)kScratchDoubles) {
+    set_error("pmx_solve_device_noalloc: %d subdomains need %zu doubles of solve scratch, the "
+              "fixed array holds %d (at most %d subdomains)", c->nsub, need, kScratchDoubles,
+              (kScratchDoubles - kSNumScal) / 6);
+    return PMX_ERR_ARG_OUTOFRANGE;
+  }
+  PMX_TRY(solve_check(c, d_b, d_x));
+  if (!c->scratch_dev) {
+    set_error("pmx_solve_device_noalloc: no solve scratch on this context's device");
+    return PMX_ERR_LIB;
+  }
+  // The PCG kernels take the partial sums and scalars from the fixed array
+  // for this call, launched eagerly: no graph (its instantiation may
+  // allocate), no timing events.  Restored on every return.
+  struct Restore {
+    pmx_ctx* c;
+    double *part, *scal;
+    int graph, timing, stages;
+    ~Restore() {
+      c->cg_part = part;
+      c->cg_scal = scal;
+      c->cg_graph = graph;
+      c->kernel_timing = timing;
+      c->log_stages = stages;
+    }
+  } restore{c, c->cg_part, c->cg_scal, c->cg_graph, c->kernel_timing, c->log_stages};
+  c->cg_scal = c->scratch_dev;
+  c->cg_part = c->scratch_dev + kSNumScal;
+  c->cg_graph = 0;
+  c->kernel_timing = 0;
+  c->log_stages = 0;
+  // The scalars start at zero (the array keeps the previous call's values).
+  PMX_HIP(hipMemsetAsync(c->cg_scal, 0, kSNumScal * sizeof(double), c->stream));
   PMX_TRY(launch_nat2brick(c, d_b, c->w_b.p, 1, c->ncell, c->ncell));
   if (x0_nonzero) PMX_TRY(launch_nat2brick(c, d_x, c->w_x.p, 1, c->ncell, c->ncell));
   const int rc = solve_core(c, x0_nonzero, rtol, max_it, its, rnorm);

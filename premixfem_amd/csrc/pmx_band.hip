@@ -206,6 +206,7 @@ int band_setup(pmx_ctx* ctx) {
   ctx->band_w = w;
   PMX_TRY(ctx->band.alloc((size_t)nt * (w + 1) * kBtt));
   PMX_TRY(ctx->band_dinv.alloc((size_t)nt * kBtt));
+  PMX_TRY(ctx->band_tmp.alloc((size_t)nc * 32));  // solve work, so a solve never allocates
   PMX_HIP(hipMemsetAsync(ctx->band.p, 0, ctx->band.n * sizeof(double), ctx->stream));
   hipLaunchKernelGGL(k_band_fill, dim3(ctx->nsub), dim3(256), 0, ctx->stream, ctx->geo, ctx->Acb.p,
                      ctx->en_lv1, w, ctx->band.p);

@@ -1467,11 +1467,11 @@ int stream_threads(const pmx_ctx* ctx) {
 
 static CgCtl make_cg(pmx_ctx* ctx) {
   CgCtl c;
-  c.scal = ctx->scal.p;
+  c.scal = ctx->cg_scal;
   c.nparts = ctx->nsub;
-  c.zr = ctx->partial.p;
-  c.pw = ctx->partial.p + 2 * (size_t)ctx->nsub;
-  c.rr = ctx->partial.p + 4 * (size_t)ctx->nsub;
+  c.zr = ctx->cg_part;
+  c.pw = ctx->cg_part + 2 * (size_t)ctx->nsub;
+  c.rr = ctx->cg_part + 4 * (size_t)ctx->nsub;
   c.abstol = ctx->cg_abstol;
   c.dtol = ctx->cg_dtol;
   c.max_it = ctx->cg_max_it;
@@ -1840,7 +1840,7 @@ static bool launch_prolong_q(pmx_ctx* ctx, S* y, const S* phi, const double* cve
 }
 
 int launch_prolong(pmx_ctx* ctx, const double* cvec, bool count_iter) {
-  double* iter = count_iter ? ctx->scal.p + kSIter : nullptr;
+  double* iter = count_iter ? ctx->cg_scal + kSIter : nullptr;
   const int nt = stream_threads(ctx);
   const bool done = ctx->pc_precision == 32
                         ? launch_prolong_q<float>(ctx, ctx->w_y32.p, ctx->phi32.p, cvec, iter)
@@ -1891,12 +1891,12 @@ int launch_cg_p_spmv(pmx_ctx* ctx, const double* z, const double* pold, double* 
 
 // r = b - A x (x == nullptr: x = 0), then the PCG start test.
 int launch_residual(pmx_ctx* ctx, const double* b, const double* x, double* r) {
-  double* part = ctx->partial.p + 4 * (size_t)ctx->nsub;
+  double* part = ctx->cg_part + 4 * (size_t)ctx->nsub;
   hipLaunchKernelGGL(k_residual, dim3(ctx->nsub), dim3(stream_threads(ctx)), 0, ctx->stream,
                      ctx->geo, b, x, r, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, part);
   PMX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, ctx->stream, part,
-                     x ? part + ctx->nsub : nullptr, ctx->nsub, ctx->scal.p, ctx->cg_rtol,
+                     x ? part + ctx->nsub : nullptr, ctx->nsub, ctx->cg_scal, ctx->cg_rtol,
                      ctx->cg_abstol);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;

@@ -430,6 +430,12 @@ struct pmx_ctx {
   pmx::DBuf<double> c_a, c_b, c_c, c_d, c_e;                       // coarse vectors
   pmx::DBuf<double> partial;                                       // per-block partial sums
   pmx::DBuf<double> scal;                                          // device scalars
+  // What the PCG kernels use for the partial sums (6 nsub) and the device
+  // scalars: partial / scal, or the library's fixed device array under
+  // pmx_solve_device_noalloc.
+  double* cg_part = nullptr;
+  double* cg_scal = nullptr;
+  double* scratch_dev = nullptr;    // the fixed array's address on this context's device
   pmx::DBuf<double> nat_buf;                                       // natural-order staging
 
   // ---- state / stats ----

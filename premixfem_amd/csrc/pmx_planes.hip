@@ -194,6 +194,9 @@ int planes_setup(pmx_ctx* ctx) {
   for (int z = 0; z < Z; ++z) npmax = std::max(npmax, ctx->pl_dof[z + 1] - ctx->pl_dof[z]);
   ctx->pl_bytes = ctx->pl_goff[Z] * sizeof(double);
   PMX_TRY(ctx->pl_G.alloc(ctx->pl_goff[Z]));
+  // the solves' work vectors (up to 32 right-hand sides) now, so a solve
+  // never allocates
+  PMX_TRY(ctx->pl_w.alloc((size_t)ctx->nc * 32));
   DBuf<double> S, H;
   DBuf<int> fail;
   PMX_TRY(S.alloc((size_t)npmax * npmax));
