@@ -80,7 +80,17 @@ def run_mode(prob, precision, steps, warmup, extra="", net=None):
         if rc not in (0,):
             _lib.check(rc)
 
-    for _ in range(warmup):
+    # The cold step: the context's first setup + solve, what one KSPSetUp +
+    # KSPSolve pays (LOBPCG block-size tuning included when the process has
+    # not tuned this subdomain size before, subdomains launched in index
+    # order); later setups launch the subdomains hardest first.
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    t_cold = time.perf_counter() - tc
+    st_cold = P.PC_get_stats(ctx)
+    for _ in range(max(0, warmup - 1)):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -127,7 +137,8 @@ def run_mode(prob, precision, steps, warmup, extra="", net=None):
     P.PC_final(tctx)
     return dict(t_step=(t1 - t0) / steps, t_setup=t_setup / steps, t_solve=t_solve / steps,
                 its=its.value, rnorm=rn.value, stats=st, kernel_ms=ms, kernel_count=cnt,
-                kernel_its=its_t.value, t_solve_eager=t_solve_eager, x=xs)
+                kernel_its=its_t.value, t_solve_eager=t_solve_eager, x=xs, t_cold=t_cold,
+                t_cold_tune=st_cold["t_eig_tune"], cold_setup=st_cold["t_stages"]["su"])
 
 
 def measured_traffic(workload, mode, kernel):
@@ -151,15 +162,16 @@ def measured_traffic(workload, mode, kernel):
     return None
 
 
-def train_warm_start(prob, nfields=4, epochs=30, batch=64, kind=None):
+def train_warm_start(prob, nfields=4, epochs=30, batch=64, kind=None, mesh=None):
     """Network for the learned mode: trained on the eigenvectors of
     ``nfields`` seeded fields of the workload's family (``kind``, by default
-    the workload's name; same mesh and subdomains, seeds 10000+)."""
+    the workload's name; seeds 10000+, disjoint from the workload's) on
+    ``mesh`` (default the workload's) with the workload's subdomain shape."""
     from premixfem_amd import cnn, problems
 
     t0 = time.perf_counter()
     kind = kind or prob.name
-    data = [cnn.sample_problem(problems.training_problem(kind, 10000 + s, prob.mesh))
+    data = [cnn.sample_problem(problems.training_problem(kind, 10000 + s, mesh or prob.mesh))
             for s in range(nfields)]
     t1 = time.perf_counter()
     net = cnn.WarmStartNet(cout=prob.en_lv1 - 1, seed=1)
@@ -226,6 +238,33 @@ def cpu_baseline(prob, gpu_its, nsample):
     t["pcg"] = (time.perf_counter() - t0) / 3 * gpu_its
     total = sum(t.values())
     return total, t
+
+
+def cpu_baseline_cpp(prob, gpu_its, nsample):
+    """The compiled FP64 CPU restatement (oracle/cpu/pmx_cpu.cpp, C++/OpenMP:
+    per-subdomain ICC(0), shift-invert Lanczos GHEPs on exact band Cholesky
+    factors, energy-form Galerkin, band Cholesky of A_c, level 3, the
+    reference's V-cycle with its five residuals, PETSc CG) on the whole
+    workload, timed in full on the host cores (at most 16: the GPU's share
+    of the box).  Returns the cpu_baseline object."""
+    from oracle import cpu_baseline as CB
+
+    threads = min(16, os.cpu_count() or 1)
+    r = CB.setup_solve(prob, threads=threads)
+    info = dict(value=round(r["t"]["total"], 3), unit="s", cores=r["threads"], kind="port",
+                sample=("the whole workload, setup + PCG to rtol 1e-8 (FP64), compiled C++/OpenMP "
+                        "restatement of the reference's algorithms (oracle/cpu/pmx_cpu.cpp)"),
+                its=r["its"], parts={k: round(v, 3) for k, v in r["t"].items() if k != "total"},
+                lanczos_unconverged=r["lanczos_unconverged"])
+    fpath = os.path.join(ROOT, "tests", "golden", f"{prob.name}.npz")
+    if os.path.exists(fpath):
+        f = np.load(fpath, allow_pickle=False)
+        xs = f["x_sample"]
+        info["rel_err_vs_fp64_oracle"] = float(np.linalg.norm(r["x"][f["cells"]] - xs) /
+                                               np.linalg.norm(xs))
+    print(f"# cpu baseline ({info['cores']} threads): {info['value']}s, its {r['its']}, "
+          f"{info['parts']}", file=sys.stderr, flush=True)
+    return info
 
 
 IMPES_MESH = (60, 220, 85)
@@ -531,7 +570,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c4",
+                    help="workload: c4 (256^3 fracture medium, the learned FP32 configuration; "
+                         "default), c2 (128^3 binary medium), c1 (32^3 GRF)")
     ap.add_argument("--cpu-sample", type=int, default=16,
                     help="subdomain eigenproblems timed for the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -548,7 +589,10 @@ def main():
                     help="batch: extra preconditioner options for every member (default: the "
                          "coarse stages applied one by one; composing them costs 2 nc^3 flops "
                          "per member, which members solved in <= 20 iterations never recover)")
-    ap.add_argument("--train-fields", type=int, default=4,
+    ap.add_argument("--train-mesh", type=int, default=64,
+                    help="learned mode: training fields of this many cells per side (same "
+                         "subdomain shape as the workload)")
+    ap.add_argument("--train-fields", type=int, default=32,
                     help="learned mode: seeded training fields of the workload's family")
     ap.add_argument("--train-epochs", type=int, default=30, help="learned mode: training epochs")
     ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
@@ -572,6 +616,10 @@ def main():
     prob = problems.config(args.config)
     print(f"# workload {prob.name}: mesh {prob.mesh} sd {prob.sd} k {prob.en_lv1}", file=sys.stderr,
           flush=True)
+    fix = None
+    fpath = os.path.join(ROOT, "tests", "golden", f"{prob.name}.npz")
+    if os.path.exists(fpath):
+        fix = np.load(fpath, allow_pickle=False)
     modes = {}
     training = None
     for m in args.modes.split(","):
@@ -579,16 +627,24 @@ def main():
         net = None
         extra = ("-eig_autotune 1 " + args.extra).strip()
         if m == "learned":
+            tm = args.train_mesh
             net, training = train_warm_start(prob, nfields=args.train_fields,
-                                             epochs=args.train_epochs)
+                                             epochs=args.train_epochs, mesh=(tm, tm, tm))
             print(f"# warm-start network: {training}", file=sys.stderr, flush=True)
             extra += " -warm_start cnn"
         r = run_mode(prob, prec, args.steps, args.warmup, extra, net)
         if net is not None:
             net.close()
+        if fix is not None:
+            cells = fix["cells"]
+            xs = fix["x_sample"]
+            r["oracle_its"] = int(fix["its"])
+            r["rel_err_vs_oracle"] = float(np.linalg.norm(r["x"][cells] - xs) / np.linalg.norm(xs))
         modes[m] = r
         print(f"# {m}: step {r['t_step']:.4f}s setup {r['t_setup']:.4f}s solve {r['t_solve']:.4f}s "
-              f"its {r['its']} lobpcg max/mean {r['stats']['eig_iters_max']}/"
+              f"its {r['its']} (oracle {r.get('oracle_its')}, x rel err "
+              f"{r.get('rel_err_vs_oracle', float('nan')):.2e}) cold {r['t_cold']:.4f}s "
+              f"(tuning {r['t_cold_tune']:.3f}s) lobpcg max/mean {r['stats']['eig_iters_max']}/"
               f"{r['stats']['eig_iters_mean']:.1f}", file=sys.stderr, flush=True)
     head = "learned" if "learned" in modes else ("fp32" if "fp32" in modes else list(modes)[0])
     hr = modes[head]
@@ -599,13 +655,34 @@ def main():
     solve_gbs = solve_bytes / hr["t_solve"] / 1e9 if hr["t_solve"] > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
-        total, parts = cpu_baseline(prob, hr["its"], args.cpu_sample)
-        cpu = dict(value=round(total, 2), unit="s", cores=os.cpu_count(), kind="port",
-                   sample=(f"numpy/SciPy FP64 oracle on the same workload: {args.cpu_sample} of "
-                           f"{int(np.prod(prob.sd))} subdomain eigenproblems timed and scaled, "
-                           "Galerkin/coarse Cholesky/level-3 timed in full, PCG timed for 3 "
-                           "iterations and scaled to the GPU iteration count"),
-                   parts={k: round(v, 3) for k, v in parts.items()})
+        cpu = cpu_baseline_cpp(prob, hr["its"], args.cpu_sample)
+    workloads = {
+        "c4": "256^3 fracture medium (lognormal sigma 1, corr 8 cells, 64 planar fractures "
+              "k=1e4), 16x16x16 subdomains of 16^3, 8 eigenvectors, pinned cell, PCG rtol 1e-8",
+        "c2": "128^3 binary box medium (3000 boxes, k=1e5), 8x8x8 subdomains of 16^3, "
+              "6 eigenvectors, pinned cell, PCG rtol 1e-8",
+        "c1": "32^3 GRF (sigma 2, corr 0.1), 4x4x4 subdomains of 8^3, 4 eigenvectors, "
+              "pinned cell, PCG rtol 1e-8",
+    }
+
+    def mode_out(r):
+        o = dict(t_step=round(r["t_step"], 5), t_setup=round(r["t_setup"], 5),
+                 t_solve=round(r["t_solve"], 5), its=r["its"],
+                 t_cold_step=round(r["t_cold"], 5), t_cold_setup=round(r["cold_setup"], 5),
+                 t_cold_lobpcg_tuning=round(r["t_cold_tune"], 5),
+                 t_solve_instrumented=round(r["t_solve_eager"], 5),
+                 lobpcg_iters_max=r["stats"]["eig_iters_max"],
+                 lobpcg_iters_mean=round(r["stats"]["eig_iters_mean"], 2),
+                 lobpcg_block=r["stats"]["eig_block"],
+                 lobpcg_fallbacks=r["stats"]["eig_fallbacks"],
+                 lobpcg_stagnated=r["stats"]["eig_stagnated"],
+                 setup_stages={k: round(v, 5) for k, v in r["stats"]["t_stages"].items()
+                               if k.startswith("su")})
+        if "oracle_its" in r:
+            o["oracle_its"] = r["oracle_its"]
+            o["rel_err_vs_fp64_oracle"] = r["rel_err_vs_oracle"]
+        return o
+
     out = {
         "metric": METRIC,
         "value": round(hr["t_step"], 5),
@@ -618,34 +695,27 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32 preconditioner / f64 CG",
-        "data": "synthetic (seeded binary box medium)",
+        "data": "synthetic (seeded fields of the workload's family)",
         "config": {
-            "workload": "128^3 binary box medium (3000 boxes, k=1e5), 8x8x8 subdomains of 16^3, "
-                        "6 eigenvectors, pinned cell, PCG rtol 1e-8" if prob.name == "c2"
-                        else prob.name,
+            "workload": workloads.get(prob.name, prob.name),
             "headline_mode": head,
             "learned_warm_start": head == "learned",
-            # the timed steps re-set up the warmed-up context, so its LOBPCG
+            # the timed steps re-set up their warmed-up context; its LOBPCG
             # launch runs the subdomains hardest first by the previous
-            # setup's iteration counts (results do not depend on the order)
-            "lobpcg_launch_order": "previous setup's iteration counts",
+            # setup's iteration counts (results do not depend on the order);
+            # t_cold_step is the context's first setup + solve
+            "lobpcg_launch_order": "previous setup's iteration counts (cold step: index order)",
             "extra_options": args.extra or None,
         },
-        "modes": {m: dict(t_step=round(r["t_step"], 5), t_setup=round(r["t_setup"], 5),
-                          t_solve=round(r["t_solve"], 5), its=r["its"],
-                          t_solve_instrumented=round(r["t_solve_eager"], 5),
-                          lobpcg_iters_max=r["stats"]["eig_iters_max"],
-                          lobpcg_fallbacks=r["stats"]["eig_fallbacks"],
-                          lobpcg_stagnated=r["stats"]["eig_stagnated"],
-                          lobpcg_iters_mean=round(r["stats"]["eig_iters_mean"], 2),
-                          setup_stages={k: round(v, 5) for k, v in r["stats"]["t_stages"].items()
-                                        if k.startswith("su")})
-                  for m, r in modes.items()},
+        "modes": {m: mode_out(r) for m, r in modes.items()},
         "speedup_vs_fp64": (round(modes["fp64"]["t_step"] / hr["t_step"], 3)
                             if "fp64" in modes and head != "fp64" else None),
         "rel_diff_vs_fp64": (float(np.linalg.norm(hr["x"] - modes["fp64"]["x"]) /
                                    np.linalg.norm(modes["fp64"]["x"]))
                              if "fp64" in modes and head != "fp64" else None),
+        "oracle_fixture": (f"tests/golden/{prob.name}.npz: FP64 oracle, {int(fix['its'])} "
+                           f"PCG iterations, pressure at {len(fix['cells'])} seeded cells"
+                           if fix is not None else None),
         "warm_start_training": training,
         "solve_hbm_gbs": round(solve_gbs, 1) if solve_gbs else None,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}

@@ -100,6 +100,8 @@ typedef struct {
   double coarse_factor_bytes; /* stored bytes of the level-2 factor (dense inverse or
                                  composite operator, band tiles, plane inverses, ICC(0)) */
   double coarse_solve_bytes;  /* bytes one level-2 solve reads from that factor */
+  double t_eig_tune;          /* seconds of the last setup spent timing LOBPCG block sizes
+                                 (-eig_autotune 1; 0 when the process had them cached) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),

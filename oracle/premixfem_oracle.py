@@ -753,7 +753,22 @@ class Oracle:
             # Cholesky, or a sparse LU (SuperLU) of the block 7-point A_c
             # when a dense matrix would be too large.
             self.chol_c = self.lu_c_sp = None
-            if self.A_c is None:
+            self.z_c = None
+            if self.pin is None and self.wd is None and lay.nbox == 1:
+                # Pure Neumann (the reference drivers' system, B6): A_c is
+                # singular with null vector z = Phi^T M 1 (the coarse
+                # coordinates of the constant).  Its pseudo-inverse on the
+                # consistent residuals: (A_c + g z z^T)^{-1} (t - z z^T t),
+                # g = trace / nc.  What MUMPS does with the singular
+                # factorisation (:894-906) is parity unpinned.
+                z = np.asarray(self.Phi.T @ self.mdiag).ravel()
+                self.z_c = z / np.linalg.norm(z)
+                if self.A_c is None:
+                    raise ValueError("the unpinned coarse pseudo-inverse needs a dense A_c")
+                g = np.trace(self.A_c) / nc
+                self.chol_c = sla.cho_factor(self.A_c + g * np.outer(self.z_c, self.z_c),
+                                             lower=True)
+            elif self.A_c is None:
                 self.lu_c_sp = spla.splu(self.A_c_sp.tocsc())
             else:
                 self.chol_c = sla.cho_factor(self.A_c, lower=True)
@@ -848,6 +863,8 @@ class Oracle:
             cs = sla.lu_solve(self.lu_c, c)
         elif self.chol_c is None:
             cs = self.lu_c_sp.solve(c)
+        elif getattr(self, "z_c", None) is not None:
+            cs = sla.cho_solve(self.chol_c, c - self.z_c * (self.z_c @ c))
         else:
             cs = sla.cho_solve(self.chol_c, c)
         y = self.Phi @ cs

@@ -1097,7 +1097,12 @@ static int setup_common(pmx_ctx* c, const double* phi_nat, const int* eigen_num,
     c->eig_iters_max = c->eig_iters_sum = 0;
   } else {
     c->h_eigen_num_given.clear();
-    if (c->eig_autotune) PMX_TRY(lobpcg_autotune(c));
+    c->t_tune = 0.0;
+    if (c->eig_autotune) {
+      const double tt = now();
+      PMX_TRY(lobpcg_autotune(c));
+      c->t_tune = now() - tt;
+    }
     if (c->warm_start == 1) {
       // learned warm start: the network's block seeds LOBPCG
       const int mb = lobpcg_block_size(c->en_lv1, c->eig_guard);
@@ -1512,6 +1517,7 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
                     : (c->use_band ? (c->use_planes ? 4 : 2) : (c->coarse_composite ? 0 : 1));
   st->coarse_stages = (int)c->coarse_prog.size();
   st->lv3_floor = c->lv3_floor;
+  st->t_eig_tune = c->t_tune;
   {
     const double nc2 = (double)c->nc * c->nc * sizeof(double);
     const double kk = (double)c->nsub * 7 * c->en_lv1 * c->en_lv1 * sizeof(double);

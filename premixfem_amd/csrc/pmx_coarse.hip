@@ -1942,6 +1942,14 @@ int level3_setup(pmx_ctx* ctx) {
   const int n = ctx->nc;
   const int k2 = std::min(ctx->en_lv2, n);
   const int m = std::min(std::max(4 * k2, k2 + 8), std::min(n, 2 * kMaxEig));
+  // Pure Neumann (no pin, no well term; the reference drivers' system): A_c
+  // is singular with the coarse constant z as exact null vector, the first
+  // eigenpair (0, z) of the level-3 problem (SLEPc finds it with its shift
+  // -1e-12, :1086-1096).  The pseudo-inverse iteration below works on z-perp
+  // and supplies the other k2-1 pairs.
+  const bool nullz = ctx->pin_brick < 0 && !ctx->have_wd && ctx->coarse_deflate &&
+                     ctx->coarse_gamma > 0.0;
+  const int k2i = nullz ? k2 - 1 : k2;
   DBuf<double> X, Y, Z, W, R, w, res;
   DBuf<int> fail;
   PMX_TRY(W.alloc((size_t)n * m));
@@ -1993,12 +2001,12 @@ int level3_setup(pmx_ctx* ctx) {
                          ctx->c_z.p, ctx->c_h.p, n, ctx->c_cs.p, Y.p);
       hipLaunchKernelGGL(k_lv3_update, dim3(m), dim3(kBlock), 0, ctx->stream, X.p, Y.p, n,
                          res.p);
-      PMX_HIP(hipMemcpyAsync(hres.data(), res.p, k2 * sizeof(double), hipMemcpyDeviceToHost,
+      PMX_HIP(hipMemcpyAsync(hres.data(), res.p, k2i * sizeof(double), hipMemcpyDeviceToHost,
                              ctx->stream));
       PMX_HIP(hipStreamSynchronize(ctx->stream));
       bool conv = true;
       double mres = 0.0;
-      for (int j = 0; j < k2; ++j) {
+      for (int j = 0; j < k2i; ++j) {
         if (!(hres[j] <= ctx->eig_tol)) conv = false;
         mres = std::max(mres, hres[j]);
       }
@@ -2018,7 +2026,7 @@ int level3_setup(pmx_ctx* ctx) {
       if (ctx->eig_debug >= 0 && (it < 20 || it % 25 == 0)) {
         PMX_HIP(hipMemcpy(hw.data(), w.p, m * sizeof(double), hipMemcpyDeviceToHost));
         printf("lv3 it %d res", it);
-        for (int j = 0; j < k2; ++j) printf(" %.3e", hres[j]);
+        for (int j = 0; j < k2i; ++j) printf(" %.3e", hres[j]);
         printf(" w");
         for (int j = 0; j < m; ++j) printf(" %.4e", hw[j]);
         printf("\n");
@@ -2051,6 +2059,7 @@ int level3_setup(pmx_ctx* ctx) {
     return PMX_ERR_USER;
   }
   ctx->lv3_iters = it;
+  if (nullz) hw.insert(hw.begin(), 0.0);
   // Adaptive count against eb_lv2 (src/PreMixFEM_3D.c:1100-1115).
   int num = k2;
   for (int j = 0; j < k2; ++j)
@@ -2063,8 +2072,16 @@ int level3_setup(pmx_ctx* ctx) {
   ctx->eigen_max_lv2 = hw[num - 1];
   ctx->h_eig_lv2.assign(hw.begin(), hw.begin() + k2);
   PMX_TRY(ctx->V2.alloc((size_t)n * num));
-  PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double), hipMemcpyDeviceToDevice,
-                         ctx->stream));
+  if (nullz) {
+    PMX_HIP(hipMemcpyAsync(ctx->V2.p, ctx->c_z.p, (size_t)n * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    if (num > 1)
+      PMX_HIP(hipMemcpyAsync(ctx->V2.p + n, X.p, (size_t)n * (num - 1) * sizeof(double),
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    PMX_HIP(hipMemcpyAsync(ctx->V2.p, X.p, (size_t)n * num * sizeof(double),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  }
   PMX_TRY(lv3_finish_energy(ctx, num));
   return PMX_SUCCESS;
 }

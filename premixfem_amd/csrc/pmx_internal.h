@@ -305,6 +305,7 @@ struct pmx_ctx {
   int eig_guard = 3;            // extra LOBPCG block vectors beyond en_lv1-1
   int quad_ok = 1;              // quad-cell stream kernels where the bricks allow (-quad 0: off)
   int eig_autotune = 0;         // time the LOBPCG block sizes on a sample and keep the fastest
+  double t_tune = 0.0;          // seconds the autotuner spent in the last setup (0: cached)
   int eig_block = 0;            // block size of the last LOBPCG solve
   double eig_sigma = 1.0e-6;    // shift of the LOBPCG preconditioner A^N + sigma M
   int log_stages = 0;           // synchronise and time every apply stage

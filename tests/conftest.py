@@ -19,3 +19,17 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_hip_first(request):
+    """GPU sessions initialise torch's HIP context before the library's first
+    call (torch provides the device buffers some tests hand to the C ABI)."""
+    if request.config.getoption("-m") and "not gpu" in request.config.getoption("-m"):
+        return
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass

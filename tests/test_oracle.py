@@ -295,3 +295,21 @@ def test_exact_level1_blocks_are_the_principal_submatrices():
     _, its_exact, _, reason = O.pcg(orc.matvec, b, orc.apply, rtol=1e-8)
     _, its_icc, _, _ = O.pcg(ref.matvec, b, ref.apply, rtol=1e-8)
     assert reason == "CONVERGED_RTOL" and its_exact <= its_icc
+
+
+def test_unpinned_coarse_solve_is_the_pseudo_inverse():
+    """The reference drivers' pure-Neumann system (B6): A_c is singular with
+    the coarse constant as null vector; the oracle's level-2 solve is its
+    pseudo-inverse on consistent vectors (checked against numpy's pinv)."""
+    mesh = (12, 12, 12)
+    k = problems.cross_kappa(mesh, 2)
+    orc = O.Oracle((1, 1, 1), mesh, k, sd=2, en_lv1=4, en_lv2=4).setup()
+    A_c = orc.A_c
+    z = orc.z_c
+    assert np.linalg.norm(A_c @ z) <= 1e-10 * np.linalg.norm(A_c)
+    t = np.random.default_rng(1).standard_normal(A_c.shape[0])
+    t -= z * (z @ t)
+    want = np.linalg.pinv(A_c, rcond=1e-12, hermitian=True) @ t
+    got = sla.cho_solve(orc.chol_c, t - z * (z @ t))
+    assert np.linalg.norm(got - want) <= 1e-8 * np.linalg.norm(want)
+    assert abs(z @ got) <= 1e-10 * np.linalg.norm(got)
