@@ -556,8 +556,13 @@ static int setup_coarse(pmx_ctx* c) {
   const double t3 = now();
   // The dense path composes the coarse stages into one nc x nc operator; the
   // ICC(0) path runs them per apply in one workgroup (no nc^2 storage).
-  if (!c->lv2_icc && !c->use_band && c->coarse_composite)
-    PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  // A single stage needs no composition: the GEMV with the deflated inverse
+  // and its separate rank-2 correction is the same operator, and keeps the
+  // large near-null term (~1/lambda_min) out of the dense entries, where its
+  // rounding would reach every component of the correction.
+  c->use_fop = !c->lv2_icc && !c->use_band && c->coarse_composite && c->coarse_prog.size() > 1;
+  if (c->use_fop) PMX_TRY(coarse_operator_setup(c, c->coarse_prog));
+  else c->Fop.release();
   c->t_stages[PMX_STAGE_SU_LV3_2] += now() - t3;
   return PMX_SUCCESS;
 }
@@ -693,7 +698,7 @@ static int apply_device(pmx_ctx* c, double* x, double* z, bool cg_partial, bool 
   double* tv = c->c_a.p;
   double* u = c->c_c.p;
   double* d = c->c_b.p;
-  if (c->lv2_icc || c->use_band || !c->coarse_composite) {
+  if (!c->use_fop) {
     PMX_TRY(launch_coarse_program(c, c->coarse_prog, tv, u, c->c_e.p, d, 6, 7, kt_begin, kt_end));
   } else {
     ev = kt_begin(c, 6);
@@ -1514,7 +1519,7 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->eig_stagnated = c->eig_stagnated;
   st->eig_block = c->eig_block;
   st->coarse_mode = c->lv2_icc ? 3
-                    : (c->use_band ? (c->use_planes ? 4 : 2) : (c->coarse_composite ? 0 : 1));
+                    : (c->use_band ? (c->use_planes ? 4 : 2) : (c->use_fop ? 0 : 1));
   st->coarse_stages = (int)c->coarse_prog.size();
   st->lv3_floor = c->lv3_floor;
   st->t_eig_tune = c->t_tune;

@@ -376,6 +376,7 @@ struct pmx_ctx {
   int coarse_composite = 1;         // dense path: one composed coarse operator (1) or the
                                     // coarse stages one by one with their residual updates (0)
   pmx::DBuf<double> Fop;            // composite coarse operator of the V-cycle (nc x nc)
+  int use_fop = 0;                  // the apply uses Fop (dense path, more than one stage)
   std::vector<int> coarse_prog;     // stage list: 2 = level 2, 3 = level 3
   // One process with an exact level-2 factor: the coarse stages after the
   // first level-2 stage vanish in exact arithmetic, and the program is cut

@@ -219,7 +219,7 @@ def test_plane_factor_matches_dense_inverse(mesh, sd, k):
     print("planes vs dense", mesh, sd, k, flush=True)
     a = runner.run(prob, extra="-coarse_band 1")
     b = runner.run(prob)
-    assert a["stats"]["coarse_mode"] == 4 and b["stats"]["coarse_mode"] == 0
+    assert a["stats"]["coarse_mode"] == 4 and b["stats"]["coarse_mode"] in (0, 1)
     err = np.linalg.norm(a["x"] - b["x"]) / np.linalg.norm(b["x"])
     print(f"planes: its {a['its']} vs {b['its']}, x rel diff {err:.2e}, lv3 "
           f"{a['stats']['eigen_min_lv2']:.6e} vs {b['stats']['eigen_min_lv2']:.6e}", flush=True)
