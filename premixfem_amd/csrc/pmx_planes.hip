@@ -24,79 +24,103 @@
 
 namespace pmx {
 
-constexpr int kPlThreads = 256;
+constexpr int kPlMaxCols = 32;    // right-hand sides of one solve (level-3 set-up blocks)
 
-// One plane step for right-hand side blockIdx.y (vectors of stride nc):
-// out = G_z w_z for the rows of subdomain s = sub0 + blockIdx.x, then
+// One plane step for the right-hand sides c0 = NCG * blockIdx.y .. c0+NCG-1
+// (vectors of stride nc) and subdomain s = sub0 + blockIdx.x of plane z, one
+// workgroup of NT threads per subdomain:
+//   out = G_z w_z for the rows of s, then
 //   fwd:  w[s + z-neighbour] -= A_c(s_up, s) out           (slot -z of s_up)
 //   bwd:  d[s] = out;  w[s - z-neighbour] -= A_c(s_dn, s) out  (slot +z of s_dn)
-// Every subdomain of plane z has at most one neighbour in the next plane
-// and no two share it, so the in-place updates of w do not collide.
-template <int KR>
-__global__ void __launch_bounds__(kPlThreads) k_pl_step(Geo g, const double* __restrict__ Acb,
-                                                        int kmax, const double* __restrict__ G,
-                                                        int np, int dof0, int sub0, int fwd,
-                                                        double* __restrict__ w,
-                                                        double* __restrict__ d, int nc,
-                                                        const double* stop) {
+// The workgroup is wide (NT = 1024 for one right-hand side: at n_p = 2048 every
+// thread issues all of its row loads at once), so one workgroup per CU keeps
+// enough loads in flight to stream G_z near HBM speed with no exchange between
+// workgroups.  Several right-hand sides share every load of G_z.  Every
+// subdomain of plane z has at most one neighbour in the next plane and no two
+// share it, so the in-place updates of w do not collide.
+template <int KR, int NCG, int NT>
+__global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict__ Acb, int kmax,
+                                                const double* __restrict__ G, int np, int dof0,
+                                                int sub0, int fwd, double* __restrict__ w,
+                                                double* __restrict__ d, int nc, int ncol,
+                                                const double* stop) {
   PMX_STOP_CHECK(stop);
-  __shared__ double red[kPlThreads / 64][KR];
-  __shared__ double outv[KR];
+  constexpr int NW = NT / 64;
+  __shared__ double red[NW][NCG * KR];
+  __shared__ double outv[NCG * KR];
   const int s = sub0 + blockIdx.x;
   const Brick b = g.bricks[s];
   const int nr = b.neig, r0 = b.dof - dof0;
-  const size_t co = (size_t)blockIdx.y * nc;
-  const double* x = w + co + dof0;
-  double acc[KR];
+  const int c0 = blockIdx.y * NCG;
+  const int ncg = min(NCG, ncol - c0);
+  double acc[NCG][KR];
 #pragma unroll
-  for (int i = 0; i < KR; ++i) acc[i] = 0.0;
+  for (int c = 0; c < NCG; ++c)
+#pragma unroll
+    for (int i = 0; i < KR; ++i) acc[c][i] = 0.0;
   const double* Gr = G + (size_t)r0 * np;
+  const double* x[NCG];
+#pragma unroll
+  for (int c = 0; c < NCG; ++c) x[c] = w + (size_t)(c0 + min(c, ncg - 1)) * nc + dof0;
   if (((np | dof0 | nc) & 1) == 0) {  // 16-byte aligned rows and segments
-    const double2* x2 = reinterpret_cast<const double2*>(x);
     const int n2 = np >> 1;
-#pragma unroll 2
-    for (int j = threadIdx.x; j < n2; j += kPlThreads) {
-      const double2 xv = x2[j];
+    for (int j = threadIdx.x; j < n2; j += NT) {
+      double2 xv[NCG];
+#pragma unroll
+      for (int c = 0; c < NCG; ++c) xv[c] = reinterpret_cast<const double2*>(x[c])[j];
 #pragma unroll
       for (int i = 0; i < KR; ++i) {
-        if (i < nr) {
+        if (KR == 8 || i < nr) {  // every brick of a KR = 8 instance has 8 rows
           const double2 gv = reinterpret_cast<const double2*>(Gr + (size_t)i * np)[j];
-          acc[i] += gv.x * xv.x + gv.y * xv.y;
+#pragma unroll
+          for (int c = 0; c < NCG; ++c) acc[c][i] += gv.x * xv[c].x + gv.y * xv[c].y;
         }
       }
     }
   } else {
-    for (int j = threadIdx.x; j < np; j += kPlThreads) {
-      const double xv = x[j];
+    for (int j = threadIdx.x; j < np; j += NT) {
+      double xv[NCG];
 #pragma unroll
-      for (int i = 0; i < KR; ++i)
-        if (i < nr) acc[i] += Gr[(size_t)i * np + j] * xv;
+      for (int c = 0; c < NCG; ++c) xv[c] = x[c][j];
+#pragma unroll
+      for (int i = 0; i < KR; ++i) {
+        if (KR == 8 || i < nr) {
+          const double gv = Gr[(size_t)i * np + j];
+#pragma unroll
+          for (int c = 0; c < NCG; ++c) acc[c][i] += gv * xv[c];
+        }
+      }
     }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < KR; ++i) {
-    double v = acc[i];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) red[wv][i] = v;
-  }
+  for (int c = 0; c < NCG; ++c)
+#pragma unroll
+    for (int i = 0; i < KR; ++i) {
+      double v = acc[c][i];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wv][c * KR + i] = v;
+    }
   __syncthreads();
-  if (threadIdx.x < KR) {
+  if (threadIdx.x < NCG * KR) {
     double v = 0.0;
-    for (int q = 0; q < kPlThreads / 64; ++q) v += red[q][threadIdx.x];
+    for (int q = 0; q < NW; ++q) v += red[q][threadIdx.x];
     outv[threadIdx.x] = v;
-    if (!fwd && (int)threadIdx.x < nr) d[co + b.dof + threadIdx.x] = v;
   }
   __syncthreads();
   const int nbi = b.nb[fwd ? 5 : 4];
-  if (nbi < 0) return;
-  const Brick o = g.bricks[nbi];
-  const int slot = fwd ? 5 : 6;  // the block of o that couples it with s
-  const double* blk = Acb + ((size_t)nbi * 7 + slot) * kmax * kmax;
-  if ((int)threadIdx.x < o.neig) {
-    double v = 0.0;
-    for (int j = 0; j < nr; ++j) v += blk[threadIdx.x * kmax + j] * outv[j];
-    w[co + o.dof + threadIdx.x] -= v;
+  const Brick o = nbi >= 0 ? g.bricks[nbi] : b;
+  const int slot_c = fwd ? 5 : 6;  // the block of o that couples it with s
+  const double* blk = nbi >= 0 ? Acb + ((size_t)nbi * 7 + slot_c) * kmax * kmax : nullptr;
+  for (int t = threadIdx.x; t < ncg * KR; t += NT) {
+    const int c = t / KR, i = t % KR;
+    const size_t co = (size_t)(c0 + c) * nc;
+    if (!fwd && i < nr) d[co + b.dof + i] = outv[c * KR + i];
+    if (nbi >= 0 && i < o.neig) {
+      double v = 0.0;
+      for (int j = 0; j < nr; ++j) v += blk[i * kmax + j] * outv[c * KR + j];
+      w[co + o.dof + i] -= v;
+    }
   }
 }
 
@@ -196,7 +220,7 @@ int planes_setup(pmx_ctx* ctx) {
   PMX_TRY(ctx->pl_G.alloc(ctx->pl_goff[Z]));
   // the solves' work vectors (up to 32 right-hand sides) now, so a solve
   // never allocates
-  PMX_TRY(ctx->pl_w.alloc((size_t)ctx->nc * 32));
+  PMX_TRY(ctx->pl_w.alloc((size_t)ctx->nc * kPlMaxCols));
   DBuf<double> S, H;
   DBuf<int> fail;
   PMX_TRY(S.alloc((size_t)npmax * npmax));
@@ -239,24 +263,35 @@ int planes_setup(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
+template <int KR, int NCG, int NT>
+static void pl_step(pmx_ctx* ctx, int z, int fwd, double* w, double* d, int ncol) {
+  const int L = ctx->sd[0] * ctx->sd[1];
+  const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
+  hipLaunchKernelGGL((k_pl_step<KR, NCG, NT>), dim3(L, (ncol + NCG - 1) / NCG), dim3(NT), 0,
+                     ctx->stream, ctx->geo, ctx->Acb.p, ctx->en_lv1,
+                     ctx->pl_G.p + ctx->pl_goff[z], np, dof0, z * L, fwd, w, d, ctx->nc, ncol,
+                     ctx->geo.stop);
+}
+
 int launch_planes_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
-  const int L = ctx->sd[0] * ctx->sd[1], Z = ctx->sd[2], nc = ctx->nc;
-  PMX_TRY(ctx->pl_w.alloc((size_t)nc * std::max(ncol, 32)));
+  const int Z = ctx->sd[2], nc = ctx->nc;
+  if (ncol > kPlMaxCols) {
+    set_error("coarse solve by planes: %d right-hand sides (at most %d)", ncol, kPlMaxCols);
+    return PMX_ERR_SUP;
+  }
   double* w = ctx->pl_w.p;
   PMX_HIP(hipMemcpyAsync(w, t, (size_t)nc * ncol * sizeof(double), hipMemcpyDeviceToDevice,
                          ctx->stream));
-  const bool wide = ctx->en_lv1 > 8;
+  // bricks of exactly 8 eigenvectors take the unguarded 8-row instances
+  const bool k8 = ctx->en_lv1 == 8 && nc == 8 * ctx->nsub;
   auto step = [&](int z, int fwd) {
-    const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
-    const double* Gz = ctx->pl_G.p + ctx->pl_goff[z];
-    if (wide)
-      hipLaunchKernelGGL(k_pl_step<kMaxEig>, dim3(L, ncol), dim3(kPlThreads), 0, ctx->stream,
-                         ctx->geo, ctx->Acb.p, ctx->en_lv1, Gz, np, dof0, z * L, fwd, w, d, nc,
-                         ctx->geo.stop);
-    else
-      hipLaunchKernelGGL(k_pl_step<8>, dim3(L, ncol), dim3(kPlThreads), 0, ctx->stream,
-                         ctx->geo, ctx->Acb.p, ctx->en_lv1, Gz, np, dof0, z * L, fwd, w, d, nc,
-                         ctx->geo.stop);
+    if (ncol == 1) {
+      if (k8) pl_step<8, 1, 1024>(ctx, z, fwd, w, d, ncol);
+      else pl_step<kMaxEig, 1, 1024>(ctx, z, fwd, w, d, ncol);
+    } else {
+      if (k8) pl_step<8, 4, 512>(ctx, z, fwd, w, d, ncol);
+      else pl_step<kMaxEig, 2, 512>(ctx, z, fwd, w, d, ncol);
+    }
   };
   for (int z = 0; z + 1 < Z; ++z) step(z, 1);
   for (int z = Z - 1; z >= 0; --z) step(z, 0);
