@@ -117,7 +117,10 @@ typedef struct {
  * factor in band storage; automatic above 12288 coarse unknowns),
  * -eig_guard g (LOBPCG guard vectors), -eig_autotune 0|1 (time the LOBPCG
  * block sizes on a sample of subdomains and keep the fastest, cached per
- * subdomain size and eigen count for the process),
+ * subdomain size and eigen count for the process), -eig_kernel 0|1 (the
+ * subdomain LOBPCG kernel: 1, default, Grams and block transforms on the FP64
+ * matrix cores with fused passes; 0, the VALU kernel; same eigenpairs to the
+ * eigensolver tolerance),
  * -coarse_collapse 0|1 (one process with an exact level-2 factor: apply only
  * the first level-2 stage of the coarse cycle, since the later level-3 and
  * level-2 stages add exactly zero in exact arithmetic; default 1),

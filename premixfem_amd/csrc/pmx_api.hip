@@ -219,6 +219,12 @@ static int parse_options(pmx_ctx* c, const char* options) {
       PMX_TRY(geti(c->eig_guard));
     } else if (k == "-bicc_prefetch") {
       PMX_TRY(geti(c->bicc_prefetch));
+    } else if (k == "-eig_kernel") {
+      PMX_TRY(geti(c->eig_kernel));
+      if (c->eig_kernel < 0 || c->eig_kernel > 1) {
+        set_error("-eig_kernel %d: 0 (VALU) or 1 (matrix cores)", c->eig_kernel);
+        return PMX_ERR_ARG_OUTOFRANGE;
+      }
     } else if (k == "-eig_order") {
       PMX_TRY(geti(c->eig_order));
     } else if (k == "-eig_debug") {

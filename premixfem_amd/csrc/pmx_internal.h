@@ -312,6 +312,7 @@ struct pmx_ctx {
   int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels
   int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
   int bicc_prefetch = 1;        // block ICC(0) level-2 solve with row data loaded a round ahead
+  int eig_kernel = 1;           // LOBPCG kernel: 1 matrix-core passes, 0 VALU
   int eig_order = 1;            // launch LOBPCG hardest-first from the previous setup's counts
   std::vector<int> h_lob_its;   // LOBPCG iterations per subdomain of the last setup
   pmx::DBuf<int> d_lob_order;

@@ -83,12 +83,13 @@ def test_apply_and_pcg_with_oracle_basis(name):
     assert err <= 1e-8, err
 
 
+@pytest.mark.parametrize("kernel", [1, 0], ids=["mfma", "valu"])
 @pytest.mark.parametrize("name", list(CASES))
-def test_full_setup_lobpcg(name):
+def test_full_setup_lobpcg(name, kernel):
     prob = CASES[name]()
-    print("case", name, flush=True)
+    print("case", name, "eig_kernel", kernel, flush=True)
     orc, x_ref, its_ref, _, _ = _oracle(prob)
-    res = runner.run(prob, keep_ctx=True)
+    res = runner.run(prob, keep_ctx=True, extra=f"-eig_kernel {kernel}")
     ctx = res["ctx"]
     try:
         phi, ev = P.PC_get_basis(ctx, 2)
