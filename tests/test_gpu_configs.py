@@ -90,6 +90,49 @@ def test_c1_fp32_preconditioner():
 
 
 # ---------------------------------------------------------------------------
+# 128^3 binary box medium (1e5 contrast), 8^3 subdomains of 16^3, 6 eigenvectors
+# ---------------------------------------------------------------------------
+def test_c2_fp64_against_oracle_fixture():
+    """The 1e5-contrast medium at full size.  Its pinned coarse matrix has
+    lambda_min(A_c) ~ 5.7e-12 (the fixture's first level-3 eigenvalue), so the
+    coarse solves amplify rounding by ~1e11 and PCG needs ~1500 iterations,
+    a count that moves by a few percent with the rounding of the coarse solve
+    alone: the oracle takes 1478 iterations with the full cycle and 1479 with
+    the cycle cut to [lv1, lv2, lv1] (both recorded in the fixture; an
+    earlier assembly order of the same oracle took 1461).  The GPU uses its own
+    subdomain eigenvectors (eigensolver tolerance 1e-8), so its rtol-1e-8
+    solution carries a different unconverged error than the oracle's: the
+    reference is the GPU's solve to rtol 1e-11, against which the GPU's
+    rtol-1e-8 pressure must be as accurate as the oracle's (within 3x) at the
+    fixture's cells.  Checked besides: the count within 5 % of the oracle's
+    and the true residual at the tolerance."""
+    prob = problems.config("c2")
+    f = _fixture("c2")
+    _check_field(prob, f)
+    print("c2 fp64", flush=True)
+    res = runner.run(prob, precision=64)
+    cells = f["cells"]
+    ex = _rel(res["x"][cells], f["x_sample"])
+    lay = O.Layout(prob.dom, prob.mesh, prob.sd)
+    T = O.transmissibilities(lay, prob.kappa)
+    pin = (tuple(prob.pin_cell), O.pin_tau(lay, T, prob.pin_cell))
+    b = prob.rhs.ravel()
+    r = np.linalg.norm(b - O.stencil_apply(lay, T, res["x"], pin)) / np.linalg.norm(b)
+    its_o, its_c = int(f["its"]), int(f["its_collapsed"])
+    tight = runner.run(prob, precision=64, rtol=1e-11)
+    xs = tight["x"][cells]
+    e_gpu, e_orc = _rel(res["x"][cells], xs), _rel(f["x_sample"], xs)
+    print(f"c2: its {res['its']} vs oracle {its_o} (cut cycle {its_c}), x vs oracle {ex:.2e}; "
+          f"true rel residual {r:.2e} (oracle {float(f['true_rel_residual']):.2e}); error vs the "
+          f"rtol-1e-11 solve ({tight['its']} its): GPU {e_gpu:.2e}, oracle {e_orc:.2e}",
+          flush=True)
+    assert res["reason"] == "CONVERGED_RTOL" and tight["reason"] == "CONVERGED_RTOL"
+    assert abs(res["its"] - its_o) <= 0.05 * its_o
+    assert r <= 2e-8
+    assert e_gpu <= 3.0 * e_orc
+
+
+# ---------------------------------------------------------------------------
 # 16^3-cell subdomains (the 128^3 and 256^3 workloads' subdomain size) on a
 # 32^3 grid: the 512-thread LOBPCG instantiations at block sizes 8 and 10
 # and the quad-cell stream kernels on 16^3 bricks, against the oracle.
