@@ -102,6 +102,11 @@ typedef struct {
   double coarse_solve_bytes;  /* bytes one level-2 solve reads from that factor */
   double t_eig_tune;          /* seconds of the last setup spent timing LOBPCG block sizes
                                  (-eig_autotune 1; 0 when the process had them cached) */
+  int eig_shape;              /* matrix-core LOBPCG workgroup shape used (-eig_shape, or
+                                 chosen by -eig_autotune 1): 0 one subdomain per CU,
+                                 1 two per CU */
+  double t_eig_kernel;        /* seconds of the last setup's subdomain LOBPCG launch, from
+                                 HIP events on the library stream (-kernel_timing; else 0) */
 } pmx_stats;
 
 /* PC_init (src/PreMixFEM_3D.c:1650-1741): domain lengths dom[3] (L, W, H),
@@ -116,11 +121,14 @@ typedef struct {
  * setup's unless -lv3_warm 0), -coarse_band 0|1 (exact level-2 Cholesky
  * factor in band storage; automatic above 12288 coarse unknowns),
  * -eig_guard g (LOBPCG guard vectors), -eig_autotune 0|1 (time the LOBPCG
- * block sizes on a sample of subdomains and keep the fastest, cached per
- * subdomain size and eigen count for the process), -eig_kernel 0|1 (the
+ * block sizes, and the workgroup shapes of the matrix-core kernel, on a
+ * sample of subdomains and keep the fastest, cached per subdomain size and
+ * eigen count for the process), -eig_kernel 0|1 (the
  * subdomain LOBPCG kernel: 1, default, Grams and block transforms on the FP64
  * matrix cores with fused passes; 0, the VALU kernel; same eigenpairs to the
- * eigensolver tolerance),
+ * eigensolver tolerance), -eig_shape 0|1 (matrix-core kernel: one
+ * 512-thread workgroup per CU on large subdomains, default, or two 256-thread
+ * ones per CU; same eigenpairs),
  * -coarse_collapse 0|1 (one process with an exact level-2 factor: apply only
  * the first level-2 stage of the coarse cycle, since the later level-3 and
  * level-2 stages add exactly zero in exact arithmetic; default 1),

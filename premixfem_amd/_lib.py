@@ -62,7 +62,8 @@ class Stats(C.Structure):
         ("eig_stagnated", C.c_int), ("eig_block", C.c_int), ("coarse_mode", C.c_int),
         ("coarse_stages", C.c_int), ("lv3_floor", C.c_double),
         ("coarse_factor_bytes", C.c_double), ("coarse_solve_bytes", C.c_double),
-        ("t_eig_tune", C.c_double),
+        ("t_eig_tune", C.c_double), ("eig_shape", C.c_int),
+        ("t_eig_kernel", C.c_double),
     ]
 
     def as_dict(self):

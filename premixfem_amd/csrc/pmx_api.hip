@@ -225,6 +225,12 @@ static int parse_options(pmx_ctx* c, const char* options) {
         set_error("-eig_kernel %d: 0 (VALU) or 1 (matrix cores)", c->eig_kernel);
         return PMX_ERR_ARG_OUTOFRANGE;
       }
+    } else if (k == "-eig_shape") {
+      PMX_TRY(geti(c->eig_shape));
+      if (c->eig_shape < 0 || c->eig_shape > 1) {
+        set_error("-eig_shape %d: 0 (one subdomain per CU) or 1 (two per CU)", c->eig_shape);
+        return PMX_ERR_ARG_OUTOFRANGE;
+      }
     } else if (k == "-eig_order") {
       PMX_TRY(geti(c->eig_order));
     } else if (k == "-eig_debug") {
@@ -1177,7 +1183,10 @@ int pmx_setup_batch(pmx_ctx* const* ctxs, int nprob) {
     if (cs[i]->eig_autotune && (tuner < 0 || cs[i]->nsub > cs[tuner]->nsub)) tuner = i;
   if (tuner >= 0) {
     PMX_TRY(lobpcg_autotune(cs[tuner]));
-    for (pmx_ctx* c : cs) c->eig_guard = cs[tuner]->eig_guard;
+    for (pmx_ctx* c : cs) {
+      c->eig_guard = cs[tuner]->eig_guard;
+      c->eig_shape = cs[tuner]->eig_shape;
+    }
   }
   PMX_TRY(lobpcg_batch(cs.data(), nprob));
   const double t_eig = now() - t1;  // one shared launch: charged to every problem
@@ -1529,6 +1538,8 @@ int pmx_get_stats(const pmx_ctx* c, pmx_stats* st) {
   st->coarse_stages = (int)c->coarse_prog.size();
   st->lv3_floor = c->lv3_floor;
   st->t_eig_tune = c->t_tune;
+  st->eig_shape = c->eig_shape;
+  st->t_eig_kernel = c->t_eig_kernel;
   {
     const double nc2 = (double)c->nc * c->nc * sizeof(double);
     const double kk = (double)c->nsub * 7 * c->en_lv1 * c->en_lv1 * sizeof(double);

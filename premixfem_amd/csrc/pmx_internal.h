@@ -309,10 +309,13 @@ struct pmx_ctx {
   int eig_block = 0;            // block size of the last LOBPCG solve
   double eig_sigma = 1.0e-6;    // shift of the LOBPCG preconditioner A^N + sigma M
   int log_stages = 0;           // synchronise and time every apply stage
-  int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels
+  int kernel_timing = 0;        // HIP-event timing of the solve-phase kernels and the LOBPCG launch
+  double t_eig_kernel = 0.0;    // seconds of the last LOBPCG launch (-kernel_timing only)
   int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
   int bicc_prefetch = 1;        // block ICC(0) level-2 solve with row data loaded a round ahead
   int eig_kernel = 1;           // LOBPCG kernel: 1 matrix-core passes, 0 VALU
+  int eig_shape = 0;            // matrix-core LOBPCG on large subdomains: 0 one per CU
+                                // (512 threads), 1 two per CU (256 threads)
   int eig_order = 1;            // launch LOBPCG hardest-first from the previous setup's counts
   std::vector<int> h_lob_its;   // LOBPCG iterations per subdomain of the last setup
   pmx::DBuf<int> d_lob_order;

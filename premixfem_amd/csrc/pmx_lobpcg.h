@@ -284,8 +284,9 @@ __device__ inline int wg_svqb(double* G, double* G1, double* Z, double* Q1, doub
 // Launches of the matrix-core kernel (pmx_lobpcg_mf.hip) for block size MB in
 // {4, 6, 8, 10}: one workgroup per subdomain (nblocks of them), threads per
 // workgroup chosen from the largest subdomain (maxn cells) as for the VALU
-// kernel.
-void launch_lobpcg_mf(int MB, const Geo& g, int nblocks, hipStream_t st, const LobArgs& a,
-                      int maxn);
+// kernel; shape 1 runs large subdomains (>= kLobBigN cells) two per CU
+// (256 threads, narrower sweep chunks) instead of one per CU.
+void launch_lobpcg_mf(int MB, int shape, const Geo& g, int nblocks, hipStream_t st,
+                      const LobArgs& a, int maxn);
 
 }  // namespace pmx

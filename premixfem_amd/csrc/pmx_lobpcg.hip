@@ -530,13 +530,16 @@ static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, f
     a.order = ctx->d_lob_order.p;
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ms) {
+  // Autotuner trials, and the real launch under -kernel_timing, are timed
+  // with HIP events on the library stream.
+  const bool timed = ms != nullptr || ctx->kernel_timing;
+  if (timed) {
     PMX_HIP(hipEventCreate(&e0));
     PMX_HIP(hipEventCreate(&e1));
     PMX_HIP(hipEventRecord(e0, ctx->stream));
   }
   if (ctx->eig_kernel == 1) {
-    launch_lobpcg_mf(MB, ctx->geo, (ctx->nsub + a.stride - 1) / a.stride, ctx->stream, a, maxn);
+    launch_lobpcg_mf(MB, ctx->eig_shape, ctx->geo, (ctx->nsub + a.stride - 1) / a.stride, ctx->stream, a, maxn);
   } else {
   switch (MB) {
     case 4:
@@ -554,10 +557,13 @@ static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, f
   }
   }
   PMX_HIP(hipGetLastError());
-  if (ms) {
+  if (timed) {
     PMX_HIP(hipEventRecord(e1, ctx->stream));
     PMX_HIP(hipEventSynchronize(e1));
-    PMX_HIP(hipEventElapsedTime(ms, e0, e1));
+    float el = 0.0f;
+    PMX_HIP(hipEventElapsedTime(&el, e0, e1));
+    if (ms) *ms = el;
+    else ctx->t_eig_kernel = 1e-3 * el;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
@@ -607,16 +613,20 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
   return lobpcg_run(ctx, x0_rows, lobpcg_block_size(ctx->en_lv1, ctx->eig_guard), 1, nullptr);
 }
 
-// Block-size autotuner of the batched LOBPCG (-eig_autotune 1).  The block
-// size MB (wanted pairs + guard vectors) trades work per iteration against
-// iterations; the best value depends on the subdomain size, the eigen count
-// and the medium.  Every admissible MB in {4, 6, 8, 10} is timed from the
-// start the setup will use (random, or the learned warm start) on the whole
-// batch (a strided sample of 1024 subdomains on
-// larger problems: the launch is timed at the occupancy it runs at); the
-// fastest sets -eig_guard, cached per (largest subdomain, en_lv1, well
-// term, warm start) for the process.  Tuning only changes how fast LOBPCG reaches the
-// tolerance.
+// Autotuner of the batched LOBPCG (-eig_autotune 1) over the block size and,
+// for the matrix-core kernel, the workgroup shape.  The block size MB
+// (wanted pairs + guard vectors) trades work per iteration against
+// iterations; the shape trades waves per subdomain against subdomains per CU
+// (one 512-thread workgroup per CU, or two 256-thread ones with narrower
+// sweep chunks on large subdomains; every workgroup held to 256 registers on
+// small ones).  The best pair depends on the subdomain size, the eigen count
+// and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed
+// from the start the setup will use (random, or the learned warm start) on
+// the whole batch (a strided sample of 1024 subdomains on larger problems:
+// the launch is timed at the occupancy it runs at); the fastest sets
+// -eig_guard and -eig_shape, cached per (largest subdomain, en_lv1, well
+// term, warm start, kernel) for the process.  Tuning only changes how fast
+// LOBPCG reaches the tolerance.
 namespace {
 struct TuneKey {
   int maxn, k, well, warm, kernel;
@@ -627,8 +637,11 @@ struct TuneKey {
     return warm != o.warm ? warm < o.warm : kernel < o.kernel;
   }
 };
-std::map<TuneKey, int>& tune_cache() {
-  static std::map<TuneKey, int> m;
+struct TuneChoice {
+  int mb, shape;
+};
+std::map<TuneKey, TuneChoice>& tune_cache() {
+  static std::map<TuneKey, TuneChoice> m;
   return m;
 }
 }  // namespace
@@ -646,27 +659,35 @@ int lobpcg_autotune(pmx_ctx* ctx) {
   if (it == tune_cache().end()) {
     const int need = ctx->have_wd ? k + 1 : k;   // at least one guard vector
     const int stride = std::max(1, (ctx->nsub + 1023) / 1024);
-    int best = 0;
+    TuneChoice best{0, 0};
     float best_ms = 0.0f;
+    const int user_shape = ctx->eig_shape;
+    const int nshape = ctx->eig_kernel == 1 ? 2 : 1;
     for (int mb : {4, 6, 8, 10}) {
       if (mb < need || mb > minn - 1) continue;
-      float ms = 0.0f;
       // with the learned warm start the trials start from it, as the setup will
       DBuf<double> x0;
       if (warm) {
         PMX_TRY(x0.alloc((size_t)ctx->ncell * mb));
         PMX_TRY(cnn_warm_start(ctx, mb, x0.p));
       }
-      PMX_TRY(lobpcg_run(ctx, warm ? x0.p : nullptr, mb, stride, &ms));
-      if (!best || ms < best_ms) {
-        best = mb;
-        best_ms = ms;
+      for (int shape = 0; shape < nshape; ++shape) {
+        float ms = 0.0f;
+        ctx->eig_shape = shape;
+        const int rc = lobpcg_run(ctx, warm ? x0.p : nullptr, mb, stride, &ms);
+        ctx->eig_shape = user_shape;
+        PMX_TRY(rc);
+        if (!best.mb || ms < best_ms) {
+          best = TuneChoice{mb, shape};
+          best_ms = ms;
+        }
       }
     }
-    if (!best) return PMX_SUCCESS;   // nothing admissible: keep -eig_guard
+    if (!best.mb) return PMX_SUCCESS;   // nothing admissible: keep -eig_guard / -eig_shape
     it = tune_cache().emplace(key, best).first;
   }
-  ctx->eig_guard = it->second - (k - 1);
+  ctx->eig_guard = it->second.mb - (k - 1);
+  if (ctx->eig_kernel == 1) ctx->eig_shape = it->second.shape;
   return PMX_SUCCESS;
 }
 
@@ -697,6 +718,7 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
     if (c->en_lv1 != k || c->eig_guard != c0->eig_guard || c->eig_tol != c0->eig_tol ||
         c->eig_maxit != c0->eig_maxit || c->eig_sigma != c0->eig_sigma ||
         c->seed != c0->seed || c->eig_stag != c0->eig_stag || c->eig_kernel != c0->eig_kernel ||
+        c->eig_shape != c0->eig_shape ||
         (c->warm_start == 1) != (c0->warm_start == 1)) {
       set_error("batched setup: problem %d has other en_lv1 / eigensolver options", i);
       return PMX_ERR_ARG_WRONG;
@@ -849,7 +871,7 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
   a.debug = -1;
   a.order = nullptr;
   if (c0->eig_kernel == 1) {
-    launch_lobpcg_mf(MB, g, S, st, a, maxn);
+    launch_lobpcg_mf(MB, c0->eig_shape, g, S, st, a, maxn);
   } else {
   switch (MB) {
     case 4:

@@ -397,12 +397,16 @@ __device__ inline void wg_symmetrize(double* A, int m, int ld) {
   __syncthreads();
 }
 
-template <int MB, int BS>
-__global__ void __launch_bounds__(BS) k_lobpcg_mf(Geo g, LobArgs a) {
+// One subdomain per workgroup of BS threads, the preconditioner sweeps on
+// chunks of CH columns.  WPE > 1 holds the registers to what WPE waves per
+// SIMD leave each wave (WPE = 1: no limit beyond the launch bounds; at 512
+// threads the bounds alone give two waves per SIMD, one workgroup per CU).
+template <int MB, int BS, int CH, int WPE>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_lobpcg_mf(Geo g, LobArgs a) {
   static_assert(3 * MB <= kKP, "LOBPCG block too wide for the update tiles");
   constexpr int LD = 3 * MB;
   constexpr int NW = BS / 64;
-  constexpr int CH = lobpcg_mf_ch(MB);
   extern __shared__ double lds[];
   const int sidx = a.order ? a.order[blockIdx.x] : blockIdx.x * a.stride;
   const Brick b = g.bricks[sidx];
@@ -816,47 +820,65 @@ __global__ void __launch_bounds__(BS) k_lobpcg_mf(Geo g, LobArgs a) {
   }
 }
 
+// Shape 1: every workgroup held to 256 registers (two waves per SIMD), and
+// large subdomains on 256 threads with sweep chunks of at most kMfPairCh
+// columns (64 KB of LDS at 4096 cells), so two subdomains share a CU and one's
+// latency-bound phases (sweeps, the small Rayleigh-Ritz algebra) overlap the
+// other's block passes.
+constexpr int kMfPairCh = 4;
+
 template <int MB>
-static size_t lobpcg_mf_lds(int maxn, int bs) {
+static size_t lobpcg_mf_lds(int maxn, int bs, int ch) {
   constexpr int LD = 3 * MB;
   const size_t small =
       (5 * LD * LD + kKP * kKP + 4 * LD + (size_t)(bs / 128) * 1024) * sizeof(double);
-  const size_t prec = (size_t)maxn * lobpcg_mf_ch(MB) * sizeof(float);
+  const size_t prec = (size_t)maxn * ch * sizeof(float);
   return std::max(small, prec);
 }
 
-template <int MB, int BS>
+template <int MB, int BS, int CH = lobpcg_mf_ch(MB), int WPE = 1>
 static void launch_mf_bs(const Geo& g, int nblocks, hipStream_t st, const LobArgs& a, int maxn) {
-  const size_t sh = lobpcg_mf_lds<MB>(maxn, BS);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lobpcg_mf<MB, BS>),
+  const size_t sh = lobpcg_mf_lds<MB>(maxn, BS, CH);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lobpcg_mf<MB, BS, CH, WPE>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  hipLaunchKernelGGL((k_lobpcg_mf<MB, BS>), dim3(nblocks), dim3(BS), sh, st, g, a);
+  hipLaunchKernelGGL((k_lobpcg_mf<MB, BS, CH, WPE>), dim3(nblocks), dim3(BS), sh, st, g, a);
 }
 
 template <int MB>
-static void launch_mf(const Geo& g, int nblocks, hipStream_t st, const LobArgs& a, int maxn) {
-  if (maxn >= kLobBigN)
-    launch_mf_bs<MB, 512>(g, nblocks, st, a, maxn);
-  else if (maxn <= kLobSmallN)
-    launch_mf_bs<MB, 128>(g, nblocks, st, a, maxn);
-  else
-    launch_mf_bs<MB, 256>(g, nblocks, st, a, maxn);
+static void launch_mf(const Geo& g, int nblocks, hipStream_t st, const LobArgs& a, int maxn,
+                      int shape) {
+  if (maxn >= kLobBigN) {
+    if (shape == 1)
+      launch_mf_bs<MB, 256, (MB < kMfPairCh ? MB : kMfPairCh), 2>(g, nblocks, st, a, maxn);
+    else
+      launch_mf_bs<MB, 512>(g, nblocks, st, a, maxn);
+  } else if (maxn <= kLobSmallN) {
+    if (shape == 1)
+      launch_mf_bs<MB, 128, lobpcg_mf_ch(MB), 2>(g, nblocks, st, a, maxn);
+    else
+      launch_mf_bs<MB, 128>(g, nblocks, st, a, maxn);
+  } else {
+    if (shape == 1)
+      launch_mf_bs<MB, 256, lobpcg_mf_ch(MB), 2>(g, nblocks, st, a, maxn);
+    else
+      launch_mf_bs<MB, 256>(g, nblocks, st, a, maxn);
+  }
 }
 
-void launch_lobpcg_mf(int MB, const Geo& g, int nblocks, hipStream_t st, const LobArgs& a,
-                      int maxn) {
+void launch_lobpcg_mf(int MB, int shape, const Geo& g, int nblocks, hipStream_t st,
+                      const LobArgs& a, int maxn) {
   switch (MB) {
     case 4:
-      launch_mf<4>(g, nblocks, st, a, maxn);
+      launch_mf<4>(g, nblocks, st, a, maxn, shape);
       break;
     case 6:
-      launch_mf<6>(g, nblocks, st, a, maxn);
+      launch_mf<6>(g, nblocks, st, a, maxn, shape);
       break;
     case 8:
-      launch_mf<8>(g, nblocks, st, a, maxn);
+      launch_mf<8>(g, nblocks, st, a, maxn, shape);
       break;
     default:
-      launch_mf<10>(g, nblocks, st, a, maxn);
+      launch_mf<10>(g, nblocks, st, a, maxn, shape);
       break;
   }
 }

@@ -208,17 +208,20 @@ def test_16cube_subdomains_lobpcg512_and_quad_kernels(brick16, mb, precision):
 
 @pytest.mark.parametrize("mb", [8, 10], ids=["MB8", "MB10"])
 def test_16cube_lobpcg_kernels_agree(brick16, mb):
-    """The matrix-core LOBPCG (-eig_kernel 1, the default) and the VALU one
-    (-eig_kernel 0) on the 512-thread 16^3 subdomains: both within 1e-8 of the
-    oracle's eigenvalues, the same PCG iteration count, and pressures within
-    the 1e-7 the eigenvector tolerance allows on this medium."""
+    """The matrix-core LOBPCG (-eig_kernel 1, the default) in both workgroup
+    shapes (one 512-thread workgroup per CU; two 256-thread ones with 4-column
+    sweep chunks, -eig_shape 1) and the VALU one (-eig_kernel 0) on the 16^3
+    subdomains: all within 1e-8 of the oracle's eigenvalues, the same PCG
+    iteration count, and pressures within the 1e-7 the eigenvector tolerance
+    allows on this medium."""
     prob, orc, x_ref, its_ref = brick16
     guard = mb - (prob.en_lv1 - 1)
     out = {}
-    for kern in (1, 0):
-        print(f"16^3 bricks MB {mb} eig_kernel {kern}", flush=True)
+    for kern, shape in ((1, 0), (0, 0), (1, 1)):
+        print(f"16^3 bricks MB {mb} eig_kernel {kern} eig_shape {shape}", flush=True)
         res = runner.run(prob, precision=64, keep_ctx=True,
-                         extra=f"-eig_guard {guard} -eig_autotune 0 -eig_kernel {kern}")
+                         extra=f"-eig_guard {guard} -eig_autotune 0 -eig_kernel {kern} "
+                               f"-eig_shape {shape}")
         try:
             _, ev = P.PC_get_basis(res["ctx"], 2)
         finally:
@@ -230,13 +233,16 @@ def test_16cube_lobpcg_kernels_agree(brick16, mb):
         print(f"  its {res['its']}, eig rel err {worst:.2e}, lobpcg its max "
               f"{st['eig_iters_max']} mean {st['eig_iters_mean']:.1f}", flush=True)
         assert st["eig_block"] == mb
+        assert st["eig_shape"] == shape
         assert worst <= 1e-8
         assert res["reason"] == "CONVERGED_RTOL"
-        out[kern] = (ev, res["x"], res["its"])
-    assert abs(out[0][2] - out[1][2]) <= 1
-    assert abs(out[1][2] - its_ref) <= 1
-    assert _rel(out[1][1], out[0][1]) <= 1e-7
-    assert np.allclose(out[1][0][:, 1:], out[0][0][:, 1:], rtol=1e-8)
+        out[(kern, shape)] = (ev, res["x"], res["its"])
+    ref = out[(1, 0)]
+    assert abs(ref[2] - its_ref) <= 1
+    for key in ((0, 0), (1, 1)):
+        assert abs(out[key][2] - ref[2]) <= 1
+        assert _rel(out[key][1], ref[1]) <= 1e-7
+        assert np.allclose(out[key][0][:, 1:], ref[0][:, 1:], rtol=1e-8)
 
 
 # ---------------------------------------------------------------------------
