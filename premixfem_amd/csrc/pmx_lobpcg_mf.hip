@@ -98,13 +98,88 @@ __device__ __forceinline__ void wg_sum(const double (&v)[NQ], int cnt, double* s
   __syncthreads();
 }
 
+// The block columns live in global memory.  The helpers below are separate
+// functions that receive them through ColSet / OutSet references, so the
+// compiler cannot infer their address space and would emit flat accesses,
+// which count against the LDS counter too: a wait for an LDS read then also
+// waits for every global load in flight, and a tile loaded ahead is waited
+// for before the current one is used.  Casting to the global address space
+// gives global loads and stores, tracked by the vector-memory counter alone.
+typedef const double __attribute__((address_space(1))) gcdouble;
+typedef double __attribute__((address_space(1))) gdouble;
+__device__ __forceinline__ const gcdouble* gptr(const double* p) {
+  return (const gcdouble*)p;
+}
+__device__ __forceinline__ gdouble* gptr(double* p) { return (gdouble*)p; }
+typedef double gd2 __attribute__((ext_vector_type(2)));
+typedef const gd2 __attribute__((address_space(1))) gcd2;
+// two consecutive doubles at a 16-byte aligned global address
+__device__ __forceinline__ gd2 ld2(const gcdouble* p) { return *reinterpret_cast<const gcd2*>(p); }
+
+// Steps [s0, s1) of wg_gram on the 16-byte path (all 16 rows present):
+// lane group g loads rows r0 + 2g, r0 + 2g + 1, r0 + 8 + 2g, r0 + 9 + 2g of
+// its columns as two 16-byte pairs, the left operand weighted when WGT; the
+// next step goes into the other buffer before this one is multiplied.
+template <int NTL, int NTR, bool WGT, class Mul>
+__device__ __forceinline__ void gram_steps(const gcdouble* const (&lp)[NTL],
+                                           const gcdouble* const (&rp)[NTR], const gcdouble* wp,
+                                           int g, int s0, int s1, Mul& mul) {
+  auto load = [&](int st, double (&av)[NTL][4], double (&bv)[NTR][4]) {
+    const int ra = (st << 4) + 2 * g, rb = ra + 8;
+    gd2 ma = {1.0, 1.0}, mb = {1.0, 1.0};
+    if (WGT) {
+      ma = ld2(wp + ra);
+      mb = ld2(wp + rb);
+    }
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) {
+      const gd2 u = ld2(lp[t] + ra), z = ld2(lp[t] + rb);
+      av[t][0] = WGT ? u.x * ma.x : u.x;
+      av[t][1] = WGT ? u.y * ma.y : u.y;
+      av[t][2] = WGT ? z.x * mb.x : z.x;
+      av[t][3] = WGT ? z.y * mb.y : z.y;
+    }
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+      const gd2 u = ld2(rp[t] + ra), z = ld2(rp[t] + rb);
+      bv[t][0] = u.x;
+      bv[t][1] = u.y;
+      bv[t][2] = z.x;
+      bv[t][3] = z.y;
+    }
+  };
+  if ((s1 - s0) & 1) {   // an odd step first, on its own
+    double a[NTL][4], b[NTR][4];
+    load(s0, a, b);
+    mul(a, b);
+    ++s0;
+  }
+  if (s0 >= s1) return;
+  // Pairs of steps with both buffers in flight: each refill is issued right
+  // after its buffer is multiplied, clamped into the range (the last refills
+  // reload steps already used, and are not multiplied), so every wait covers
+  // exactly the buffer about to be used.
+  double a0[NTL][4], b0[NTR][4], a1[NTL][4], b1[NTR][4];
+  load(s0, a0, b0);
+  load(s0 + 1, a1, b1);
+  for (int st = s0; st < s1; st += 2) {
+    mul(a0, b0);
+    load(min(st + 2, s1 - 2), a0, b0);
+    mul(a1, b1);
+    load(min(st + 3, s1 - 1), a1, b1);
+  }
+}
+
 // G[i * ldg + j] = sum_r L_i[r] wgt[r] R_j[r] for i < nl, j < nr
 // (nl <= 16 NTL, nr <= 16 NTR; wgt may be null).  Each wave takes a
 // contiguous range of 16-row steps; within a step lane group g = lane / 16
 // feeds rows 8 (i / 2) + 2 g + i % 2 to MFMA i (i = 0..3), so every load is
 // a 16-byte pair of rows (vec: n and the subdomain offset even, which makes
-// every column and the weights 16-byte aligned).  The waves' partial Grams
-// are added in a fixed tree order through red ((waves / 2) * NTL * NTR * 256
+// every column and the weights 16-byte aligned).  The next step's rows are
+// loaded while the current step is multiplied.  Lanes beyond the set's
+// columns read its first column (finite data) and their entries are dropped;
+// rows beyond n are read clamped and zeroed.  The waves' partial Grams are
+// added in a fixed tree order through red ((waves / 2) * NTL * NTR * 256
 // doubles), so the result does not depend on timing.
 template <int NTL, int NTR>
 __device__ void wg_gram(const ColSet& L, int nl, const ColSet& R, int nr,
@@ -112,74 +187,25 @@ __device__ void wg_gram(const ColSet& L, int nl, const ColSet& R, int nr,
                         int ldg) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int cl = lane & 15, g = lane >> 4;
-  const double* lp[NTL];
-  const double* rp[NTR];
+  const gcdouble* lp[NTL];
+  const gcdouble* rp[NTR];
 #pragma unroll
-  for (int t = 0; t < NTL; ++t) lp[t] = cl + 16 * t < nl ? col_of(L, cl + 16 * t, n) : nullptr;
+  for (int t = 0; t < NTL; ++t) {
+    const int c = cl + 16 * t;
+    lp[t] = gptr(col_of(L, c < nl ? c : 0, n));
+  }
 #pragma unroll
-  for (int t = 0; t < NTR; ++t) rp[t] = cl + 16 * t < nr ? col_of(R, cl + 16 * t, n) : nullptr;
+  for (int t = 0; t < NTR; ++t) {
+    const int c = cl + 16 * t;
+    rp[t] = gptr(col_of(R, c < nr ? c : 0, n));
+  }
+  const gcdouble* wp = wgt ? gptr(wgt) : nullptr;
   mfma_d4 acc[NTL][NTR];
 #pragma unroll
   for (int tl = 0; tl < NTL; ++tl)
 #pragma unroll
     for (int tr = 0; tr < NTR; ++tr) acc[tl][tr] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-  const int steps = (n + 15) >> 4, per = (steps + nwv - 1) / nwv;
-  const int s0 = wv * per, s1 = min(steps, s0 + per);
-  for (int st = s0; st < s1; ++st) {
-    const int r0 = st << 4;
-    double av[NTL][4], bv[NTR][4], m[4];
-    if (vec && r0 + 16 <= n) {
-      const int ra = r0 + 2 * g, rb = r0 + 8 + 2 * g;
-#pragma unroll
-      for (int t = 0; t < NTL; ++t) {
-        double2 u = {0.0, 0.0}, z = {0.0, 0.0};
-        if (lp[t]) {
-          u = *reinterpret_cast<const double2*>(lp[t] + ra);
-          z = *reinterpret_cast<const double2*>(lp[t] + rb);
-        }
-        av[t][0] = u.x;
-        av[t][1] = u.y;
-        av[t][2] = z.x;
-        av[t][3] = z.y;
-      }
-#pragma unroll
-      for (int t = 0; t < NTR; ++t) {
-        double2 u = {0.0, 0.0}, z = {0.0, 0.0};
-        if (rp[t]) {
-          u = *reinterpret_cast<const double2*>(rp[t] + ra);
-          z = *reinterpret_cast<const double2*>(rp[t] + rb);
-        }
-        bv[t][0] = u.x;
-        bv[t][1] = u.y;
-        bv[t][2] = z.x;
-        bv[t][3] = z.y;
-      }
-      if (wgt) {
-        const double2 u = *reinterpret_cast<const double2*>(wgt + ra);
-        const double2 z = *reinterpret_cast<const double2*>(wgt + rb);
-        m[0] = u.x;
-        m[1] = u.y;
-        m[2] = z.x;
-        m[3] = z.y;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = r0 + 8 * (i >> 1) + 2 * g + (i & 1);
-        const bool ok = r < n;
-#pragma unroll
-        for (int t = 0; t < NTL; ++t) av[t][i] = (lp[t] && ok) ? lp[t][r] : 0.0;
-#pragma unroll
-        for (int t = 0; t < NTR; ++t) bv[t][i] = (rp[t] && ok) ? rp[t][r] : 0.0;
-        m[i] = (wgt && ok) ? wgt[r] : 1.0;
-      }
-    }
-    if (wgt) {
-#pragma unroll
-      for (int t = 0; t < NTL; ++t)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) av[t][i] *= m[i];
-    }
+  auto mul = [&](const double (&av)[NTL][4], const double (&bv)[NTR][4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -188,6 +214,42 @@ __device__ void wg_gram(const ColSet& L, int nl, const ColSet& R, int nr,
         for (int tr = 0; tr < NTR; ++tr)
           acc[tl][tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[tl][i], bv[tr][i], acc[tl][tr], 0,
                                                              0, 0);
+  };
+  // Full 16-row steps on the 16-byte path: branch-free loads of the next
+  // step into the other of two buffers while this one is multiplied.
+  const int full = vec ? n >> 4 : 0;
+  if (full > 0) {
+    const int per = (full + nwv - 1) / nwv;
+    const int s0 = wv * per, s1 = min(full, s0 + per);
+    if (wp)
+      gram_steps<NTL, NTR, true>(lp, rp, wp, g, s0, s1, mul);
+    else
+      gram_steps<NTL, NTR, false>(lp, rp, wp, g, s0, s1, mul);
+  }
+  // Remaining rows (every row when not on the 16-byte path): row by row,
+  // clamped and zeroed beyond n, one step per wave in turn.
+  const int steps = (n + 15) >> 4;
+  for (int st = full + wv; st < steps; st += nwv) {
+    const int r0 = st << 4;
+    double av[NTL][4], bv[NTR][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + 8 * (i >> 1) + 2 * g + (i & 1);
+      const bool ok = r < n;
+      const int rc = ok ? r : n - 1;
+      const double m = wp ? wp[rc] : 1.0;
+#pragma unroll
+      for (int t = 0; t < NTL; ++t) {
+        const double x = lp[t][rc];
+        av[t][i] = ok ? x * m : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) {
+        const double x = rp[t][rc];
+        bv[t][i] = ok ? x : 0.0;
+      }
+    }
+    mul(av, bv);
   }
   constexpr int NE = NTL * NTR * 4;
   for (int h = nwv >> 1; h >= 1; h >>= 1) {
@@ -229,53 +291,84 @@ __device__ void wg_gram(const ColSet& L, int nl, const ColSet& R, int nr,
 // Out(:, c) = sum_k In(:, k) CF[k * kKP + c] for c < nout (<= 16 NTO),
 // k < nin (<= 32), CF in LDS.  Computed as D^T = CF^T In^T: one 16-row tile
 // per step, lane l % 16 a row and l / 16 a column group, so loads and stores
-// are 128-byte runs of a column.  Out may share columns with In: every row
-// tile is read whole by the wave that owns it before that wave writes it.
+// are 128-byte runs of a column.  The coefficients are read into registers
+// once; the next tile is loaded before this tile is multiplied and stored.
+// Lanes beyond the input columns read its first column (finite data) against
+// a zero coefficient; rows beyond n are read clamped and not stored.  Out may
+// share columns with In: every row tile is read whole by the wave that owns
+// it before that wave writes it.
 template <int NTO>
 __device__ void wg_update(const ColSet& In, int nin, const double* CF, const OutSet& Out,
                           int nout, int n) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   const int cl = lane & 15, g = lane >> 4;
-  const double* ip[8];
+  const gcdouble* ip[8];
+  double af[8][NTO];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) ip[q] = 4 * q + g < nin ? col_of(In, 4 * q + g, n) : nullptr;
+  for (int q = 0; q < 8; ++q) {
+    const int k = 4 * q + g;
+    ip[q] = gptr(col_of(In, k < nin ? k : 0, n));
+#pragma unroll
+    for (int t = 0; t < NTO; ++t) {
+      const int c = 16 * t + cl;
+      af[q][t] = (k < nin && c < nout) ? CF[k * kKP + c] : 0.0;
+    }
+  }
+  gdouble* op[NTO][4];
+#pragma unroll
+  for (int t = 0; t < NTO; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = 16 * t + g + 4 * v;
+      op[t][v] = c < nout ? gptr(col_of(Out, c, n)) : nullptr;
+    }
   const int tiles = (n + 15) >> 4, per = (tiles + nwv - 1) / nwv;
   const int t0 = wv * per, t1 = min(tiles, t0 + per);
-  double bv[8];
   auto load = [&](int tt, double (&dst)[8]) {
-    const int row = (tt << 4) + cl;
-    const bool ok = row < n;
+    const int row = min((tt << 4) + cl, n - 1);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dst[q] = (ip[q] && ok) ? ip[q][row] : 0.0;
+    for (int q = 0; q < 8; ++q) dst[q] = ip[q][row];
   };
-  if (t0 < t1) load(t0, bv);
-  for (int tt = t0; tt < t1; ++tt) {
-    // the next tile's loads are issued before this tile's stores (other rows)
-    double nb[8];
-    if (tt + 1 < t1) load(tt + 1, nb);
+  auto tile = [&](int tt, const double (&bv)[8]) {
     mfma_d4 acc[NTO];
 #pragma unroll
     for (int t = 0; t < NTO; ++t) acc[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 8; ++q)
 #pragma unroll
-      for (int t = 0; t < NTO; ++t) {
-        const int k = 4 * q + g, c = 16 * t + cl;
-        const double af = (k < nin && c < nout) ? CF[k * kKP + c] : 0.0;
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bv[q], acc[t], 0, 0, 0);
-      }
+      for (int t = 0; t < NTO; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q][t], bv[q], acc[t], 0, 0, 0);
     const int row = (tt << 4) + cl;
     if (row < n) {
 #pragma unroll
       for (int t = 0; t < NTO; ++t)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int c = 16 * t + g + 4 * v;
-          if (c < nout) col_of(Out, c, n)[row] = acc[t][v];
-        }
+        for (int v = 0; v < 4; ++v)
+          if (op[t][v]) op[t][v][row] = acc[t][v];
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) bv[q] = nb[q];
+  };
+  int ta = t0;
+  if ((t1 - ta) & 1) {   // an odd tile first, on its own
+    double b[8];
+    load(ta, b);
+    tile(ta, b);
+    ++ta;
+  }
+  if (ta < t1) {
+    // Pairs of tiles with both buffers in flight: each refill is issued right
+    // after its buffer's tile is stored, clamped into the range (the last
+    // refills reload tiles already done, and are not used), so every wait
+    // covers exactly the buffer about to be multiplied.  A refill reads rows
+    // of a later tile than the ones stored before it.
+    double b0[8], b1[8];
+    load(ta, b0);
+    load(ta + 1, b1);
+    for (int tt = ta; tt < t1; tt += 2) {
+      tile(tt, b0);
+      load(min(tt + 2, t1 - 2), b0);
+      tile(tt + 1, b1);
+      load(min(tt + 3, t1 - 1), b1);
+    }
   }
   __syncthreads();
 }
