@@ -71,8 +71,8 @@ __device__ inline void wg_matmul(const double* A, const double* B, double* C, in
 // coupling is significant relative to the diagonal (threshold Jacobi, which
 // keeps small eigenvalues relatively accurate); a sweep without rotations
 // ends the iteration.
-__device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
-                                double* rot, int m, int ld, int max_sweeps = 30) {
+__device__ inline int wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
+                               double* rot, int m, int ld, int max_sweeps = 30) {
   __shared__ int s_rot[2];
   double* cc = rot;
   double* ss = rot + ld;
@@ -82,18 +82,26 @@ __device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, 
   if (threadIdx.x == 0) s_rot[0] = s_rot[1] = 0;
   __syncthreads();
   const int mp = (m + 1) & ~1;
+  const int i0 = threadIdx.x / m, j0 = threadIdx.x % m;   // this thread's first element
+  const int di = blockDim.x / m, dj = blockDim.x % m;     // and the step to its next one
   double *Ac = A0, *An = A1, *Qc = Q0, *Qn = Q1;
+  int sweeps = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     const int fl = sweep & 1;
+    sweeps = sweep + 1;
     for (int r = 0; r < mp - 1; ++r) {
       for (int k = threadIdx.x; k < mp / 2; k += blockDim.x) {
+        // round-robin pairing (r + k, r - k) mod (mp - 1), with mp - 1 paired
+        // to r; r, k < mp - 1, so one conditional add or subtract wraps
         int p, q;
         if (k == 0) {
           p = r;
           q = mp - 1;
         } else {
-          p = (r + k) % (mp - 1);
-          q = (r - k + (mp - 1)) % (mp - 1);
+          p = r + k;
+          if (p >= mp - 1) p -= mp - 1;
+          q = r - k;
+          if (q < 0) q += mp - 1;
         }
         if (p > q) {
           const int t = p;
@@ -102,14 +110,14 @@ __device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, 
         }
         if (q < m) {
           const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
-          double c = 1.0, s = 0.0;
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq))) {
-            s_rot[fl] = 1;
-            const double th = (aqq - app) / (2.0 * apq);
-            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
-          }
+          // the threshold test and the rotation are computed side by side
+          // (no branch between them); apq = 0 takes the identity
+          const bool big = fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq));
+          const double th = (aqq - app) / (2.0 * (apq != 0.0 ? apq : 1.0));
+          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          const double cr = 1.0 / sqrt(t * t + 1.0);
+          const double c = big ? cr : 1.0, s = big ? t * cr : 0.0;
+          if (big) s_rot[fl] = 1;
           cc[p] = c;
           cc[q] = c;
           ss[p] = -s;
@@ -125,8 +133,8 @@ __device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, 
       __syncthreads();
       // every reader of the previous sweep's flag is past this point
       if (r == 0 && threadIdx.x == 0) s_rot[fl ^ 1] = 0;
-      for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int i = e / m, j = e % m;
+      // element e = i m + j, stepped by the block size without a division
+      for (int e = threadIdx.x, i = i0, j = j0; e < m * m; e += blockDim.x) {
         const int ip = part[i], jp = part[j];
         const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
         double x;
@@ -137,6 +145,12 @@ __device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, 
               si * sj * Ac[ip * ld + jp];
         An[i * ld + j] = x;
         Qn[i * ld + j] = cj * Qc[i * ld + j] + sj * Qc[i * ld + jp];
+        i += di;
+        j += dj;
+        if (j >= m) {
+          j -= m;
+          ++i;
+        }
       }
       __syncthreads();
       double* t = Ac;
@@ -175,6 +189,7 @@ __device__ inline void wg_syev2(double* A0, double* A1, double* Q0, double* Q1, 
   for (int e = threadIdx.x; e < m * m; e += blockDim.x)
     Q0[(e / m) * ld + e % m] = An[(e / m) * ld + e % m];
   __syncthreads();
+  return sweeps;
 }
 
 }  // namespace pmx

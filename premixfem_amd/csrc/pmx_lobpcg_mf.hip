@@ -563,9 +563,12 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   int it = 0, status = 0, np = 0;
   // phase timers of the traced subdomain (thread 0), kept in LDS
   __shared__ unsigned long long tph[16];
+  __shared__ int tsw;   // Rayleigh-Ritz Jacobi sweeps of the traced subdomain
   const bool trace = sidx == a.debug && threadIdx.x == 0;
-  if (trace)
+  if (trace) {
     for (int i = 0; i < 16; ++i) tph[i] = 0;
+    tsw = 0;
+  }
 #define PMX_PHASE(i)                                                   \
   if (trace) {                                                         \
     const unsigned long long tn = __builtin_amdgcn_s_memrealtime();    \
@@ -817,7 +820,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         __syncthreads();
         wg_symmetrize(Z, ns, LD);
         PMX_PHASE(11)
-        wg_syev2(Z, TM, Q, GA, w, rot, ns, LD);  // eigenvectors in Q
+        {
+          const int sw = wg_syev2(Z, TM, Q, GA, w, rot, ns, LD);  // eigenvectors in Q
+          if (trace) tsw += sw;
+        }
         PMX_PHASE(12)
         for (int e = threadIdx.x; e < ns * MB; e += BS) {
           const int i = e / MB, c = e % MB;
@@ -873,10 +879,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   if (trace) {
     printf("lobpcg-mf phases (us) its %d: resid %.0f sweeps %.0f deflate %.0f W,AW %.0f "
            "conv %.0f | ortho gram1 %.0f upd1 %.0f gram2 %.0f svqb %.0f upd2 %.0f | gram %.0f | "
-           "rr chol %.0f syev %.0f P %.0f | update %.0f\n",
+           "rr chol %.0f syev %.0f (%d sweeps) P %.0f | update %.0f\n",
            it, tph[0] * 0.01, tph[1] * 0.01, tph[2] * 0.01, tph[3] * 0.01, tph[4] * 0.01,
            tph[5] * 0.01, tph[6] * 0.01, tph[7] * 0.01, tph[8] * 0.01, tph[9] * 0.01,
-           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tph[13] * 0.01, tph[14] * 0.01);
+           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tsw, tph[13] * 0.01, tph[14] * 0.01);
   }
 #undef PMX_PHASE
   // ---- outputs ----
