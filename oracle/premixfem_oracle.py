@@ -587,8 +587,6 @@ class Oracle:
         if eg_op not in (EIG_OP_MOD, EIG_OP_MEAN, EIG_OP_DIAG):
             raise ValueError("eg_op (:1690)")
         self.lay = Layout(dom, mesh, sd, procs)
-        if self.lay.nbox > 1 and (use_2level or lv2_icc):
-            raise ValueError("process boxes: the 2-level and ICC(0) variants are not covered")
         self.kappa = np.asarray(kappa, dtype=np.float64)
         self.en_lv1, self.en_lv2, self.eg_op = int(en_lv1), int(en_lv2), int(eg_op)
         self.pin_cell = pin_cell
@@ -726,10 +724,14 @@ class Oracle:
                                  shape=(M * N * P, nc))
         A_c = self.energy_gram(self.Phi).tocsr()
         self.dof_box = np.repeat([lay.sub_box(s) for s in range(lay.nsub)], self.eigen_num_lv1)
-        if lay.nbox > 1:
+        if lay.nbox > 1 and not self.use_2level:
             # _PC_setup_lv2_J (:806-888) couples only the subdomains of one
             # process: the blocks between boxes are dropped, the faces
-            # between boxes stay on the diagonal blocks.
+            # between boxes stay on the diagonal blocks.  So does the spe10
+            # fork's ICC(0) level 2 (spe10.c:1074-1205, one SBAIJ matrix per
+            # process).  The 2-level variant's A_c keeps every coupling
+            # (_PC_setup_lv2_c, :423-735: the neighbours' dof offsets are
+            # exchanged, :448-479, and one matrix spans all processes).
             A_c = A_c.tocoo()
             same = self.dof_box[A_c.row] == self.dof_box[A_c.col]
             A_c = sp.csr_matrix((A_c.data[same], (A_c.row[same], A_c.col[same])), shape=A_c.shape)

@@ -493,12 +493,17 @@ static int setup_coarse(pmx_ctx* c) {
   // level-2 factor: the dense inverse, or the block ICC(0) of the spe10 fork.
   PMX_TRY(c->Acb.alloc((size_t)c->nsub * 7 * c->en_lv1 * c->en_lv1));
   for (DBuf<double>* b : {&c->c_a, &c->c_b, &c->c_c, &c->c_d, &c->c_e}) PMX_TRY(b->alloc(c->nc));
-  c->use_band = !c->lv2_icc && !c->use_2level && c->nbox == 1 &&
+  c->use_band = !c->lv2_icc && !c->use_2level &&
                  (c->coarse_band == 1 || (c->coarse_band < 0 && c->nc > kBandAutoNc));
+  if (c->nbox > 1 && c->use_band && !c->coarse_planes) {
+    set_error("-procs_virtual: the band Cholesky coarse factor is not built for process boxes");
+    return PMX_ERR_SUP;
+  }
   // With process boxes A_c keeps only the couplings inside a box, and every
   // box block is nonsingular through its faces to the other boxes: no
-  // near-null direction to deflate.
-  if (c->nbox > 1) c->coarse_deflate = 0;
+  // near-null direction to deflate.  (The 2-level variant's A_c keeps every
+  // coupling, as on one process.)
+  if (c->nbox > 1 && !c->use_2level) c->coarse_deflate = 0;
   if (c->lv2_icc || c->use_band) {
     for (DBuf<double>* b : {&c->Ac, &c->Ainv, &c->Fop}) b->release();
     PMX_TRY(launch_galerkin(c));
@@ -560,8 +565,8 @@ static int setup_coarse(pmx_ctx* c) {
   double t1 = now();
   c->t_stages[PMX_STAGE_SU_LV2_2] += t1 - t0;
   if (!c->use_2level) {
-    PMX_TRY((c->lv2_icc || c->use_band) ? level3_setup_icc(c)
-                                        : (c->nbox > 1 ? level3_setup_boxes(c) : level3_setup(c)));
+    PMX_TRY(c->nbox > 1 ? level3_setup_boxes(c)
+                        : ((c->lv2_icc || c->use_band) ? level3_setup_icc(c) : level3_setup(c)));
     double t2 = now();
     c->t_stages[PMX_STAGE_SU_LV3_1] += t2 - t1;
   }
@@ -787,9 +792,9 @@ int pmx_init(pmx_ctx** out, const double* dom, const int* mesh, const char* opti
     c->sd[a] *= c->pv[a];
   }
   c->nbox = c->pv[0] * c->pv[1] * c->pv[2];
-  if (c->nbox > 1 && (c->use_2level || c->lv2_icc || c->coarse_band == 1)) {
-    set_error("-procs_virtual: the 2-level, ICC(0) level-2 and band variants are not built for "
-              "process boxes");
+  if (c->nbox > 1 && c->coarse_band == 1 && !c->coarse_planes) {
+    set_error("-procs_virtual: the band Cholesky coarse factor (-coarse_planes 0) is not built "
+              "for process boxes; the plane factor is");
     return fail(PMX_ERR_SUP);
   }
   if (c->en_lv1 < 1) {
@@ -1664,7 +1669,8 @@ int pmx_get_coarse_matrix(const pmx_ctx* c, double* out) {
                       hipMemcpyDeviceToHost));
     return PMX_SUCCESS;
   }
-  // -lv2_icc keeps block rows only: expand them.
+  // -lv2_icc and the plane factor keep block rows only: expand them (with
+  // process boxes, the level-2 matrix keeps only the blocks inside a box).
   const int k = c->en_lv1;
   std::vector<double> blk(c->Acb.n);
   PMX_HIP(hipMemcpy(blk.data(), c->Acb.p, blk.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1675,6 +1681,7 @@ int pmx_get_coarse_matrix(const pmx_ctx* c, double* out) {
       const int o = slot == 0 ? s : b.nb[slot - 1];
       if (o < 0) continue;
       const Brick& ob = c->h_bricks[o];
+      if (ob.box != b.box) continue;
       const double* B = blk.data() + ((size_t)s * 7 + slot) * k * k;
       for (int i = 0; i < b.neig; ++i)
         for (int j = 0; j < ob.neig; ++j) out[(size_t)(b.dof + i) * c->nc + ob.dof + j] = B[i * k + j];

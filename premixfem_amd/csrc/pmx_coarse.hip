@@ -29,7 +29,9 @@ namespace pmx {
 // residual updates between stages stand for.  inner = 1 builds the box-local
 // Neumann matrix A_i_inner of level 3 (:939-1073) into Ac instead: faces
 // between boxes are dropped from the diagonal blocks too, and Acb is not
-// written (pass nullptr).
+// written (pass nullptr).  inner = 2 keeps every coupling in Ac as well: the
+// 2-level variant's coarse matrix, assembled over all processes with the
+// neighbours' dof offsets (_PC_setup_lv2_c, :423-735).
 __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
                            const double* __restrict__ Tx, const double* __restrict__ Ty,
                            const double* __restrict__ Tz, int nc, double* __restrict__ Ac,
@@ -66,7 +68,7 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         const int lext = (d >> 1) == 0 ? b.nx : ((d >> 1) == 1 ? b.ny : b.nz);
         const bool inside = (d & 1) ? (lcoord < lext - 1) : (lcoord > 0);
         if (!inside) {
-          if (!inner || g.bricks[b.nb[d]].box == b.box) sdiag += t;
+          if (inner != 1 || g.bricks[b.nb[d]].box == b.box) sdiag += t;
         } else if ((d & 1) == 0) {
           // Interior face with the lower neighbour: T (phi_c - phi_nb)(...)^T
           // from the differences of the basis values, as the reference
@@ -160,7 +162,7 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
         for (int j = 0; j < o.neig; ++j) {
           const double v = block_sum(acc[a][j], red);
           if (threadIdx.x == 0) {
-            if (Ac && o.box == b.box) {
+            if (Ac && (o.box == b.box || inner == 2)) {
               Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
               Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
             }
@@ -181,7 +183,7 @@ int launch_galerkin(pmx_ctx* ctx) {
   PMX_HIP(hipMemsetAsync(ctx->Acb.p, 0, sizeof(double) * ctx->Acb.n, ctx->stream));
   hipLaunchKernelGGL(k_galerkin, dim3(ctx->nsub), dim3(kBlock), 0, ctx->stream, ctx->geo,
                      ctx->phi.p, ctx->en_lv1, ctx->T[0].p, ctx->T[1].p, ctx->T[2].p, ctx->nc,
-                     ctx->Ac.p, ctx->Acb.p, 0);
+                     ctx->Ac.p, ctx->Acb.p, ctx->use_2level ? 2 : 0);
   PMX_HIP(hipGetLastError());
   return PMX_SUCCESS;
 }
@@ -613,7 +615,10 @@ __global__ void __launch_bounds__(64 * kBiccWaves) k_bicc_factor(
         for (int e = lane; e < k * k; e += 64) D[e] = blk[(e / k) * kmax + e % k];
       __syncthreads();
       for (int dir = 0; dir < 6; dir += 2) {
-        const int t = act ? b.nb[dir] : -1;
+        // process boxes: the ICC(0) of each box's block (no coupling between
+        // boxes, spe10.c:1074-1205 assembles A_c per process)
+        int t = act ? b.nb[dir] : -1;
+        if (t >= 0 && g.bricks[t].box != b.box) t = -1;
         const int kt = t >= 0 ? g.bricks[t].neig : 0;
         const double* B = blk + (size_t)(dir + 1) * kk;  // A_st, columns over t's dofs
         const double* Dt = dinv + (size_t)(t >= 0 ? t : 0) * kk;
@@ -964,7 +969,10 @@ int bicc_setup(pmx_ctx* ctx) {
     sl.neig = b.neig;
     sl.pad = 0;
     for (int a = 0; a < 3; ++a) {
-      const int lo = b.nb[2 * a], hi = b.nb[2 * a + 1];
+      // process boxes: neighbours in another box do not couple
+      int lo = b.nb[2 * a], hi = b.nb[2 * a + 1];
+      if (lo >= 0 && ctx->h_bricks[lo].box != b.box) lo = -1;
+      if (hi >= 0 && ctx->h_bricks[hi].box != b.box) hi = -1;
       sl.lpos[a] = lo >= 0 ? pos[lo] : -1;
       sl.lk[a] = lo >= 0 ? ctx->h_bricks[lo].neig : 0;
       sl.hpos[a] = hi >= 0 ? pos[hi] : -1;

@@ -15,6 +15,10 @@
 // spread over the whole device with one workgroup per subdomain of the
 // plane, instead of one small launch per 64-wide tile column of a band
 // factor.  The C blocks are read from the block rows Acb (slots -z / +z).
+// With process boxes the factored matrix keeps only the couplings inside a
+// box (_PC_setup_lv2_J, src/PreMixFEM_3D.c:806-888): blocks between
+// subdomains of two boxes are skipped here, while Acb keeps them for the
+// coarse residual updates.
 // At 256^3 cells with 16^3 subdomains of 16^3 cells and k = 8: 16 planes of
 // n_p = 2048, G_z 33.5 MB each (0.54 GB), 31 GEMVs per solve.
 #include <cmath>
@@ -108,7 +112,9 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
     outv[threadIdx.x] = v;
   }
   __syncthreads();
-  const int nbi = b.nb[fwd ? 5 : 4];
+  // process boxes: A_c keeps no coupling between subdomains of two boxes
+  int nbi = b.nb[fwd ? 5 : 4];
+  if (nbi >= 0 && g.bricks[nbi].box != b.box) nbi = -1;
   const Brick o = nbi >= 0 ? g.bricks[nbi] : b;
   const int slot_c = fwd ? 5 : 6;  // the block of o that couples it with s
   const double* blk = nbi >= 0 ? Acb + ((size_t)nbi * 7 + slot_c) * kmax * kmax : nullptr;
@@ -137,6 +143,7 @@ __global__ void k_pl_gather(Geo g, const double* __restrict__ Acb, int kmax, int
     const int o = slot == 0 ? s : b.nb[slot - 1];
     if (o < 0) continue;
     const Brick ob = g.bricks[o];
+    if (ob.box != b.box) continue;   // process boxes: no coupling between boxes
     if (i >= b.neig || j >= ob.neig) continue;
     S[(size_t)(b.dof - dof0 + i) * np + (ob.dof - dof0 + j)] =
         Acb[(size_t)s * 7 * kk + slot * kk + i * kmax + j];
@@ -153,6 +160,12 @@ __global__ void k_pl_cg(Geo g, const double* __restrict__ Acb, int kmax, int sub
   const int sd = b.nb[4];
   if (sd < 0) return;
   const Brick bd = g.bricks[sd];
+  if (bd.box != b.box) {
+    // process boxes: no coupling with the lower neighbour, a zero row block
+    for (int col = threadIdx.x; col < npp; col += blockDim.x)
+      for (int i = 0; i < b.neig; ++i) H[(size_t)(b.dof - dof0 + i) * npp + col] = 0.0;
+    return;
+  }
   const double* blk = Acb + ((size_t)s * 7 + 5) * kmax * kmax;
   for (int col = threadIdx.x; col < npp; col += blockDim.x) {
     double gv[kMaxEig];
@@ -174,6 +187,7 @@ __global__ void k_pl_schur(Geo g, const double* __restrict__ Acb, int kmax, int 
   const int sd = b.nb[4];
   if (sd < 0) return;
   const Brick bd = g.bricks[sd];
+  if (bd.box != b.box) return;   // process boxes: no coupling between boxes
   const double* blk = Acb + ((size_t)sp * 7 + 5) * kmax * kmax;
   for (int i = blockIdx.y * blockDim.x + threadIdx.x; i < np; i += gridDim.y * blockDim.x) {
     double hv[kMaxEig];

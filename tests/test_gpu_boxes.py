@@ -82,6 +82,76 @@ def test_boxes_match_oracle(name, mesh, k1, k2, procs, kw, extra):
     assert err <= 1e-8
 
 
+VARIANTS = [
+    # name, mesh, en_lv1, en_lv2, procs, oracle kwargs, extra options
+    ("2x1x2-2level", (13, 10, 9), 3, 3, (2, 1, 2), dict(use_2level=True), "-use_2level"),
+    ("2x2x1-2level-noshift", (11, 13, 7), 3, 2, (2, 2, 1),
+     dict(use_2level=True, no_shift_A_cc=True), "-use_2level -no_shift_A_cc"),
+    ("2x1x2-icc", (13, 10, 9), 3, 3, (2, 1, 2), dict(lv2_icc=True), "-lv2_icc"),
+    ("2x2x2-icc-noshift", (16, 16, 16), 4, 4, (2, 2, 2), dict(lv2_icc=True, no_shift_A_cc=True),
+     "-lv2_icc -no_shift_A_cc"),
+    ("2x1x2-planes", (13, 10, 9), 3, 3, (2, 1, 2), {}, "-coarse_band 1"),
+    ("2x2x2-planes-W", (16, 16, 16), 4, 4, (2, 2, 2), dict(use_W_cycle=True),
+     "-coarse_band 1 -use_W_cycle"),
+]
+
+
+@pytest.mark.parametrize("name,mesh,k1,k2,procs,kw,extra", VARIANTS,
+                         ids=[c[0] for c in VARIANTS])
+def test_box_variants_match_oracle(name, mesh, k1, k2, procs, kw, extra):
+    """Process boxes with the 2-level variant (one coarse matrix over all
+    processes with the couplings between them, _PC_setup_lv2_c :423-735),
+    the spe10 fork's ICC(0) level 2 (one SBAIJ factor per process,
+    spe10.c:1074-1205) and the exact coarse solve by z-planes of the
+    box-local A_c: y = B x within 1e-10, level-3 eigenvalues within 1e-9
+    (none for 2-level), the 2-level A_c within 1e-12, PCG iterations within
+    1 and the pressure within 1e-8 of the oracle (given its basis)."""
+    prob = _prob(mesh, k1, k2, 17)
+    opts = _opts(procs, extra)
+    print("boxes", name, mesh, opts, flush=True)
+    orc = _oracle(prob, procs, kw)
+    x_ref, its_ref, _, reason_ref = O.pcg(orc.matvec, prob.rhs.ravel(), orc.apply, rtol=prob.rtol)
+    phi, num, ev = O.basis_columns(orc)
+    res = runner.run(prob, basis=(phi, num, ev), extra=opts, keep_ctx=True)
+    ctx, shell = res["ctx"], res["shell"]
+    two_level = kw.get("use_2level", False)
+    try:
+        Ac = P.PC_get_coarse_matrix(ctx)
+        ev3 = None if two_level else P.PC_get_basis(ctx, 3)[1]
+        x = np.random.default_rng(3).standard_normal(prob.ncell)
+        y = P.PC_apply_vec(shell, x)
+    finally:
+        P.PC_final(ctx)
+    y_ref = orc.apply(x)
+    e_apply = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    err = np.linalg.norm(res["x"] - x_ref) / np.linalg.norm(x_ref)
+    e_ac = np.abs(Ac - orc.A_c).max() / np.abs(orc.A_c).max()
+    msg = (f"{name}: A_c {e_ac:.1e}, apply {e_apply:.1e}, its {res['its']} vs {its_ref}, "
+           f"x {err:.1e}")
+    assert e_ac <= 1e-12
+    if not two_level:
+        lam_ref = np.concatenate([lam[:n] for lam, n in zip(orc.eig_lv2_box,
+                                                            orc.eigen_num_lv2_box)])
+        e_lam = np.abs(ev3 - lam_ref).max() / np.abs(lam_ref).max()
+        msg += f", lv3 eig {e_lam:.1e}"
+        assert len(ev3) == orc.eigen_num_lv2
+        assert e_lam <= 1e-9
+    print(msg, flush=True)
+    assert e_apply <= 1e-10
+    assert res["reason"] == "CONVERGED_RTOL" and reason_ref == "CONVERGED_RTOL"
+    assert abs(res["its"] - its_ref) <= 1
+    assert err <= 1e-8
+
+
+def test_box_band_factor_refused():
+    """The band Cholesky coarse factor has no process-box form: refused with
+    PETSC_ERR_SUP, the plane factor is the exact solve offered instead."""
+    prob = _prob((13, 10, 9), 3, 3, 17)
+    with pytest.raises(P.PMXError) as e:
+        runner.run(prob, extra=_opts((2, 1, 2), "-coarse_band 1 -coarse_planes 0"))
+    assert e.value.code == 56
+
+
 @pytest.mark.parametrize("precision", [64, 32])
 def test_boxes_full_gpu_setup(precision):
     """The whole setup on the GPU (its own LOBPCG basis) against the oracle's

@@ -168,3 +168,69 @@ def test_one_box_is_the_one_process_path():
                  en_lv2=3, pin_cell=(MESH[0] - 1, 0, 0)).setup()
     assert np.array_equal(a.A_c, b.A_c)
     assert np.array_equal(a.V2, b.V2)
+
+
+def test_two_level_keeps_every_coupling():
+    """The 2-level variant on P processes (_PC_setup_lv2_c, :423-735) builds
+    one coarse matrix over all processes, the neighbours' dof offsets
+    exchanged (:448-479): A_c = Phi^T A Phi with the couplings between
+    boxes, plus the NIL shift (:690-692), against a dense product."""
+    o = _orc(use_2level=True)
+    Phi = o.Phi.toarray()
+    full = Phi.T @ o.A.toarray() @ Phi
+    assert np.allclose(o.A_c, full, rtol=1e-12, atol=1e-14 * np.abs(full).max())
+    db = o.dof_box
+    d0, d1 = np.flatnonzero(db == 0), np.flatnonzero(db == 1)
+    assert np.abs(o.A_c[np.ix_(d0, d1)]).max() > 1e-3 * np.abs(full).max()
+    # the apply is the exact coarse solve between two level-1 stages
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal(o.A.shape[0])
+    y1 = o.apply_lv1(r)
+    t = Phi.T @ (r - o.matvec(y1))
+    y2 = y1 + Phi @ np.linalg.solve(full + O.NIL * np.eye(full.shape[0]), t)
+    y = y2 + o.apply_lv1(r - o.matvec(y2))
+    assert np.allclose(o.apply(r), y, rtol=1e-9, atol=1e-12 * np.abs(y).max())
+
+
+def test_block_icc_on_boxes_is_the_boxes_own_ic0():
+    """The spe10 fork's level 2 on P processes is one SBAIJ matrix per process
+    (spe10.c:1074-1205) with PETSc ICC(0): per box, the textbook scalar
+    IC(0) of that box's block of A_c on its block pattern (from the box
+    alone), and nothing between boxes."""
+    import math
+
+    o = _orc(lv2_icc=True)
+    A = o.A_c_sp.toarray()
+    d = o.dof_idx
+    lay = o.lay
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(A.shape[0])
+    y = o.bicc.solve(r)
+    for b in range(lay.nbox):
+        subs = [s for s in range(lay.nsub) if lay.sub_box(s) == b]
+        idx = np.concatenate([np.arange(d[s], d[s + 1]) for s in subs])
+        pos = {s: i for i, s in enumerate(subs)}
+        # block pattern: self and face neighbours inside the box
+        sdx, sdy, _ = lay.sd
+        pat = np.zeros((idx.size, idx.size), dtype=bool)
+        off = np.concatenate([[0], np.cumsum([d[s + 1] - d[s] for s in subs])])
+        for s in subs:
+            for t in (s, s - 1, s + 1, s - sdx, s + sdx, s - sdx * sdy, s + sdx * sdy):
+                if t in pos and (abs(t - s) != 1 or t // sdx == s // sdx) and \
+                        (abs(t - s) != sdx or t // (sdx * sdy) == s // (sdx * sdy)):
+                    i, j = pos[s], pos[t]
+                    pat[off[i]:off[i + 1], off[j]:off[j + 1]] = True
+        a = A[np.ix_(idx, idx)]
+        n = idx.size
+        L = np.zeros((n, n))
+        for i in range(n):
+            for j in range(i + 1):
+                if pat[i, j]:
+                    s = a[i, j] - L[i, :j] @ L[j, :j]
+                    L[i, j] = math.sqrt(s) if i == j else s / L[j, j]
+        np.testing.assert_allclose(L @ (L.T @ y[idx]), r[idx], rtol=1e-9,
+                                   atol=1e-9 * np.abs(r).max())
+    # the preconditioner symmetric and positive on the boxes
+    u, v = rng.standard_normal((2, o.A.shape[0]))
+    assert abs(u @ o.apply(v) - v @ o.apply(u)) <= 1e-10 * abs(u @ o.apply(v))
+    assert u @ o.apply(u) > 0
