@@ -2020,10 +2020,15 @@ int level3_setup(pmx_ctx* ctx) {
       }
       // Rounding floor: the near-null direction of the pinned coarse matrix
       // is amplified ~1/lambda_min by the inverse, which bounds the
-      // attainable shift-invert residual.  Accept a stagnated residual below
-      // 1e-4 (no 2x improvement over the last 10 iterations).
+      // attainable shift-invert residual (7e-8 on the 128^3 1e5-contrast
+      // medium, lambda_min(A_c) ~ 6e-12).  A residual that stagnates (no 2x
+      // improvement over the last 10 iterations) within -eig_stag x
+      // -eig_tol (100 x 1e-8 by default, the bound of the subdomain
+      // eigensolver and of the sparse coarse path) is accepted and reported
+      // in lv3_floor; above it the iteration runs on to its cap and fails
+      // as SLEPc's nconv < nev does (:1098).
       hist.push_back(mres);
-      if (!conv && it >= 20 && mres <= 1e-4) {
+      if (!conv && it >= 20 && mres <= ctx->eig_stag * ctx->eig_tol) {
         double best_old = 1e300;
         for (size_t q = 0; q + 10 < hist.size(); ++q) best_old = std::min(best_old, hist[q]);
         if (mres > 0.5 * best_old) {

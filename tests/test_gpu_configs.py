@@ -110,7 +110,19 @@ def test_c2_fp64_against_oracle_fixture():
     f = _fixture("c2")
     _check_field(prob, f)
     print("c2 fp64", flush=True)
-    res = runner.run(prob, precision=64)
+    res = runner.run(prob, precision=64, keep_ctx=True)
+    try:
+        _, ev3 = P.PC_get_basis(res["ctx"], 3)
+    finally:
+        P.PC_final(res["ctx"])
+    st = res["stats"]
+    # Level 3: the fixture's eigenvalues of A_c (the first is the pinned
+    # near-null one, ~5.7e-12), reached within the accepted rounding floor.
+    e_lv3 = np.abs(ev3 - f["eig_lv2"]) / f["eig_lv2"]
+    print(f"c2 level 3: eigenvalues rel err {e_lv3}, {st['lv3_iters']} its, floor "
+          f"{st['lv3_floor']:.2e}", flush=True)
+    assert np.all(e_lv3 <= 1e-5)
+    assert st["lv3_floor"] <= 100 * 1e-8
     cells = f["cells"]
     ex = _rel(res["x"][cells], f["x_sample"])
     lay = O.Layout(prob.dom, prob.mesh, prob.sd)
