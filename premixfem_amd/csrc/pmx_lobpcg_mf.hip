@@ -288,6 +288,167 @@ __device__ void wg_gram(const ColSet& L, int nl, const ColSet& R, int nr,
   __syncthreads();
 }
 
+// GB = S^T diag(wgt) S and GA = S^T (A S) in one pass over S and A S
+// (ns <= 16 NT columns each; GB[i * ldg + j], GA[i * ldg + j]): every row of
+// S is loaded once and serves as the left operand of both products and,
+// weighted, as the right operand of the first.  Same step layout,
+// pipelining and fixed-order wave reduction as wg_gram (the two Grams
+// reduced one after the other through red).  HAS_A = false computes GB
+// alone.
+template <int NT, bool HAS_A = true>
+__device__ void wg_gram_ma(const ColSet& S, const ColSet& AS, int ns,
+                           const double* __restrict__ wgt, int n, bool vec, double* red,
+                           double* GB, double* GA, int ldg) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  const int cl = lane & 15, g = lane >> 4;
+  const gcdouble* sp[NT];
+  const gcdouble* ap[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int c = cl + 16 * t < ns ? cl + 16 * t : 0;
+    sp[t] = gptr(col_of(S, c, n));
+    ap[t] = HAS_A ? gptr(col_of(AS, c, n)) : sp[t];
+  }
+  const gcdouble* wp = gptr(wgt);
+  mfma_d4 accB[NT][NT], accA[NT][NT];
+#pragma unroll
+  for (int tl = 0; tl < NT; ++tl)
+#pragma unroll
+    for (int tr = 0; tr < NT; ++tr) {
+      accB[tl][tr] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+      accA[tl][tr] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    }
+  // sv: rows of S, av: rows of A S, m: weights of the four rows
+  auto mul = [&](const double (&sv)[NT][4], const double (&av)[NT][4], const double (&m)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double sw[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) sw[t] = sv[t][i] * m[i];
+#pragma unroll
+      for (int tl = 0; tl < NT; ++tl)
+#pragma unroll
+        for (int tr = 0; tr < NT; ++tr) {
+          accB[tl][tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[tl][i], sw[tr], accB[tl][tr], 0,
+                                                              0, 0);
+          if (HAS_A)
+            accA[tl][tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(sv[tl][i], av[tr][i],
+                                                                accA[tl][tr], 0, 0, 0);
+        }
+    }
+  };
+  const int full = vec ? n >> 4 : 0;
+  if (full > 0) {
+    const int per = (full + nwv - 1) / nwv;
+    int s0 = wv * per;
+    const int s1 = min(full, s0 + per);
+    auto load = [&](int st, double (&sv)[NT][4], double (&av)[NT][4], double (&m)[4]) {
+      const int ra = (st << 4) + 2 * g, rb = ra + 8;
+      const gd2 ma = ld2(wp + ra), mb = ld2(wp + rb);
+      m[0] = ma.x;
+      m[1] = ma.y;
+      m[2] = mb.x;
+      m[3] = mb.y;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const gd2 u = ld2(sp[t] + ra), z = ld2(sp[t] + rb);
+        sv[t][0] = u.x;
+        sv[t][1] = u.y;
+        sv[t][2] = z.x;
+        sv[t][3] = z.y;
+        if (HAS_A) {
+          const gd2 x = ld2(ap[t] + ra), y = ld2(ap[t] + rb);
+          av[t][0] = x.x;
+          av[t][1] = x.y;
+          av[t][2] = y.x;
+          av[t][3] = y.y;
+        }
+      }
+    };
+    if (s0 < s1 && ((s1 - s0) & 1)) {   // an odd step first, on its own
+      double sv[NT][4], av[NT][4], m[4];
+      load(s0, sv, av, m);
+      mul(sv, av, m);
+      ++s0;
+    }
+    if (s0 < s1) {
+      // pairs of steps with both buffers in flight, refills clamped into the
+      // range (the last ones reload used steps and are not multiplied)
+      double s0v[NT][4], a0v[NT][4], m0[4], s1v[NT][4], a1v[NT][4], m1[4];
+      load(s0, s0v, a0v, m0);
+      load(s0 + 1, s1v, a1v, m1);
+      for (int st = s0; st < s1; st += 2) {
+        mul(s0v, a0v, m0);
+        load(min(st + 2, s1 - 2), s0v, a0v, m0);
+        mul(s1v, a1v, m1);
+        load(min(st + 3, s1 - 1), s1v, a1v, m1);
+      }
+    }
+  }
+  // remaining rows (every row off the 16-byte path), clamped and zeroed
+  const int steps = (n + 15) >> 4;
+  for (int st = full + wv; st < steps; st += nwv) {
+    const int r0 = st << 4;
+    double sv[NT][4], av[NT][4], m[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + 8 * (i >> 1) + 2 * g + (i & 1);
+      const bool ok = r < n;
+      const int rc = ok ? r : n - 1;
+      m[i] = ok ? wp[rc] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double x = sp[t][rc];
+        sv[t][i] = ok ? x : 0.0;
+        if (HAS_A) {
+          const double y = ap[t][rc];
+          av[t][i] = ok ? y : 0.0;
+        }
+      }
+    }
+    mul(sv, av, m);
+  }
+  constexpr int NE = NT * NT * 4;
+  auto reduce_store = [&](mfma_d4 (&acc)[NT][NT], double* G) {
+    for (int h = nwv >> 1; h >= 1; h >>= 1) {
+      if (wv >= h && wv < 2 * h) {
+        double* dst = red + (size_t)(wv - h) * NE * 64;
+#pragma unroll
+        for (int tl = 0; tl < NT; ++tl)
+#pragma unroll
+          for (int tr = 0; tr < NT; ++tr)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dst[((tl * NT + tr) * 4 + q) * 64 + lane] = acc[tl][tr][q];
+      }
+      __syncthreads();
+      if (wv < h) {
+        const double* src = red + (size_t)wv * NE * 64;
+#pragma unroll
+        for (int tl = 0; tl < NT; ++tl)
+#pragma unroll
+          for (int tr = 0; tr < NT; ++tr)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[tl][tr][q] += src[((tl * NT + tr) * 4 + q) * 64 + lane];
+      }
+      __syncthreads();
+    }
+    if (wv == 0) {
+#pragma unroll
+      for (int tl = 0; tl < NT; ++tl)
+#pragma unroll
+        for (int tr = 0; tr < NT; ++tr)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int i = 16 * tl + g + 4 * q, j = 16 * tr + cl;
+            if (i < ns && j < ns) G[i * ldg + j] = acc[tl][tr][q];
+          }
+    }
+    __syncthreads();
+  };
+  reduce_store(accB, GB);
+  if (HAS_A) reduce_store(accA, GA);
+}
+
 // Out(:, c) = sum_k In(:, k) CF[k * kKP + c] for c < nout (<= 16 NTO),
 // k < nin (<= 32), CF in LDS.  Computed as D^T = CF^T In^T: one 16-row tile
 // per step, lane l % 16 a row and l / 16 a column group, so loads and stores
@@ -789,13 +950,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         // ---- G: S^T M S and S^T A S, S = [X W P] ----
         const int ns = MB + nw + np;
         const ColSet S = colset(X, MB, W, nw, P, np), AS = colset(AX, MB, AW, nw, AP, np);
-        if (ns > 16) {
-          wg_gram<2, 2>(S, ns, S, ns, mass, n, vec, red, GB, LD);
-          wg_gram<2, 2>(S, ns, AS, ns, nullptr, n, vec, red, GA, LD);
-        } else {
-          wg_gram<1, 1>(S, ns, S, ns, mass, n, vec, red, GB, LD);
-          wg_gram<1, 1>(S, ns, AS, ns, nullptr, n, vec, red, GA, LD);
-        }
+        if (ns > 16)
+          wg_gram_ma<2>(S, AS, ns, mass, n, vec, red, GB, GA, LD);
+        else
+          wg_gram_ma<1>(S, AS, ns, mass, n, vec, red, GB, GA, LD);
         wg_symmetrize(GA, ns, LD);
         wg_symmetrize(GB, ns, LD);
         PMX_PHASE(10)
