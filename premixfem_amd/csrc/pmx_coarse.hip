@@ -189,50 +189,55 @@ int launch_galerkin(pmx_ctx* ctx) {
 }
 
 // ---------------------------------------------------------------------------
-// Dense SPD inversion by the block sweep operator: for each 128-wide pivot
+// Dense SPD inversion by the block sweep operator: for each SW-wide pivot
 // block k with D = A_kk^{-1}:  A_ij -= A_ik D A_kj (i,j != k),
 // A_ik <- A_ik D, A_kj <- D A_kj, A_kk <- -D.  After all blocks A = -A^{-1}.
 // Pivots are Schur complements of an SPD matrix, so no pivoting is needed.
+// The pivot width trades the one-workgroup pivot inversions (SW steps of
+// one barrier each, every thread reading a row segment of SW / 4 values per
+// step) against the passes of the trailing update over the matrix (n / SW):
+// 64 up to n = kSwWideN, 128 above.
 // ---------------------------------------------------------------------------
-constexpr int kSw = 128;   // pivot block
-constexpr int kSwT = 64;   // trailing-update tile (pivot blocks are whole tiles)
+constexpr int kSwT = 64;          // trailing-update tile (pivot blocks are whole tiles)
+constexpr int kSwWideN = 4096;    // 128-wide pivot blocks above this size
 
-// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to 128):
-// 512 threads, thread t holds row t/4, columns 32 (t%4) .. +31 in registers;
+// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to SW):
+// 4 SW threads, thread t holds row t/4, columns (SW/4)(t%4) .. in registers;
 // one barrier per elimination step (the row / column broadcast buffers
 // alternate).  The unsymmetrised inverse goes through Dt (global scratch) to
 // be symmetrised.
-constexpr int kPivThreads = 512;
-__global__ void __launch_bounds__(kPivThreads) k_sweep_pivot(const double* __restrict__ A, int n,
-                                                             int k0, int bs,
-                                                             double* __restrict__ D,
-                                                             double* __restrict__ Dt) {
-  __shared__ double2 rowv2[2][kSw / 2];
-  __shared__ double colv[2][kSw];
+template <int SW>
+__global__ void __launch_bounds__(4 * SW) k_sweep_pivot(const double* __restrict__ A, int n,
+                                                        int k0, int bs,
+                                                        double* __restrict__ D,
+                                                        double* __restrict__ Dt) {
+  constexpr int CW = SW / 4;   // columns per thread
+  __shared__ double2 rowv2[2][SW / 2];
+  __shared__ double colv[2][SW];
   const int r = threadIdx.x >> 2, q = threadIdx.x & 3;
-  double m[32];
+  double m[CW];
 #pragma unroll
-  for (int c = 0; c < 32; ++c) {
-    const int j = q * 32 + c;
+  for (int c = 0; c < CW; ++c) {
+    const int j = q * CW + c;
     m[c] = (r < bs && j < bs) ? A[(size_t)(k0 + r) * n + k0 + j] : (r == j ? 1.0 : 0.0);
   }
   for (int tq = 0; tq < 4; ++tq) {
 #pragma unroll
-    for (int tc = 0; tc < 32; ++tc) {
-      const int t = tq * 32 + tc, par = t & 1;
+    for (int tc = 0; tc < CW; ++tc) {
+      const int t = tq * CW + tc, par = t & 1;
       if (q == tq) colv[par][r] = m[tc];
       if (r == t) {
 #pragma unroll
-        for (int c = 0; c < 32; c += 2) rowv2[par][q * 16 + c / 2] = double2{m[c], m[c + 1]};
+        for (int c = 0; c < CW; c += 2) rowv2[par][q * (CW / 2) + c / 2] = double2{m[c], m[c + 1]};
       }
       __syncthreads();
       const double* rowv = reinterpret_cast<const double*>(rowv2[par]);
       const double piv = 1.0 / rowv[t];
       const double f = colv[par][r];
 #pragma unroll
-      for (int c = 0; c < 32; c += 2) {
-        const double2 rv = rowv2[par][q * 16 + c / 2];
-        const int j = q * 32 + c;
+      for (int c = 0; c < CW; c += 2) {
+        const double2 rv = rowv2[par][q * (CW / 2) + c / 2];
+        const int j = q * CW + c;
         const double rt0 = (j == t ? 1.0 : rv.x) * piv;
         const double rt1 = (j + 1 == t ? 1.0 : rv.y) * piv;
         m[c] = r == t ? rt0 : (j == t ? 0.0 : m[c]) - f * rt0;
@@ -241,49 +246,51 @@ __global__ void __launch_bounds__(kPivThreads) k_sweep_pivot(const double* __res
     }
   }
 #pragma unroll
-  for (int c = 0; c < 32; ++c) Dt[r * kSw + q * 32 + c] = m[c];
+  for (int c = 0; c < CW; ++c) Dt[r * SW + q * CW + c] = m[c];
   __syncthreads();
-  for (int e = threadIdx.x; e < kSw * kSw; e += blockDim.x) {
-    const int i = e / kSw, j = e % kSw;
+  for (int e = threadIdx.x; e < SW * SW; e += blockDim.x) {
+    const int i = e / SW, j = e % SW;
     // Symmetrise the pivot inverse.
-    D[e] = (i < bs && j < bs) ? 0.5 * (Dt[i * kSw + j] + Dt[j * kSw + i]) : 0.0;
+    D[e] = (i < bs && j < bs) ? 0.5 * (Dt[i * SW + j] + Dt[j * SW + i]) : 0.0;
   }
 }
 
 // P[r][t] = A[r][k0+t] for t < bs, 0 beyond.
+template <int SW>
 __global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int bs,
                               double* __restrict__ P) {
-  const size_t tot = (size_t)n * kSw;
+  const size_t tot = (size_t)n * SW;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
        e += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)(e / kSw), t = (int)(e % kSw);
+    const int r = (int)(e / SW), t = (int)(e % SW);
     P[e] = t < bs ? A[(size_t)r * n + k0 + t] : 0.0;
   }
 }
 
 // Trailing update A_ij -= U_i P_j^T on 64x64 tiles (FP64 MFMA, K = bs); the
 // tiles of the pivot block rows / columns take U, U^T and -D.
+template <int SW>
 __global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, int n, int k0,
                                                       int bs, const double* __restrict__ P,
                                                       const double* __restrict__ U,
                                                       const double* __restrict__ D) {
   using TL = MfmaTile<2, 2, 2, 2>;
-  static_assert(TL::TM == kSwT && TL::TN == kSwT && kSw % kSwT == 0,
+  static_assert(TL::TM == kSwT && TL::TN == kSwT && SW % kSwT == 0,
                 "pivot blocks are whole tiles");
   __shared__ double lds[TL::LDS_DOUBLES];
   const int I0 = blockIdx.y * kSwT, J0 = blockIdx.x * kSwT;
-  const bool prow = I0 >= k0 && I0 < k0 + kSw, pcol = J0 >= k0 && J0 < k0 + kSw;
+  const bool prow = I0 >= k0 && I0 < k0 + SW, pcol = J0 >= k0 && J0 < k0 + SW;
   if (prow || pcol) {
     for (int e = threadIdx.x; e < kSwT * kSwT; e += blockDim.x) {
       const int i = I0 + e / kSwT, j = J0 + e % kSwT;
       if (i >= n || j >= n) continue;
       double v;
       if (prow && pcol)
-        v = -D[(i - k0) * kSw + (j - k0)];
+        v = -D[(i - k0) * SW + (j - k0)];
       else if (prow)
-        v = U[(size_t)j * kSw + (i - k0)];
+        v = U[(size_t)j * SW + (i - k0)];
       else
-        v = U[(size_t)i * kSw + (j - k0)];
+        v = U[(size_t)i * SW + (j - k0)];
       A[(size_t)i * n + j] = v;
     }
     return;
@@ -294,7 +301,7 @@ __global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, in
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mfma_d4{0.0, 0.0, 0.0, 0.0};
   // op(A) = U (rows I0..), op(B) = P^T (P row j = column j of op(B)).
-  mfma_tile_acc<2, 2, 2, 2, true>(U, kSw, P, kSw, n, n, bs, I0, J0, lds, acc);
+  mfma_tile_acc<2, 2, 2, 2, true>(U, SW, P, SW, n, n, bs, I0, J0, lds, acc);
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -323,27 +330,30 @@ int coarse_invert(pmx_ctx* ctx, const double* A, double* Ainv, double shift) {
   return spd_invert(ctx->stream, ctx->nc, A, Ainv, shift);
 }
 
-int spd_invert(hipStream_t stream, int n, const double* A, double* Ainv, double shift) {
+template <int SW>
+static int spd_invert_sw(hipStream_t stream, int n, const double* A, double* Ainv,
+                         double shift) {
   DBuf<double> D, Dt, P, U;
-  PMX_TRY(D.alloc(kSw * kSw));
-  PMX_TRY(Dt.alloc(kSw * kSw));
-  PMX_TRY(P.alloc((size_t)n * kSw));
-  PMX_TRY(U.alloc((size_t)n * kSw));
+  PMX_TRY(D.alloc(SW * SW));
+  PMX_TRY(Dt.alloc(SW * SW));
+  PMX_TRY(P.alloc((size_t)n * SW));
+  PMX_TRY(U.alloc((size_t)n * SW));
   const int nblk = 1024;
   hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, stream, A, Ainv, n, shift,
                      1.0);
   PMX_HIP(hipGetLastError());
   const int nt = (n + kSwT - 1) / kSwT;
-  const int npb = std::min((n * kSw + 255) / 256, 2048);
-  for (int k0 = 0; k0 < n; k0 += kSw) {
-    const int bs = std::min(kSw, n - k0);
-    hipLaunchKernelGGL(k_sweep_pivot, dim3(1), dim3(kPivThreads), 0, stream, Ainv, n, k0,
-                       bs, D.p, Dt.p);
-    hipLaunchKernelGGL(k_sweep_panel, dim3(npb), dim3(256), 0, stream, Ainv, n, k0, bs, P.p);
-    // U = P D (the pivot columns times the pivot inverse)
+  const int npb = std::min((n * SW + 255) / 256, 2048);
+  for (int k0 = 0; k0 < n; k0 += SW) {
+    const int bs = std::min(SW, n - k0);
+    hipLaunchKernelGGL(k_sweep_pivot<SW>, dim3(1), dim3(4 * SW), 0, stream, Ainv, n, k0, bs,
+                       D.p, Dt.p);
+    hipLaunchKernelGGL(k_sweep_panel<SW>, dim3(npb), dim3(256), 0, stream, Ainv, n, k0, bs,
+                       P.p);
+    // U = P D (the pivot columns times the pivot inverse; one 128-column tile)
     hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 4, false>), dim3(1, (n + 63) / 64), dim3(256), 0,
-                       stream, n, kSw, kSw, 1.0, P.p, kSw, D.p, kSw, 0.0, U.p, kSw);
-    hipLaunchKernelGGL(k_sweep_update, dim3(nt, nt), dim3(256), 0, stream, Ainv, n, k0, bs,
+                       stream, n, SW, SW, 1.0, P.p, SW, D.p, SW, 0.0, U.p, SW);
+    hipLaunchKernelGGL(k_sweep_update<SW>, dim3(nt, nt), dim3(256), 0, stream, Ainv, n, k0, bs,
                        P.p, U.p, D.p);
     PMX_HIP(hipGetLastError());
   }
@@ -352,6 +362,11 @@ int spd_invert(hipStream_t stream, int n, const double* A, double* Ainv, double 
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipStreamSynchronize(stream));
   return PMX_SUCCESS;
+}
+
+int spd_invert(hipStream_t stream, int n, const double* A, double* Ainv, double shift) {
+  return n <= kSwWideN ? spd_invert_sw<64>(stream, n, A, Ainv, shift)
+                       : spd_invert_sw<128>(stream, n, A, Ainv, shift);
 }
 
 // ---------------------------------------------------------------------------
