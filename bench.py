@@ -162,11 +162,13 @@ def run_mode(prob, precision, steps, warmup, extra="", net=None):
 
 def measured_traffic(workload, mode, kernel):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (FP32 preconditioner),
-    calibrated against a copy kernel; None when no summary covers it."""
-    if workload != "c2" or mode == "fp64":
-        return None
-    path = os.path.join(ROOT, "profiles", "c2_fp32_solve_hbm_traffic.txt")
+    FETCH_SIZE / WRITE_SIZE summary of this workload and mode
+    (profiles/<workload>_<mode>_hbm_traffic.txt, or the 128^3 FP32 solve
+    summary), calibrated against a copy kernel; None when no summary covers
+    it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{mode}_hbm_traffic.txt")
+    if not os.path.exists(path) and workload == "c2" and mode == "fp32":
+        path = os.path.join(ROOT, "profiles", "c2_fp32_solve_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
