@@ -39,10 +39,13 @@ constexpr int kPlMaxCols = 32;    // right-hand sides of one solve (level-3 set-
 // The workgroup is wide (NT = 1024 for one right-hand side: at n_p = 2048 every
 // thread issues all of its row loads at once), so one workgroup per CU keeps
 // enough loads in flight to stream G_z near HBM speed with no exchange between
-// workgroups.  Several right-hand sides share every load of G_z.  Every
+// workgroups.  Several right-hand sides share every load of G_z: NCG per
+// thread, and NGRP groups of threads reading the same G rows in step (the
+// level-3 set-up's 16-column block solves stream G_z once, not four times).
+// Every
 // subdomain of plane z has at most one neighbour in the next plane and no two
 // share it, so the in-place updates of w do not collide.
-template <int KR, int NCG, int NT>
+template <int KR, int NCG, int NT, int NGRP = 1>
 __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict__ Acb, int kmax,
                                                 const double* __restrict__ G, int np, int dof0,
                                                 int sub0, int fwd, double* __restrict__ w,
@@ -50,13 +53,20 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
                                                 const double* stop) {
   PMX_STOP_CHECK(stop);
   constexpr int NW = NT / 64;
+  constexpr int TPG = NT / NGRP;           // threads per group of NCG right-hand sides
+  static_assert(TPG % 64 == 0, "groups of whole waves");
+  constexpr int NO = NGRP * NCG * KR;      // outputs of the workgroup
   __shared__ double red[NW][NCG * KR];
-  __shared__ double outv[NCG * KR];
+  __shared__ double outv[NO];
   const int s = sub0 + blockIdx.x;
   const Brick b = g.bricks[s];
   const int nr = b.neig, r0 = b.dof - dof0;
-  const int c0 = blockIdx.y * NCG;
-  const int ncg = min(NCG, ncol - c0);
+  // The NGRP groups take consecutive blocks of NCG right-hand sides and read
+  // the same G rows at the same time, so HBM streams G_z once for all of them.
+  const int grp = threadIdx.x / TPG, tg = threadIdx.x % TPG;
+  const int cb = blockIdx.y * NGRP * NCG;  // first right-hand side of the workgroup
+  const int c0 = cb + grp * NCG;
+  const int ncg = min(NCG, ncol - c0);     // <= 0: an idle group
   double acc[NCG][KR];
 #pragma unroll
   for (int c = 0; c < NCG; ++c)
@@ -65,10 +75,11 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
   const double* Gr = G + (size_t)r0 * np;
   const double* x[NCG];
 #pragma unroll
-  for (int c = 0; c < NCG; ++c) x[c] = w + (size_t)(c0 + min(c, ncg - 1)) * nc + dof0;
+  for (int c = 0; c < NCG; ++c)
+    x[c] = w + (size_t)(ncg > 0 ? c0 + min(c, ncg - 1) : 0) * nc + dof0;
   if (((np | dof0 | nc) & 1) == 0) {  // 16-byte aligned rows and segments
     const int n2 = np >> 1;
-    for (int j = threadIdx.x; j < n2; j += NT) {
+    for (int j = tg; j < n2; j += TPG) {
       double2 xv[NCG];
 #pragma unroll
       for (int c = 0; c < NCG; ++c) xv[c] = reinterpret_cast<const double2*>(x[c])[j];
@@ -82,7 +93,7 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
       }
     }
   } else {
-    for (int j = threadIdx.x; j < np; j += NT) {
+    for (int j = tg; j < np; j += TPG) {
       double xv[NCG];
 #pragma unroll
       for (int c = 0; c < NCG; ++c) xv[c] = x[c][j];
@@ -106,10 +117,13 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
       if (lane == 0) red[wv][c * KR + i] = v;
     }
   __syncthreads();
-  if (threadIdx.x < NCG * KR) {
+  // outv[q * NCG * KR + e]: group q's sum over its waves, in wave order
+  constexpr int WPG = TPG / 64;
+  for (int t = threadIdx.x; t < NO; t += NT) {
+    const int q = t / (NCG * KR), e = t % (NCG * KR);
     double v = 0.0;
-    for (int q = 0; q < NW; ++q) v += red[q][threadIdx.x];
-    outv[threadIdx.x] = v;
+    for (int u = 0; u < WPG; ++u) v += red[q * WPG + u][e];
+    outv[t] = v;
   }
   __syncthreads();
   // process boxes: A_c keeps no coupling between subdomains of two boxes
@@ -118,9 +132,10 @@ __global__ void __launch_bounds__(NT) k_pl_step(Geo g, const double* __restrict_
   const Brick o = nbi >= 0 ? g.bricks[nbi] : b;
   const int slot_c = fwd ? 5 : 6;  // the block of o that couples it with s
   const double* blk = nbi >= 0 ? Acb + ((size_t)nbi * 7 + slot_c) * kmax * kmax : nullptr;
-  for (int t = threadIdx.x; t < ncg * KR; t += NT) {
+  const int nout = min(NGRP * NCG, ncol - cb) * KR;
+  for (int t = threadIdx.x; t < nout; t += NT) {
     const int c = t / KR, i = t % KR;
-    const size_t co = (size_t)(c0 + c) * nc;
+    const size_t co = (size_t)(cb + c) * nc;
     if (!fwd && i < nr) d[co + b.dof + i] = outv[c * KR + i];
     if (nbi >= 0 && i < o.neig) {
       double v = 0.0;
@@ -277,11 +292,12 @@ int planes_setup(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
-template <int KR, int NCG, int NT>
+template <int KR, int NCG, int NT, int NGRP = 1>
 static void pl_step(pmx_ctx* ctx, int z, int fwd, double* w, double* d, int ncol) {
   const int L = ctx->sd[0] * ctx->sd[1];
   const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
-  hipLaunchKernelGGL((k_pl_step<KR, NCG, NT>), dim3(L, (ncol + NCG - 1) / NCG), dim3(NT), 0,
+  constexpr int per = NCG * NGRP;
+  hipLaunchKernelGGL((k_pl_step<KR, NCG, NT, NGRP>), dim3(L, (ncol + per - 1) / per), dim3(NT), 0,
                      ctx->stream, ctx->geo, ctx->Acb.p, ctx->en_lv1,
                      ctx->pl_G.p + ctx->pl_goff[z], np, dof0, z * L, fwd, w, d, ctx->nc, ncol,
                      ctx->geo.stop);
@@ -302,9 +318,13 @@ int launch_planes_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
     if (ncol == 1) {
       if (k8) pl_step<8, 1, 1024>(ctx, z, fwd, w, d, ncol);
       else pl_step<kMaxEig, 1, 1024>(ctx, z, fwd, w, d, ncol);
-    } else {
+    } else if (ncol <= 4) {
       if (k8) pl_step<8, 4, 512>(ctx, z, fwd, w, d, ncol);
       else pl_step<kMaxEig, 2, 512>(ctx, z, fwd, w, d, ncol);
+    } else {
+      // up to 16 right-hand sides per workgroup in 4 groups of 4
+      if (k8) pl_step<8, 4, 1024, 4>(ctx, z, fwd, w, d, ncol);
+      else pl_step<kMaxEig, 2, 1024, 4>(ctx, z, fwd, w, d, ncol);
     }
   };
   for (int z = 0; z + 1 < Z; ++z) step(z, 1);
