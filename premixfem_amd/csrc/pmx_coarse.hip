@@ -18,7 +18,10 @@ namespace pmx {
 // subdomain (and the pin / well term) adds T phi phi^T; off-diagonal block
 // with the upper neighbour s' across a face: -sum_face T phi_s,i phi_s',j.  One workgroup per subdomain, rows in chunks
 // of 4 to bound registers; only the upper triangle of the diagonal block is
-// reduced and mirrored, so A_c is exactly symmetric.
+// reduced and mirrored, so A_c is exactly symmetric.  The 4 x kMaxEig sums
+// of a chunk are reduced together: a register reduce-scatter across the wave
+// leaves lane L with the wave's total of sum L, one LDS exchange adds the
+// waves -- one barrier pair per chunk instead of one per entry.
 // Output: dense nc x nc (row-major) and block rows Acb[s][slot][kmax][kmax]
 // with slot 0 = self, 1..6 = -x,+x,-y,+y,-z,+z.
 // ---------------------------------------------------------------------------
@@ -32,30 +35,86 @@ namespace pmx {
 // written (pass nullptr).  inner = 2 keeps every coupling in Ac as well: the
 // 2-level variant's coarse matrix, assembled over all processes with the
 // neighbours' dof offsets (_PC_setup_lv2_c, :423-735).
-__global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
+// One stage of the wave reduce-scatter: N values per lane -> N / 2, lane
+// bit N / 2 picks the upper or lower half to keep, the partner across that
+// bit supplies its copy of the kept half.
+template <int N>
+__device__ __forceinline__ void rs_stage(double* v, int lane) {
+  constexpr int H = N / 2;
+  const bool hi = (lane & H) != 0;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const double send = hi ? v[k] : v[k + H];
+    const double keep = hi ? v[k + H] : v[k];
+    v[k] = keep + __shfl_xor(send, H, 64);
+  }
+}
+
+// Sums acc[4][kMaxEig] (64 values per thread) over the workgroup; on return
+// threads 0..63 hold the total of entry threadIdx.x = a * kMaxEig + j.
+// red holds (blockDim.x / 64) x 64 doubles.  Ends with a barrier, so red can
+// be reused at once.
+__device__ __forceinline__ double chunk_sum(double (&acc)[4][kMaxEig], double* red) {
+  static_assert(4 * kMaxEig == 64, "one accumulator per lane");
+  double* v = &acc[0][0];
+  const int lane = threadIdx.x & 63;
+  rs_stage<64>(v, lane);
+  rs_stage<32>(v, lane);
+  rs_stage<16>(v, lane);
+  rs_stage<8>(v, lane);
+  rs_stage<4>(v, lane);
+  rs_stage<2>(v, lane);
+  red[threadIdx.x] = v[0];
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x < 64)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += red[w * 64 + threadIdx.x];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kBlock) k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
                            const double* __restrict__ Tx, const double* __restrict__ Ty,
                            const double* __restrict__ Tz, int nc, double* __restrict__ Ac,
                            double* __restrict__ Acb, int inner) {
-  __shared__ double red[8];
+  __shared__ double red[kBlock];
   const int s = blockIdx.x;
   const Brick b = g.bricks[s];
   const int n = b.nx * b.ny * b.nz;
   const int ke = b.neig;
   const double* TT3[3] = {Tx, Ty, Tz};
   double* blk = Acb ? Acb + (size_t)s * 7 * kmax * kmax : nullptr;
-  for (int i0 = 0; i0 < ke; i0 += 4) {
+  // Unrolled so that the row offset i0 is a constant.  The basis rows are
+  // read again for every rank-1 term (L1 hits) rather than held, which keeps
+  // the 64 accumulators and a few values in registers.
+#pragma unroll
+  for (int ic = 0; ic < kMaxEig / 4; ++ic) {
+    const int i0 = 4 * ic;
+    if (i0 >= ke) break;
     double acc[4][kMaxEig];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int j = 0; j < kMaxEig; ++j) acc[a][j] = 0.0;
+    // acc[a][j] += w u_(i0+a) u_j with u = pa - pb (pb may be null).
+    auto rank1 = [&](double w, const double* pa, const double* pb) {
+      double ui[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        ui[a] = i0 + a < ke ? w * (pa[i0 + a] - (pb ? pb[i0 + a] : 0.0)) : 0.0;
+#pragma unroll
+      for (int j = 0; j < kMaxEig; ++j) {
+        if (j < ke) {
+          const double uj = pa[j] - (pb ? pb[j] : 0.0);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a][j] += ui[a] * uj;
+        }
+      }
+    };
     for (int li = threadIdx.x; li < n; li += blockDim.x) {
       const int lx = li % b.nx, ly = (li / b.nx) % b.ny, lz = li / (b.nx * b.ny);
       const int c = b.off + li;
       const double* pc = phi + (size_t)c * kmax;
-      double pcv[kMaxEig];
-#pragma unroll
-      for (int j = 0; j < kMaxEig; ++j) pcv[j] = j < ke ? pc[j] : 0.0;
       // Faces to other subdomains, the pin and the well term act on the
       // diagonal block as d phi phi^T.
       double sdiag = 0.0;
@@ -73,50 +132,29 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
           // Interior face with the lower neighbour: T (phi_c - phi_nb)(...)^T
           // from the differences of the basis values, as the reference
           // assembles it -- no cancellation near the null space.
-          const double* pn = phi + (size_t)nbi * kmax;
-          double del[kMaxEig];
-#pragma unroll
-          for (int j = 0; j < kMaxEig; ++j) del[j] = j < ke ? pcv[j] - pn[j] : 0.0;
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            if (i0 + a >= ke) break;
-            const double di = t * del[i0 + a];
-#pragma unroll
-            for (int j = 0; j < kMaxEig; ++j) acc[a][j] += di * del[j];
-          }
+          rank1(t, pc, phi + (size_t)nbi * kmax);
         }
       }
       if (c == g.pin) sdiag += g.tau;
       if (g.wd) sdiag += g.wd[c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (i0 + a >= ke) break;
-        const double pi = sdiag * pcv[i0 + a];
-#pragma unroll
-        for (int j = 0; j < kMaxEig; ++j) acc[a][j] += pi * pcv[j];
-      }
+      rank1(sdiag, pc, nullptr);
     }
-    for (int a = 0; a < 4; ++a) {
-      const int i = i0 + a;
-      if (i >= ke) break;
-      for (int j = i; j < ke; ++j) {
-        const double v = block_sum(acc[a][j], red);
-        if (threadIdx.x == 0) {
-          if (Ac) {
-            Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
-            Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
-          }
-          if (Acb) {
-            blk[i * kmax + j] = v;
-            blk[j * kmax + i] = v;
-          }
-        }
+    const double v = chunk_sum(acc, red);
+    const int i = i0 + (int)(threadIdx.x / kMaxEig), j = threadIdx.x % kMaxEig;
+    if (threadIdx.x < 64 && i < ke && j >= i && j < ke) {
+      if (Ac) {
+        Ac[(size_t)(b.dof + i) * nc + b.dof + j] = v;
+        Ac[(size_t)(b.dof + j) * nc + b.dof + i] = v;
+      }
+      if (Acb) {
+        blk[i * kmax + j] = v;
+        blk[j * kmax + i] = v;
       }
     }
   }
   // Faces to the upper neighbours.
   for (int dir = 1; dir < 6; dir += 2) {
-    const int sn = b.nb[dir];
+    const int sn = g.bricks[s].nb[dir];  // from memory: b stays in registers
     if (sn < 0) continue;
     const Brick o = g.bricks[sn];
     const int axis = dir >> 1;
@@ -156,21 +194,16 @@ __global__ void k_galerkin(Geo g, const double* __restrict__ phi, int kmax,
             if (j < o.neig) acc[a][j] -= pi * pn[j];
         }
       }
-      for (int a = 0; a < 4; ++a) {
-        const int i = i0 + a;
-        if (i >= ke) break;
-        for (int j = 0; j < o.neig; ++j) {
-          const double v = block_sum(acc[a][j], red);
-          if (threadIdx.x == 0) {
-            if (Ac && (o.box == b.box || inner == 2)) {
-              Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
-              Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
-            }
-            if (Acb) {
-              blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
-              Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
-            }
-          }
+      const double v = chunk_sum(acc, red);
+      const int i = i0 + (int)(threadIdx.x / kMaxEig), j = threadIdx.x % kMaxEig;
+      if (threadIdx.x < 64 && i < ke && j < o.neig) {
+        if (Ac && (o.box == b.box || inner == 2)) {
+          Ac[(size_t)(b.dof + i) * nc + o.dof + j] = v;
+          Ac[(size_t)(o.dof + j) * nc + b.dof + i] = v;
+        }
+        if (Acb) {
+          blk[(size_t)(dir + 1) * kmax * kmax + i * kmax + j] = v;
+          Acb[((size_t)sn * 7 + dir) * kmax * kmax + j * kmax + i] = v;  // slot of -dir
         }
       }
     }
