@@ -64,50 +64,57 @@ __device__ inline void wg_matmul(const double* A, const double* B, double* C, in
   __syncthreads();
 }
 
+// Pair k (0 <= k < mp / 2) of round r (0 <= r < mp - 1) of the round-robin
+// ordering over mp (even) indices: (r, mp - 1) for k = 0, else
+// (r + k, r - k) mod (mp - 1); returned with p < q.  Over the mp - 1 rounds
+// every index pair meets once.  q >= m marks an index without a partner.
+__device__ __forceinline__ void jacobi_pair(int r, int k, int mp, int& p, int& q) {
+  if (k == 0) {
+    p = r;
+    q = mp - 1;
+  } else {
+    p = r + k;
+    if (p >= mp - 1) p -= mp - 1;
+    q = r - k;
+    if (q < 0) q += mp - 1;
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
 // Symmetric eigendecomposition by parallel cyclic Jacobi with two barriers
 // per round (ping-pong buffers).  Input in A0 (destroyed); on return the
 // eigenvectors are in Q0 (columns, ascending eigenvalues w).  A1, Q1 are
 // scratch; rot holds 3*ld doubles.  Rotations are applied only where the
 // coupling is significant relative to the diagonal (threshold Jacobi, which
 // keeps small eigenvalues relatively accurate); a sweep without rotations
-// ends the iteration.
+// ends the iteration.  A round computes the rotations of its mp / 2
+// disjoint pairs once, then applies them as A <- R^T A R, Q <- Q R in
+// units of a 2 x 2 block (pair a rows, pair b columns) of A and of Q, so a
+// unit reads its operands in one batch.
 __device__ inline int wg_syev2(double* A0, double* A1, double* Q0, double* Q1, double* w,
                                double* rot, int m, int ld, int max_sweeps = 30) {
   __shared__ int s_rot[2];
-  double* cc = rot;
-  double* ss = rot + ld;
-  int* part = reinterpret_cast<int*>(rot + 2 * ld);
+  double* pcs = rot;       // pair k: cosine
+  double* psn = rot + ld;  // pair k: sine (the second index's; the first takes -s)
   for (int e = threadIdx.x; e < m * m; e += blockDim.x)
     Q0[(e / m) * ld + e % m] = (e / m == e % m) ? 1.0 : 0.0;
   if (threadIdx.x == 0) s_rot[0] = s_rot[1] = 0;
   __syncthreads();
-  const int mp = (m + 1) & ~1;
-  const int i0 = threadIdx.x / m, j0 = threadIdx.x % m;   // this thread's first element
-  const int di = blockDim.x / m, dj = blockDim.x % m;     // and the step to its next one
+  const int mp = (m + 1) & ~1, np2 = mp >> 1;
   double *Ac = A0, *An = A1, *Qc = Q0, *Qn = Q1;
   int sweeps = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     const int fl = sweep & 1;
     sweeps = sweep + 1;
     for (int r = 0; r < mp - 1; ++r) {
-      for (int k = threadIdx.x; k < mp / 2; k += blockDim.x) {
-        // round-robin pairing (r + k, r - k) mod (mp - 1), with mp - 1 paired
-        // to r; r, k < mp - 1, so one conditional add or subtract wraps
+      for (int k = threadIdx.x; k < np2; k += blockDim.x) {
         int p, q;
-        if (k == 0) {
-          p = r;
-          q = mp - 1;
-        } else {
-          p = r + k;
-          if (p >= mp - 1) p -= mp - 1;
-          q = r - k;
-          if (q < 0) q += mp - 1;
-        }
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
+        jacobi_pair(r, k, mp, p, q);
+        double c = 1.0, s = 0.0;
         if (q < m) {
           const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
           // the threshold test and the rotation are computed side by side
@@ -116,40 +123,51 @@ __device__ inline int wg_syev2(double* A0, double* A1, double* Q0, double* Q1, d
           const double th = (aqq - app) / (2.0 * (apq != 0.0 ? apq : 1.0));
           const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
           const double cr = 1.0 / sqrt(t * t + 1.0);
-          const double c = big ? cr : 1.0, s = big ? t * cr : 0.0;
+          c = big ? cr : 1.0;
+          s = big ? t * cr : 0.0;
           if (big) s_rot[fl] = 1;
-          cc[p] = c;
-          cc[q] = c;
-          ss[p] = -s;
-          ss[q] = s;
-          part[p] = q;
-          part[q] = p;
-        } else if (p < m) {
-          cc[p] = 1.0;
-          ss[p] = 0.0;
-          part[p] = p;
         }
+        pcs[k] = c;
+        psn[k] = s;
       }
       __syncthreads();
       // every reader of the previous sweep's flag is past this point
       if (r == 0 && threadIdx.x == 0) s_rot[fl ^ 1] = 0;
-      // element e = i m + j, stepped by the block size without a division
-      for (int e = threadIdx.x, i = i0, j = j0; e < m * m; e += blockDim.x) {
-        const int ip = part[i], jp = part[j];
-        const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
-        double x;
-        if (ip == j && ip != i)
-          x = 0.0;
-        else
-          x = ci * cj * Ac[i * ld + j] + ci * sj * Ac[i * ld + jp] + si * cj * Ac[ip * ld + j] +
-              si * sj * Ac[ip * ld + jp];
-        An[i * ld + j] = x;
-        Qn[i * ld + j] = cj * Qc[i * ld + j] + sj * Qc[i * ld + jp];
-        i += di;
-        j += dj;
-        if (j >= m) {
-          j -= m;
-          ++i;
+      for (int u = threadIdx.x; u < np2 * np2; u += blockDim.x) {
+        const int ka = u / np2, kb = u - ka * np2;
+        int pa, qa, pb, qb;
+        jacobi_pair(r, ka, mp, pa, qa);
+        jacobi_pair(r, kb, mp, pb, qb);
+        const bool ha = qa < m, hb = qb < m;
+        // element (i, j): ci cj a_ij + ci sj a_i,j' + si cj a_i',j + si sj a_i',j'
+        // with (c, -s) at the first index of a pair and (c, s) at the second,
+        // i' and j' the partners; an index without a partner has (1, 0) and
+        // is its own partner
+        const int qa1 = ha ? qa : pa, qb1 = hb ? qb : pb;
+        const double ca = pcs[ka], sa = psn[ka], cb = pcs[kb], sb = psn[kb];
+        const double a00 = Ac[pa * ld + pb], a01 = Ac[pa * ld + qb1];
+        const double a10 = Ac[qa1 * ld + pb], a11 = Ac[qa1 * ld + qb1];
+        const double q00 = Qc[pa * ld + pb], q01 = Qc[pa * ld + qb1];
+        const double q10 = Qc[qa1 * ld + pb], q11 = Qc[qa1 * ld + qb1];
+        const double sa0 = ha ? -sa : 0.0, sb0 = hb ? -sb : 0.0;
+        auto elem = [](double ci, double si, double cj, double sj, double aij, double aijp,
+                       double aipj, double aipjp) {
+          return ci * cj * aij + ci * sj * aijp + si * cj * aipj + si * sj * aipjp;
+        };
+        const bool diag = ka == kb;  // the annihilated couplings
+        An[pa * ld + pb] = elem(ca, sa0, cb, sb0, a00, a01, a10, a11);
+        Qn[pa * ld + pb] = cb * q00 + sb0 * q01;
+        if (hb) {
+          An[pa * ld + qb] = (diag && ha) ? 0.0 : elem(ca, sa0, cb, sb, a01, a00, a11, a10);
+          Qn[pa * ld + qb] = cb * q01 + sb * q00;
+        }
+        if (ha) {
+          An[qa * ld + pb] = diag ? 0.0 : elem(ca, sa, cb, sb0, a10, a11, a00, a01);
+          Qn[qa * ld + pb] = cb * q10 + sb0 * q11;
+          if (hb) {
+            An[qa * ld + qb] = elem(ca, sa, cb, sb, a11, a10, a01, a00);
+            Qn[qa * ld + qb] = cb * q11 + sb * q10;
+          }
         }
       }
       __syncthreads();

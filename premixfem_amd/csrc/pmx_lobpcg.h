@@ -23,6 +23,11 @@ constexpr int kLobWaves = 8;   // most waves per workgroup (LDS reduction slots)
 constexpr int kLobBigN = 2048;
 constexpr int kLobSmallN = 512;   // at most this many cells: 128-thread workgroups
 constexpr int kStagIts = 40;   // iterations without progress before the stagnation test
+// AX and AP follow X and P by block updates whose rounding errors
+// accumulate; they are recomputed from A every kRefreshIts iterations once
+// the best residual is within kRefreshNear x tol.
+constexpr int kRefreshIts = 10;
+constexpr double kRefreshNear = 1e3;
 
 struct LobArgs {
   int kwant, maxit, kmax;

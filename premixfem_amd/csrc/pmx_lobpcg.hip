@@ -146,6 +146,12 @@ __global__ void __launch_bounds__(BS) k_lobpcg(Geo g, LobArgs a) {
       }
       __syncthreads();
       for (it = 0; it < a.maxit; ++it) {
+        // Near convergence the drift of the updated AX and AP from A X and
+        // A P sets the residual's floor and can turn the iteration unstable.
+        if (it > 0 && it % kRefreshIts == 0 && s_rbest <= kRefreshNear * a.tol) {
+          wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, X, AX, MB);
+          if (np > 0) wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, P, AP, np);
+        }
         // ---- W = T (AX - M X Lambda), deflated ----
         for (int e = threadIdx.x; e < MB * n; e += blockDim.x) {
           const int j = e / n, r = e % n;

