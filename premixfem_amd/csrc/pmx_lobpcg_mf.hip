@@ -539,36 +539,44 @@ __device__ void wg_update(const ColSet& In, int nin, const double* CF, const Out
 // position and factor) of the next hyperplane before working on the current
 // one, and reads the neighbours of all its columns before writing any, so a
 // plane costs one LDS round trip rather than one per column; pl holds the
-// plane offsets (in LDS when they fit).  A plane wider than the workgroup
-// loads its extra cells on the spot.
-template <int CH>
-__device__ void mf_sweeps(const BrickView& v, int nplanes, const int* pl, const int* pc,
+// plane offsets, typed with its address space (LDS when they fit, else
+// global): through a generic pointer they would be flat loads, which wait
+// for the global loads in flight.  A plane wider than the workgroup loads
+// its extra cells on the spot.
+typedef const int __attribute__((address_space(3))) lcint;
+typedef const int __attribute__((address_space(1))) gcint;
+template <int CH, typename PLI>
+__device__ void mf_sweeps(const BrickView& v, int nplanes, const PLI* pl, const int* pc,
                           const float4* __restrict__ pf, const float4* __restrict__ pb, float* lv,
                           int ch) {
   const int n = v.n, tid = threadIdx.x, bs = blockDim.x;
   for (int dirn = 0; dirn < 2; ++dirn) {
     const bool fwd = dirn == 0;
     const float4* fac = (fwd ? pf : pb) + v.off;
-    int s = fwd ? 0 : nplanes - 1;
-    int pos = pl[s] + tid, end = pl[s + 1];
-    int pk = 0;
-    float4 f = {0.f, 0.f, 0.f, 0.f};
-    if (pos < end) {
-      pk = pc[pos];
-      f = fac[pos];
-    }
-    for (int k = 0; k < nplanes; ++k) {
-      int npos = 0, nend = 0, npk = 0;
-      float4 nf = {0.f, 0.f, 0.f, 0.f};
-      if (k + 1 < nplanes) {
-        const int s2 = fwd ? k + 1 : nplanes - 2 - k;
-        npos = pl[s2] + tid;
-        nend = pl[s2 + 1];
-        if (npos < nend) {
-          npk = pc[npos];
-          nf = fac[npos];
+    // this thread's first cell of plane k in sweep order: position range and
+    // its packed coordinates and factor (zero beyond the plane)
+    auto fetch = [&](int k, int& p0, int& p1, int& pk, float4& f) {
+      p0 = 0;
+      p1 = 0;
+      pk = 0;
+      f = float4{0.f, 0.f, 0.f, 0.f};
+      if (k < nplanes) {
+        const int sk = fwd ? k : nplanes - 1 - k;
+        p0 = pl[sk] + tid;
+        p1 = pl[sk + 1];
+        if (p0 < p1) {
+          pk = pc[p0];
+          f = fac[p0];
         }
       }
+    };
+    int pos, end, pk;
+    float4 f;
+    fetch(0, pos, end, pk, f);
+    for (int k = 0; k < nplanes; ++k) {
+      int npos, nend, npk;
+      float4 nf;
+      fetch(k + 1, npos, nend, npk, nf);
       for (int p = pos; p < end; p += bs) {
         int pkk = pk;
         float4 ff = f;
@@ -710,11 +718,9 @@ k_lobpcg_mf(Geo g, LobArgs a) {
     __syncthreads();
   }
   // hyperplane offsets of the sweeps, in LDS when they fit
-  const int* pl = ps;
-  if (b.nplanes <= kMfPlanes) {
+  const bool pl_lds = b.nplanes <= kMfPlanes;
+  if (pl_lds)
     for (int i = threadIdx.x; i <= b.nplanes; i += BS) s_pl[i] = ps[i];
-    pl = s_pl;
-  }
   __syncthreads();
   const double msum = msum_s;
   // A subdomain with a well term has no null vector: all en_lv1 pairs are
@@ -811,7 +817,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           }
           __syncthreads();
           PMX_PHASE(0)
-          mf_sweeps<CH>(v, b.nplanes, pl, pc, a.pf, a.pb, pv, ch);
+          if (pl_lds)
+            mf_sweeps<CH>(v, b.nplanes, (const lcint*)s_pl, pc, a.pf, a.pb, pv, ch);
+          else
+            mf_sweeps<CH>(v, b.nplanes, (const gcint*)ps, pc, a.pf, a.pb, pv, ch);
           PMX_PHASE(1)
           double acc[CH];
 #pragma unroll
