@@ -1341,6 +1341,11 @@ int launch_lv3_apply(pmx_ctx* ctx, const double* t, double* d) {
 // order), the small Cholesky / eigenproblems in one workgroup.
 constexpr int kGramRows = 2048;   // rows per workgroup of a partial Gram
 constexpr int kLv3Max = 32;       // largest LOBPCG basis [X W P] (3 m)
+// Stagnation window of the shift-invert subspace iterations of level 3:
+// they gain a constant factor per iteration (4x on the 256^3 fracture
+// medium), so kLv3StagWin iterations without a 2x gain mean the residual sits
+// on its rounding floor.
+constexpr int kLv3StagWin = 5;
 
 // part[(g*ma + i)*mb + j] = sum over rows of chunk g of A[:, i] . B[:, j].
 __global__ void __launch_bounds__(256) k_gram_part(const double* __restrict__ A, int ma,
@@ -2076,9 +2081,10 @@ int level3_setup(pmx_ctx* ctx) {
       // in lv3_floor; above it the iteration runs on to its cap and fails
       // as SLEPc's nconv < nev does (:1098).
       hist.push_back(mres);
-      if (!conv && it >= 20 && mres <= ctx->eig_stag * ctx->eig_tol) {
+      if (!conv && (int)hist.size() > kLv3StagWin && mres <= ctx->eig_stag * ctx->eig_tol) {
         double best_old = 1e300;
-        for (size_t q = 0; q + 10 < hist.size(); ++q) best_old = std::min(best_old, hist[q]);
+        for (size_t q = 0; q + kLv3StagWin < hist.size(); ++q)
+          best_old = std::min(best_old, hist[q]);
         if (mres > 0.5 * best_old) {
           conv = true;
           ctx->lv3_floor = mres;
@@ -2160,13 +2166,14 @@ int level3_setup(pmx_ctx* ctx) {
 // (P dropped when that fails), Rayleigh-Ritz on S^T A_c S, X = S Q_{:,1:m},
 // P = S_{:,m+1:} Q_{m+1:,1:m}.  m >= en_lv2 + 4 vectors; converged when
 // ||A_c x - w x|| <= eig_tol |w| for the en_lv2 wanted pairs (x of unit
-// norm); a residual that stagnates (no 2x gain over 20 iterations) below
-// 100 eig_tol is accepted and reported in lv3_floor.  Warm start: the
+// norm); a residual that stagnates (no 2x gain over 20 iterations) within
+// eig_stag x eig_tol is accepted and reported in lv3_floor.  Warm start: the
 // previous setup's eigenvectors when the coarse space has the same size.
 // With the exact band factor instead of ICC(0) the same loop runs the
 // residual-corrected shift-invert subspace iteration of the dense path
 // (basis X - A_c^{-1}(A_c X - X diag(w)) only, m = max(4 en_lv2, en_lv2 + 8))
-// and tests the shift-inverted residual ||x - w A_c^{-1} x||.
+// and tests the shift-inverted residual ||x - w A_c^{-1} x||, whose
+// stagnation window is kLv3StagWin iterations.
 int level3_setup_icc(pmx_ctx* ctx) {
   const int n = ctx->nc;
   const int k2 = std::min(ctx->en_lv2, n);
@@ -2233,7 +2240,7 @@ int level3_setup_icc(pmx_ctx* ctx) {
       np = 0;
       PMX_TRY(wk.orthonormalize(S.p, ns, tmp.p, &bad));
     }
-    if (bad && !hist.empty() && hist.back() <= 100.0 * ctx->eig_tol) {
+    if (bad && !hist.empty() && hist.back() <= ctx->eig_stag * ctx->eig_tol) {
       // The new directions are rounding noise of converged pairs: accept.
       conv = true;
       ctx->lv3_floor = hist.back();
@@ -2288,9 +2295,14 @@ int level3_setup_icc(pmx_ctx* ctx) {
       conv = true;
       break;
     }
-    if (it >= 40 && mres <= 100.0 * ctx->eig_tol) {
+    // Stagnation within -eig_stag x -eig_tol: no 2x gain over the window
+    // (kLv3StagWin iterations with the exact factor, 20 for the ICC(0)
+    // LOBPCG, whose progress is less regular).
+    const int win = ctx->use_band ? kLv3StagWin : 20;
+    if ((int)hist.size() > (ctx->use_band ? win : 2 * win) &&
+        mres <= ctx->eig_stag * ctx->eig_tol) {
       double best_old = 1e300;
-      for (size_t q = 0; q + 20 < hist.size(); ++q) best_old = std::min(best_old, hist[q]);
+      for (size_t q = 0; q + win < hist.size(); ++q) best_old = std::min(best_old, hist[q]);
       if (mres > 0.5 * best_old) {
         conv = true;
         ctx->lv3_floor = mres;
