@@ -28,6 +28,9 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));   // one MFMA 16x16 
 
 constexpr int kKP = 32;   // rows / columns of the coefficient matrix of a block update
 constexpr int kMfPlanes = 255;   // hyperplanes whose offsets are kept in LDS
+// Largest |x^T M w'| / ||w'||_M (x a column of [X P], w' a column of W after
+// one projection) at which the second projection is left out.
+constexpr double kSkip2 = 1e-10;
 
 // Columns per sweep chunk: the whole block up to 8 (one sweep pair per
 // iteration), else two halves; the FP32 chunk of a 16^3 subdomain then takes
@@ -701,7 +704,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   __shared__ double lam[MB], nrm[MB], msum_s, s_rbest;
   __shared__ double s_red[NW * MB], s_dc[CH];
   __shared__ int s_pl[kMfPlanes + 1];
-  __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag, s_well;
+  __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag, s_well, s_skip2;
 
   {
     double s = 0.0, wsum = 0.0;
@@ -731,10 +734,12 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   // phase timers of the traced subdomain (thread 0), kept in LDS
   __shared__ unsigned long long tph[16];
   __shared__ int tsw;   // Rayleigh-Ritz Jacobi sweeps of the traced subdomain
+  __shared__ int tsk;   // its iterations without the second projection
   const bool trace = sidx == a.debug && threadIdx.x == 0;
   if (trace) {
     for (int i = 0; i < 16; ++i) tph[i] = 0;
     tsw = 0;
+    tsk = 0;
   }
 #define PMX_PHASE(i)                                                   \
   if (trace) {                                                         \
@@ -927,6 +932,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         wg_update<1>(sxpw, nk, CF, outset(W, MB), MB, n);
         wg_update<1>(axpw, nk, CF, outset(AW, MB), MB, n);
         PMX_PHASE(6)
+        if (threadIdx.x == 0) s_skip2 = 1;
         if (nk > 16)
           wg_gram<2, 1>(sxpw, nk, colset(W, MB), MB, mass, n, vec, red, GB, LD);
         else
@@ -938,16 +944,29 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           for (int k = 0; k < nxp; ++k) s -= GB[k * LD + i] * GB[k * LD + j];
           GA[i * LD + j] = s;
         }
+        // The second projection's coefficients a', b' are rounding-level
+        // when the first pass met no cancellation: below kSkip2 of the
+        // column's M-norm it is left out (the Rayleigh-Ritz uses the exact
+        // Gram of [X W P], which such a coupling barely perturbs) and the
+        // SVQB update reads W and AW alone.
+        for (int e = threadIdx.x; e < nxp * MB; e += BS) {
+          const int k = e / MB, j = e % MB;
+          if (fabs(GB[k * LD + j]) > kSkip2 * sqrt(fabs(GB[(nxp + j) * LD + j]))) s_skip2 = 0;
+        }
         __syncthreads();
+        const bool skip2 = s_skip2 != 0;
+        if (trace && skip2) ++tsk;
         PMX_PHASE(7)
         const int nw = wg_svqb(GA, Q, Z, TM, w, rot, MB, LD, 1e-12);
         if (nw == 0) {
           status = 3;
           break;
         }
-        // CF = [-(a', b') Z; Z]: the second projection and the SVQB in one pass
-        for (int e = threadIdx.x; e < nk * nw; e += BS) {
-          const int k = e / nw, c = e % nw;
+        // CF = [-(a', b') Z; Z]: the second projection and the SVQB in one
+        // pass (CF = Z alone when the projection is left out)
+        const int k0 = skip2 ? nxp : 0;
+        for (int e = threadIdx.x; e < (nk - k0) * nw; e += BS) {
+          const int k = k0 + e / nw, c = e % nw;
           double val;
           if (k < nxp) {
             val = 0.0;
@@ -955,12 +974,17 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           } else {
             val = Z[(k - nxp) * LD + c];
           }
-          CF[k * kKP + c] = val;
+          CF[(k - k0) * kKP + c] = val;
         }
         __syncthreads();
         PMX_PHASE(8)
-        wg_update<1>(sxpw, nk, CF, outset(W, nw), nw, n);
-        wg_update<1>(axpw, nk, CF, outset(AW, nw), nw, n);
+        if (skip2) {
+          wg_update<1>(colset(W, MB), MB, CF, outset(W, nw), nw, n);
+          wg_update<1>(colset(AW, MB), MB, CF, outset(AW, nw), nw, n);
+        } else {
+          wg_update<1>(sxpw, nk, CF, outset(W, nw), nw, n);
+          wg_update<1>(axpw, nk, CF, outset(AW, nw), nw, n);
+        }
         PMX_PHASE(9)
         // ---- G: S^T M S and S^T A S, S = [X W P] ----
         const int ns = MB + nw + np;
@@ -1052,10 +1076,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   if (trace) {
     printf("lobpcg-mf phases (us) its %d: resid %.0f sweeps %.0f deflate %.0f W,AW %.0f "
            "conv %.0f | ortho gram1 %.0f upd1 %.0f gram2 %.0f svqb %.0f upd2 %.0f | gram %.0f | "
-           "rr chol %.0f syev %.0f (%d sweeps) P %.0f | update %.0f\n",
+           "rr chol %.0f syev %.0f (%d sweeps) P %.0f | update %.0f | one projection %d\n",
            it, tph[0] * 0.01, tph[1] * 0.01, tph[2] * 0.01, tph[3] * 0.01, tph[4] * 0.01,
            tph[5] * 0.01, tph[6] * 0.01, tph[7] * 0.01, tph[8] * 0.01, tph[9] * 0.01,
-           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tsw, tph[13] * 0.01, tph[14] * 0.01);
+           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tsw, tph[13] * 0.01, tph[14] * 0.01, tsk);
   }
 #undef PMX_PHASE
   // ---- outputs ----
