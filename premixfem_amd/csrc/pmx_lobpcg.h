@@ -153,14 +153,15 @@ __device__ inline void wg_apply_AN(const BrickView& v, const double* __restrict_
     const int o2 = ly > 0 ? li - v.nx : li, o3 = ly < v.ny - 1 ? li + v.nx : li;
     const int o4 = lz > 0 ? li - v.nxny : li, o5 = lz < v.nz - 1 ? li + v.nxny : li;
     const double wc = wd ? wd[c] : 0.0;
+    // Columns beyond nb read column 0 (in range) and store nothing, so the
+    // loads of all NB columns are issued together.
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      if (j >= nb) break;
-      const double* x = V + (size_t)j * n;
+      const double* x = V + (size_t)(j < nb ? j : 0) * n;
       const double xc = x[li];
-      Y[(size_t)j * n + li] = tl0 * (xc - x[o0]) + th0 * (xc - x[o1]) + tl1 * (xc - x[o2]) +
-                              th1 * (xc - x[o3]) + tl2 * (xc - x[o4]) + th2 * (xc - x[o5]) +
-                              wc * xc;
+      const double y = tl0 * (xc - x[o0]) + th0 * (xc - x[o1]) + tl1 * (xc - x[o2]) +
+                       th1 * (xc - x[o3]) + tl2 * (xc - x[o4]) + th2 * (xc - x[o5]) + wc * xc;
+      if (j < nb) Y[(size_t)j * n + li] = y;
     }
   });
   __syncthreads();

@@ -28,9 +28,6 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));   // one MFMA 16x16 
 
 constexpr int kKP = 32;   // rows / columns of the coefficient matrix of a block update
 constexpr int kMfPlanes = 255;   // hyperplanes whose offsets are kept in LDS
-// Largest |x^T M w'| / ||w'||_M (x a column of [X P], w' a column of W after
-// one projection) at which the second projection is left out.
-constexpr double kSkip2 = 1e-10;
 
 // Columns per sweep chunk: the whole block up to 8 (one sweep pair per
 // iteration), else two halves; the FP32 chunk of a 16^3 subdomain then takes
@@ -542,44 +539,36 @@ __device__ void wg_update(const ColSet& In, int nin, const double* CF, const Out
 // position and factor) of the next hyperplane before working on the current
 // one, and reads the neighbours of all its columns before writing any, so a
 // plane costs one LDS round trip rather than one per column; pl holds the
-// plane offsets, typed with its address space (LDS when they fit, else
-// global): through a generic pointer they would be flat loads, which wait
-// for the global loads in flight.  A plane wider than the workgroup loads
-// its extra cells on the spot.
-typedef const int __attribute__((address_space(3))) lcint;
-typedef const int __attribute__((address_space(1))) gcint;
-template <int CH, typename PLI>
-__device__ void mf_sweeps(const BrickView& v, int nplanes, const PLI* pl, const int* pc,
+// plane offsets (in LDS when they fit).  A plane wider than the workgroup
+// loads its extra cells on the spot.
+template <int CH>
+__device__ void mf_sweeps(const BrickView& v, int nplanes, const int* pl, const int* pc,
                           const float4* __restrict__ pf, const float4* __restrict__ pb, float* lv,
                           int ch) {
   const int n = v.n, tid = threadIdx.x, bs = blockDim.x;
   for (int dirn = 0; dirn < 2; ++dirn) {
     const bool fwd = dirn == 0;
     const float4* fac = (fwd ? pf : pb) + v.off;
-    // this thread's first cell of plane k in sweep order: position range and
-    // its packed coordinates and factor (zero beyond the plane)
-    auto fetch = [&](int k, int& p0, int& p1, int& pk, float4& f) {
-      p0 = 0;
-      p1 = 0;
-      pk = 0;
-      f = float4{0.f, 0.f, 0.f, 0.f};
-      if (k < nplanes) {
-        const int sk = fwd ? k : nplanes - 1 - k;
-        p0 = pl[sk] + tid;
-        p1 = pl[sk + 1];
-        if (p0 < p1) {
-          pk = pc[p0];
-          f = fac[p0];
+    int s = fwd ? 0 : nplanes - 1;
+    int pos = pl[s] + tid, end = pl[s + 1];
+    int pk = 0;
+    float4 f = {0.f, 0.f, 0.f, 0.f};
+    if (pos < end) {
+      pk = pc[pos];
+      f = fac[pos];
+    }
+    for (int k = 0; k < nplanes; ++k) {
+      int npos = 0, nend = 0, npk = 0;
+      float4 nf = {0.f, 0.f, 0.f, 0.f};
+      if (k + 1 < nplanes) {
+        const int s2 = fwd ? k + 1 : nplanes - 2 - k;
+        npos = pl[s2] + tid;
+        nend = pl[s2 + 1];
+        if (npos < nend) {
+          npk = pc[npos];
+          nf = fac[npos];
         }
       }
-    };
-    int pos, end, pk;
-    float4 f;
-    fetch(0, pos, end, pk, f);
-    for (int k = 0; k < nplanes; ++k) {
-      int npos, nend, npk;
-      float4 nf;
-      fetch(k + 1, npos, nend, npk, nf);
       for (int p = pos; p < end; p += bs) {
         int pkk = pk;
         float4 ff = f;
@@ -704,7 +693,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   __shared__ double lam[MB], nrm[MB], msum_s, s_rbest;
   __shared__ double s_red[NW * MB], s_dc[CH];
   __shared__ int s_pl[kMfPlanes + 1];
-  __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag, s_well, s_skip2;
+  __shared__ int s_conv, s_flag, s_lock[MB], s_itbest, s_stag, s_well;
 
   {
     double s = 0.0, wsum = 0.0;
@@ -721,9 +710,11 @@ k_lobpcg_mf(Geo g, LobArgs a) {
     __syncthreads();
   }
   // hyperplane offsets of the sweeps, in LDS when they fit
-  const bool pl_lds = b.nplanes <= kMfPlanes;
-  if (pl_lds)
+  const int* pl = ps;
+  if (b.nplanes <= kMfPlanes) {
     for (int i = threadIdx.x; i <= b.nplanes; i += BS) s_pl[i] = ps[i];
+    pl = s_pl;
+  }
   __syncthreads();
   const double msum = msum_s;
   // A subdomain with a well term has no null vector: all en_lv1 pairs are
@@ -734,12 +725,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   // phase timers of the traced subdomain (thread 0), kept in LDS
   __shared__ unsigned long long tph[16];
   __shared__ int tsw;   // Rayleigh-Ritz Jacobi sweeps of the traced subdomain
-  __shared__ int tsk;   // its iterations without the second projection
   const bool trace = sidx == a.debug && threadIdx.x == 0;
   if (trace) {
     for (int i = 0; i < 16; ++i) tph[i] = 0;
     tsw = 0;
-    tsk = 0;
   }
 #define PMX_PHASE(i)                                                   \
   if (trace) {                                                         \
@@ -822,10 +811,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           }
           __syncthreads();
           PMX_PHASE(0)
-          if (pl_lds)
-            mf_sweeps<CH>(v, b.nplanes, (const lcint*)s_pl, pc, a.pf, a.pb, pv, ch);
-          else
-            mf_sweeps<CH>(v, b.nplanes, (const gcint*)ps, pc, a.pf, a.pb, pv, ch);
+          mf_sweeps<CH>(v, b.nplanes, pl, pc, a.pf, a.pb, pv, ch);
           PMX_PHASE(1)
           double acc[CH];
 #pragma unroll
@@ -932,7 +918,6 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         wg_update<1>(sxpw, nk, CF, outset(W, MB), MB, n);
         wg_update<1>(axpw, nk, CF, outset(AW, MB), MB, n);
         PMX_PHASE(6)
-        if (threadIdx.x == 0) s_skip2 = 1;
         if (nk > 16)
           wg_gram<2, 1>(sxpw, nk, colset(W, MB), MB, mass, n, vec, red, GB, LD);
         else
@@ -944,29 +929,16 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           for (int k = 0; k < nxp; ++k) s -= GB[k * LD + i] * GB[k * LD + j];
           GA[i * LD + j] = s;
         }
-        // The second projection's coefficients a', b' are rounding-level
-        // when the first pass met no cancellation: below kSkip2 of the
-        // column's M-norm it is left out (the Rayleigh-Ritz uses the exact
-        // Gram of [X W P], which such a coupling barely perturbs) and the
-        // SVQB update reads W and AW alone.
-        for (int e = threadIdx.x; e < nxp * MB; e += BS) {
-          const int k = e / MB, j = e % MB;
-          if (fabs(GB[k * LD + j]) > kSkip2 * sqrt(fabs(GB[(nxp + j) * LD + j]))) s_skip2 = 0;
-        }
         __syncthreads();
-        const bool skip2 = s_skip2 != 0;
-        if (trace && skip2) ++tsk;
         PMX_PHASE(7)
         const int nw = wg_svqb(GA, Q, Z, TM, w, rot, MB, LD, 1e-12);
         if (nw == 0) {
           status = 3;
           break;
         }
-        // CF = [-(a', b') Z; Z]: the second projection and the SVQB in one
-        // pass (CF = Z alone when the projection is left out)
-        const int k0 = skip2 ? nxp : 0;
-        for (int e = threadIdx.x; e < (nk - k0) * nw; e += BS) {
-          const int k = k0 + e / nw, c = e % nw;
+        // CF = [-(a', b') Z; Z]: the second projection and the SVQB in one pass
+        for (int e = threadIdx.x; e < nk * nw; e += BS) {
+          const int k = e / nw, c = e % nw;
           double val;
           if (k < nxp) {
             val = 0.0;
@@ -974,17 +946,12 @@ k_lobpcg_mf(Geo g, LobArgs a) {
           } else {
             val = Z[(k - nxp) * LD + c];
           }
-          CF[(k - k0) * kKP + c] = val;
+          CF[k * kKP + c] = val;
         }
         __syncthreads();
         PMX_PHASE(8)
-        if (skip2) {
-          wg_update<1>(colset(W, MB), MB, CF, outset(W, nw), nw, n);
-          wg_update<1>(colset(AW, MB), MB, CF, outset(AW, nw), nw, n);
-        } else {
-          wg_update<1>(sxpw, nk, CF, outset(W, nw), nw, n);
-          wg_update<1>(axpw, nk, CF, outset(AW, nw), nw, n);
-        }
+        wg_update<1>(sxpw, nk, CF, outset(W, nw), nw, n);
+        wg_update<1>(axpw, nk, CF, outset(AW, nw), nw, n);
         PMX_PHASE(9)
         // ---- G: S^T M S and S^T A S, S = [X W P] ----
         const int ns = MB + nw + np;
@@ -1076,10 +1043,10 @@ k_lobpcg_mf(Geo g, LobArgs a) {
   if (trace) {
     printf("lobpcg-mf phases (us) its %d: resid %.0f sweeps %.0f deflate %.0f W,AW %.0f "
            "conv %.0f | ortho gram1 %.0f upd1 %.0f gram2 %.0f svqb %.0f upd2 %.0f | gram %.0f | "
-           "rr chol %.0f syev %.0f (%d sweeps) P %.0f | update %.0f | one projection %d\n",
+           "rr chol %.0f syev %.0f (%d sweeps) P %.0f | update %.0f\n",
            it, tph[0] * 0.01, tph[1] * 0.01, tph[2] * 0.01, tph[3] * 0.01, tph[4] * 0.01,
            tph[5] * 0.01, tph[6] * 0.01, tph[7] * 0.01, tph[8] * 0.01, tph[9] * 0.01,
-           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tsw, tph[13] * 0.01, tph[14] * 0.01, tsk);
+           tph[10] * 0.01, tph[11] * 0.01, tph[12] * 0.01, tsw, tph[13] * 0.01, tph[14] * 0.01);
   }
 #undef PMX_PHASE
   // ---- outputs ----
