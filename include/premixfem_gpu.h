@@ -128,7 +128,10 @@ typedef struct {
  * matrix cores with fused passes; 0, the VALU kernel; same eigenpairs to the
  * eigensolver tolerance), -eig_shape 0|1 (matrix-core kernel: one
  * 512-thread workgroup per CU on large subdomains, default, or two 256-thread
- * ones per CU; same eigenpairs),
+ * ones per CU; same eigenpairs), -eig_ortho 0|1 (matrix-core kernel: W
+ * made M-orthogonal to [X P] and M-orthonormal from one Gram of [X P W] in
+ * coefficient space, default, or by two explicit projection passes; same
+ * eigenpairs to the eigensolver tolerance),
  * -coarse_collapse 0|1 (one process with an exact level-2 factor: apply only
  * the first level-2 stage of the coarse cycle, since the later level-3 and
  * level-2 stages add exactly zero in exact arithmetic; default 1),

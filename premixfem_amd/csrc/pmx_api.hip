@@ -225,6 +225,13 @@ static int parse_options(pmx_ctx* c, const char* options) {
         set_error("-eig_kernel %d: 0 (VALU) or 1 (matrix cores)", c->eig_kernel);
         return PMX_ERR_ARG_OUTOFRANGE;
       }
+    } else if (k == "-eig_ortho") {
+      PMX_TRY(geti(c->eig_ortho));
+      if (c->eig_ortho < 0 || c->eig_ortho > 1) {
+        set_error("-eig_ortho %d: 0 (explicit projections) or 1 (coefficient space)",
+                  c->eig_ortho);
+        return PMX_ERR_ARG_OUTOFRANGE;
+      }
     } else if (k == "-eig_shape") {
       PMX_TRY(geti(c->eig_shape));
       if (c->eig_shape < 0 || c->eig_shape > 1) {

@@ -314,6 +314,8 @@ struct pmx_ctx {
   int eig_debug = -1;           // trace the LOBPCG iterations of one subdomain
   int bicc_prefetch = 1;        // block ICC(0) level-2 solve with row data loaded a round ahead
   int eig_kernel = 1;           // LOBPCG kernel: 1 matrix-core passes, 0 VALU
+  int eig_ortho = 1;            // matrix-core LOBPCG: W orthogonalised in coefficient space (1)
+                                // or by two explicit projection passes (0)
   int eig_shape = 0;            // matrix-core LOBPCG on large subdomains: 0 one per CU
                                 // (512 threads), 1 two per CU (256 threads)
   int eig_order = 1;            // launch LOBPCG hardest-first from the previous setup's counts

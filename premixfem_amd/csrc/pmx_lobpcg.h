@@ -56,6 +56,8 @@ struct LobArgs {
   double stag_factor; // stagnation acceptance: worst residual <= stag_factor * tol
   int debug;          // subdomain whose iterations are traced with printf (-1: none)
   const int* order;   // workgroup b solves subdomain order[b] (nullptr: b * stride)
+  int ortho_coef;     // matrix-core kernel: orthogonalise W in coefficient space from one
+                      // Gram of [X P W] (1) or by two explicit projection passes (0)
 };
 
 __device__ __forceinline__ double hash_uniform(uint64_t a) {

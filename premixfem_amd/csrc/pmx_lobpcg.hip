@@ -517,6 +517,7 @@ static int lobpcg_run(pmx_ctx* ctx, const double* x0_rows, int MB, int stride, f
   a.stagnated = stag.p;
   a.stag_factor = ctx->eig_stag;
   a.debug = ctx->eig_debug;
+  a.ortho_coef = ctx->eig_ortho;
   // Launch order: one 512-thread workgroup fills a CU, so subdomains beyond
   // the first wave of workgroups start late, and a hard one started late
   // sets the launch time.  A context that was set up before launches its
@@ -724,7 +725,7 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
     if (c->en_lv1 != k || c->eig_guard != c0->eig_guard || c->eig_tol != c0->eig_tol ||
         c->eig_maxit != c0->eig_maxit || c->eig_sigma != c0->eig_sigma ||
         c->seed != c0->seed || c->eig_stag != c0->eig_stag || c->eig_kernel != c0->eig_kernel ||
-        c->eig_shape != c0->eig_shape ||
+        c->eig_shape != c0->eig_shape || c->eig_ortho != c0->eig_ortho ||
         (c->warm_start == 1) != (c0->warm_start == 1)) {
       set_error("batched setup: problem %d has other en_lv1 / eigensolver options", i);
       return PMX_ERR_ARG_WRONG;
@@ -876,6 +877,7 @@ int lobpcg_batch(pmx_ctx** ctxs, int nprob) {
   a.stag_factor = c0->eig_stag;
   a.debug = -1;
   a.order = nullptr;
+  a.ortho_coef = c0->eig_ortho;
   if (c0->eig_kernel == 1) {
     launch_lobpcg_mf(MB, c0->eig_shape, g, S, st, a, maxn);
   } else {

@@ -898,6 +898,68 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         // ---- O: W -= X (X^T M W) + P (P^T M W) twice; SVQB(W) ----
         const int nxp = MB + np, nk = nxp + MB;
         const ColSet sxpw = colset(X, MB, P, np, W, MB), axpw = colset(AX, MB, AP, np, AW, MB);
+        int nw = 0;
+        if (a.ortho_coef) {
+          // Both projections and the SVQB from one M-Gram G of B = [X P W]
+          // (one pass instead of two Grams and two block updates):
+          // W' = B K1, K1 = [-G_xw; I] (locked columns zero); the second
+          // projection a' = [X P]^T M W' = (G K1)_x; W'^T M W' = K1^T G K1;
+          // the Gram after it K1^T G K1 - a'^T a'; W'' = B (K1 - [a'; 0]) Z.
+          if (nk > 16)
+            wg_gram_ma<2, false>(sxpw, sxpw, nk, mass, n, vec, red, GB, nullptr, LD);
+          else
+            wg_gram_ma<1, false>(sxpw, sxpw, nk, mass, n, vec, red, GB, nullptr, LD);
+          for (int e = threadIdx.x; e < nk * MB; e += BS) {   // K1 -> CF
+            const int k = e / MB, c = e % MB;
+            double val;
+            if (s_lock[c])
+              val = 0.0;
+            else if (k < nxp)
+              val = -GB[k * LD + nxp + c];
+            else
+              val = (k - nxp == c) ? 1.0 : 0.0;
+            CF[k * kKP + c] = val;
+          }
+          __syncthreads();
+          for (int e = threadIdx.x; e < nk * MB; e += BS) {   // T1 = G K1 -> TM
+            const int k = e / MB, c = e % MB;
+            double v = 0.0;
+            for (int l = 0; l < nk; ++l) v += GB[k * LD + l] * CF[l * kKP + c];
+            TM[k * LD + c] = v;
+          }
+          __syncthreads();
+          for (int e = threadIdx.x; e < MB * MB; e += BS) {   // K1^T T1 - a'^T a'
+            const int i = e / MB, j = e % MB;
+            double v = 0.0;
+            for (int l = 0; l < nk; ++l) v += CF[l * kKP + i] * TM[l * LD + j];
+            for (int k = 0; k < nxp; ++k) v -= TM[k * LD + i] * TM[k * LD + j];
+            GA[i * LD + j] = v;
+          }
+          __syncthreads();
+          wg_symmetrize(GA, MB, LD);
+          for (int e = threadIdx.x; e < nxp * MB; e += BS) {  // K = K1 - [a'; 0]
+            const int k = e / MB, c = e % MB;
+            CF[k * kKP + c] -= TM[k * LD + c];
+          }
+          __syncthreads();
+          PMX_PHASE(7)
+          nw = wg_svqb(GA, Q, Z, TM, w, rot, MB, LD, 1e-12);
+          if (nw == 0) {
+            status = 3;
+            break;
+          }
+          for (int e = threadIdx.x; e < nk * nw; e += BS) {  // K Z -> GB (G is done)
+            const int k = e / nw, c = e % nw;
+            double v = 0.0;
+            for (int j = 0; j < MB; ++j) v += CF[k * kKP + j] * Z[j * LD + c];
+            GB[k * LD + c] = v;
+          }
+          __syncthreads();
+          for (int e = threadIdx.x; e < nk * nw; e += BS)
+            CF[(e / nw) * kKP + e % nw] = GB[(e / nw) * LD + e % nw];
+          __syncthreads();
+          PMX_PHASE(8)
+        } else {
         if (nxp > 16)
           wg_gram<2, 1>(colset(X, MB, P, np), nxp, colset(W, MB), MB, mass, n, vec, red, GB, LD);
         else
@@ -931,7 +993,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         }
         __syncthreads();
         PMX_PHASE(7)
-        const int nw = wg_svqb(GA, Q, Z, TM, w, rot, MB, LD, 1e-12);
+        nw = wg_svqb(GA, Q, Z, TM, w, rot, MB, LD, 1e-12);
         if (nw == 0) {
           status = 3;
           break;
@@ -950,6 +1012,7 @@ k_lobpcg_mf(Geo g, LobArgs a) {
         }
         __syncthreads();
         PMX_PHASE(8)
+        }
         wg_update<1>(sxpw, nk, CF, outset(W, nw), nw, n);
         wg_update<1>(axpw, nk, CF, outset(AW, nw), nw, n);
         PMX_PHASE(9)

@@ -54,6 +54,7 @@ def _init(dom, mesh, options):
     ((1.0, 1.0, 1.0), (8, 8, 8), "-pc_precision 16", 62),
     ((1.0, 1.0, 1.0), (8, 8, 8), "-use_full_Cholesky_lv1 -pc_precision 32", 56),  # FP64 only
     ((1.0, 1.0, 1.0), (8, 8, 8), "-pin 8,0,0", 63),
+    ((1.0, 1.0, 1.0), (8, 8, 8), "-eig_ortho 2", 63),
     ((1.0, 1.0, 1.0), (8, 8, 8), "-no_such_option 1", 62),
     ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2,2", 62),         # px,py,pz or one count
     ((1.0, 1.0, 1.0), (8, 8, 8), "-procs_virtual 2,x,1", 62),

@@ -11,9 +11,10 @@ the pressure within 1e-10 relative; where the GPU uses its own subdomain
 eigenvectors (converged to the eigensolver tolerance 1e-8, like SLEPc's)
 rather than the oracle's, the preconditioner differs from the oracle's by
 that much, and on the 1e5-contrast media the pressure within 1e-7.  FP32
-preconditioner inside the FP64 CG: iterations within 2, pressure within 1e-6
-(the FP32 rounding of the apply, 1e-7 relative, enters every search
-direction).
+preconditioner inside the FP64 CG: iterations within 2 (0 to 4 above on the
+pinned 1e5-contrast bricks, the spread measured over LOBPCG seeds), pressure
+within 1e-6 (the FP32 rounding of the apply, 1e-7 relative, enters every
+search direction).
 """
 from __future__ import annotations
 
@@ -225,18 +226,20 @@ def test_16cube_subdomains_lobpcg512_and_quad_kernels(brick16, mb, precision):
 def test_16cube_lobpcg_kernels_agree(brick16, mb):
     """The matrix-core LOBPCG (-eig_kernel 1, the default) in both workgroup
     shapes (one 512-thread workgroup per CU; two 256-thread ones with 4-column
-    sweep chunks, -eig_shape 1) and the VALU one (-eig_kernel 0) on the 16^3
-    subdomains: all within 1e-8 of the oracle's eigenvalues, the same PCG
-    iteration count, and pressures within the 1e-7 the eigenvector tolerance
-    allows on this medium."""
+    sweep chunks, -eig_shape 1), with W orthogonalised in coefficient space
+    (-eig_ortho 1, the default) or by two explicit projection passes, and the
+    VALU one (-eig_kernel 0) on the 16^3 subdomains: all within 1e-8 of the
+    oracle's eigenvalues, the same PCG iteration count, and pressures within
+    the 1e-7 the eigenvector tolerance allows on this medium."""
     prob, orc, x_ref, its_ref = brick16
     guard = mb - (prob.en_lv1 - 1)
     out = {}
-    for kern, shape in ((1, 0), (0, 0), (1, 1)):
-        print(f"16^3 bricks MB {mb} eig_kernel {kern} eig_shape {shape}", flush=True)
+    for kern, shape, ortho in ((1, 0, 1), (0, 0, 1), (1, 1, 1), (1, 1, 0)):
+        print(f"16^3 bricks MB {mb} eig_kernel {kern} eig_shape {shape} eig_ortho {ortho}",
+              flush=True)
         res = runner.run(prob, precision=64, keep_ctx=True,
                          extra=f"-eig_guard {guard} -eig_autotune 0 -eig_kernel {kern} "
-                               f"-eig_shape {shape}")
+                               f"-eig_shape {shape} -eig_ortho {ortho}")
         try:
             _, ev = P.PC_get_basis(res["ctx"], 2)
         finally:
@@ -251,10 +254,10 @@ def test_16cube_lobpcg_kernels_agree(brick16, mb):
         assert st["eig_shape"] == shape
         assert worst <= 1e-8
         assert res["reason"] == "CONVERGED_RTOL"
-        out[(kern, shape)] = (ev, res["x"], res["its"])
-    ref = out[(1, 0)]
+        out[(kern, shape, ortho)] = (ev, res["x"], res["its"])
+    ref = out[(1, 0, 1)]
     assert abs(ref[2] - its_ref) <= 1
-    for key in ((0, 0), (1, 1)):
+    for key in ((0, 0, 1), (1, 1, 1), (1, 1, 0)):
         assert abs(out[key][2] - ref[2]) <= 1
         assert _rel(out[key][1], ref[1]) <= 1e-7
         assert np.allclose(out[key][0][:, 1:], ref[0][:, 1:], rtol=1e-8)
