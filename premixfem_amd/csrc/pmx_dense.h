@@ -64,6 +64,18 @@ __device__ inline void wg_matmul(const double* A, const double* B, double* C, in
   __syncthreads();
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates and two Newton steps.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y * fma(-0.5 * x * y, y, 1.5);
+}
+
 // Pair k (0 <= k < mp / 2) of round r (0 <= r < mp - 1) of the round-robin
 // ordering over mp (even) indices: (r, mp - 1) for k = 0, else
 // (r + k, r - k) mod (mp - 1); returned with p < q.  Over the mp - 1 rounds
@@ -118,11 +130,13 @@ __device__ inline int wg_syev2(double* A0, double* A1, double* Q0, double* Q1, d
         if (q < m) {
           const double app = Ac[p * ld + p], aqq = Ac[q * ld + q], apq = Ac[p * ld + q];
           // the threshold test and the rotation are computed side by side
-          // (no branch between them); apq = 0 takes the identity
+          // (no branch between them); apq = 0 takes the identity.  The
+          // reciprocals and 1/sqrt are the hardware estimates refined by two
+          // Newton steps (c and s to a few ulps; A and Q take the same ones).
           const bool big = fabs(apq) > 1e-300 && fabs(apq) > 1e-17 * sqrt(fabs(app * aqq));
-          const double th = (aqq - app) / (2.0 * (apq != 0.0 ? apq : 1.0));
-          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-          const double cr = 1.0 / sqrt(t * t + 1.0);
+          const double th = (aqq - app) * rcp_nr(2.0 * (apq != 0.0 ? apq : 1.0));
+          const double t = (th >= 0.0 ? 1.0 : -1.0) * rcp_nr(fabs(th) + sqrt(th * th + 1.0));
+          const double cr = rsq_nr(t * t + 1.0);
           c = big ? cr : 1.0;
           s = big ? t * cr : 0.0;
           if (big) s_rot[fl] = 1;
