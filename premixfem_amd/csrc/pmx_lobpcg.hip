@@ -629,12 +629,14 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 // small ones).  The best pair depends on the subdomain size, the eigen count
 // and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed
 // from the start the setup will use (random, or the learned warm start) on
-// the whole batch (a strided sample of 1024 subdomains on larger problems:
-// the launch is timed at the occupancy it runs at); the fastest sets
+// the whole batch (a strided sample of 512 subdomains on larger problems,
+// one full round of workgroups at two per CU: the launch is timed at the
+// occupancy it runs at); the fastest sets
 // -eig_guard and -eig_shape, cached per (largest subdomain, en_lv1, well
 // term, warm start, kernel) for the process.  Tuning only changes how fast
 // LOBPCG reaches the tolerance.
 namespace {
+constexpr int kTuneSample = 512;   // subdomains timed per trial on larger problems
 struct TuneKey {
   int maxn, k, well, warm, kernel;
   bool operator<(const TuneKey& o) const {
@@ -665,7 +667,7 @@ int lobpcg_autotune(pmx_ctx* ctx) {
   auto it = tune_cache().find(key);
   if (it == tune_cache().end()) {
     const int need = ctx->have_wd ? k + 1 : k;   // at least one guard vector
-    const int stride = std::max(1, (ctx->nsub + 1023) / 1024);
+    const int stride = std::max(1, (ctx->nsub + kTuneSample - 1) / kTuneSample);
     TuneChoice best{0, 0};
     float best_ms = 0.0f;
     const int user_shape = ctx->eig_shape;
