@@ -130,7 +130,8 @@ typedef struct {
  * 512-thread workgroup per CU on large subdomains, default, or two 256-thread
  * ones per CU; same eigenpairs), -eig_ortho 0|1 (matrix-core kernel: W
  * made M-orthogonal to [X P] and M-orthonormal from one Gram of [X P W] in
- * coefficient space, default, or by two explicit projection passes; same
+ * coefficient space, default, or by two explicit projection passes, not
+ * available for subdomains of at most 512 cells or in pmx_setup_batch; same
  * eigenpairs to the eigensolver tolerance),
  * -coarse_collapse 0|1 (one process with an exact level-2 factor: apply only
  * the first level-2 stage of the coarse cycle, since the later level-3 and

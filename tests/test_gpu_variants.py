@@ -227,3 +227,13 @@ def test_plane_factor_matches_dense_inverse(mesh, sd, k):
     assert abs(a["its"] - b["its"]) <= 1
     assert err <= 1e-9
     assert a["stats"]["eigen_max_lv2"] == pytest.approx(b["stats"]["eigen_max_lv2"], rel=1e-6)
+
+
+def test_explicit_projections_refused_on_small_subdomains():
+    """-eig_ortho 0 (the matrix-core LOBPCG's explicit projections) is off for
+    subdomains of at most 512 cells: refused with PETSC_ERR_SUP at setup,
+    before any launch."""
+    prob = problems.config("small")
+    with pytest.raises(P.PMXError) as e:
+        runner.run(prob, extra="-eig_ortho 0 -eig_autotune 0")
+    assert e.value.code == 56
