@@ -637,13 +637,15 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 // and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed
 // from the start the setup will use (random, or the learned warm start) on
 // the whole batch (a strided sample of 512 subdomains on larger problems,
-// one full round of workgroups at two per CU: the launch is timed at the
+// one full round of workgroups at two per CU, 1024 for small subdomains,
+// whose 128-thread workgroups run more per CU: the launch is timed at the
 // occupancy it runs at); the fastest sets
 // -eig_guard and -eig_shape, cached per (largest subdomain, en_lv1, well
 // term, warm start, kernel) for the process.  Tuning only changes how fast
 // LOBPCG reaches the tolerance.
 namespace {
 constexpr int kTuneSample = 512;   // subdomains timed per trial on larger problems
+                                   // (twice that for subdomains of <= kLobSmallN cells)
 struct TuneKey {
   int maxn, k, well, warm, kernel;
   bool operator<(const TuneKey& o) const {
@@ -674,7 +676,9 @@ int lobpcg_autotune(pmx_ctx* ctx) {
   auto it = tune_cache().find(key);
   if (it == tune_cache().end()) {
     const int need = ctx->have_wd ? k + 1 : k;   // at least one guard vector
-    const int stride = std::max(1, (ctx->nsub + kTuneSample - 1) / kTuneSample);
+    // small subdomains run more workgroups per CU: twice the sample
+    const int sample = maxn <= kLobSmallN ? 2 * kTuneSample : kTuneSample;
+    const int stride = std::max(1, (ctx->nsub + sample - 1) / sample);
     TuneChoice best{0, 0};
     float best_ms = 0.0f;
     const int user_shape = ctx->eig_shape;
