@@ -1272,17 +1272,17 @@ int launch_coarse_update(pmx_ctx* ctx, const double* d, double* t, double* u, in
   return PMX_SUCCESS;
 }
 
-// Y = A_c X for m column vectors (column-major, stride nc).
+// Y = A_c X for m column vectors (column-major, stride nc); one thread per
+// row and column (grid y = the column), so the columns' rows run in
+// parallel rather than one after the other in a thread.
 __global__ void k_coarse_spmm(Geo g, const double* __restrict__ Acb, int kmax,
-                              const double* __restrict__ X, double* __restrict__ Y, int nc,
-                              int m) {
+                              const double* __restrict__ X, double* __restrict__ Y, int nc) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gid / kmax, i = gid % kmax;
+  const int s = gid / kmax, i = gid % kmax, c = blockIdx.y;
   if (s >= g.nsub) return;
   const Brick& b = g.bricks[s];
   if (i >= b.neig) return;
-  for (int c = 0; c < m; ++c)
-    Y[(size_t)c * nc + b.dof + i] = coarse_row(g, Acb, kmax, s, i, X + (size_t)c * nc);
+  Y[(size_t)c * nc + b.dof + i] = coarse_row(g, Acb, kmax, s, i, X + (size_t)c * nc);
 }
 
 // d = V (A_cc^{-1} (V^T t)) with the LU of A_cc (level-3 apply,
@@ -2107,8 +2107,8 @@ int level3_setup(pmx_ctx* ctx) {
                        wk.wall.p, fail.p);
     PMX_TRY(wk.xform(Y.p, m, m, 0, 0, m, W.p));
     std::swap(Y, W);
-    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
-                       ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n, m);
+    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm, m), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->Acb.p, ctx->en_lv1, Y.p, Z.p, n);
     PMX_TRY(wk.gram(Y.p, m, Z.p, m));
     hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, wk.G.p, m, 1, wk.T.p,
                        w.p, fail.p);
@@ -2252,8 +2252,8 @@ int level3_setup_icc(pmx_ctx* ctx) {
       return PMX_ERR_USER;
     }
     // Rayleigh-Ritz on S^T A_c S.
-    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm), dim3(kBlock), 0, ctx->stream, ctx->geo,
-                       ctx->Acb.p, ctx->en_lv1, S.p, Z.p, n, ns);
+    hipLaunchKernelGGL(k_coarse_spmm, dim3(nspmm, ns), dim3(kBlock), 0, ctx->stream, ctx->geo,
+                       ctx->Acb.p, ctx->en_lv1, S.p, Z.p, n);
     PMX_TRY(wk.gram(S.p, ns, Z.p, ns));
     hipLaunchKernelGGL(k_lv3_small, dim3(1), dim3(256), 0, ctx->stream, wk.G.p, ns, 1, wk.T.p,
                        wk.wall.p, wk.flag.p);
