@@ -1999,6 +1999,29 @@ __global__ void k_lv3_sires(const double* __restrict__ Y, const double* __restri
   if (threadIdx.x == 0) res[j] = sqrt(s);
 }
 
+// The level-3 iterations' random start block: count standard normal values
+// of the context's seed (mt19937_64 seeded with seed + 7), generated on the
+// host once and kept on the device for the setups that follow (the same
+// values every setup; generating them took ~10 ms of host time per setup at
+// nc = 32768, m = 16).
+static int lv3_random_block(pmx_ctx* ctx, size_t count, double* dst) {
+  const unsigned long long seed = (unsigned long long)ctx->seed + 7;
+  if (ctx->lv3_rand.n != count || ctx->lv3_rand_seed != seed || !ctx->lv3_rand.p) {
+    std::vector<double> h(count);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd;
+    for (auto& v : h) v = nd(rng);
+    PMX_TRY(ctx->lv3_rand.alloc(count));
+    PMX_HIP(hipMemcpyAsync(ctx->lv3_rand.p, h.data(), count * sizeof(double),
+                           hipMemcpyHostToDevice, ctx->stream));
+    PMX_HIP(hipStreamSynchronize(ctx->stream));   // h is freed on return
+    ctx->lv3_rand_seed = seed;
+  }
+  PMX_HIP(hipMemcpyAsync(dst, ctx->lv3_rand.p, count * sizeof(double), hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  return PMX_SUCCESS;
+}
+
 int level3_setup(pmx_ctx* ctx) {
   const int n = ctx->nc;
   const int k2 = std::min(ctx->en_lv2, n);
@@ -2032,12 +2055,7 @@ int level3_setup(pmx_ctx* ctx) {
   PMX_TRY(wk.T.alloc(kLv3Max * kLv3Max));
   PMX_TRY(wk.wall.alloc(kLv3Max));
   PMX_TRY(wk.flag.alloc(1));
-  std::vector<double> h((size_t)n * m);
-  std::mt19937_64 rng(ctx->seed + 7);
-  std::normal_distribution<double> nd;
-  for (auto& v : h) v = nd(rng);
-  PMX_HIP(hipMemcpyAsync(X.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                         ctx->stream));
+  PMX_TRY(lv3_random_block(ctx, (size_t)n * m, X.p));
   PMX_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), ctx->stream));
   const int nspmm = (ctx->nsub * ctx->en_lv1 + kBlock - 1) / kBlock;
   std::vector<double> hw(m), hres(m), hist;
@@ -2200,12 +2218,7 @@ int level3_setup_icc(pmx_ctx* ctx) {
   const size_t colb = (size_t)n * sizeof(double);
 
   // Initial block: previous eigenvectors (if any) and M^{-1} of random columns.
-  std::vector<double> h((size_t)n * m);
-  std::mt19937_64 rng(ctx->seed + 7);
-  std::normal_distribution<double> nd;
-  for (auto& v : h) v = nd(rng);
-  PMX_HIP(hipMemcpyAsync(R.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                         ctx->stream));
+  PMX_TRY(lv3_random_block(ctx, (size_t)n * m, R.p));
   PMX_TRY(launch_coarse_prec(ctx, R.p, S.p, m));
   const bool warm = ctx->lv3_warm && ctx->V2.p && ctx->lv3_prev_n == n && ctx->eigen_num_lv2 > 0;
   if (warm)

@@ -429,6 +429,8 @@ struct pmx_ctx {
   int lv3_iters = 0;
   int lv3_warm = 1;                 // -lv2_icc: start level 3 from the previous eigenvectors
   int lv3_prev_n = 0;               // coarse size of the previous level-3 solve
+  pmx::DBuf<double> lv3_rand;       // level-3 random start block (seeded), kept
+  unsigned long long lv3_rand_seed = 0;  // for the setups that follow
   double lv3_floor = 0.0;           // attained residual when the level-3 solver stagnated
 
   // ---- work vectors ----
