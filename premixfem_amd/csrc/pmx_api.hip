@@ -635,9 +635,13 @@ static int setup_fine(pmx_ctx* c) {
     c->geo.pin = -1;
     c->geo.tau = 0.0;
   }
-  PMX_TRY(c->fac_f.alloc((size_t)c->ncell * 4));
-  PMX_TRY(c->fac_b.alloc((size_t)c->ncell * 4));
-  PMX_TRY(launch_dic_factor(c, 0, 0.0, c->fac_f.p, c->fac_b.p, false));
+  // The FP64 factor serves the FP64 preconditioner and the exact level-1
+  // blocks; the FP32 preconditioner sweeps its own FP32 factor only.
+  if (c->pc_precision != 32 || c->use_full_Cholesky_lv1) {
+    PMX_TRY(c->fac_f.alloc((size_t)c->ncell * 4));
+    PMX_TRY(c->fac_b.alloc((size_t)c->ncell * 4));
+    PMX_TRY(launch_dic_factor(c, 0, 0.0, c->fac_f.p, c->fac_b.p, false));
+  }
   c->geo.l1L = c->geo.l1U = nullptr;
   c->geo.l1off = nullptr;
   if (c->use_full_Cholesky_lv1) PMX_TRY(chol_lv1_setup(c));
