@@ -234,16 +234,15 @@ int launch_galerkin(pmx_ctx* ctx) {
 constexpr int kSwT = 64;          // trailing-update tile (pivot blocks are whole tiles)
 constexpr int kSwWideN = 4096;    // 128-wide pivot blocks above this size
 
-// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to SW):
-// 4 SW threads, thread t holds row t/4, columns (SW/4)(t%4) .. in registers;
-// one barrier per elimination step (the row / column broadcast buffers
-// alternate).  The unsymmetrised inverse goes through Dt (global scratch) to
-// be symmetrised.
+// Gauss-Jordan inverse of the bs x bs pivot block (identity-padded to SW),
+// read from src (leading dimension ld): 4 SW threads, thread t holds row
+// t/4, columns (SW/4)(t%4) .. in registers; one barrier per elimination step
+// (the row / column broadcast buffers alternate).  The unsymmetrised inverse
+// goes through Dt (global scratch) to be symmetrised into D.
 template <int SW>
-__global__ void __launch_bounds__(4 * SW) k_sweep_pivot(const double* __restrict__ A, int n,
-                                                        int k0, int bs,
-                                                        double* __restrict__ D,
-                                                        double* __restrict__ Dt) {
+__device__ __forceinline__ void sweep_pivot_block(const double* src, int ld, int bs,
+                                                  double* __restrict__ D,
+                                                  double* __restrict__ Dt) {
   constexpr int CW = SW / 4;   // columns per thread
   __shared__ double2 rowv2[2][SW / 2];
   __shared__ double colv[2][SW];
@@ -252,7 +251,7 @@ __global__ void __launch_bounds__(4 * SW) k_sweep_pivot(const double* __restrict
 #pragma unroll
   for (int c = 0; c < CW; ++c) {
     const int j = q * CW + c;
-    m[c] = (r < bs && j < bs) ? A[(size_t)(k0 + r) * n + k0 + j] : (r == j ? 1.0 : 0.0);
+    m[c] = (r < bs && j < bs) ? src[(size_t)r * ld + j] : (r == j ? 1.0 : 0.0);
   }
   for (int tq = 0; tq < 4; ++tq) {
 #pragma unroll
@@ -288,6 +287,14 @@ __global__ void __launch_bounds__(4 * SW) k_sweep_pivot(const double* __restrict
   }
 }
 
+template <int SW>
+__global__ void __launch_bounds__(4 * SW) k_sweep_pivot(const double* __restrict__ A, int n,
+                                                        int k0, int bs,
+                                                        double* __restrict__ D,
+                                                        double* __restrict__ Dt) {
+  sweep_pivot_block<SW>(A + (size_t)k0 * n + k0, n, bs, D, Dt);
+}
+
 // P[r][t] = A[r][k0+t] for t < bs, 0 beyond.
 template <int SW>
 __global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int bs,
@@ -302,11 +309,16 @@ __global__ void k_sweep_panel(const double* __restrict__ A, int n, int k0, int b
 
 // Trailing update A_ij -= U_i P_j^T on 64x64 tiles (FP64 MFMA, K = bs); the
 // tiles of the pivot block rows / columns take U, U^T and -D.
+// With SW = kSwT the workgroup of the next pivot block's tile (knext, a
+// tile of its own) inverts that block right after updating it, into Dn: the
+// next step's pivot then costs no launch and overlaps this update.
 template <int SW>
 __global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, int n, int k0,
                                                       int bs, const double* __restrict__ P,
                                                       const double* __restrict__ U,
-                                                      const double* __restrict__ D) {
+                                                      const double* __restrict__ D, int knext,
+                                                      double* __restrict__ Dn,
+                                                      double* __restrict__ Dt) {
   using TL = MfmaTile<2, 2, 2, 2>;
   static_assert(TL::TM == kSwT && TL::TN == kSwT && SW % kSwT == 0,
                 "pivot blocks are whole tiles");
@@ -346,6 +358,14 @@ __global__ void __launch_bounds__(256) k_sweep_update(double* __restrict__ A, in
         const int i = I0 + r, j = J0 + c;
         if (i < n && j < n) A[(size_t)i * n + j] -= acc[rb][cb][v];
       }
+  if constexpr (SW == kSwT) {
+    if (knext < n && I0 == knext && J0 == knext) {
+      // the workgroup reads back its own stores (the barrier's workgroup
+      // fences order them; the workgroup's waves share one L1)
+      __syncthreads();
+      sweep_pivot_block<SW>(A + (size_t)knext * n + knext, n, min(SW, n - knext), Dn, Dt);
+    }
+  }
 }
 
 __global__ void k_copy_shift(const double* __restrict__ A, double* __restrict__ B, int n,
@@ -367,7 +387,7 @@ template <int SW>
 static int spd_invert_sw(hipStream_t stream, int n, const double* A, double* Ainv,
                          double shift) {
   DBuf<double> D, Dt, P, U;
-  PMX_TRY(D.alloc(SW * SW));
+  PMX_TRY(D.alloc(2 * SW * SW));   // two pivot inverses: this step's and the next
   PMX_TRY(Dt.alloc(SW * SW));
   PMX_TRY(P.alloc((size_t)n * SW));
   PMX_TRY(U.alloc((size_t)n * SW));
@@ -377,17 +397,23 @@ static int spd_invert_sw(hipStream_t stream, int n, const double* A, double* Ain
   PMX_HIP(hipGetLastError());
   const int nt = (n + kSwT - 1) / kSwT;
   const int npb = std::min((n * SW + 255) / 256, 2048);
-  for (int k0 = 0; k0 < n; k0 += SW) {
+  // With 64-wide pivots the update of step k inverts the pivot block of step
+  // k + 1 (k_sweep_update); only the first one is a launch of its own.
+  constexpr bool fused = SW == kSwT;
+  for (int k0 = 0, kk = 0; k0 < n; k0 += SW, ++kk) {
     const int bs = std::min(SW, n - k0);
-    hipLaunchKernelGGL(k_sweep_pivot<SW>, dim3(1), dim3(4 * SW), 0, stream, Ainv, n, k0, bs,
-                       D.p, Dt.p);
+    double* Dk = D.p + (size_t)(kk & 1) * SW * SW;
+    double* Dnext = D.p + (size_t)((kk + 1) & 1) * SW * SW;
+    if (!fused || kk == 0)
+      hipLaunchKernelGGL(k_sweep_pivot<SW>, dim3(1), dim3(4 * SW), 0, stream, Ainv, n, k0, bs,
+                         Dk, Dt.p);
     hipLaunchKernelGGL(k_sweep_panel<SW>, dim3(npb), dim3(256), 0, stream, Ainv, n, k0, bs,
                        P.p);
     // U = P D (the pivot columns times the pivot inverse; one 128-column tile)
     hipLaunchKernelGGL((k_gemm_mfma<2, 2, 2, 4, false>), dim3(1, (n + 63) / 64), dim3(256), 0,
-                       stream, n, SW, SW, 1.0, P.p, SW, D.p, SW, 0.0, U.p, SW);
+                       stream, n, SW, SW, 1.0, P.p, SW, Dk, SW, 0.0, U.p, SW);
     hipLaunchKernelGGL(k_sweep_update<SW>, dim3(nt, nt), dim3(256), 0, stream, Ainv, n, k0, bs,
-                       P.p, U.p, D.p);
+                       P.p, U.p, Dk, fused ? k0 + SW : n, Dnext, Dt.p);
     PMX_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_copy_shift, dim3(nblk), dim3(kBlock), 0, stream, Ainv, Ainv, n, 0.0,
