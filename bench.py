@@ -644,8 +644,8 @@ def main():
                     help="setup_solve: options appended to every mode's (recorded in the JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus must be 1 "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py: the preconditioner runs on one GPU here; --gpus must be 1 "
+              "(the process-box sharding across GPUs is not run)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "impes":
