@@ -1371,7 +1371,7 @@ constexpr int kLv3Max = 32;       // largest LOBPCG basis [X W P] (3 m)
 // they gain a constant factor per iteration (4x on the 256^3 fracture
 // medium), so kLv3StagWin iterations without a 2x gain mean the residual sits
 // on its rounding floor.
-constexpr int kLv3StagWin = 5;
+constexpr int kLv3StagWin = 3;
 
 // part[(g*ma + i)*mb + j] = sum over rows of chunk g of A[:, i] . B[:, j].
 __global__ void __launch_bounds__(256) k_gram_part(const double* __restrict__ A, int ma,
