@@ -28,6 +28,8 @@
 
 namespace pmx {
 
+typedef double pl_d4 __attribute__((ext_vector_type(4)));   // an MFMA accumulator
+
 constexpr int kPlMaxCols = 32;    // right-hand sides of one solve (level-3 set-up blocks)
 
 // One plane step for the right-hand sides c0 = NCG * blockIdx.y .. c0+NCG-1
@@ -292,6 +294,87 @@ int planes_setup(pmx_ctx* ctx) {
   return PMX_SUCCESS;
 }
 
+// The same plane step for up to 16 right-hand sides on the FP64 matrix
+// cores, for bricks of 8 eigenvectors: a workgroup of 4 waves takes two
+// subdomains of the plane (16 rows of G_z) and 16 right-hand sides, C(16 x
+// 16) = G_z[rows, :] W[:, cols] by v_mfma_f64_16x16x4f64 with the n_p
+// columns of G_z split over the waves.  Lane l feeds row l % 16 and column
+// l % 16 with four consecutive k of one 16-k block (k = 16 b + 4 (l / 16) +
+// j for the j-th product), so each operand is one 32-byte load; the waves'
+// partial tiles are added in wave order.  Rows of an absent second
+// subdomain and columns beyond ncol are computed on valid data and dropped.
+// Requires n_p % 16 == 0 (the plane's column blocks) and 32-byte aligned
+// plane and column offsets (multiples of 8 dofs).
+__global__ void __launch_bounds__(256) k_pl_step_mfma(Geo g, const double* __restrict__ Acb,
+                                                      int kmax, const double* __restrict__ G,
+                                                      int np, int dof0, int sub0, int L, int fwd,
+                                                      double* __restrict__ w,
+                                                      double* __restrict__ d, int nc, int ncol,
+                                                      const double* stop) {
+  PMX_STOP_CHECK(stop);
+  __shared__ double red[4][4][64];    // [wave][item][lane]
+  __shared__ double outv[2][16][8];   // [subdomain][right-hand side][row]
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int li = lane & 15, g4 = lane >> 4;
+  const int sA = sub0 + 2 * (int)blockIdx.x;
+  const bool has2 = 2 * (int)blockIdx.x + 1 < L;
+  const int cb = (int)blockIdx.y * 16;
+  // this lane's row of G_z (rows 8..15: the second subdomain, or the first
+  // one's again when there is none) and right-hand side
+  const int sr = (li < 8 || !has2) ? sA : sA + 1;
+  const int rowG = g.bricks[sr].dof - dof0 + (li & 7);
+  const bool rowok = li < 8 || has2;
+  const int col = cb + li < ncol ? cb + li : cb;
+  const double* Gr = G + (size_t)rowG * np;
+  const double* Wc = w + (size_t)col * nc + dof0;
+  const int nkb = np >> 4, per = (nkb + 3) >> 2;
+  const int kb0 = wv * per, kb1 = min(nkb, kb0 + per);
+  pl_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int kb = kb0; kb < kb1; ++kb) {
+    const int k = (kb << 4) + 4 * g4;
+    const double4 a = *reinterpret_cast<const double4*>(Gr + k);
+    const double4 b = *reinterpret_cast<const double4*>(Wc + k);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rowok ? a.x : 0.0, b.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rowok ? a.y : 0.0, b.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rowok ? a.z : 0.0, b.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rowok ? a.w : 0.0, b.w, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[wv][v][lane] = acc[v];
+  __syncthreads();
+  if (wv == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double x = ((red[0][v][lane] + red[1][v][lane]) + red[2][v][lane]) + red[3][v][lane];
+      const int r16 = g4 + 4 * v;   // item v of lane l: row l / 16 + 4 v, column l % 16
+      outv[r16 >> 3][li][r16 & 7] = x;
+    }
+  }
+  __syncthreads();
+  // epilogue as k_pl_step: thread t -> subdomain t / 128, right-hand side
+  // (t / 8) % 16, row t % 8
+  const int t = threadIdx.x;
+  const int half = t >> 7, c = (t >> 3) & 15, i = t & 7;
+  if (half == 1 && !has2) return;
+  if (cb + c >= ncol) return;
+  const Brick b = g.bricks[sA + half];
+  const int nr = b.neig;
+  int nbi = b.nb[fwd ? 5 : 4];
+  if (nbi >= 0 && g.bricks[nbi].box != b.box) nbi = -1;
+  const size_t co = (size_t)(cb + c) * nc;
+  if (!fwd && i < nr) d[co + b.dof + i] = outv[half][c][i];
+  if (nbi >= 0) {
+    const Brick o = g.bricks[nbi];
+    if (i < o.neig) {
+      const int slot_c = fwd ? 5 : 6;
+      const double* blk = Acb + ((size_t)nbi * 7 + slot_c) * kmax * kmax;
+      double v = 0.0;
+      for (int j = 0; j < nr; ++j) v += blk[i * kmax + j] * outv[half][c][j];
+      w[co + o.dof + i] -= v;
+    }
+  }
+}
+
 template <int KR, int NCG, int NT, int NGRP = 1>
 static void pl_step(pmx_ctx* ctx, int z, int fwd, double* w, double* d, int ncol) {
   const int L = ctx->sd[0] * ctx->sd[1];
@@ -322,9 +405,20 @@ int launch_planes_solve(pmx_ctx* ctx, const double* t, double* d, int ncol) {
       if (k8) pl_step<8, 4, 512>(ctx, z, fwd, w, d, ncol);
       else pl_step<kMaxEig, 2, 512>(ctx, z, fwd, w, d, ncol);
     } else {
-      // up to 16 right-hand sides per workgroup in 4 groups of 4
-      if (k8) pl_step<8, 4, 1024, 4>(ctx, z, fwd, w, d, ncol);
-      else pl_step<kMaxEig, 2, 1024, 4>(ctx, z, fwd, w, d, ncol);
+      const int L = ctx->sd[0] * ctx->sd[1];
+      const int dof0 = ctx->pl_dof[z], np = ctx->pl_dof[z + 1] - dof0;
+      if (k8 && np % 16 == 0 && dof0 % 8 == 0) {
+        // 16 right-hand sides per workgroup on the matrix cores
+        hipLaunchKernelGGL(k_pl_step_mfma, dim3((L + 1) / 2, (ncol + 15) / 16), dim3(256), 0,
+                           ctx->stream, ctx->geo, ctx->Acb.p, ctx->en_lv1,
+                           ctx->pl_G.p + ctx->pl_goff[z], np, dof0, z * L, L, fwd, w, d, nc,
+                           ncol, ctx->geo.stop);
+      } else if (k8) {
+        // up to 16 right-hand sides per workgroup in 4 groups of 4
+        pl_step<8, 4, 1024, 4>(ctx, z, fwd, w, d, ncol);
+      } else {
+        pl_step<kMaxEig, 2, 1024, 4>(ctx, z, fwd, w, d, ncol);
+      }
     }
   };
   for (int z = 0; z + 1 < Z; ++z) step(z, 1);
