@@ -634,7 +634,10 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 // (one 512-thread workgroup per CU, or two 256-thread ones with narrower
 // sweep chunks on large subdomains; every workgroup held to 256 registers on
 // small ones).  The best pair depends on the subdomain size, the eigen count
-// and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed
+// and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed,
+// except that a shape more than 10 % slower than the other at one block size
+// is left out at the larger ones (the shape ranking held across block sizes
+// on every workload measured), each trial starting
 // from the start the setup will use (random, or the learned warm start) on
 // the whole batch (a strided sample of 512 subdomains on larger problems,
 // one full round of workgroups at two per CU, 1024 for small subdomains,
@@ -646,6 +649,8 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 namespace {
 constexpr int kTuneSample = 512;   // subdomains timed per trial on larger problems
                                    // (twice that for subdomains of <= kLobSmallN cells)
+constexpr float kShapeMargin = 1.1f;   // a shape this much slower at one block size
+                                       // is not timed at the larger ones
 struct TuneKey {
   int maxn, k, well, warm, kernel;
   bool operator<(const TuneKey& o) const {
@@ -683,6 +688,7 @@ int lobpcg_autotune(pmx_ctx* ctx) {
     float best_ms = 0.0f;
     const int user_shape = ctx->eig_shape;
     const int nshape = ctx->eig_kernel == 1 ? 2 : 1;
+    int drop_shape = -1;   // a shape that lost one block size by more than kShapeMargin
     for (int mb : {4, 6, 8, 10}) {
       if (mb < need || mb > minn - 1) continue;
       // with the learned warm start the trials start from it, as the setup will
@@ -691,16 +697,23 @@ int lobpcg_autotune(pmx_ctx* ctx) {
         PMX_TRY(x0.alloc((size_t)ctx->ncell * mb));
         PMX_TRY(cnn_warm_start(ctx, mb, x0.p));
       }
+      float shape_ms[2] = {0.0f, 0.0f};
       for (int shape = 0; shape < nshape; ++shape) {
+        if (shape == drop_shape) continue;
         float ms = 0.0f;
         ctx->eig_shape = shape;
         const int rc = lobpcg_run(ctx, warm ? x0.p : nullptr, mb, stride, &ms);
         ctx->eig_shape = user_shape;
         PMX_TRY(rc);
+        shape_ms[shape] = ms;
         if (!best.mb || ms < best_ms) {
           best = TuneChoice{mb, shape};
           best_ms = ms;
         }
+      }
+      if (nshape == 2 && drop_shape < 0) {
+        if (shape_ms[0] > kShapeMargin * shape_ms[1]) drop_shape = 0;
+        else if (shape_ms[1] > kShapeMargin * shape_ms[0]) drop_shape = 1;
       }
     }
     if (!best.mb) return PMX_SUCCESS;   // nothing admissible: keep -eig_guard / -eig_shape
