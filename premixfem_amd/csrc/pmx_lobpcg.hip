@@ -636,8 +636,9 @@ int lobpcg_setup(pmx_ctx* ctx, const double* x0_rows) {
 // small ones).  The best pair depends on the subdomain size, the eigen count
 // and the medium.  Every admissible (MB in {4, 6, 8, 10}, shape) is timed,
 // except that a shape more than 10 % slower than the other at one block size
-// is left out at the larger ones (the shape ranking held across block sizes
-// on every workload measured), each trial starting
+// is left out at the larger ones (on the workloads measured such a shape
+// either stayed slower at the larger blocks or won them back by under 10 %
+// where the smaller block was faster still), each trial starting
 // from the start the setup will use (random, or the learned warm start) on
 // the whole batch (a strided sample of 512 subdomains on larger problems,
 // one full round of workgroups at two per CU, 1024 for small subdomains,
