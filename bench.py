@@ -597,10 +597,20 @@ def solve_roofline(prob, r, s, workload, head):
                                          block=lob["mb"], shape=lob["shape"],
                                          subdomain_iterations=round(lob["sub_iters"], 1))
         if lob["t"] * 1e6 > dk["avg_us"] * dk["launches_per_step"]:
-            roofline = dict(kernel="k_lobpcg_mf", bound="mfma", achieved=round(tf, 2),
-                            peak=FP64_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
-                            frac=round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+            # the bound the model's intensity (flop per algorithmic byte) sets
+            # against FP64's ridge, 78.6 TFLOP/s / 8 TB/s = 9.8 flop/B
+            ridge = FP64_MFMA_PEAK_TFLOPS * 1e3 / HBM_PEAK_GBS
+            if fl / by >= ridge:
+                lead = dict(bound="mfma", achieved=round(tf, 2), peak=FP64_MFMA_PEAK_TFLOPS,
+                            unit="TFLOP/s", frac=round(tf / FP64_MFMA_PEAK_TFLOPS, 4))
+            else:
+                lead = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS,
+                            unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4))
+            roofline = dict(kernel="k_lobpcg_mf", **lead,
                             traffic=measured_traffic(workload, head, "k_lobpcg_mf"),
+                            intensity_model=round(fl / by, 2),
+                            tflops_model=round(tf, 2),
+                            mfma_frac_model=round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
                             hbm_gbs_model=round(gbs, 1),
                             hbm_frac_model=round(gbs / HBM_PEAK_GBS, 4),
                             solve_kernel=dict(kernel=dom, achieved=round(dk["gbs"], 1),
@@ -638,7 +648,7 @@ def main():
                          "subdomain shape as the workload)")
     ap.add_argument("--train-fields", type=int, default=32,
                     help="learned mode: seeded training fields of the workload's family")
-    ap.add_argument("--train-epochs", type=int, default=30, help="learned mode: training epochs")
+    ap.add_argument("--train-epochs", type=int, default=180, help="learned mode: training epochs")
     ap.add_argument("--precision", type=int, default=32, help="impes: preconditioner precision")
     ap.add_argument("--extra", default="",
                     help="setup_solve: options appended to every mode's (recorded in the JSON)")
